@@ -1,0 +1,456 @@
+"""`RocmBackend`: the object-safe hardware trait of the reference, over the C ABI.
+
+Mirrors `trait GpuBackend` (crates/oxidelake-device/src/backend.rs:44-198)
+method for method; every call goes to liboxide_rocm.so (include/oxide_rocm.h).
+Errors mirror `EngineError` (crates/oxidelake-core/src/error.rs:17-73):
+`Unsupported` is the per-batch fallback signal, every other error fails the
+query. Host batches are pyarrow RecordBatches (the reference's arrow-rs
+RecordBatch), moved through the Arrow C Data Interface.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+import pyarrow as pa
+
+from . import plan
+from ._capi import (
+    EALLOCATION,
+    EDEVICE,
+    EEXECUTION,
+    EFORMAT,
+    EPLAN,
+    EUNSUPPORTED,
+    Agg,
+    AggSpec,
+    ArrowArray,
+    ArrowSchema,
+    Column,
+    lib,
+)
+from .plan import AggregateSpec, DistanceMetric, Encoded, ExprT
+
+
+# ------------------------------------------------------------------ errors
+class EngineError(Exception):
+    kind = "Execution"
+
+    def is_unsupported(self) -> bool:
+        return False
+
+
+class Unsupported(EngineError):
+    kind = "Unsupported"
+
+    def __init__(self, feature: str, detail: str):
+        super().__init__(f"unsupported: {feature} ({detail})")
+        self.feature = feature
+        self.detail = detail
+
+    def is_unsupported(self) -> bool:
+        return True
+
+
+class DeviceError(EngineError):
+    kind = "Device"
+
+
+class ExecutionError(EngineError):
+    kind = "Execution"
+
+
+class PlanError(EngineError):
+    kind = "Plan"
+
+
+class FormatError(EngineError):
+    kind = "Format"
+
+
+class AllocationError(EngineError):
+    kind = "Allocation"
+
+
+def check(rc: int) -> None:
+    if rc == 0:
+        return
+    msg = (lib.oxr_last_error() or b"").decode()
+    feat = (lib.oxr_last_feature() or b"").decode()
+    if rc == EUNSUPPORTED:
+        raise Unsupported(feat or "rocm", msg)
+    cls = {
+        EDEVICE: DeviceError,
+        EEXECUTION: ExecutionError,
+        EPLAN: PlanError,
+        EFORMAT: FormatError,
+        EALLOCATION: AllocationError,
+    }.get(rc, EngineError)
+    raise cls(f"rocm device error: {msg}" if rc == EDEVICE else msg)
+
+
+# ------------------------------------------------------------------ handles
+class Stream:
+    """`DeviceStream` (handles.rs): an owned HIP stream or a borrowed one."""
+
+    def __init__(self, handle: int, backend: "RocmBackend"):
+        self.handle = handle
+        self.backend = backend
+
+    def synchronize(self) -> None:
+        check(lib.oxr_stream_sync(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.oxr_stream_destroy(self.handle)
+            self.handle = None
+
+
+class _CudaArrayView:
+    """Zero-copy `__cuda_array_interface__` over a device column."""
+
+    def __init__(self, ptr: int, shape, typestr: str, owner):
+        self.__cuda_array_interface__ = {
+            "shape": tuple(shape),
+            "typestr": typestr,
+            "data": (ptr, False),
+            "version": 2,
+            "strides": None,
+        }
+        self._owner = owner
+
+
+_TYPESTR = {plan.INT32: "<i4", plan.INT64: "<i8", plan.FLOAT32: "<f4", plan.FLOAT64: "<f8",
+            plan.F32_VECTOR: "<f4"}
+_WIDTH = {plan.INT32: 4, plan.INT64: 8, plan.FLOAT32: 4, plan.FLOAT64: 8}
+
+
+class DeviceBatch:
+    """`DeviceBatch` (handles.rs:130): a batch resident in HBM."""
+
+    def __init__(self, handle: int, keepalive=None):
+        self.handle = handle
+        self._keep = keepalive
+
+    @property
+    def num_rows(self) -> int:
+        return int(lib.oxr_batch_rows(self.handle))
+
+    @property
+    def num_columns(self) -> int:
+        return int(lib.oxr_batch_ncols(self.handle))
+
+    def name(self, i: int) -> str:
+        return lib.oxr_batch_name(self.handle, i).decode()
+
+    def column(self, i: int) -> Column:
+        c = Column()
+        check(lib.oxr_batch_column(self.handle, i, C.byref(c)))
+        return c
+
+    def torch_column(self, i: int):
+        """Zero-copy torch views (values, validity-bytes or None) of column i."""
+        import torch
+
+        c = self.column(i)
+        shape = (c.len, c.dim) if c.type == plan.F32_VECTOR else (c.len,)
+        if c.len == 0:
+            dt = {"<i4": torch.int32, "<i8": torch.int64, "<f4": torch.float32,
+                  "<f8": torch.float64}[_TYPESTR[c.type]]
+            return torch.empty(shape, dtype=dt, device="cuda"), None
+        vals = torch.as_tensor(_CudaArrayView(c.values, shape, _TYPESTR[c.type], self), device="cuda")
+        valid = None
+        if c.validity:
+            valid = torch.as_tensor(
+                _CudaArrayView(c.validity, ((c.len + 7) // 8,), "|u1", self), device="cuda")
+        return vals, valid
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.oxr_batch_free(self.handle)
+            self.handle = None
+
+
+class JoinTable:
+    """`DeviceJoinTable` (handles.rs:224)."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    def info(self) -> tuple[int, int, bool]:
+        d, r, dense = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib.oxr_join_table_info(self.handle, C.byref(d), C.byref(r), C.byref(dense)))
+        return int(d.value), int(r.value), bool(dense.value)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.oxr_join_table_free(self.handle)
+            self.handle = None
+
+
+class JoinMatches:
+    """`DeviceJoinMatches` (handles.rs:303): one probe batch's matches."""
+
+    def __init__(self, handle: int, total: int, probe_rows: int):
+        self.handle = handle
+        self.total = total
+        self.probe_rows = probe_rows
+
+    def __len__(self) -> int:
+        return self.total
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.oxr_join_matches_free(self.handle)
+            self.handle = None
+
+
+@dataclass
+class MemoryInfo:
+    total_bytes: int
+    free_bytes: int
+    pinned_supported: bool
+
+    @property
+    def used_bytes(self) -> int:
+        return max(0, self.total_bytes - self.free_bytes)
+
+
+def _cstrs(names: Optional[Sequence[str]], n: int):
+    if names is None:
+        return None, None
+    raw = [s.encode() for s in names]
+    arr = (C.c_char_p * max(1, n))(*raw)
+    return arr, raw
+
+
+class RocmBackend:
+    """The MI355X backend behind the reference's `GpuBackend` trait."""
+
+    def __init__(self, ordinal: int = 0):
+        why = C.create_string_buffer(256)
+        rc = lib.oxr_probe(ordinal, why, 256)
+        if rc != 0:
+            raise Unsupported("rocm.probe", why.value.decode() or "no gfx950 device")
+        dev = C.c_void_p()
+        check(lib.oxr_open(ordinal, C.byref(dev)))
+        self.dev = dev.value
+        self.ordinal = ordinal
+
+    # -- identity (backend.rs:46-53)
+    def kind(self) -> str:
+        return "rocm"
+
+    def device_id(self) -> int:
+        return self.ordinal
+
+    def memory_info(self) -> MemoryInfo:
+        t, f, p = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib.oxr_mem_info(self.dev, C.byref(t), C.byref(f), C.byref(p)))
+        return MemoryInfo(int(t.value), int(f.value), bool(p.value))
+
+    # -- streams (backend.rs:66, 83)
+    def create_stream(self) -> Stream:
+        s = C.c_void_p()
+        check(lib.oxr_stream_create(self.dev, C.byref(s)))
+        return Stream(s.value, self)
+
+    def wrap_stream(self, hip_stream: int) -> Stream:
+        s = C.c_void_p()
+        check(lib.oxr_stream_wrap(self.dev, C.c_void_p(hip_stream), C.byref(s)))
+        return Stream(s.value, self)
+
+    def torch_stream(self) -> Stream:
+        import torch
+
+        return self.wrap_stream(torch.cuda.current_stream().cuda_stream)
+
+    def synchronize(self, stream: Stream) -> None:
+        stream.synchronize()
+
+    # -- batches (backend.rs:86-97)
+    def upload(self, stream: Stream, batch: pa.RecordBatch) -> DeviceBatch:
+        schema, array = ArrowSchema(), ArrowArray()
+        batch._export_to_c(C.addressof(array), C.addressof(schema))
+        out = C.c_void_p()
+        try:
+            check(lib.oxr_upload(stream.handle, C.byref(schema), C.byref(array), C.byref(out)))
+        finally:
+            _release(array)
+            _release(schema)
+        return DeviceBatch(out.value)
+
+    def download(self, stream: Stream, batch: DeviceBatch) -> pa.RecordBatch:
+        schema, array = ArrowSchema(), ArrowArray()
+        check(lib.oxr_download(stream.handle, batch.handle, C.byref(schema), C.byref(array)))
+        return pa.RecordBatch._import_from_c(C.addressof(array), C.addressof(schema))
+
+    def wrap(self, columns: Sequence[tuple], names: Optional[Sequence[str]] = None,
+             rows: Optional[int] = None) -> DeviceBatch:
+        """A batch over device tensors (no copy): each column is
+        (values_tensor, validity_bytes_tensor_or_None, oxr_type[, dim])."""
+        cols = (Column * max(1, len(columns)))()
+        keep = []
+        nrows = rows
+        for i, spec in enumerate(columns):
+            t, valid, typ = spec[0], spec[1], spec[2]
+            dim = spec[3] if len(spec) > 3 else (t.shape[1] if typ == plan.F32_VECTOR else 0)
+            n = t.shape[0]
+            nrows = n if nrows is None else nrows
+            cols[i].type = typ
+            cols[i].dim = dim
+            cols[i].len = n
+            cols[i].values = t.data_ptr()
+            cols[i].validity = valid.data_ptr() if valid is not None else None
+            keep.extend([t, valid])
+        arr, raw = _cstrs(names, len(columns))
+        out = C.c_void_p()
+        check(lib.oxr_batch_wrap(self.dev, cols, arr, len(columns), nrows or 0, C.byref(out)))
+        return DeviceBatch(out.value, keepalive=(keep, raw))
+
+    # -- operators
+    def supports_predicate(self, predicate: ExprT | None) -> bool:
+        """backend.rs:182-186."""
+        enc = Encoded(predicate)
+        return bool(lib.oxr_supports_predicate(enc.ref()))
+
+    def supports_hash_join(self) -> bool:
+        return True
+
+    def supports_aggregate(self) -> bool:
+        return True
+
+    def filter_project(self, stream: Stream, batch: DeviceBatch, predicate: ExprT | None,
+                       projection: Sequence, names: Optional[Sequence[str]] = None) -> DeviceBatch:
+        """backend.rs:98-102. `projection` items are column indices (the
+        reference's form) or expressions."""
+        exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
+        if names is None:
+            names = [batch.name(p.index) if isinstance(p, plan.Col) else f"p{i}"
+                     for i, p in enumerate(exprs)]
+        pred = Encoded(predicate)
+        parr, penc = plan.encode_many(exprs)
+        narr, raw = _cstrs(names, len(exprs))
+        out = C.c_void_p()
+        check(lib.oxr_filter_project(stream.handle, batch.handle, pred.ref(), parr, len(exprs),
+                                     narr, C.byref(out)))
+        return DeviceBatch(out.value)
+
+    def prepare_join_build(self, stream: Stream, build: DeviceBatch, key: int) -> JoinTable:
+        """backend.rs:129-142 (the table is built on the device)."""
+        out = C.c_void_p()
+        check(lib.oxr_join_build(stream.handle, build.handle, key, C.byref(out)))
+        return JoinTable(out.value)
+
+    def probe_join(self, stream: Stream, probe: DeviceBatch, key: int, table: JoinTable) -> JoinMatches:
+        """backend.rs:153-163."""
+        out, total = C.c_void_p(), C.c_uint64()
+        check(lib.oxr_join_probe(stream.handle, table.handle, probe.handle, key, C.byref(out),
+                                 C.byref(total)))
+        return JoinMatches(out.value, int(total.value), probe.num_rows)
+
+    def hash_join(self, stream: Stream, left: DeviceBatch, right: DeviceBatch, table: JoinTable,
+                  matches: JoinMatches, first: int, rows: int,
+                  projection: Optional[Sequence[int]] = None) -> DeviceBatch:
+        """backend.rs:114-118."""
+        out = C.c_void_p()
+        if projection is None:
+            proj, n = None, -1
+        else:
+            proj = (C.c_int32 * max(1, len(projection)))(*projection)
+            n = len(projection)
+        check(lib.oxr_join_gather(stream.handle, left.handle, right.handle, table.handle,
+                                  matches.handle, first, rows, proj, n, C.byref(out)))
+        return DeviceBatch(out.value)
+
+    def aggregate(self, stream: Stream, batch: DeviceBatch, spec: AggregateSpec) -> DeviceBatch:
+        """backend.rs:166-170."""
+        cspec, keep = _agg_spec(spec)
+        out = C.c_void_p()
+        check(lib.oxr_aggregate(stream.handle, batch.handle, C.byref(cspec), C.byref(out)))
+        return DeviceBatch(out.value)
+
+    def vector_distance(self, stream: Stream, batch: DeviceBatch, column: int,
+                        query: Sequence[float], metric: DistanceMetric,
+                        output_name: str) -> DeviceBatch:
+        """backend.rs:173-177."""
+        q = np.ascontiguousarray(np.asarray(query, dtype=np.float32))
+        out = C.c_void_p()
+        check(lib.oxr_vector_distance(stream.handle, batch.handle, column,
+                                      q.ctypes.data_as(C.POINTER(C.c_float)), len(q), int(metric),
+                                      output_name.encode(), C.byref(out)))
+        return DeviceBatch(out.value)
+
+    def vector_topk(self, stream: Stream, batch: DeviceBatch, column: int, queries: np.ndarray,
+                    metric: DistanceMetric, k: int) -> tuple[np.ndarray, np.ndarray]:
+        q = np.ascontiguousarray(np.asarray(queries, dtype=np.float32))
+        if q.ndim == 1:
+            q = q[None, :]
+        nq, dim = q.shape
+        rows = np.empty((nq, k), dtype=np.int64)
+        dist = np.empty((nq, k), dtype=np.float32)
+        check(lib.oxr_vector_topk(stream.handle, batch.handle, column,
+                                  q.ctypes.data_as(C.POINTER(C.c_float)), nq, dim, int(metric), k,
+                                  rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  dist.ctypes.data_as(C.POINTER(C.c_float))))
+        return rows, dist
+
+
+class AggState:
+    """Streaming aggregation state resident in HBM (oxr_agg_*)."""
+
+    def __init__(self, backend: RocmBackend, stream: Stream, like: DeviceBatch, spec: AggregateSpec):
+        self.cspec, self._keep = _agg_spec(spec)
+        out = C.c_void_p()
+        check(lib.oxr_agg_create(stream.handle, like.handle, C.byref(self.cspec), C.byref(out)))
+        self.handle = out.value
+
+    def update(self, stream: Stream, batch: DeviceBatch) -> None:
+        check(lib.oxr_agg_update(stream.handle, self.handle, batch.handle))
+
+    def update_join(self, stream: Stream, table: JoinTable, build: DeviceBatch, build_group: int,
+                    probe: DeviceBatch, probe_key: int) -> None:
+        check(lib.oxr_agg_update_join(stream.handle, self.handle, table.handle, build.handle,
+                                      build_group, probe.handle, probe_key))
+
+    def finish(self, stream: Stream) -> DeviceBatch:
+        out = C.c_void_p()
+        check(lib.oxr_agg_finish(stream.handle, self.handle, C.byref(out)))
+        return DeviceBatch(out.value)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.oxr_agg_free(self.handle)
+            self.handle = None
+
+
+def _agg_spec(spec: AggregateSpec):
+    aggs = (Agg * max(1, len(spec.aggregates)))()
+    keep = []
+    for i, a in enumerate(spec.aggregates):
+        enc = Encoded(a.input)
+        keep.append(enc)
+        aggs[i].func = int(a.func)
+        aggs[i].input = enc.expr
+        if a.name is not None:
+            nm = a.name.encode()
+            keep.append(nm)
+            aggs[i].name = nm
+        else:
+            aggs[i].name = None
+    flt = None
+    if spec.filter is not None:
+        fenc = Encoded(spec.filter)
+        keep.append(fenc)
+        flt = C.pointer(fenc.expr)
+    cspec = AggSpec(spec.group_by, len(spec.aggregates), aggs, flt, spec.groups_hint)
+    keep.append(aggs)
+    return cspec, keep
+
+
+def _release(struct) -> None:
+    if struct.release:
+        fn = C.CFUNCTYPE(None, C.c_void_p)(struct.release)
+        fn(C.addressof(struct))

@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every symbol include/oxide_rocm.h declares
+(no device calls: this runs without a GPU)."""
+import ctypes as C
+
+from oxidelake_amd._capi import LIB_PATH, SIGNATURES, exported_symbols
+
+
+def test_library_is_built():
+    assert LIB_PATH.exists(), "run __graft_entry__.build() first"
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(str(LIB_PATH))
+    missing = [s for s in exported_symbols() if not hasattr(lib, s)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_bindings_cover_the_header():
+    assert set(exported_symbols()) <= set(SIGNATURES), set(exported_symbols()) - set(SIGNATURES)
+
+
+def test_probe_without_a_device_is_a_typed_decline():
+    from oxidelake_amd._capi import lib
+
+    why = C.create_string_buffer(256)
+    rc = lib.oxr_probe(0, why, 256)
+    if rc != 0:  # no GPU in this container: a reason, never a crash
+        assert rc == 1
+        assert why.value
+
+
+def test_supports_predicate_is_host_only():
+    from oxidelake_amd.backend import RocmBackend  # noqa: F401  (import only)
+    from oxidelake_amd._capi import lib
+    from oxidelake_amd import plan
+
+    C_ = plan.Comparison
+    p = plan.Encoded(plan.and_(plan.cmp(plan.col(0), C_.GtEq, plan.Lit.int64(2)),
+                               plan.cmp(plan.col(1), C_.Lt, plan.Lit.float64(4.0))))
+    assert lib.oxr_supports_predicate(p.ref()) == 1
+    bad = plan.Encoded(plan.and_(plan.col(0), plan.col(1)))  # AND of non-booleans
+    assert lib.oxr_supports_predicate(bad.ref()) == 0

@@ -1,0 +1,172 @@
+"""GPU parity: fused filter + projection vs the CPU oracle.
+
+Bit-exact: the selected rows, their order, every projected value (bit
+pattern) and every NULL must equal the oracle's restatement of
+crates/oxidelake-device/src/cpu/kernels.rs:99-152.
+"""
+import numpy as np
+import pyarrow as pa
+import pytest
+
+from oxidelake_amd import datagen, plan
+from oxidelake_amd.backend import ExecutionError, Unsupported
+from oxidelake_amd.plan import Comparison as Cmp
+from oxidelake_amd.plan import Lit, Predicate, col
+from tests.util import assert_batches_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(backend, stream, oracle, batch, pred, proj, names=None):
+    dev = backend.upload(stream, batch)
+    out = backend.filter_project(stream, dev, pred, proj, names)
+    got = backend.download(stream, out)
+    want = oracle.filter_project(batch, pred, proj, names)
+    return got, want
+
+
+def ref_batch():
+    # kernels.rs:938-973
+    return pa.record_batch({
+        "k": pa.array([1, 2, None, 2, 5], pa.int64()),
+        "v": pa.array([0.5, None, 2.5, 3.5, -1.0], pa.float64()),
+    })
+
+
+def test_reference_known_answer(backend, stream, oracle):
+    # kernels.rs:976-988: output k = [2, 5]
+    pred = Predicate.and_(Predicate.compare(0, Cmp.GtEq, Lit.int64(2)),
+                          Predicate.compare(1, Cmp.Lt, Lit.float64(4.0)))
+    got, want = run_both(backend, stream, oracle, ref_batch(), pred, [1, 0])
+    assert got.column(1).to_pylist() == [2, 5]
+    assert_batches_equal(got, want)
+
+
+def test_float_order_as_datafusion(backend, stream, oracle):
+    # kernels.rs:990-1021
+    vals = [float("nan"), -float("nan"), float("inf"), -float("inf"), 0.0, -0.0, 4.0]
+    b = pa.record_batch({"v": pa.array(vals, pa.float64())})
+    cases = [
+        (Cmp.Gt, 4.0, [0, 2]),
+        (Cmp.Lt, -float("inf"), [1]),
+        (Cmp.Eq, float("nan"), [0]),
+        (Cmp.Eq, 0.0, [4, 5]),
+        (Cmp.Eq, -0.0, [4, 5]),
+        (Cmp.Lt, 0.0, [1, 3]),
+        (Cmp.GtEq, -0.0, [0, 2, 4, 5, 6]),
+    ]
+    idx = pa.record_batch({"v": b.column(0), "row": pa.array(np.arange(7), pa.int64())})
+    for op, x, rows in cases:
+        got, want = run_both(backend, stream, oracle, idx, Predicate.compare(0, op, Lit.float64(x)), [1])
+        assert got.column(0).to_pylist() == rows, (op, x)
+        assert_batches_equal(got, want)
+
+
+PREDICATES = [
+    Predicate.compare(0, Cmp.GtEq, Lit.int64(0)),
+    Predicate.compare(1, Cmp.Lt, Lit.int64(3)),
+    Predicate.compare(1, Cmp.Eq, Lit.int32(-4)),
+    Predicate.compare(2, Cmp.Gt, Lit.float64(0.5)),
+    Predicate.compare(2, Cmp.LtEq, Lit.float32(-0.0)),
+    Predicate.compare(3, Cmp.GtEq, Lit.float64(float("nan"))),
+    Predicate.compare(3, Cmp.Lt, Lit.float64(-float("inf"))),
+    Predicate.compare(3, Cmp.NotEq, Lit.float64(0.0)),
+    Predicate.and_(Predicate.compare(0, Cmp.GtEq, Lit.int64(2)), Predicate.compare(3, Cmp.Gt, Lit.float64(4.0))),
+    Predicate.or_(Predicate.compare(0, Cmp.Lt, Lit.int64(-40)), Predicate.compare(2, Cmp.Gt, Lit.float64(1.0))),
+    plan.or_(plan.and_(Predicate.compare(1, Cmp.Lt, Lit.int64(5)), Predicate.compare(2, Cmp.Gt, Lit.float64(0.5))),
+             plan.cmp(plan.mod(col(0), Lit.int64(7)), Cmp.Eq, Lit.int64(3))),
+    plan.cmp(plan.add(col(0), col(1)), Cmp.Gt, Lit.int64(10)),
+    plan.cmp(plan.mul(col(2), col(3)), Cmp.LtEq, Lit.float64(0.25)),
+    plan.cmp(col(0), Cmp.Lt, col(1)),
+    plan.cmp(Lit.int64(3), Cmp.Gt, col(0)),
+]
+
+PROJECTIONS = [
+    [0],
+    [4, 0],
+    [col(3), plan.add(plan.mul(col(1), Lit.int64(2)), Lit.int64(1)), plan.mul(col(2), col(3))],
+    [plan.sub(col(0), col(1)), plan.div(col(3), col(2)), plan.mod(col(1), Lit.int32(3)),
+     plan.mul(col(2), Lit.float32(1.5)), plan.add(col(1), Lit.float64(0.25))],
+    [plan.div(col(0), Lit.int64(3)), plan.mul(col(0), col(0))],
+]
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 64, 1000, 4097, 50_000])
+@pytest.mark.parametrize("seed", [1, 42])
+def test_random_parity(backend, stream, oracle, n, seed):
+    rng = np.random.default_rng(seed * 1000 + n)
+    batch = datagen.random_batch(rng, n)
+    for pi, pred in enumerate(PREDICATES):
+        for qi, proj in enumerate(PROJECTIONS):
+            if (pi + qi) % 3 and n > 1000:
+                continue
+            got, want = run_both(backend, stream, oracle, batch, pred, proj)
+            assert_batches_equal(got, want, names=False)
+
+
+def test_no_predicate_keeps_every_row(backend, stream, oracle):
+    rng = np.random.default_rng(3)
+    batch = datagen.random_batch(rng, 3000)
+    got, want = run_both(backend, stream, oracle, batch, None, [0, 1, 2, 3, 4])
+    assert got.num_rows == 3000
+    assert_batches_equal(got, want)
+
+
+def test_sliced_input(backend, stream, oracle):
+    rng = np.random.default_rng(5)
+    batch = datagen.random_batch(rng, 999).slice(13, 700)
+    pred = Predicate.compare(2, Cmp.Gt, Lit.float64(0.0))
+    got, want = run_both(backend, stream, oracle, batch, pred, [0, 2, 4])
+    assert_batches_equal(got, want)
+
+
+def test_divide_by_zero_is_an_error(backend, stream):
+    b = pa.record_batch({"x": pa.array([4, 5, 6], pa.int64()), "y": pa.array([1, 0, 2], pa.int64())})
+    dev = backend.upload(stream, b)
+    with pytest.raises(ExecutionError):
+        backend.filter_project(stream, dev, None, [plan.div(col(0), col(1))])
+    # a zero divisor in a NULL row is not an error
+    b = pa.record_batch({"x": pa.array([4, 5, 6], pa.int64()),
+                         "y": pa.array([1, None, 2], pa.int64())})
+    b = pa.record_batch({"x": b.column(0), "y": pa.array([1, 0, 2], pa.int64(), mask=np.array([False, True, False]))})
+    dev = backend.upload(stream, b)
+    out = backend.download(stream, backend.filter_project(stream, dev, None, [plan.div(col(0), col(1))]))
+    assert out.column(0).to_pylist() == [4, None, 3]
+
+
+def test_unsupported_column_is_declined(backend, stream):
+    b = pa.record_batch({"s": pa.array(["a", "b"])})
+    with pytest.raises(Unsupported) as e:
+        backend.upload(stream, b)
+    assert e.value.feature == "device.column_type"
+
+
+def test_vector_null_element_is_declined(backend, stream):
+    # columns.rs:211-241: a NULL element inside a valid row cannot travel
+    child = pa.array([1.0, None, 3.0, 4.0], pa.float32())
+    v = pa.FixedSizeListArray.from_arrays(child, 2)
+    with pytest.raises(Unsupported) as e:
+        backend.upload(stream, pa.record_batch({"v": v}))
+    assert e.value.feature == "device.vector_null_element"
+    # under a NULL row it is fine
+    v = pa.FixedSizeListArray.from_arrays(child, 2, mask=pa.array([True, False]))
+    dev = backend.upload(stream, pa.record_batch({"v": v}))
+    assert dev.num_rows == 2
+
+
+@pytest.mark.parametrize("threshold", [50, 500])
+def test_config2_query_parity(backend, stream, oracle, threshold):
+    import torch
+
+    t = datagen.config2_torch(300_001, seed=7)
+    pred, proj = datagen.config2_query(threshold)
+    dev = datagen.config2_wrap(backend, t)
+    s = backend.torch_stream()
+    out = backend.filter_project(s, dev, pred, proj, ["a", "b2", "cd"])
+    got = backend.download(s, out)
+    torch.cuda.synchronize()
+    want = oracle.filter_project(datagen.config2_arrow(t), pred, proj, ["a", "b2", "cd"])
+    assert_batches_equal(got, want)
+    sel = got.num_rows / 300_001
+    p_a = (threshold / 1000) * 0.95 * 0.3085
+    assert abs(sel - (p_a + 1 / 7 - p_a / 7)) < 0.01

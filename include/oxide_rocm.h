@@ -338,6 +338,35 @@ int oxr_vector_topk(oxr_stream *s, const oxr_batch *input, int32_t column,
                     int32_t metric, int32_t k, int64_t *out_rows,
                     float *out_dist);
 
+/* --------------------------------------------------------- synthetic tables */
+/* Seeded synthetic columns written straight into HBM, for benchmarks and
+ * tests (the reference's counterpart is its `gen-data` table generator,
+ * crates/oxidelake-storage/src/datagen.rs:89-182, which writes Parquet on
+ * the host). Every value is a pure function of (seed, column index, row):
+ * SplitMix64 draws, so a table is reproducible at any size. */
+enum {
+  OXR_GEN_UNIFORM_INT = 1,   /* integers in [a, b) */
+  OXR_GEN_UNIFORM_REAL = 2,  /* reals in [a, b) */
+  OXR_GEN_NORMAL = 3,        /* N(a, b^2) */
+  OXR_GEN_LOGNORMAL = 4,     /* exp(N(a, b^2)) */
+  OXR_GEN_ZIPF = 5,          /* Zipf exponent a over ranks 1..b, written as rank-1 */
+  OXR_GEN_SEQUENCE = 6,      /* a + row */
+  OXR_GEN_PERMUTATION = 7,   /* a + a random permutation of 0..rows-1 */
+  OXR_GEN_MIXTURE = 8,       /* vectors: one of a N(0,1) centroids + N(0, b^2) noise */
+  OXR_GEN_MIXTURE_NORMALIZED = 9 /* the same, L2-normalised */
+};
+typedef struct oxr_gen_spec {
+  int32_t type;      /* OXR_INT32 .. OXR_F32_VECTOR */
+  int32_t dist;      /* OXR_GEN_* */
+  int32_t dim;       /* vector size for OXR_F32_VECTOR */
+  int32_t pad;
+  double a, b;       /* distribution parameters */
+  double null_frac;  /* probability a row is NULL (0 = no validity bitmap) */
+  uint64_t seed;
+} oxr_gen_spec;
+int oxr_generate(oxr_stream *s, const oxr_gen_spec *cols, const char *const *names,
+                 int32_t ncols, uint64_t rows, oxr_batch **out);
+
 /* ---------------------------------------------------------------- profiling */
 /* Per-kernel device time from HIP events recorded on the launching stream
  * around every kernel launch while enabled (off by default; benchmarks turn

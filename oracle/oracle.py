@@ -336,9 +336,17 @@ class GroupedAggregator:
         varr = (C.c_void_p * max(1, len(outs)))(*[v.ctypes.data for v in ovalid])
         lib().ora_agg_finish(self.h, _ptr(keys), _ptr(kvalid), oarr, varr)
         key_t = pa.int32() if self.key_type == plan.INT32 else pa.int64()
-        cols = [pa.array(keys[:g].astype(np.int32 if self.key_type == plan.INT32 else np.int64),
-                         type=key_t, mask=~kvalid[:g].astype(bool))]
-        fields = [pa.field(self.key_name, key_t)]
+        cols, fields = [], []
+        if self.spec.group_by >= 0:
+            cols.append(pa.array(keys[:g].astype(np.int32 if self.key_type == plan.INT32 else np.int64),
+                                 type=key_t, mask=~kvalid[:g].astype(bool)))
+            fields.append(pa.field(self.key_name, key_t))
+        elif g == 0:
+            # SQL: a global aggregate over no rows is one row (COUNT 0, the rest NULL)
+            g = 1
+            for a, o, v in zip(self.spec.aggregates, outs, ovalid):
+                o[0] = 0
+                v[0] = a.func in (AggregateFunction.Count, AggregateFunction.CountStar)
         for o, v, ot, nm in zip(outs, ovalid, otypes, self.names):
             at = {plan.INT32: pa.int32(), plan.INT64: pa.int64(), plan.FLOAT32: pa.float32(),
                   plan.FLOAT64: pa.float64()}[ot]

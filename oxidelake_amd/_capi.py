@@ -61,6 +61,19 @@ ArrowArray._fields_ = [
 ]
 
 
+class GenSpec(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("dist", C.c_int32),
+        ("dim", C.c_int32),
+        ("pad", C.c_int32),
+        ("a", C.c_double),
+        ("b", C.c_double),
+        ("null_frac", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
 class Agg(C.Structure):
     _fields_ = [
         ("func", C.c_int32),
@@ -138,6 +151,7 @@ SIGNATURES: dict[str, tuple] = {
         [P, P, I32, C.POINTER(C.c_float), I32, I32, I32, I32, C.POINTER(C.c_int64),
          C.POINTER(C.c_float)],
     ),
+    "oxr_generate": (C.c_int, [P, C.POINTER(GenSpec), C.POINTER(C.c_char_p), I32, U64, PP]),
     "oxr_profile_enable": (None, [C.c_int]),
     "oxr_profile_reset": (None, []),
     "oxr_profile_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),

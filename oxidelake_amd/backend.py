@@ -29,6 +29,7 @@ from ._capi import (
     ArrowArray,
     ArrowSchema,
     Column,
+    GenSpec,
     lib,
 )
 from .plan import AggregateSpec, DistanceMetric, Encoded, ExprT
@@ -108,20 +109,6 @@ class Stream:
             self.handle = None
 
 
-class _CudaArrayView:
-    """Zero-copy `__cuda_array_interface__` over a device column."""
-
-    def __init__(self, ptr: int, shape, typestr: str, owner):
-        self.__cuda_array_interface__ = {
-            "shape": tuple(shape),
-            "typestr": typestr,
-            "data": (ptr, False),
-            "version": 2,
-            "strides": None,
-        }
-        self._owner = owner
-
-
 _TYPESTR = {plan.INT32: "<i4", plan.INT64: "<i8", plan.FLOAT32: "<f4", plan.FLOAT64: "<f8",
             plan.F32_VECTOR: "<f4"}
 _WIDTH = {plan.INT32: 4, plan.INT64: 8, plan.FLOAT32: 4, plan.FLOAT64: 8}
@@ -149,23 +136,6 @@ class DeviceBatch:
         c = Column()
         check(lib.oxr_batch_column(self.handle, i, C.byref(c)))
         return c
-
-    def torch_column(self, i: int):
-        """Zero-copy torch views (values, validity-bytes or None) of column i."""
-        import torch
-
-        c = self.column(i)
-        shape = (c.len, c.dim) if c.type == plan.F32_VECTOR else (c.len,)
-        if c.len == 0:
-            dt = {"<i4": torch.int32, "<i8": torch.int64, "<f4": torch.float32,
-                  "<f8": torch.float64}[_TYPESTR[c.type]]
-            return torch.empty(shape, dtype=dt, device="cuda"), None
-        vals = torch.as_tensor(_CudaArrayView(c.values, shape, _TYPESTR[c.type], self), device="cuda")
-        valid = None
-        if c.validity:
-            valid = torch.as_tensor(
-                _CudaArrayView(c.validity, ((c.len + 7) // 8,), "|u1", self), device="cuda")
-        return vals, valid
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -262,11 +232,6 @@ class RocmBackend:
         check(lib.oxr_stream_wrap(self.dev, C.c_void_p(hip_stream), C.byref(s)))
         return Stream(s.value, self)
 
-    def torch_stream(self) -> Stream:
-        import torch
-
-        return self.wrap_stream(torch.cuda.current_stream().cuda_stream)
-
     def synchronize(self, stream: Stream) -> None:
         stream.synchronize()
 
@@ -289,8 +254,9 @@ class RocmBackend:
 
     def wrap(self, columns: Sequence[tuple], names: Optional[Sequence[str]] = None,
              rows: Optional[int] = None) -> DeviceBatch:
-        """A batch over device tensors (no copy): each column is
-        (values_tensor, validity_bytes_tensor_or_None, oxr_type[, dim])."""
+        """A batch over device memory the caller owns (no copy): each column is
+        (values, validity_or_None, oxr_type[, dim]) where values/validity are
+        objects with `data_ptr()` and `shape` (e.g. framework tensors)."""
         cols = (Column * max(1, len(columns)))()
         keep = []
         nrows = rows
@@ -309,6 +275,18 @@ class RocmBackend:
         out = C.c_void_p()
         check(lib.oxr_batch_wrap(self.dev, cols, arr, len(columns), nrows or 0, C.byref(out)))
         return DeviceBatch(out.value, keepalive=(keep, raw))
+
+    def generate(self, stream: Stream, columns: Sequence, rows: int) -> DeviceBatch:
+        """A seeded synthetic table written straight into HBM (oxr_generate).
+        `columns` are datagen.GenColumn specs."""
+        specs = (GenSpec * max(1, len(columns)))()
+        for i, c in enumerate(columns):
+            specs[i] = GenSpec(c.type, c.dist, c.dim, 0, float(c.a), float(c.b), float(c.null_frac),
+                               int(c.seed) & 0xFFFFFFFFFFFFFFFF)
+        arr, raw = _cstrs([c.name for c in columns], len(columns))
+        out = C.c_void_p()
+        check(lib.oxr_generate(stream.handle, specs, arr, len(columns), rows, C.byref(out)))
+        return DeviceBatch(out.value)
 
     # -- operators
     def supports_predicate(self, predicate: ExprT | None) -> bool:

@@ -26,12 +26,17 @@ SOURCES = [
     "aggregate.hip",
     "join.hip",
     "vector.hip",
+    "datagen.hip",
     "pending.hip",
 ]
 
 # Sources whose float arithmetic must not be contracted into FMA: the
 # filter and aggregate evaluate SQL expressions, where a*b+c is two roundings.
 NO_CONTRACT = {"expr.hip", "filter.hip", "aggregate.hip", "join.hip"}
+
+# Sources that accumulate FP64 sums with hardware atomics (global and LDS
+# add-f64 instead of compare-and-swap loops).
+FP_ATOMICS = {"aggregate.hip", "join.hip"}
 
 ARCH = "gfx950"
 
@@ -63,6 +68,8 @@ def _compile(src: str) -> Path:
     flags = common_flags()
     if src in NO_CONTRACT:
         flags.append("-ffp-contract=off")
+    if src in FP_ATOMICS:
+        flags.append("-munsafe-fp-atomics")
     cmd = [hipcc(), *flags, "-c", str(CSRC / src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

@@ -1,0 +1,931 @@
+// Grouped hash aggregation: SUM / COUNT / MIN / MAX / COUNT(*) by one key.
+//
+// Replaces `ops::aggregate` (crates/oxidelake-device/src/cuda/ops.rs:623-1161)
+// and its kernels (kernels/cuda/aggregation.cu), which size a table by the
+// batch's rows, insert keys in one launch, number groups in another, run one
+// launch per aggregate function (each re-probing the table) and merge the
+// per-batch partial on the host. Here:
+//  - the group table lives in HBM across batches (streaming state), open
+//    addressing with a single 64-bit CAS on the key (INT64_MIN marks an empty
+//    slot; the NULL key and the key INT64_MIN itself have dedicated slots), so
+//    no claimed/published spin protocol is needed;
+//  - one kernel per batch evaluates the optional fused filter, the key and
+//    every aggregate's input expression, and folds each row into an
+//    LDS-privatised block table (hot keys aggregate in LDS), which is flushed
+//    to the HBM table with one atomic per (group, aggregate) per flush;
+//  - SUM over integers wraps in Int64; SUM over floats is an FP64 atomic sum
+//    from +0.0 (order differs from the CPU's sequential fold: the parity
+//    contract is 1e-12 relative); MIN/MAX of floats are integer min/max over
+//    totalOrder keys (kernels.rs:820-838).
+// Semantics: crates/oxidelake-device/src/cpu/kernels.rs:478-818.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <memory>
+
+#include "common.h"
+#include "expr.h"
+
+namespace oxr {
+namespace {
+
+constexpr int V = 4;
+constexpr int kBlock = 256;
+constexpr int kTileRows = kBlock * V;
+constexpr int kMaxAggs = 8;
+constexpr int kLdsSlots = 1024;       // block table slots (power of two)
+constexpr int kLdsAggs = 4;           // aggregates pre-aggregated in LDS
+constexpr int kLdsProbe = 8;          // LDS probe limit before going global
+constexpr int64_t kEmpty = INT64_MIN; // empty-slot marker
+
+enum : uint8_t {
+  A_COUNT = 1,
+  A_SUM_I64,
+  A_SUM_F64,
+  A_MIN_I64,
+  A_MAX_I64,
+  A_MIN_F64,  // over totalOrder keys of f64
+  A_MAX_F64,
+  A_MIN_F32,  // over totalOrder keys of f32 (sign-extended)
+  A_MAX_F32,
+  A_COUNT_STAR
+};
+
+__host__ __device__ inline int64_t acc_init(uint8_t kind) {
+  switch (kind) {
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32: return INT64_MAX;
+    case A_MAX_I64:
+    case A_MAX_F64:
+    case A_MAX_F32: return INT64_MIN;
+  }
+  return 0;
+}
+
+__host__ __device__ inline uint64_t fmix64(uint64_t h) {
+  // MurmurHash3 finalizer
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return h;
+}
+
+struct AggDesc {
+  uint8_t kind;
+  uint8_t in_type;   // value type of the input expression
+  uint8_t nullable;  // input can be NULL (then `has` flags are kept)
+  uint8_t pad;
+  int16_t col;       // bare column input, or -1
+  uint16_t pstart, plen;
+  int16_t pad2;
+};
+
+struct Table {
+  uint64_t cap;       // power of two; slots cap (NULL key) and cap+1 (INT64_MIN key) follow
+  int64_t *keys;      // cap+2
+  int64_t *acc[kMaxAggs];
+  uint32_t *has[kMaxAggs];  // cap+2 each, or NULL
+  uint32_t *seen;     // [0] NULL-key group, [1] INT64_MIN group
+  uint64_t *distinct; // new keys claimed
+  uint32_t *overflow; // a key found no slot
+};
+
+struct Params {
+  uint64_t n;
+  DCol cols[kMaxCols];
+  Ins ins[kMaxIns];
+  Leaf leaves[kMaxLeaves];
+  uint8_t bprog[kMaxBool];
+  int32_t nb;
+  int32_t key_col;   // -1: one global group
+  int32_t naggs;
+  int32_t lds_aggs;  // aggregates folded in LDS (0 = all global)
+  AggDesc aggs[kMaxAggs];
+  Table t;
+  uint32_t *error;   // division by zero
+};
+
+__device__ inline void acc_apply_global(uint8_t kind, int64_t *addr, int64_t v) {
+  switch (kind) {
+    case A_COUNT:
+    case A_COUNT_STAR:
+    case A_SUM_I64:
+      atomicAdd((unsigned long long *)addr, (unsigned long long)v);
+      break;
+    case A_SUM_F64:
+      unsafeAtomicAdd((double *)addr, as_f64((uint64_t)v));
+      break;
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32:
+      atomicMin((long long *)addr, (long long)v);
+      break;
+    default:
+      atomicMax((long long *)addr, (long long)v);
+      break;
+  }
+}
+
+__device__ inline void acc_apply_lds(uint8_t kind, int64_t *addr, int64_t v) {
+  switch (kind) {
+    case A_COUNT:
+    case A_COUNT_STAR:
+    case A_SUM_I64:
+      atomicAdd((unsigned long long *)addr, (unsigned long long)v);
+      break;
+    case A_SUM_F64:
+      unsafeAtomicAdd((double *)addr, as_f64((uint64_t)v));
+      break;
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32:
+      atomicMin((long long *)addr, (long long)v);
+      break;
+    default:
+      atomicMax((long long *)addr, (long long)v);
+      break;
+  }
+}
+
+// The per-row contribution of a value of `in_type` to an accumulator.
+__device__ inline int64_t contribution(uint8_t kind, uint8_t in_type, uint64_t v) {
+  switch (kind) {
+    case A_COUNT:
+    case A_COUNT_STAR: return 1;
+    case A_SUM_I64:
+    case A_MIN_I64:
+    case A_MAX_I64: return (int64_t)v;  // ints arrive sign-extended
+    case A_SUM_F64:
+      return (int64_t)(in_type == OXR_FLOAT32 ? f64_bits((double)as_f32(v)) : v);
+    case A_MIN_F64:
+    case A_MAX_F64: {
+      // totalOrder without zero normalisation (kernels.rs:527-538)
+      int64_t b = (int64_t)v;
+      return b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1);
+    }
+    default: {
+      int32_t b = (int32_t)(uint32_t)v;
+      return (int64_t)(b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1));
+    }
+  }
+}
+
+// Finds (or claims) the HBM slot of `key` (not NULL, not INT64_MIN).
+__device__ inline uint64_t global_slot(const Table &t, int64_t key, bool *ok) {
+  const uint64_t mask = t.cap - 1;
+  uint64_t h = fmix64((uint64_t)key) & mask;
+  for (uint64_t probe = 0; probe < t.cap; probe++) {
+    int64_t cur = __hip_atomic_load(&t.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == key) return h;
+    if (cur == kEmpty) {
+      unsigned long long old = atomicCAS((unsigned long long *)&t.keys[h],
+                                         (unsigned long long)kEmpty, (unsigned long long)key);
+      if ((int64_t)old == kEmpty) {
+        atomicAdd((unsigned long long *)t.distinct, 1ull);
+        return h;
+      }
+      if ((int64_t)old == key) return h;
+    }
+    h = (h + 1) & mask;
+  }
+  *ok = false;
+  return 0;
+}
+
+__device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, bool *ok) {
+  if (!key_valid) {
+    t.seen[0] = 1;
+    return t.cap;
+  }
+  if (key == kEmpty) {
+    t.seen[1] = 1;
+    return t.cap + 1;
+  }
+  return global_slot(t, key, ok);
+}
+
+__global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
+  __shared__ int64_t s_keys[kLdsSlots + 2];
+  __shared__ int64_t s_acc[kLdsAggs][kLdsSlots + 2];
+  __shared__ uint32_t s_has[kLdsAggs][kLdsSlots + 2];
+  __shared__ uint32_t s_used;
+  __shared__ uint32_t s_flush;
+
+  const int tid = threadIdx.x;
+  const int la = p.lds_aggs;
+  for (int i = tid; i < kLdsSlots + 2; i += kBlock) {
+    s_keys[i] = kEmpty;
+    for (int a = 0; a < la; a++) {
+      s_acc[a][i] = acc_init(p.aggs[a].kind);
+      s_has[a][i] = 0;
+    }
+  }
+  if (tid == 0) {
+    s_used = 0;
+    s_flush = 0;
+  }
+  __syncthreads();
+
+  bool div0 = false, ok = true;
+  const uint64_t ntiles = (p.n + kTileRows - 1) / kTileRows;
+
+  auto flush = [&]() {
+    // every used LDS slot -> one global update per aggregate, then clear
+    for (int i = tid; i < kLdsSlots + 2; i += kBlock) {
+      const int64_t k = s_keys[i];
+      const bool used = i < kLdsSlots ? k != kEmpty : k != kEmpty;  // specials carry a marker
+      if (used) {
+        uint64_t gs;
+        if (i == kLdsSlots) gs = slot_of(p.t, false, 0, &ok);
+        else if (i == kLdsSlots + 1) gs = slot_of(p.t, true, kEmpty, &ok);
+        else gs = global_slot(p.t, k, &ok);
+        if (ok) {
+          for (int a = 0; a < la; a++) {
+            const AggDesc d = p.aggs[a];
+            const int64_t v = s_acc[a][i];
+            if (v != acc_init(d.kind) || d.kind == A_SUM_F64)
+              acc_apply_global(d.kind, &p.t.acc[a][gs], v);
+            if (p.t.has[a] && s_has[a][i]) p.t.has[a][gs] = 1;
+          }
+        }
+      }
+      s_keys[i] = kEmpty;
+      for (int a = 0; a < la; a++) {
+        s_acc[a][i] = acc_init(p.aggs[a].kind);
+        s_has[a][i] = 0;
+      }
+    }
+  };
+
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t row0 = tile * kTileRows + (uint64_t)tid * V;
+    uint32_t sel = eval_predicate<V>(p.leaves, p.bprog, p.nb, p.ins, p.cols, row0, p.n, &div0);
+    uint64_t key[V];
+    uint32_t kvalid = (1u << V) - 1;
+    if (p.key_col >= 0) {
+      // a NULL key is a group of its own: validity only picks the slot
+      load_rows<V>(p.cols[p.key_col], row0, p.n, key, kvalid);
+    } else {
+      // one global group, kept in the slot the NULL key would use
+#pragma unroll
+      for (int q = 0; q < V; q++) key[q] = 0;
+      kvalid = 0;
+    }
+    if (sel) {
+      // LDS slots of this lane's selected rows (-1: go global)
+      int lslot[V];
+#pragma unroll
+      for (int q = 0; q < V; q++) {
+        lslot[q] = -1;
+        if (!((sel >> q) & 1)) continue;
+        if (la == 0) continue;
+        const bool kv = (kvalid >> q) & 1;
+        const int64_t k = (int64_t)key[q];
+        if (!kv) {
+          lslot[q] = kLdsSlots;
+          s_keys[kLdsSlots] = 0;  // marks the NULL-key slot used
+          continue;
+        }
+        if (k == kEmpty) {
+          lslot[q] = kLdsSlots + 1;
+          s_keys[kLdsSlots + 1] = 0;
+          continue;
+        }
+        uint32_t h = (uint32_t)fmix64((uint64_t)k) & (kLdsSlots - 1);
+        for (int pr = 0; pr < kLdsProbe; pr++) {
+          int64_t cur = s_keys[h];
+          if (cur == k) {
+            lslot[q] = (int)h;
+            break;
+          }
+          if (cur == kEmpty) {
+            unsigned long long old = atomicCAS((unsigned long long *)&s_keys[h],
+                                               (unsigned long long)kEmpty, (unsigned long long)k);
+            if ((int64_t)old == kEmpty) {
+              atomicAdd(&s_used, 1u);
+              lslot[q] = (int)h;
+              break;
+            }
+            if ((int64_t)old == k) {
+              lslot[q] = (int)h;
+              break;
+            }
+          }
+          h = (h + 1) & (kLdsSlots - 1);
+        }
+      }
+      // global slots for rows the block table could not take
+      uint64_t gslot[V];
+      bool need_global = false;
+#pragma unroll
+      for (int q = 0; q < V; q++) need_global |= ((sel >> q) & 1) && lslot[q] < 0;
+      const bool all_global = la < p.naggs;
+#pragma unroll
+      for (int q = 0; q < V; q++) {
+        gslot[q] = 0;
+        if (((sel >> q) & 1) && (lslot[q] < 0 || all_global))
+          gslot[q] = slot_of(p.t, (kvalid >> q) & 1, (int64_t)key[q], &ok);
+      }
+      (void)need_global;
+      for (int a = 0; a < p.naggs; a++) {
+        const AggDesc d = p.aggs[a];
+        uint64_t v[V];
+        uint32_t vm = (1u << V) - 1;
+        if (d.kind == A_COUNT_STAR) {
+#pragma unroll
+          for (int q = 0; q < V; q++) v[q] = 0;
+        } else if (d.col >= 0) {
+          load_rows<V>(p.cols[d.col], row0, p.n, v, vm);
+        } else {
+          uint64_t r[kRegs][V];
+          uint32_t m[kRegs] = {0, 0, 0, 0};
+          run_program<V>(p.ins, d.pstart, d.plen, p.cols, row0, p.n, r, m, &div0);
+#pragma unroll
+          for (int q = 0; q < V; q++) v[q] = r[0][q];
+          vm = m[0];
+        }
+#pragma unroll
+        for (int q = 0; q < V; q++) {
+          if (!((sel >> q) & 1)) continue;
+          const bool valid = (vm >> q) & 1;
+          if (!valid && d.kind != A_COUNT_STAR) continue;
+          const int64_t c = contribution(d.kind, d.in_type, v[q]);
+          if (a < la && lslot[q] >= 0) {
+            acc_apply_lds(d.kind, &s_acc[a][lslot[q]], c);
+            if (d.nullable) s_has[a][lslot[q]] = 1;
+          } else {
+            if (!ok) continue;
+            acc_apply_global(d.kind, &p.t.acc[a][gslot[q]], c);
+            if (p.t.has[a]) p.t.has[a][gslot[q]] = 1;
+          }
+        }
+      }
+    }
+    if (la) {
+      __syncthreads();
+      if (s_used > kLdsSlots / 2) {
+        flush();
+        __syncthreads();
+        if (tid == 0) s_used = 0;
+        __syncthreads();
+      }
+    }
+  }
+  if (la) {
+    __syncthreads();
+    flush();
+  }
+  if (div0) atomicOr(p.error, 1u);
+  if (!ok) atomicOr(p.t.overflow, 1u);
+}
+
+// ------------------------------------------------------------------ finish
+// Marks occupied slots and compacts (key, slot) of ordinary keys.
+__global__ void compact_kernel(Table t, int64_t *out_keys, uint64_t *out_slots, uint64_t *count) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= t.cap) return;
+  const int64_t k = t.keys[i];
+  if (k == kEmpty) return;
+  const uint64_t at = atomicAdd((unsigned long long *)count, 1ull);
+  out_keys[at] = k;
+  out_slots[at] = i;
+}
+
+struct OutCol {
+  void *values;
+  uint32_t *valid;
+  uint8_t kind, in_type, out_type, nullable;
+};
+
+// Row j of the output: group slots[j]; optional special rows appended.
+__global__ void gather_kernel(Table t, const int64_t *keys_sorted, const uint64_t *slots_sorted,
+                              uint64_t g_ordinary, int has_min_key, int has_null, int key_type,
+                              void *out_key, uint32_t *out_key_valid, OutCol *outs, int naggs,
+                              uint64_t total, int empty_group) {
+  const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  uint64_t slot;
+  int64_t key;
+  bool key_valid = true;
+  // order: the INT64_MIN key first (smallest), ordinary keys ascending, NULL last
+  if (has_min_key && j == 0) {
+    slot = t.cap + 1;
+    key = INT64_MIN;
+  } else if (has_null && j == total - 1) {
+    slot = t.cap;
+    key = 0;
+    key_valid = false;
+  } else {
+    const uint64_t r = j - (has_min_key ? 1 : 0);
+    slot = slots_sorted[r];
+    key = keys_sorted[r];
+  }
+  if (out_key) {
+    if (key_type == OXR_INT32) ((int32_t *)out_key)[j] = (int32_t)key;
+    else ((int64_t *)out_key)[j] = key;
+    if (key_valid) atomicOr(&out_key_valid[j >> 5], 1u << (j & 31));
+  }
+  for (int a = 0; a < naggs; a++) {
+    const OutCol o = outs[a];
+    const int64_t v = t.acc[a][slot];
+    bool valid;
+    if (o.kind == A_COUNT || o.kind == A_COUNT_STAR) valid = true;
+    else if (t.has[a]) valid = t.has[a][slot] != 0;
+    else valid = !empty_group;  // a non-nullable input: every row of a group holds a value
+    uint64_t outv;
+    switch (o.kind) {
+      case A_MIN_F64:
+      case A_MAX_F64: {
+        int64_t b = v ^ (int64_t)(((uint64_t)(v >> 63)) >> 1);
+        outv = (uint64_t)b;
+        break;
+      }
+      case A_MIN_F32:
+      case A_MAX_F32: {
+        int32_t k32 = (int32_t)v;
+        int32_t b = k32 ^ (int32_t)(((uint32_t)(k32 >> 31)) >> 1);
+        outv = (uint64_t)(uint32_t)b;
+        break;
+      }
+      default: outv = (uint64_t)v;
+    }
+    if (o.out_type == OXR_INT32 || o.out_type == OXR_FLOAT32) ((uint32_t *)o.values)[j] = (uint32_t)outv;
+    else ((uint64_t *)o.values)[j] = outv;
+    if (o.valid && valid) atomicOr(&o.valid[j >> 5], 1u << (j & 31));
+  }
+}
+
+__global__ void fill64_kernel(int64_t *p, uint64_t n, int64_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Re-inserts every occupied slot of `from` into the empty table `to`.
+__global__ void rehash_kernel(Table from, Table to, int naggs) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= from.cap + 2) return;
+  uint64_t dst;
+  bool ok = true;
+  if (i < from.cap) {
+    const int64_t k = from.keys[i];
+    if (k == kEmpty) return;
+    dst = global_slot(to, k, &ok);
+    if (!ok) {
+      atomicOr(to.overflow, 1u);
+      return;
+    }
+  } else {
+    dst = i - from.cap + to.cap;  // special slots keep their role
+  }
+  for (int a = 0; a < naggs; a++) {
+    to.acc[a][dst] = from.acc[a][i];
+    if (from.has[a]) to.has[a][dst] = from.has[a][i];
+  }
+}
+
+int fill64(oxr_stream *s, int64_t *p, uint64_t n, int64_t v) {
+  if (!n) return OXR_OK;
+  unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill64_kernel, dim3(blocks), dim3(256), 0, s->s, p, n, v);
+  OXR_HIP(hipGetLastError());
+  return OXR_OK;
+}
+
+uint64_t next_pow2(uint64_t x) {
+  uint64_t c = 16;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+}  // namespace
+}  // namespace oxr
+
+using namespace oxr;
+
+struct oxr_agg_state {
+  oxr_device *dev = nullptr;
+  oxr::StreamCore *owner = nullptr;
+  int32_t key_col = -1;
+  int32_t key_type = OXR_INT64;
+  std::string key_name;
+  uint64_t hint = 0;
+  Program prog;  // filter + aggregate input programs
+  std::vector<AggDesc> aggs;
+  std::vector<int32_t> out_types;
+  std::vector<std::string> names;
+  std::vector<int32_t> col_types;  // schema the spec was compiled against
+  Table t{};
+  void *mem = nullptr;  // one arena block holding the table
+  uint64_t groups = 0;  // distinct ordinary keys seen (after the last update)
+};
+
+namespace oxr {
+namespace {
+
+int table_alloc(oxr_stream *s, oxr_agg_state *st, uint64_t cap, Table *out, void **mem) {
+  const int na = (int)st->aggs.size();
+  const uint64_t slots = cap + 2;
+  size_t bytes = slots * 8 * (1 + na) + 64;
+  for (int a = 0; a < na; a++)
+    if (st->aggs[a].nullable) bytes += slots * 4 + 16;
+  bytes += 64;  // seen, distinct, overflow
+  void *m = nullptr;
+  OXR_TRY(stream_alloc(s, bytes, &m));
+  char *p = (char *)m;
+  Table t{};
+  t.cap = cap;
+  t.keys = (int64_t *)p;
+  p += slots * 8;
+  for (int a = 0; a < na; a++) {
+    t.acc[a] = (int64_t *)p;
+    p += slots * 8;
+  }
+  p = (char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+  for (int a = 0; a < na; a++) {
+    t.has[a] = nullptr;
+    if (st->aggs[a].nullable) {
+      t.has[a] = (uint32_t *)p;
+      p += slots * 4;
+    }
+  }
+  p = (char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+  t.seen = (uint32_t *)p;
+  t.distinct = (uint64_t *)(p + 16);
+  t.overflow = (uint32_t *)(p + 24);
+  int rc = fill64(s, t.keys, slots, kEmpty);
+  for (int a = 0; a < na && rc == OXR_OK; a++) rc = fill64(s, t.acc[a], slots, acc_init(st->aggs[a].kind));
+  for (int a = 0; a < na && rc == OXR_OK; a++)
+    if (t.has[a]) rc = fill_zero(s, t.has[a], slots * 4);
+  if (rc == OXR_OK) rc = fill_zero(s, t.seen, 32);
+  if (rc != OXR_OK) {
+    stream_release(s, m);
+    return rc;
+  }
+  *out = t;
+  *mem = m;
+  return OXR_OK;
+}
+
+// Grows the table so that `more` further distinct keys keep the load <= 1/2.
+int ensure_capacity(oxr_stream *s, oxr_agg_state *st, uint64_t more) {
+  uint64_t want = next_pow2(2 * (st->groups + more));
+  if (st->mem && want <= st->t.cap) return OXR_OK;
+  Table nt;
+  void *nm = nullptr;
+  OXR_TRY(table_alloc(s, st, std::max<uint64_t>(want, 16), &nt, &nm));
+  if (st->mem) {
+    const uint64_t slots = st->t.cap + 2;
+    hipLaunchKernelGGL(rehash_kernel, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s->s, st->t,
+                       nt, (int)st->aggs.size());
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      stream_release(s, nm);
+      return fail_hip(e, "rehash_kernel launch");
+    }
+    // the distinct count carries over
+    e = hipMemcpyAsync(nt.distinct, st->t.distinct, 8, hipMemcpyDeviceToDevice, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(nt.seen, st->t.seen, 8, hipMemcpyDeviceToDevice, s->s);
+    if (e != hipSuccess) {
+      stream_release(s, nm);
+      return fail_hip(e, "hipMemcpyAsync(rehash counters)");
+    }
+    stream_release(s, st->mem);
+  }
+  st->t = nt;
+  st->mem = nm;
+  return OXR_OK;
+}
+
+const char *func_name(int32_t f) {
+  switch (f) {
+    case OXR_SUM: return "SUM";
+    case OXR_COUNT:
+    case OXR_COUNT_STAR: return "COUNT";
+    case OXR_MIN: return "MIN";
+    case OXR_MAX: return "MAX";
+  }
+  return "?";
+}
+
+int compile_spec(const oxr_batch *like, const oxr_agg_spec *spec, oxr_agg_state *st) {
+  if (spec->naggs < 0 || spec->naggs > kMaxAggs)
+    return fail(OXR_EUNSUPPORTED, "rocm.aggregate",
+                "at most " + std::to_string(kMaxAggs) + " aggregates per operator");
+  if ((int)like->cols.size() > kMaxCols)
+    return fail(OXR_EUNSUPPORTED, "rocm.aggregate", "too many input columns");
+  st->key_col = spec->group_by;
+  if (spec->group_by >= 0) {
+    if (spec->group_by >= (int)like->cols.size())
+      return fail(OXR_EPLAN, "rocm.aggregate", "group key column out of range");
+    const int32_t kt = like->cols[spec->group_by].type;
+    if (kt != OXR_INT64 && kt != OXR_INT32)
+      return fail(OXR_EUNSUPPORTED, "cpu.int64_column",
+                  "group key column has a type other than Int64/Int32");
+    st->key_type = kt;
+    st->key_name = like->names[spec->group_by];
+  }
+  OXR_TRY(compile_predicate(spec->filter, like->cols, &st->prog));
+  for (int i = 0; i < spec->naggs; i++) {
+    const oxr_agg &g = spec->aggs[i];
+    AggDesc d{};
+    d.col = -1;
+    std::string in_name = "*";
+    int32_t in_type = OXR_INT64;
+    if (g.func != OXR_COUNT_STAR) {
+      ValueProg vp;
+      OXR_TRY(compile_value(&g.input, like->cols, &st->prog, &vp));
+      in_type = vp.type;
+      d.col = vp.col;
+      d.pstart = vp.start;
+      d.plen = vp.len;
+      d.nullable = vp.nullable;
+      in_name = vp.col >= 0 ? like->names[vp.col] : "expr";
+      if (in_type == OXR_F32_VECTOR)
+        return fail(OXR_EUNSUPPORTED, "rocm.aggregate", "aggregate over a vector column");
+    }
+    d.in_type = (uint8_t)in_type;
+    const bool fl = in_type == OXR_FLOAT32 || in_type == OXR_FLOAT64;
+    int32_t out_t = in_type;
+    switch (g.func) {
+      case OXR_COUNT: d.kind = A_COUNT; out_t = OXR_INT64; break;
+      case OXR_COUNT_STAR: d.kind = A_COUNT_STAR; out_t = OXR_INT64; break;
+      case OXR_SUM:
+        d.kind = fl ? A_SUM_F64 : A_SUM_I64;
+        out_t = fl ? OXR_FLOAT64 : OXR_INT64;
+        break;
+      case OXR_MIN:
+        d.kind = in_type == OXR_FLOAT64 ? A_MIN_F64 : in_type == OXR_FLOAT32 ? A_MIN_F32 : A_MIN_I64;
+        break;
+      case OXR_MAX:
+        d.kind = in_type == OXR_FLOAT64 ? A_MAX_F64 : in_type == OXR_FLOAT32 ? A_MAX_F32 : A_MAX_I64;
+        break;
+      default:
+        return fail(OXR_EPLAN, "rocm.aggregate", "unknown aggregate function");
+    }
+    st->aggs.push_back(d);
+    st->out_types.push_back(out_t);
+    st->names.push_back(g.name ? std::string(g.name)
+                               : std::string(func_name(g.func)) + "(" + in_name + ")");
+  }
+  for (auto &c : like->cols) st->col_types.push_back(c.type);
+  st->hint = spec->groups_hint;
+  return OXR_OK;
+}
+
+int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in) {
+  if (in->cols.size() != st->col_types.size())
+    return fail(OXR_EPLAN, "rocm.aggregate", "batch does not match the aggregation's schema");
+  for (size_t i = 0; i < in->cols.size(); i++)
+    if (in->cols[i].type != st->col_types[i])
+      return fail(OXR_EPLAN, "rocm.aggregate", "batch column types differ from the spec's");
+  const uint64_t n = in->rows;
+  if (n == 0) return OXR_OK;
+  // keep the load at most one half for every key this batch might add
+  uint64_t more = st->key_col < 0 ? 1 : n;
+  if (st->hint) more = std::min<uint64_t>(more, st->hint > st->groups ? st->hint - st->groups : 16);
+  OXR_TRY(ensure_capacity(s, st, more));
+
+  std::unique_ptr<Params> pp(new Params());
+  Params &P = *pp;
+  std::memset(&P, 0, sizeof P);
+  P.n = n;
+  for (size_t i = 0; i < in->cols.size(); i++) {
+    const DevCol &c = in->cols[i];
+    if (n && ((uintptr_t)c.values % 16) != 0)
+      return fail(OXR_EUNSUPPORTED, "rocm.aggregate", "column values must be 16-byte aligned");
+    P.cols[i].values = c.values;
+    P.cols[i].valid = (const uint32_t *)c.validity;
+    P.cols[i].type = c.type;
+    P.cols[i].dim = c.dim;
+  }
+  std::copy(st->prog.ins.begin(), st->prog.ins.end(), P.ins);
+  std::copy(st->prog.leaves.begin(), st->prog.leaves.end(), P.leaves);
+  std::copy(st->prog.bprog.begin(), st->prog.bprog.end(), P.bprog);
+  P.nb = (int32_t)st->prog.bprog.size();
+  P.key_col = st->key_col;
+  P.naggs = (int32_t)st->aggs.size();
+  P.lds_aggs = P.naggs <= kLdsAggs ? P.naggs : 0;
+  for (int a = 0; a < P.naggs; a++) P.aggs[a] = st->aggs[a];
+  P.t = st->t;
+  void *err = nullptr;
+  OXR_TRY(stream_alloc(s, 64, &err));
+  P.error = (uint32_t *)err;
+  int rc = fill_zero(s, err, 64);
+  const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
+  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * 8);
+  if (rc == OXR_OK) {
+    ProfScope prof(s->s, "aggregate");
+    hipLaunchKernelGGL(aggregate_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail_hip(e, "aggregate_kernel launch");
+  }
+  uint64_t host[4] = {0, 0, 0, 0};  // error, distinct, overflow
+  if (rc == OXR_OK) {
+    hipError_t e = hipMemcpyAsync(&host[0], err, 4, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&host[1], st->t.distinct, 8, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&host[2], st->t.overflow, 4, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "aggregate result");
+  }
+  stream_release(s, err);
+  if (rc != OXR_OK) return rc;
+  if ((uint32_t)host[0]) return fail(OXR_EEXECUTION, "rocm.aggregate", "Divide by zero error");
+  if ((uint32_t)host[2])
+    return fail(OXR_EUNSUPPORTED, "rocm.aggregate",
+                "the group table overflowed: more distinct keys than groups_hint allowed");
+  st->groups = host[1];
+  return OXR_OK;
+}
+
+int run_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {
+  if (!st->mem) OXR_TRY(ensure_capacity(s, st, 1));
+  const Table &t = st->t;
+  // ordinary keys
+  const uint64_t cap = t.cap;
+  const uint64_t g = st->groups;
+  void *buf = nullptr;
+  const size_t gb = (g ? g : 1) * 8;
+  OXR_TRY(stream_alloc(s, gb * 4 + 64, &buf));
+  int64_t *k_in = (int64_t *)buf;
+  uint64_t *s_in = (uint64_t *)((char *)buf + gb);
+  int64_t *k_out = (int64_t *)((char *)buf + 2 * gb);
+  uint64_t *s_out = (uint64_t *)((char *)buf + 3 * gb);
+  uint64_t *count = (uint64_t *)((char *)buf + 4 * gb);
+  int rc = fill_zero(s, count, 8);
+  if (rc == OXR_OK && g) {
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s->s, t, k_in,
+                       s_in, count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail_hip(e, "compact_kernel launch");
+  }
+  uint32_t seen[2] = {0, 0};
+  if (rc == OXR_OK) {
+    hipError_t e = hipMemcpyAsync(seen, t.seen, 8, hipMemcpyDeviceToHost, s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "aggregate seen flags");
+  }
+  // sort ordinary keys ascending (stable radix sort of signed keys)
+  void *tmp = nullptr;
+  if (rc == OXR_OK && g > 1) {
+    size_t tmp_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k_in, k_out, s_in, s_out,
+                                                      (int)g, 0, 64, s->s);
+    if (e == hipSuccess) {
+      rc = stream_alloc(s, tmp_bytes + 16, &tmp);
+      if (rc == OXR_OK) {
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k_in, k_out, s_in, s_out, (int)g, 0,
+                                               64, s->s);
+      }
+    }
+    if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "radix sort of group keys");
+  } else if (rc == OXR_OK && g == 1) {
+    hipError_t e = hipMemcpyAsync(k_out, k_in, 8, hipMemcpyDeviceToDevice, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(s_out, s_in, 8, hipMemcpyDeviceToDevice, s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "hipMemcpyAsync");
+  }
+  if (rc == OXR_OK) {
+    hipError_t e = hipStreamSynchronize(s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "aggregate finish");
+  }
+  const bool global_only = st->key_col < 0;
+  const int has_null = seen[0] ? 1 : 0;
+  const int has_min = seen[1] ? 1 : 0;
+  uint64_t total = g + has_null + has_min;
+  if (global_only) total = 1;  // one row even over no input (SQL global aggregate)
+  std::unique_ptr<oxr_batch> ob(new_batch(s, total));
+  auto cleanup = [&]() {
+    for (auto &c : ob->cols) {
+      stream_release(s, c.values);
+      stream_release(s, c.validity);
+    }
+    stream_release(s, tmp);
+    stream_release(s, buf);
+  };
+  if (rc != OXR_OK) {
+    cleanup();
+    return rc;
+  }
+  void *key_vals = nullptr;
+  uint32_t *key_valid = nullptr;
+  if (!global_only) {
+    DevCol kc;
+    rc = alloc_column(s, st->key_type, 0, total, true, &kc);
+    if (rc != OXR_OK) {
+      cleanup();
+      return rc;
+    }
+    ob->cols.push_back(kc);
+    ob->names.push_back(st->key_name);
+    key_vals = kc.values;
+    key_valid = (uint32_t *)kc.validity;
+  }
+  std::vector<OutCol> outs(st->aggs.size());
+  for (size_t a = 0; a < st->aggs.size(); a++) {
+    DevCol c;
+    const bool nullable = !(st->aggs[a].kind == A_COUNT || st->aggs[a].kind == A_COUNT_STAR);
+    rc = alloc_column(s, st->out_types[a], 0, total, nullable, &c);
+    if (rc != OXR_OK) {
+      cleanup();
+      return rc;
+    }
+    ob->cols.push_back(c);
+    ob->names.push_back(st->names[a]);
+    outs[a].values = c.values;
+    outs[a].valid = (uint32_t *)c.validity;
+    outs[a].kind = st->aggs[a].kind;
+    outs[a].in_type = st->aggs[a].in_type;
+    outs[a].out_type = (uint8_t)st->out_types[a];
+    outs[a].nullable = nullable;
+  }
+  void *douts = nullptr;
+  rc = stream_alloc(s, sizeof(OutCol) * std::max<size_t>(1, outs.size()), &douts);
+  if (rc == OXR_OK && !outs.empty()) {
+    hipError_t e = hipMemcpyAsync(douts, outs.data(), sizeof(OutCol) * outs.size(),
+                                  hipMemcpyHostToDevice, s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "hipMemcpyAsync(out columns)");
+  }
+  if (rc == OXR_OK && total) {
+    Table tt = t;
+    int hm = has_min, hn = has_null;
+    uint64_t go = g;
+    if (global_only) {
+      // the one group sits in the NULL-key slot (key_valid false routes there)
+      hm = 0;
+      hn = 1;
+      go = 0;
+      // a global aggregate over no rows: COUNT 0, the rest NULL (slot untouched)
+    }
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s->s, tt,
+                       k_out, s_out, go, hm, hn, st->key_type, key_vals, key_valid,
+                       (OutCol *)douts, (int)outs.size(), total, global_only && !seen[0] ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail_hip(e, "gather_kernel launch");
+  }
+  if (rc == OXR_OK) {
+    hipError_t e = hipStreamSynchronize(s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "aggregate gather");
+  }
+  stream_release(s, douts);
+  if (rc != OXR_OK) {
+    cleanup();
+    return rc;
+  }
+  stream_release(s, tmp);
+  stream_release(s, buf);
+  *out = ob.release();
+  return OXR_OK;
+}
+
+}  // namespace
+}  // namespace oxr
+
+extern "C" {
+
+int oxr_agg_create(oxr_stream *s, const oxr_batch *like, const oxr_agg_spec *spec,
+                   oxr_agg_state **out) {
+  clear_error();
+  if (!s || !like || !spec || !out || (spec->naggs > 0 && !spec->aggs))
+    return fail(OXR_EPLAN, "rocm.aggregate", "null argument");
+  std::unique_ptr<oxr_agg_state> st(new oxr_agg_state());
+  st->dev = s->dev;
+  st->owner = retain(s->core);
+  OXR_TRY(compile_spec(like, spec, st.get()));
+  if (st->hint) OXR_TRY(ensure_capacity(s, st.get(), st->hint));
+  *out = st.release();
+  return OXR_OK;
+}
+
+int oxr_agg_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *input) {
+  clear_error();
+  if (!s || !st || !input) return fail(OXR_EPLAN, "rocm.aggregate", "null argument");
+  return run_update(s, st, input);
+}
+
+int oxr_agg_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {
+  clear_error();
+  if (!s || !st || !out) return fail(OXR_EPLAN, "rocm.aggregate", "null argument");
+  return run_finish(s, st, out);
+}
+
+void oxr_agg_free(oxr_agg_state *st) {
+  if (!st) return;
+  if (st->mem) st->dev->arena->release(st->mem, st->owner->s);
+  release(st->owner);
+  delete st;
+}
+
+int oxr_aggregate(oxr_stream *s, const oxr_batch *input, const oxr_agg_spec *spec,
+                  oxr_batch **out) {
+  clear_error();
+  if (!s || !input || !spec || !out) return fail(OXR_EPLAN, "rocm.aggregate", "null argument");
+  oxr_agg_state *st = nullptr;
+  OXR_TRY(oxr_agg_create(s, input, spec, &st));
+  std::unique_ptr<oxr_agg_state, void (*)(oxr_agg_state *)> guard(st, oxr_agg_free);
+  OXR_TRY(run_update(s, st, input));
+  return run_finish(s, st, out);
+}
+
+}  // extern "C"

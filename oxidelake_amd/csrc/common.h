@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -74,10 +75,24 @@ struct oxr_device {
   oxr::Arena *arena = nullptr;
 };
 
-struct oxr_stream {
-  oxr_device *dev = nullptr;
+namespace oxr {
+// A HIP stream shared by its `oxr_stream` handle and by everything allocated
+// on it (batches, join tables, matches, aggregation states): those free
+// their memory in the stream's order, so the stream lives until the last of
+// them is freed, whenever the caller destroys its handle.
+struct StreamCore {
   hipStream_t s = nullptr;
   bool owned = false;
+  std::atomic<int> refs{1};
+};
+StreamCore *retain(StreamCore *c);
+void release(StreamCore *c);
+}  // namespace oxr
+
+struct oxr_stream {
+  oxr_device *dev = nullptr;
+  hipStream_t s = nullptr;  // == core->s
+  oxr::StreamCore *core = nullptr;
 };
 
 struct oxr_buffer {
@@ -131,7 +146,7 @@ class ProfScope {
 
 struct oxr_batch {
   oxr_device *dev = nullptr;
-  hipStream_t owner_stream = nullptr;
+  oxr::StreamCore *owner = nullptr;  // stream the owned columns were allocated on
   uint64_t rows = 0;
   std::vector<oxr::DevCol> cols;
   std::vector<std::string> names;

@@ -357,10 +357,10 @@ __device__ inline uint32_t eval_predicate(const Leaf *leaves, const uint8_t *bpr
                                           const Ins *ins, const DCol *cols, uint64_t row0,
                                           uint64_t n, bool *div0) {
   const uint32_t all = (1u << V) - 1;
-  if (nb == 0) {
-    uint32_t inb = row0 + V <= n ? all : (row0 >= n ? 0u : ((1u << (uint32_t)(n - row0)) - 1));
-    return inb;
-  }
+  // rows at or past n are never selected, whatever the leaves say (a leaf
+  // comparing two literals is true everywhere)
+  const uint32_t inb = row0 + V <= n ? all : (row0 >= n ? 0u : ((1u << (uint32_t)(n - row0)) - 1));
+  if (nb == 0) return inb;
   uint64_t st = 0;  // stack of V-bit masks
   for (int k = 0; k < nb; k++) {
     const uint8_t b = bprog[k];
@@ -374,7 +374,7 @@ __device__ inline uint32_t eval_predicate(const Leaf *leaves, const uint8_t *bpr
       st = (st & ~(uint64_t)all) | res;
     }
   }
-  return (uint32_t)(st & all);
+  return (uint32_t)(st & all) & inb;
 }
 
 }  // namespace oxr

@@ -187,10 +187,25 @@ int fill_zero(oxr_stream *s, void *p, size_t bytes) {
   return OXR_OK;
 }
 
+StreamCore *retain(StreamCore *c) {
+  if (c) c->refs.fetch_add(1, std::memory_order_relaxed);
+  return c;
+}
+
+void release(StreamCore *c) {
+  if (!c) return;
+  if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  if (c->owned) {
+    hipStreamSynchronize(c->s);
+    hipStreamDestroy(c->s);
+  }
+  delete c;
+}
+
 oxr_batch *new_batch(oxr_stream *s, uint64_t rows) {
   oxr_batch *b = new oxr_batch();
   b->dev = s->dev;
-  b->owner_stream = s->s;
+  b->owner = retain(s->core);
   b->rows = rows;
   return b;
 }
@@ -385,8 +400,10 @@ int oxr_stream_create(oxr_device *dev, oxr_stream **out) {
   OXR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   oxr_stream *st = new oxr_stream();
   st->dev = dev;
+  st->core = new StreamCore();
+  st->core->s = s;
+  st->core->owned = true;
   st->s = s;
-  st->owned = true;
   *out = st;
   return OXR_OK;
 }
@@ -395,18 +412,18 @@ int oxr_stream_wrap(oxr_device *dev, void *hip_stream, oxr_stream **out) {
   if (!dev || !out) return fail(OXR_EPLAN, "rocm.stream", "null argument");
   oxr_stream *st = new oxr_stream();
   st->dev = dev;
-  st->s = (hipStream_t)hip_stream;
-  st->owned = false;
+  st->core = new StreamCore();
+  st->core->s = (hipStream_t)hip_stream;
+  st->core->owned = false;
+  st->s = st->core->s;
   *out = st;
   return OXR_OK;
 }
 
 void oxr_stream_destroy(oxr_stream *s) {
   if (!s) return;
-  if (s->owned) {
-    hipStreamSynchronize(s->s);
-    hipStreamDestroy(s->s);
-  }
+  // the HIP stream itself goes when the last batch/table allocated on it does
+  release(s->core);
   delete s;
 }
 
@@ -725,7 +742,7 @@ int oxr_download(oxr_stream *s, const oxr_batch *batch, struct ArrowSchema *out_
       a->n_children = 1;
       a->children = p->children.data();
       ExportedSchema *sp = (ExportedSchema *)sc->private_data;
-      sp->children.push_back(make_schema("f", "item", false));
+      sp->children.push_back(make_schema("f", "item", true));
       sc->n_children = 1;
       sc->children = sp->children.data();
     } else {
@@ -857,11 +874,13 @@ int oxr_profile_get(const char *kernel, double *total_ms, uint64_t *launches) {
 
 void oxr_batch_free(oxr_batch *b) {
   if (!b) return;
+  hipStream_t s = b->owner ? b->owner->s : nullptr;
   for (auto &c : b->cols) {
     if (!c.owned) continue;
-    b->dev->arena->release(c.values, b->owner_stream);
-    b->dev->arena->release(c.validity, b->owner_stream);
+    b->dev->arena->release(c.values, s);
+    b->dev->arena->release(c.validity, s);
   }
+  release(b->owner);
   delete b;
 }
 

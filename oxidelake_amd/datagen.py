@@ -1,12 +1,15 @@
-"""Seeded synthetic tables for the benchmark workloads.
+"""Seeded synthetic tables for the benchmark workloads, generated in HBM.
 
-Device-side generation (torch on the GPU) for the HBM-resident full-size
-tables, and conversion of the same tensors to pyarrow RecordBatches for the
-CPU oracle at test sizes. The reference's own demo generator is SplitMix64
-(crates/oxidelake-storage/src/datagen.rs:89-95); `splitmix64` below restates
-it for the pinned demo-table fixture.
+Each table is a list of `GenColumn` specs handed to the library's counter-
+based generator (oxr_generate: SplitMix64 draws per (seed, column, row)), so
+the full-size tables never cross PCIe. Tests copy them to the host
+(`RocmBackend.download`) to run the CPU oracle on the same rows. The
+reference's own demo generator is SplitMix64 too
+(crates/oxidelake-storage/src/datagen.rs:89-95); `splitmix64` restates it.
 """
 from __future__ import annotations
+
+from dataclasses import dataclass
 
 import numpy as np
 import pyarrow as pa
@@ -14,6 +17,9 @@ import pyarrow as pa
 from . import plan
 
 MASK64 = (1 << 64) - 1
+
+GEN_UNIFORM_INT, GEN_UNIFORM_REAL, GEN_NORMAL, GEN_LOGNORMAL, GEN_ZIPF, GEN_SEQUENCE, \
+    GEN_PERMUTATION, GEN_MIXTURE, GEN_MIXTURE_NORMALIZED = range(1, 10)
 
 
 def splitmix64(state: int) -> tuple[int, int]:
@@ -25,40 +31,49 @@ def splitmix64(state: int) -> tuple[int, int]:
     return state, z ^ (z >> 31)
 
 
-def pack_validity_torch(valid):
-    """Bool tensor (n) -> LSB-first bitmap bytes, padded to whole 32-bit words."""
-    import torch
-
-    n = valid.shape[0]
-    words = (n + 31) // 32
-    pad = words * 32 - n
-    v = valid.to(torch.uint8)
-    if pad:
-        v = torch.cat([v, torch.zeros(pad, dtype=torch.uint8, device=v.device)])
-    v = v.view(-1, 8).to(torch.int32)
-    w = torch.tensor([1, 2, 4, 8, 16, 32, 64, 128], dtype=torch.int32, device=v.device)
-    return (v * w).sum(dim=1).to(torch.uint8).contiguous()
+@dataclass
+class GenColumn:
+    name: str
+    type: int
+    dist: int
+    a: float = 0.0
+    b: float = 0.0
+    null_frac: float = 0.0
+    dim: int = 0
+    seed: int = 42
 
 
-def config2_torch(n: int, seed: int = 42, device: str = "cuda", null_frac: float = 0.05):
-    """Config 2's table: a Int64 U[0,2^40), b Int32 U[0,1000), c Float32 N(0,1)
-    with `null_frac` NULLs, d Float64 lognormal(0,1). Returns a dict of
-    tensors: a, b, c, c_valid (bool), c_bits (bitmap), d."""
-    import torch
+# ---------------------------------------------------------------- configs
+def config1_columns(seed: int = 42) -> list[GenColumn]:
+    """1M-row table: id Int64 0..N, k Int32 U[0,16), v Float64 U[0,8),
+    emb FixedSizeList<Float32, 8> i.i.d. N(0,1); no NULLs."""
+    return [GenColumn("id", plan.INT64, GEN_SEQUENCE, 0, 0, seed=seed),
+            GenColumn("k", plan.INT32, GEN_UNIFORM_INT, 0, 16, seed=seed),
+            GenColumn("v", plan.FLOAT64, GEN_UNIFORM_REAL, 0.0, 8.0, seed=seed),
+            GenColumn("emb", plan.F32_VECTOR, GEN_MIXTURE, 0, 1.0, dim=8, seed=seed)]
 
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    a = torch.randint(0, 1 << 40, (n,), dtype=torch.int64, device=device, generator=g)
-    b = torch.randint(0, 1000, (n,), dtype=torch.int32, device=device, generator=g)
-    c = torch.randn(n, dtype=torch.float32, device=device, generator=g)
-    c_valid = torch.rand(n, device=device, generator=g) >= null_frac
-    d = torch.randn(n, dtype=torch.float64, device=device, generator=g).exp_()
-    return {"a": a, "b": b, "c": c, "c_valid": c_valid, "c_bits": pack_validity_torch(c_valid), "d": d}
+
+def config1_spec() -> plan.AggregateSpec:
+    """SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k (k = col 1, v = col 2)."""
+    C = plan.Comparison
+    return plan.AggregateSpec(
+        1, [plan.Aggregate(plan.AggregateFunction.Sum, plan.col(2))],
+        filter=plan.and_(plan.cmp(plan.col(1), C.GtEq, plan.Lit.int64(2)),
+                         plan.cmp(plan.col(2), C.Lt, plan.Lit.float64(4.0))))
+
+
+def config2_columns(seed: int = 42, null_frac: float = 0.05) -> list[GenColumn]:
+    """a Int64 U[0,2^40), b Int32 U[0,1000), c Float32 N(0,1) with 5% NULLs,
+    d Float64 lognormal(0,1)."""
+    return [GenColumn("a", plan.INT64, GEN_UNIFORM_INT, 0, float(1 << 40), seed=seed),
+            GenColumn("b", plan.INT32, GEN_UNIFORM_INT, 0, 1000, seed=seed),
+            GenColumn("c", plan.FLOAT32, GEN_NORMAL, 0.0, 1.0, null_frac=null_frac, seed=seed),
+            GenColumn("d", plan.FLOAT64, GEN_LOGNORMAL, 0.0, 1.0, seed=seed)]
 
 
 def config2_query(b_threshold: int = 50, c_threshold: float = 0.5, modulus: int = 7):
-    """SELECT a, b*2+1, c*d FROM t WHERE b < T AND c > 0.5 OR a % 7 = 3
-    over columns (a, b, c, d) = (0, 1, 2, 3). Returns (predicate, projection)."""
+    """SELECT a, b*2+1, c*d FROM t WHERE b < T AND c > 0.5 OR a % 7 = 3 over
+    columns (a, b, c, d) = (0, 1, 2, 3). Returns (predicate, projection)."""
     C = plan.Comparison
     pred = plan.or_(
         plan.and_(plan.cmp(plan.col(1), C.Lt, plan.Lit.int64(b_threshold)),
@@ -71,31 +86,50 @@ def config2_query(b_threshold: int = 50, c_threshold: float = 0.5, modulus: int 
     return pred, proj
 
 
-def config2_arrow(t: dict) -> pa.RecordBatch:
-    """The same table on the host, as a RecordBatch (a, b, c, d)."""
-    a = t["a"].cpu().numpy()
-    b = t["b"].cpu().numpy()
-    c = t["c"].cpu().numpy()
-    cv = t["c_valid"].cpu().numpy()
-    d = t["d"].cpu().numpy()
-    return pa.record_batch({
-        "a": pa.array(a, pa.int64()),
-        "b": pa.array(b, pa.int32()),
-        "c": pa.array(c, pa.float32(), mask=~cv),
-        "d": pa.array(d, pa.float64()),
-    })
+def config2_selectivity(b_threshold: int, c_threshold: float = 0.5, null_frac: float = 0.05,
+                        modulus: int = 7) -> float:
+    """Expected selectivity: P(A) + 1/m - P(A)/m with P(A) = T/1000 * (1-nulls) * P(c > t)."""
+    from math import erf, sqrt
+
+    pc = 0.5 * (1 - erf(c_threshold / sqrt(2)))
+    pa_ = (b_threshold / 1000) * (1 - null_frac) * pc
+    return pa_ + 1 / modulus - pa_ / modulus
 
 
-def config2_wrap(backend, t: dict):
-    """Config 2's tensors as a device batch (no copy)."""
-    return backend.wrap([
-        (t["a"], None, plan.INT64),
-        (t["b"], None, plan.INT32),
-        (t["c"], t["c_bits"], plan.FLOAT32),
-        (t["d"], None, plan.FLOAT64),
-    ], names=["a", "b", "c", "d"])
+def config3_columns(fact_rows: int, dim_rows: int, groups: int, seed: int = 42):
+    """fact: fk Int64 Zipf(1.1) over dim_rows keys, qty Int32 U[1,100],
+    price Float64 U[1,1000]; dim: pk Int64 a permutation of 0..dim_rows-1,
+    grp Int32 U[0, groups)."""
+    fact = [GenColumn("fk", plan.INT64, GEN_ZIPF, 1.1, float(dim_rows), seed=seed),
+            GenColumn("qty", plan.INT32, GEN_UNIFORM_INT, 1, 101, seed=seed),
+            GenColumn("price", plan.FLOAT64, GEN_UNIFORM_REAL, 1.0, 1000.0, seed=seed)]
+    dim = [GenColumn("pk", plan.INT64, GEN_PERMUTATION, 0, 0, seed=seed + 1),
+           GenColumn("grp", plan.INT32, GEN_UNIFORM_INT, 0, groups, seed=seed + 1)]
+    return fact, dim
 
 
+def config3_spec() -> plan.AggregateSpec:
+    """SUM(qty), SUM(qty*price), COUNT(*) over the fact columns (qty = 1,
+    price = 2); the group key comes from the dimension side."""
+    F = plan.AggregateFunction
+    return plan.AggregateSpec(-1, [
+        plan.Aggregate(F.Sum, plan.col(1), "SUM(qty)"),
+        plan.Aggregate(F.Sum, plan.mul(plan.col(1), plan.col(2)), "SUM(qty*price)"),
+        plan.Aggregate(F.CountStar, None, "COUNT(*)"),
+    ])
+
+
+def config4_columns(seed: int = 42, dim: int = 768, centroids: int = 1024, sigma: float = 0.3):
+    """Embeddings: a Gaussian mixture of N(0,1) centroids, sigma per point,
+    L2-normalised."""
+    return [GenColumn("emb", plan.F32_VECTOR, GEN_MIXTURE_NORMALIZED, centroids, sigma, dim=dim, seed=seed)]
+
+
+def table(backend, stream, columns: list[GenColumn], rows: int):
+    return backend.generate(stream, columns, rows)
+
+
+# ------------------------------------------------------------ host helpers
 def random_batch(rng: np.random.Generator, n: int, null_frac: float = 0.1,
                  specials: bool = True) -> pa.RecordBatch:
     """A mixed-type batch for parity tests: i Int64, j Int32, f Float32,

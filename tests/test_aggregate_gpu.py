@@ -1,0 +1,188 @@
+"""GPU parity: grouped aggregation vs the CPU oracle.
+
+Keys, COUNT, integer SUM/MIN/MAX and float MIN/MAX: bit-exact. Float SUM:
+within 1e-12 relative of the oracle's compensated (exact) sum — the device
+adds in a different order than the CPU's sequential fold, and that order is
+the only divergence (crates/oxidelake-device/src/cpu/kernels.rs:478-818).
+"""
+import math
+import struct
+
+import numpy as np
+import pyarrow as pa
+import pytest
+
+from oxidelake_amd import datagen, plan
+from oxidelake_amd.backend import AggState
+from oxidelake_amd.plan import AggregateFunction as F
+from oxidelake_amd.plan import Aggregate, AggregateSpec, Lit, col
+from oxidelake_amd.plan import Comparison as Cmp
+from tests.test_oracle_golden import EDGE_ROWS, EDGE_SPEC, edge_batch
+from tests.util import assert_arrays_equal
+
+pytestmark = pytest.mark.gpu
+
+SUM_REL = 1e-12
+
+
+def fbits(v):
+    return struct.unpack("<Q", struct.pack("<d", v))[0]
+
+
+def assert_agg_equal(got: pa.RecordBatch, want: pa.RecordBatch, spec: AggregateSpec):
+    assert got.num_rows == want.num_rows, f"{got.num_rows} groups != {want.num_rows}"
+    assert got.num_columns == want.num_columns
+    key_cols = 0 if spec.group_by < 0 else 1
+    if key_cols:
+        assert_arrays_equal(got.column(0), want.column(0), "key")
+    for i, a in enumerate(spec.aggregates):
+        g, w = got.column(key_cols + i), want.column(key_cols + i)
+        assert g.type == w.type, (i, g.type, w.type)
+        if a.func == F.Sum and w.type == pa.float64():
+            gv, wv = g.to_pylist(), w.to_pylist()
+            for r, (x, y) in enumerate(zip(gv, wv)):
+                assert (x is None) == (y is None), (i, r, x, y)
+                if x is None:
+                    continue
+                if math.isnan(y) or math.isinf(y):
+                    assert (math.isnan(x) and math.isnan(y)) or x == y, (i, r, x, y)
+                else:
+                    assert abs(x - y) <= SUM_REL * max(abs(y), 1e-300), (i, r, x, y, abs(x - y) / max(abs(y), 1e-300))
+        else:
+            assert_arrays_equal(g, w, f"aggregate {i} ({got.schema.field(key_cols + i).name})")
+
+
+def device_aggregate(backend, stream, batch, spec):
+    dev = backend.upload(stream, batch)
+    return backend.download(stream, backend.aggregate(stream, dev, spec))
+
+
+def test_reference_known_answer(backend, stream, oracle):
+    # kernels.rs:1123-1158
+    b = pa.record_batch({"k": pa.array([1, 2, None, 2, 5], pa.int64()),
+                         "v": pa.array([0.5, None, 2.5, 3.5, -1.0], pa.float64())})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.Count, col(1)),
+                             Aggregate(F.Min, col(1)), Aggregate(F.Max, col(0))])
+    got = device_aggregate(backend, stream, b, spec)
+    assert got.column(0).to_pylist() == [1, 2, 5, None]
+    assert got.column(1).to_pylist() == [0.5, 3.5, -1.0, 2.5]
+    assert got.column(2).to_pylist() == [1, 1, 1, 1]
+    assert got.column(4).to_pylist() == [1, 2, 5, None]
+    assert got.schema.field(1).name == "SUM(v)" and got.schema.field(4).name == "MAX(k)"
+    assert_agg_equal(got, oracle.aggregate(b, spec, sum_mode=1), spec)
+
+
+def test_edge_values_wrap_and_total_order(backend, stream, oracle):
+    # kernels.rs:1209-1288: wrapped sums, NaN of both signs, signed zeros
+    b = edge_batch(EDGE_ROWS)
+    got = device_aggregate(backend, stream, b, EDGE_SPEC)
+    assert got.column(1).to_pylist() == [-7, 4, None, -(2**63) + 7, -2]
+    assert_agg_equal(got, oracle.aggregate(b, EDGE_SPEC, sum_mode=1), EDGE_SPEC)
+
+
+def test_sum_overflow_rows(backend, stream):
+    # conformance.rs:306-356
+    I64MAX = 2**63 - 1
+    rows = [(1, I64MAX - 1), (2, 5), (1, I64MAX - 1), (3, -(2**63)), (4, I64MAX), (3, -1),
+            (None, I64MAX), (4, 1), (None, I64MAX), (4, -2)]
+    b = pa.record_batch({"k": pa.array([r[0] for r in rows], pa.int64()),
+                         "x": pa.array([r[1] for r in rows], pa.int64())})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.Count, col(1))])
+    got = device_aggregate(backend, stream, b, spec)
+    rendered = ["|".join("NULL" if v is None else str(v) for v in row)
+                for row in zip(*[c.to_pylist() for c in got.columns])]
+    assert rendered == ["1|-4|2", "2|5|1", f"3|{I64MAX}|2", f"4|{I64MAX - 1}|3", "NULL|-2|2"]
+
+
+def test_empty_batch_has_no_groups(backend, stream):
+    b = pa.record_batch({"k": pa.array([], pa.int64()), "v": pa.array([], pa.float64())})
+    got = device_aggregate(backend, stream, b, AggregateSpec(0, [Aggregate(F.Count, col(1))]))
+    assert got.num_rows == 0 and got.num_columns == 2
+
+
+def test_int64_min_key_is_a_group(backend, stream, oracle):
+    b = pa.record_batch({"k": pa.array([-(2**63), 5, -(2**63), None, 5], pa.int64()),
+                         "x": pa.array([1, 2, 3, 4, 5], pa.int64())})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.CountStar)])
+    got = device_aggregate(backend, stream, b, spec)
+    assert got.column(0).to_pylist() == [-(2**63), 5, None]
+    assert got.column(1).to_pylist() == [4, 7, 4]
+    assert_agg_equal(got, oracle.aggregate(b, spec), spec)
+
+
+SPECS = [
+    AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.Count, col(1)), Aggregate(F.Min, col(1)),
+                      Aggregate(F.Max, col(1))]),
+    AggregateSpec(1, [Aggregate(F.Sum, col(3)), Aggregate(F.Min, col(3)), Aggregate(F.Max, col(3)),
+                      Aggregate(F.CountStar)]),
+    AggregateSpec(0, [Aggregate(F.Sum, col(2)), Aggregate(F.Min, col(2)), Aggregate(F.Max, col(2)),
+                      Aggregate(F.Count, col(2))]),
+    AggregateSpec(1, [Aggregate(F.Sum, plan.mul(col(0), col(1))), Aggregate(F.Sum, plan.mul(col(2), col(3))),
+                      Aggregate(F.CountStar), Aggregate(F.Min, plan.add(col(1), Lit.int32(1))),
+                      Aggregate(F.Max, col(0)), Aggregate(F.Count, col(3))]),
+    AggregateSpec(-1, [Aggregate(F.Sum, col(0)), Aggregate(F.Count, col(3)), Aggregate(F.Max, col(3))]),
+    AggregateSpec(0, [Aggregate(F.Sum, col(3))],
+                  filter=plan.and_(plan.cmp(col(1), Cmp.GtEq, Lit.int64(-5)),
+                                   plan.cmp(col(3), Cmp.Lt, Lit.float64(4.0)))),
+]
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 1000, 50_000])
+@pytest.mark.parametrize("si", range(len(SPECS)))
+def test_random_parity(backend, stream, oracle, n, si):
+    rng = np.random.default_rng(100 + n)
+    batch = datagen.random_batch(rng, n).select([0, 1, 2, 3])
+    spec = SPECS[si]
+    got = device_aggregate(backend, stream, batch, spec)
+    want = oracle.aggregate(batch, spec, sum_mode=1)
+    assert_agg_equal(got, want, spec)
+
+
+def test_many_groups_and_streaming_state(backend, stream, oracle):
+    rng = np.random.default_rng(8)
+    n = 200_000
+    k = rng.integers(0, 60_000, n).astype(np.int64)
+    x = rng.integers(-(2**40), 2**40, n).astype(np.int64)
+    v = rng.random(n) * 1000
+    batch = pa.record_batch({"k": pa.array(k), "x": pa.array(x), "v": pa.array(v)})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.Sum, col(2)), Aggregate(F.Min, col(1)),
+                             Aggregate(F.Max, col(2)), Aggregate(F.CountStar)])
+    want = oracle.aggregate(batch, spec, sum_mode=1)
+    got = device_aggregate(backend, stream, batch, spec)
+    assert_agg_equal(got, want, spec)
+    # the same rows streamed through a resident state in uneven batches
+    dev_batches = [backend.upload(stream, batch.slice(a, b - a))
+                   for a, b in [(0, 7), (7, 50_000), (50_000, 50_001), (50_001, n)]]
+    st = AggState(backend, stream, dev_batches[0], spec)
+    for d in dev_batches:
+        st.update(stream, d)
+    got = backend.download(stream, st.finish(stream))
+    assert_agg_equal(got, want, spec)
+
+
+def test_config1_query(backend, stream, oracle):
+    # SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k over 1M rows
+    rng = np.random.default_rng(42)
+    n = 1_000_000
+    k = rng.integers(0, 16, n).astype(np.int32)
+    v = rng.random(n) * 8
+    batch = pa.record_batch({"k": pa.array(k), "v": pa.array(v)})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1))],
+                         filter=plan.and_(plan.cmp(col(0), Cmp.GtEq, Lit.int32(2)),
+                                          plan.cmp(col(1), Cmp.Lt, Lit.float64(4.0))))
+    got = device_aggregate(backend, stream, batch, spec)
+    want = oracle.aggregate(batch, spec, sum_mode=1)
+    assert got.num_rows == 14
+    assert got.column(0).type == pa.int32()
+    assert_agg_equal(got, want, spec)
+    exact = [math.fsum(v[(k == g) & (v < 4.0)]) for g in range(2, 16)]
+    assert_agg_equal(got, pa.record_batch({"k": got.column(0), "SUM(v)": pa.array(exact)}), spec)
+
+
+def test_global_aggregate_over_no_rows_is_one_row(backend, stream, oracle):
+    b = pa.record_batch({"x": pa.array([], pa.int64()), "v": pa.array([], pa.float64())})
+    spec = AggregateSpec(-1, [Aggregate(F.Sum, col(0)), Aggregate(F.Count, col(1)), Aggregate(F.Max, col(1)),
+                              Aggregate(F.CountStar)])
+    got = device_aggregate(backend, stream, b, spec)
+    assert [c.to_pylist() for c in got.columns] == [[None], [0], [None], [0]]
+    assert_agg_equal(got, oracle.aggregate(b, spec), spec)

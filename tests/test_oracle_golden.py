@@ -382,3 +382,13 @@ def test_splitmix64_known_values():
         s, z = datagen.splitmix64(s)
         out.append(z)
     assert out == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_global_aggregate_is_one_row(oracle):
+    b = pa.record_batch({"x": pa.array([], pa.int64())})
+    spec = AggregateSpec(-1, [Aggregate(F.Sum, col(0)), Aggregate(F.CountStar)])
+    out = oracle.aggregate(b, spec)
+    assert out.num_columns == 2 and [c.to_pylist() for c in out.columns] == [[None], [0]]
+    b = pa.record_batch({"x": pa.array([3, None, 4], pa.int64())})
+    out = oracle.aggregate(b, spec)
+    assert [c.to_pylist() for c in out.columns] == [[7], [3]]

@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 IN_BYTES_PER_ROW = 8 + 4 + 4 + 1 / 8 + 8      # a, b, c (+validity bit), d
 OUT_BYTES_PER_ROW = 8 + 8 + 8 + 1 / 8         # a, b*2+1 (Int64), c*d (+validity)
+BATCH_ROWS = 1_000_000                        # 1M-row batches (a multiple of 32 rows)
 
 
 def parse():
@@ -104,18 +105,30 @@ def main():
     dev = datagen.table(be, stream, datagen.config2_columns(seed=42), n)
 
     def run(threshold):
+        # 1M-row batches, each filtered and compacted on its own, one launch
         pred, proj = datagen.config2_query(threshold)
-        return be.filter_project(stream, dev, pred, proj, ["a", "b2", "cd"])
+        return be.filter_project_batches(stream, dev, BATCH_ROWS, pred, proj, ["a", "b2", "cd"])
+
+    def rows_of(outs):
+        return sum(o.num_rows for o in outs)
 
     # full-size parity (count and checksums against numpy over the whole table)
     host = be.download(stream, dev)
-    out = run(args.b_threshold)
-    sel = out.num_rows
-    ok = full_size_check(host, be.download(stream, out), args.b_threshold)
-    out = None
+    outs = run(args.b_threshold)
+    sel = rows_of(outs)
+    import pyarrow as pa
+
+    out_host = pa.Table.from_batches([be.download(stream, o) for o in outs]).combine_chunks().to_batches()[0]
+    ok = full_size_check(host, out_host, args.b_threshold)
+    outs = out_host = None
     del host
     # the generator is counter-based: these are the full table's first 2M rows
     sample = be.download(stream, datagen.table(be, stream, datagen.config2_columns(seed=42), 2_000_000))
+
+    # launch shape picked by the project autotuner on this device and table
+    from oxidelake_amd import autotune
+
+    tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
 
     lib.oxr_profile_enable(0)
     for _ in range(args.warmup):
@@ -162,6 +175,7 @@ def main():
         "kernel_ms": kernel_ms,
         "hbm_gbs": achieved,
         "parity_full_size": ok,
+        "autotune": tuned,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -175,7 +189,7 @@ def main():
         sweep = {}
         for thr in (500, 1000):
             o = run(thr)
-            s = o.num_rows
+            s = rows_of(o)
             o = None
             stream.synchronize()
             lib.oxr_profile_reset()

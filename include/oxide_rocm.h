@@ -237,6 +237,19 @@ int oxr_filter_project(oxr_stream *s, const oxr_batch *input,
                        const oxr_expr *predicate, const oxr_expr *projection,
                        int32_t nproj, const char *const *names, oxr_batch **out);
 
+/* The same over many batches in one launch: `input` read as consecutive
+ * batches of `batch_rows` rows (a multiple of 32; the last may be short),
+ * each filtered and compacted on its own, as the reference's operator does
+ * batch by batch (crates/oxidelake-compute/src/exec/filter.rs:182-210), but
+ * without a launch, a host round trip and an allocation per batch. Writes
+ * ceil(rows / batch_rows) result batches (in input order) to outs[], which
+ * has room for max_batches; *nbatches is that count. The result batches share
+ * their device buffers; each may be freed independently. */
+int oxr_filter_project_batches(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows,
+                               const oxr_expr *predicate, const oxr_expr *projection,
+                               int32_t nproj, const char *const *names, oxr_batch **outs,
+                               uint64_t max_batches, uint64_t *nbatches);
+
 /* -------------------------------------------------------------- hash join */
 
 /* `GpuBackend::prepare_join_build` (backend.rs:129-142, args.rs:21-31): the
@@ -366,6 +379,16 @@ typedef struct oxr_gen_spec {
 } oxr_gen_spec;
 int oxr_generate(oxr_stream *s, const oxr_gen_spec *cols, const char *const *names,
                  int32_t ncols, uint64_t rows, oxr_batch **out);
+
+/* ------------------------------------------------------------------ tuning */
+/* Launch shapes and table sizes the autotuner picks (no counterpart in the
+ * reference, whose CUDA launches are fixed at 256 threads, cuda/ops.rs:42).
+ * Knobs: "filter.v" / "filter.g" (rows per lane, row groups per lane of the
+ * filter kernel), "agg.grid" (blocks per CU of the aggregate kernel),
+ * "vector.rows" (rows per block of the vector scan). A negative value resets
+ * a knob to its default; oxr_tune_get reports -1 for a knob never set. */
+int oxr_tune_set(const char *knob, int64_t value);
+int oxr_tune_get(const char *knob, int64_t *value);
 
 /* ---------------------------------------------------------------- profiling */
 /* Per-kernel device time from HIP events recorded on the launching stream

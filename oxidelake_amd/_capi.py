@@ -130,6 +130,11 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [P, P, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(C.c_char_p), PP],
     ),
+    "oxr_filter_project_batches": (
+        C.c_int,
+        [P, P, U64, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(C.c_char_p), PP, U64,
+         C.POINTER(U64)],
+    ),
     "oxr_join_build": (C.c_int, [P, P, I32, PP]),
     "oxr_join_table_info": (C.c_int, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(C.c_int)]),
     "oxr_join_table_free": (None, [P]),
@@ -152,6 +157,8 @@ SIGNATURES: dict[str, tuple] = {
          C.POINTER(C.c_float)],
     ),
     "oxr_generate": (C.c_int, [P, C.POINTER(GenSpec), C.POINTER(C.c_char_p), I32, U64, PP]),
+    "oxr_tune_set": (C.c_int, [C.c_char_p, C.c_int64]),
+    "oxr_tune_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "oxr_profile_enable": (None, [C.c_int]),
     "oxr_profile_reset": (None, []),
     "oxr_profile_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]),

@@ -1,0 +1,89 @@
+"""Project autotuner: picks kernel launch shapes by timing them on the device.
+
+Knobs live in the library (oxr_tune_set). The tuner sweeps the instantiated
+candidates of a knob group on a representative batch, timing each with the
+library's HIP-event kernel profiler (oxr_profile_*), keeps the fastest and
+records the choice in a JSON cache keyed by device and workload, so a later
+process applies it without re-timing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from pathlib import Path
+from typing import Callable, Iterable, Sequence
+
+from ._capi import lib
+
+CACHE = Path(os.environ.get("OXR_TUNE_CACHE", Path.home() / ".cache" / "oxidelake_amd" / "tune.json"))
+
+FILTER_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (4, 1)]
+AGG_GRIDS = [2, 4, 8, 16]
+
+
+def set_knob(name: str, value: int) -> None:
+    lib.oxr_tune_set(name.encode(), int(value))
+
+
+def get_knob(name: str) -> int:
+    v = C.c_int64()
+    lib.oxr_tune_get(name.encode(), C.byref(v))
+    return int(v.value)
+
+
+def kernel_ms(kernel: str, run: Callable[[], object], reps: int) -> float:
+    """Average device time of `kernel` over `reps` calls of `run`."""
+    run()  # warm (allocations, code objects)
+    lib.oxr_profile_reset()
+    lib.oxr_profile_enable(1)
+    try:
+        for _ in range(reps):
+            run()
+    finally:
+        lib.oxr_profile_enable(0)
+    ms, n = C.c_double(), C.c_uint64()
+    lib.oxr_profile_get(kernel.encode(), C.byref(ms), C.byref(n))
+    lib.oxr_profile_reset()
+    return ms.value / max(1, n.value)
+
+
+def sweep(kernel: str, knobs: Sequence[str], candidates: Iterable[Sequence[int]],
+          run: Callable[[], object], reps: int = 3) -> dict:
+    """Times every candidate setting of `knobs`, applies the fastest."""
+    timings = {}
+    for cand in candidates:
+        for k, v in zip(knobs, cand):
+            set_knob(k, v)
+        timings[",".join(map(str, cand))] = kernel_ms(kernel, run, reps)
+    best = min(timings, key=timings.get)
+    for k, v in zip(knobs, best.split(",")):
+        set_knob(k, int(v))
+    return {"knobs": dict(zip(knobs, map(int, best.split(",")))), "ms": timings}
+
+
+def tune_filter(run: Callable[[], object], reps: int = 3) -> dict:
+    return sweep("filter_project", ["filter.v", "filter.g"], FILTER_SHAPES, run, reps)
+
+
+def tune_aggregate(run: Callable[[], object], reps: int = 3) -> dict:
+    return sweep("aggregate", ["agg.grid"], [(g,) for g in AGG_GRIDS], run, reps)
+
+
+def save(key: str, result: dict, path: Path = CACHE) -> None:
+    path.parent.mkdir(parents=True, exist_ok=True)
+    data = json.loads(path.read_text()) if path.exists() else {}
+    data[key] = result
+    path.write_text(json.dumps(data, indent=1))
+
+
+def apply(key: str, path: Path = CACHE) -> bool:
+    """Applies a cached choice; False when none is cached for `key`."""
+    if not path.exists():
+        return False
+    entry = json.loads(path.read_text()).get(key)
+    if not entry:
+        return False
+    for k, v in entry["knobs"].items():
+        set_knob(k, v)
+    return True

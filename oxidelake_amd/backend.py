@@ -316,6 +316,25 @@ class RocmBackend:
                                      narr, C.byref(out)))
         return DeviceBatch(out.value)
 
+    def filter_project_batches(self, stream: Stream, batch: DeviceBatch, batch_rows: int,
+                               predicate: ExprT | None, projection: Sequence,
+                               names: Optional[Sequence[str]] = None) -> list[DeviceBatch]:
+        """`batch` read as consecutive batches of `batch_rows` rows, each
+        filtered on its own, all in one launch (oxr_filter_project_batches)."""
+        exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
+        if names is None:
+            names = [batch.name(p.index) if isinstance(p, plan.Col) else f"p{i}"
+                     for i, p in enumerate(exprs)]
+        pred = Encoded(predicate)
+        parr, penc = plan.encode_many(exprs)
+        narr, raw = _cstrs(names, len(exprs))
+        cap = max(1, -(-batch.num_rows // batch_rows))
+        outs = (C.c_void_p * cap)()
+        nb = C.c_uint64()
+        check(lib.oxr_filter_project_batches(stream.handle, batch.handle, batch_rows, pred.ref(), parr,
+                                             len(exprs), narr, outs, cap, C.byref(nb)))
+        return [DeviceBatch(outs[i]) for i in range(nb.value)]
+
     def prepare_join_build(self, stream: Stream, build: DeviceBatch, key: int) -> JoinTable:
         """backend.rs:129-142 (the table is built on the device)."""
         out = C.c_void_p()

@@ -29,7 +29,7 @@
 namespace oxr {
 namespace {
 
-constexpr int V = 4;
+constexpr int V = 1;  // rows per lane (register use grows with it)
 constexpr int kBlock = 256;
 constexpr int kTileRows = kBlock * V;
 constexpr int kMaxAggs = 8;
@@ -78,9 +78,7 @@ struct AggDesc {
   uint8_t in_type;   // value type of the input expression
   uint8_t nullable;  // input can be NULL (then `has` flags are kept)
   uint8_t pad;
-  int16_t col;       // bare column input, or -1
-  uint16_t pstart, plen;
-  int16_t pad2;
+  ValueProg vp;      // the input: a slot or a program
 };
 
 struct Table {
@@ -99,8 +97,11 @@ struct Params {
   Ins ins[kMaxIns];
   Leaf leaves[kMaxLeaves];
   uint8_t bprog[kMaxBool];
+  DivMagic magic[kMaxMagic];
+  int8_t slot_cols[kMaxSlots];
+  int32_t nslots;
   int32_t nb;
-  int32_t key_col;   // -1: one global group
+  int32_t key_slot;  // -1: one global group
   int32_t naggs;
   int32_t lds_aggs;  // aggregates folded in LDS (0 = all global)
   AggDesc aggs[kMaxAggs];
@@ -207,6 +208,7 @@ __device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, 
   return global_slot(t, key, ok);
 }
 
+template <int NS>
 __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
   __shared__ int64_t s_keys[kLdsSlots + 2];
   __shared__ int64_t s_acc[kLdsAggs][kLdsSlots + 2];
@@ -262,17 +264,24 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
 
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t row0 = tile * kTileRows + (uint64_t)tid * V;
-    uint32_t sel = eval_predicate<V>(p.leaves, p.bprog, p.nb, p.ins, p.cols, row0, p.n, &div0);
+    Slots<V, NS> S;
+    load_slots<V, NS>(p.cols, p.slot_cols, p.nslots, row0, p.n, S);
+    uint32_t sel = eval_predicate<V, NS>(p.leaves, p.bprog, p.nb, p.ins, p.cols, S, p.magic, row0,
+                                         p.n, &div0);
     uint64_t key[V];
-    uint32_t kvalid = (1u << V) - 1;
-    if (p.key_col >= 0) {
-      // a NULL key is a group of its own: validity only picks the slot
-      load_rows<V>(p.cols[p.key_col], row0, p.n, key, kvalid);
-    } else {
-      // one global group, kept in the slot the NULL key would use
+    uint32_t kvalid = 0;  // one global group: the slot a NULL key would use
 #pragma unroll
-      for (int q = 0; q < V; q++) key[q] = 0;
-      kvalid = 0;
+    for (int q = 0; q < V; q++) key[q] = 0;
+    if (p.key_slot >= 0) {
+      // a NULL key is a group of its own: validity only picks the slot
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        if (p.key_slot == k) {
+#pragma unroll
+          for (int q = 0; q < V; q++) key[q] = S.v[k][q];
+          kvalid = S.m[k];
+        }
+      }
     }
     if (sel) {
       // LDS slots of this lane's selected rows (-1: go global)
@@ -337,15 +346,8 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
         if (d.kind == A_COUNT_STAR) {
 #pragma unroll
           for (int q = 0; q < V; q++) v[q] = 0;
-        } else if (d.col >= 0) {
-          load_rows<V>(p.cols[d.col], row0, p.n, v, vm);
         } else {
-          uint64_t r[kRegs][V];
-          uint32_t m[kRegs] = {0, 0, 0, 0};
-          run_program<V>(p.ins, d.pstart, d.plen, p.cols, row0, p.n, r, m, &div0);
-#pragma unroll
-          for (int q = 0; q < V; q++) v[q] = r[0][q];
-          vm = m[0];
+          eval_value<V, NS>(d.vp, p.ins, S, p.magic, v, vm, &div0);
         }
 #pragma unroll
         for (int q = 0; q < V; q++) {
@@ -510,6 +512,7 @@ struct oxr_agg_state {
   oxr_device *dev = nullptr;
   oxr::StreamCore *owner = nullptr;
   int32_t key_col = -1;
+  int32_t key_slot = -1;
   int32_t key_type = OXR_INT64;
   std::string key_name;
   uint64_t hint = 0;
@@ -627,21 +630,19 @@ int compile_spec(const oxr_batch *like, const oxr_agg_spec *spec, oxr_agg_state 
                   "group key column has a type other than Int64/Int32");
     st->key_type = kt;
     st->key_name = like->names[spec->group_by];
+    st->key_slot = st->prog.slot_of(spec->group_by);
   }
   OXR_TRY(compile_predicate(spec->filter, like->cols, &st->prog));
   for (int i = 0; i < spec->naggs; i++) {
     const oxr_agg &g = spec->aggs[i];
     AggDesc d{};
-    d.col = -1;
     std::string in_name = "*";
     int32_t in_type = OXR_INT64;
     if (g.func != OXR_COUNT_STAR) {
       ValueProg vp;
       OXR_TRY(compile_value(&g.input, like->cols, &st->prog, &vp));
       in_type = vp.type;
-      d.col = vp.col;
-      d.pstart = vp.start;
-      d.plen = vp.len;
+      d.vp = vp;
       d.nullable = vp.nullable;
       in_name = vp.col >= 0 ? like->names[vp.col] : "expr";
       if (in_type == OXR_F32_VECTOR)
@@ -705,8 +706,11 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in) {
   std::copy(st->prog.ins.begin(), st->prog.ins.end(), P.ins);
   std::copy(st->prog.leaves.begin(), st->prog.leaves.end(), P.leaves);
   std::copy(st->prog.bprog.begin(), st->prog.bprog.end(), P.bprog);
+  std::copy(st->prog.magic.begin(), st->prog.magic.end(), P.magic);
+  for (size_t i = 0; i < st->prog.slot_cols.size(); i++) P.slot_cols[i] = (int8_t)st->prog.slot_cols[i];
+  P.nslots = (int32_t)st->prog.slot_cols.size();
   P.nb = (int32_t)st->prog.bprog.size();
-  P.key_col = st->key_col;
+  P.key_slot = st->key_col >= 0 ? st->key_slot : -1;
   P.naggs = (int32_t)st->aggs.size();
   P.lds_aggs = P.naggs <= kLdsAggs ? P.naggs : 0;
   for (int a = 0; a < P.naggs; a++) P.aggs[a] = st->aggs[a];
@@ -716,10 +720,15 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in) {
   P.error = (uint32_t *)err;
   int rc = fill_zero(s, err, 64);
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
-  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * 8);
+  const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.grid", 8));
+  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   if (rc == OXR_OK) {
     ProfScope prof(s->s, "aggregate");
-    hipLaunchKernelGGL(aggregate_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P);
+    switch (slot_bucket(P.nslots)) {
+      case 2: hipLaunchKernelGGL(aggregate_kernel<2>, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P); break;
+      case 4: hipLaunchKernelGGL(aggregate_kernel<4>, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P); break;
+      default: hipLaunchKernelGGL(aggregate_kernel<8>, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P); break;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail_hip(e, "aggregate_kernel launch");
   }

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -129,6 +130,10 @@ void stream_release(oxr_stream *s, void *p);
 // captured hipGraph.
 int fill_zero(oxr_stream *s, void *p, size_t bytes);
 
+// Tuning knobs (oxr_tune_set): launch shapes and table sizes chosen by the
+// autotuner; `dflt` when a knob was never set.
+int64_t tune_get(const char *knob, int64_t dflt);
+
 // Kernel timing (oxr_profile_*): brackets one launch with HIP events when
 // profiling is enabled; a no-op otherwise.
 class ProfScope {
@@ -150,6 +155,7 @@ struct oxr_batch {
   uint64_t rows = 0;
   std::vector<oxr::DevCol> cols;
   std::vector<std::string> names;
+  std::shared_ptr<void> holder;  // buffers shared with other views of one result
 };
 
 namespace oxr {

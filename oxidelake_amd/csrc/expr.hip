@@ -111,6 +111,17 @@ struct Emitter {
   const std::vector<DevCol> &cols;
   Program *p;
 
+  // Result type of nodes [begin, end) (a value subtree), without emitting.
+  int32_t type_of(int begin, int end) {
+    const oxr_expr_node &nd = e->nodes[end - 1];
+    if (nd.kind == OXR_X_COLUMN) return cols[nd.column].type;
+    if (nd.kind == OXR_X_LITERAL) return nd.type;
+    if (nd.kind == OXR_X_CMP || nd.kind == OXR_X_AND || nd.kind == OXR_X_OR) return 0;
+    int mid;
+    split(e, begin, end, &mid);
+    return coerce(type_of(begin, mid), type_of(mid, end - 1));
+  }
+
   // Emits code computing nodes [begin, end) into register `reg`; returns
   // its type, or 0 on failure (error recorded).
   int32_t emit(int begin, int end, int reg, bool *nullable) {
@@ -126,8 +137,15 @@ struct Emitter {
     Ins in;
     std::memset(&in, 0, sizeof in);
     if (nd.kind == OXR_X_COLUMN) {
+      const int slot = p->slot_of(nd.column);
+      if (slot < 0) {
+        fail(OXR_EUNSUPPORTED, "rocm.expr",
+             "an operator reads at most " + std::to_string(kMaxSlots) + " distinct columns");
+        return 0;
+      }
       in.op = I_LOAD;
       in.col = nd.column;
+      in.a = (uint8_t)slot;
       in.type = (uint8_t)cols[nd.column].type;
       in.dst = (uint8_t)reg;
       p->ins.push_back(in);
@@ -153,6 +171,36 @@ struct Emitter {
     }
     int mid;
     split(e, begin, end, &mid);
+    // integer / or % by a non-zero integer literal: multiply-high by a magic
+    // number instead of a division
+    const oxr_expr_node &rhs = e->nodes[mid];
+    const int64_t dlit = rhs.type == OXR_INT32 ? (int64_t)(int32_t)rhs.lit_i : rhs.lit_i;
+    if ((nd.kind == OXR_X_DIV || nd.kind == OXR_X_MOD) && end - 1 - mid == 1 &&
+        rhs.kind == OXR_X_LITERAL && is_int(rhs.type) && dlit != 0 && is_int(type_of(begin, mid)) &&
+        (int)p->magic.size() < kMaxMagic) {
+      const int64_t d = dlit;
+      int32_t ta = emit(begin, mid, reg, nullable);
+      if (!ta) return 0;
+      {
+        const int32_t t = coerce(ta, rhs.type);
+        if (!cast(reg, ta, t)) return 0;
+        Ins dl;
+        std::memset(&dl, 0, sizeof dl);
+        const bool i32 = t == OXR_INT32;
+        dl.op = nd.kind == OXR_X_DIV ? (i32 ? I_DIVL_I32 : I_DIVL_I64) : (i32 ? I_MODL_I32 : I_MODL_I64);
+        dl.type = (uint8_t)t;
+        dl.dst = (uint8_t)reg;
+        dl.a = (uint8_t)reg;
+        dl.lit = (int64_t)p->magic.size();
+        p->magic.push_back(div_magic(d));
+        if ((int)p->ins.size() >= kMaxIns) {
+          fail(OXR_EUNSUPPORTED, "rocm.expr", "expression has too many instructions");
+          return 0;
+        }
+        p->ins.push_back(dl);
+        return t;
+      }
+    }
     int32_t ta = emit(begin, mid, reg, nullable);
     if (!ta) return 0;
     int32_t tb = emit(mid, end - 1, reg + 1, nullable);
@@ -160,21 +208,24 @@ struct Emitter {
     const int32_t t = coerce(ta, tb);
     if (!cast(reg, ta, t) || !cast(reg + 1, tb, t)) return 0;
     std::memset(&in, 0, sizeof in);
+    int k;
     switch (nd.kind) {
-      case OXR_X_ADD: in.op = I_ADD; break;
-      case OXR_X_SUB: in.op = I_SUB; break;
-      case OXR_X_MUL: in.op = I_MUL; break;
-      case OXR_X_DIV: in.op = I_DIV; break;
-      case OXR_X_MOD: in.op = I_MOD; break;
+      case OXR_X_ADD: k = 0; break;
+      case OXR_X_SUB: k = 1; break;
+      case OXR_X_MUL: k = 2; break;
+      case OXR_X_DIV: k = 3; break;
+      case OXR_X_MOD: k = 4; break;
       default:
         fail(OXR_EPLAN, "rocm.expr", "a boolean inside a value expression");
         return 0;
     }
-    if ((in.op == I_DIV || in.op == I_MOD) && is_int(t)) p->has_div = true;
+    // fused opcode: the operation within its type's block of five
+    const int base = t == OXR_INT32 ? I_ADD_I32 : t == OXR_INT64 ? I_ADD_I64
+                     : t == OXR_FLOAT32 ? I_ADD_F32 : I_ADD_F64;
+    in.op = (uint8_t)(base + k);
+    if (k >= 3 && is_int(t)) p->has_div = true;
     in.type = (uint8_t)t;
     in.dst = (uint8_t)reg;
-    in.a = (uint8_t)reg;
-    in.b = (uint8_t)(reg + 1);
     if ((int)p->ins.size() >= kMaxIns) {
       fail(OXR_EUNSUPPORTED, "rocm.expr", "expression has too many instructions");
       return 0;
@@ -183,19 +234,25 @@ struct Emitter {
     return t;
   }
 
+  // Widens register `reg` from `from` to `to` (coercion never narrows).
+  // Int32 values are held sign-extended, so Int32 -> Int64 is free.
   bool cast(int reg, int32_t from, int32_t to) {
-    if (from == to) return true;
+    if (from == to || (from == OXR_INT32 && to == OXR_INT64)) return true;
     if ((int)p->ins.size() >= kMaxIns) {
       fail(OXR_EUNSUPPORTED, "rocm.expr", "expression has too many instructions");
       return false;
     }
     Ins in;
     std::memset(&in, 0, sizeof in);
-    in.op = I_CAST;
-    in.from = (uint8_t)from;
+    if (to == OXR_FLOAT32 && is_int(from)) in.op = I_CAST_I_F32;
+    else if (to == OXR_FLOAT64 && is_int(from)) in.op = I_CAST_I_F64;
+    else if (to == OXR_FLOAT64 && from == OXR_FLOAT32) in.op = I_CAST_F32_F64;
+    else {
+      fail(OXR_EPLAN, "rocm.expr", "unexpected narrowing cast");
+      return false;
+    }
     in.type = (uint8_t)to;
     in.dst = (uint8_t)reg;
-    in.a = (uint8_t)reg;
     p->ins.push_back(in);
     return true;
   }
@@ -242,7 +299,12 @@ int compile_leaf(const oxr_expr *e, const std::vector<DevCol> &cols, Program *p,
     const int32_t ct = cols[c.column].type;
     if (ct == OXR_F32_VECTOR)
       return fail(OXR_EUNSUPPORTED, "rocm.filter", "comparison on a vector column");
+    const int slot = p->slot_of(c.column);
+    if (slot < 0)
+      return fail(OXR_EUNSUPPORTED, "rocm.expr",
+                  "an operator reads at most " + std::to_string(kMaxSlots) + " distinct columns");
     lf.fast = 1;
+    lf.slot = (uint8_t)slot;
     lf.col = (int16_t)c.column;
     lf.ctype = (uint8_t)coerce(ct, lit.type);
     lf.key = literal_key(lit, lf.ctype);
@@ -312,6 +374,13 @@ int compile_value(const oxr_expr *e, const std::vector<DevCol> &cols, Program *p
     v.col = (int16_t)e->nodes[0].column;
     v.type = cols[v.col].type;
     v.nullable = cols[v.col].validity != nullptr;
+    if (v.type != OXR_F32_VECTOR) {
+      const int slot = p->slot_of(v.col);
+      if (slot < 0)
+        return fail(OXR_EUNSUPPORTED, "rocm.expr",
+                    "an operator reads at most " + std::to_string(kMaxSlots) + " distinct columns");
+      v.slot = (int16_t)slot;
+    }
     *out = v;
     return OXR_OK;
   }

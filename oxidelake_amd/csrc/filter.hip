@@ -5,16 +5,18 @@
 // AND per conjunction, a block count, a device->host copy of the counts with
 // a host scan, a scatter of row indices and one gather per projected column.
 // Here one kernel per batch:
-//   1. each lane evaluates the whole predicate for V=4 consecutive rows in
-//      registers (expr.h), loading each column with 16-byte vector loads;
-//   2. a wave64 prefix sum of the per-lane counts plus a block scan give each
+//   1. each lane loads every column the query reads for its 2 x 4
+//      consecutive rows with 16-byte vector loads, all in flight at once;
+//   2. it evaluates the whole predicate in registers (expr.h);
+//   3. a wave64 prefix sum of the per-lane counts plus a block scan give each
 //      selected row its rank inside the tile;
-//   3. a single-pass decoupled look-back over the tiles (tile ids taken from
+//   4. a single-pass decoupled look-back over the tiles (tile ids taken from
 //      an atomic counter, so every tile a lane waits on is already running)
 //      gives the tile's output offset — no host round trip;
-//   4. each lane computes the projections of its selected rows and stores
-//      them at their compacted positions; output validity bits are assembled
-//      in LDS and written as whole words (boundary words with atomicOr).
+//   5. each lane computes the projections of its selected rows from the same
+//      registers and stores them at their compacted positions; output
+//      validity bits are assembled in LDS and written as whole words
+//      (boundary words with atomicOr).
 // Semantics: crates/oxidelake-device/src/cpu/kernels.rs:99-152.
 #include <hip/hip_runtime.h>
 
@@ -26,12 +28,25 @@
 namespace oxr {
 namespace {
 
-constexpr int V = 4;             // consecutive rows per lane
+// Launch shape: V consecutive rows per lane, G row groups per lane, so a
+// tile is kBlock * V * G rows. Shapes are instantiated below and picked per
+// call (the autotuner's choice, else the default).
 constexpr int kBlock = 256;      // threads per block (4 waves)
-constexpr int kGroups = 2;       // row groups per lane
-constexpr int kTile = kBlock * V * kGroups;  // rows per tile
 constexpr int kWaves = kBlock / 64;
-constexpr int kTileWords = kTile / 32 + 2;   // output validity words a tile can touch
+
+struct FilterShape {
+  int v, g;
+};
+
+// The instantiated shapes; knobs "filter.v"/"filter.g" pick one (default
+// one row per lane, two row groups: at most ~90 VGPRs for four columns).
+FilterShape filter_shape() {
+  const int v = (int)tune_get("filter.v", 1), g = (int)tune_get("filter.g", 2);
+  const FilterShape known[] = {{1, 1}, {1, 2}, {1, 4}, {2, 1}, {2, 2}, {4, 1}};
+  for (const FilterShape &k : known)
+    if (k.v == v && k.g == g) return k;
+  return {1, 2};
+}
 
 constexpr uint64_t kFlagA = 1ull << 62;  // tile aggregate available
 constexpr uint64_t kFlagP = 2ull << 62;  // inclusive prefix available
@@ -40,10 +55,8 @@ constexpr uint64_t kValMask = (1ull << 62) - 1;
 struct Out {
   void *values;
   uint32_t *valid;  // NULL when the projection cannot be NULL
-  int32_t type, dim;
-  int16_t col;      // >= 0: copy this input column
-  uint16_t pstart, plen;
-  int16_t pad;
+  ValueProg vp;     // slot, or program; vp.col for vector columns
+  int32_t dim;
 };
 
 struct Params {
@@ -53,11 +66,16 @@ struct Params {
   Ins ins[kMaxIns];
   Leaf leaves[kMaxLeaves];
   uint8_t bprog[kMaxBool];
+  DivMagic magic[kMaxMagic];
+  int8_t slot_cols[kMaxSlots];
+  int32_t nslots;
   Out outs[kMaxOuts];
   int32_t nb, nouts;
   uint64_t *status;   // ntiles words, zeroed
   uint32_t *counter;  // tile ticket, zeroed
-  uint64_t *result;   // [0] = selected rows, [1] = error flags
+  uint64_t *result;   // [0] = error flags
+  uint64_t *counts;   // selected rows per batch
+  uint64_t batch_rows, tiles_per_batch;
 };
 
 __device__ inline uint64_t atomic_load_u64(uint64_t *p) {
@@ -83,33 +101,35 @@ __device__ inline uint64_t wave_sum64(uint64_t x) {
 }
 
 // Decoupled look-back, run by one whole wave. Publishes `agg` for `tile`
-// and returns the exclusive prefix of the tiles before it.
-__device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t agg, int lane) {
-  if (tile == 0) {
-    if (lane == 0) atomic_store_u64(&status[0], kFlagP | agg);
+// and returns the exclusive prefix of the tiles of its batch before it
+// (`first` is the batch's first tile: every batch is compacted on its own).
+__device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg, int lane) {
+  if (tile == first) {
+    if (lane == 0) atomic_store_u64(&status[tile], kFlagP | agg);
     return 0;
   }
   if (lane == 0) atomic_store_u64(&status[tile], kFlagA | agg);
   uint64_t excl = 0;
   int64_t end = (int64_t)tile - 1;  // nearest predecessor in this window
+  const int64_t lo = (int64_t)first;
   while (true) {
     const int64_t idx = end - lane;
     uint64_t st = 0;
-    if (idx >= 0) {
+    if (idx >= lo) {
       // every predecessor took its ticket before this tile did, so it is
       // running and will publish: the wait ends
       do {
         st = atomic_load_u64(&status[idx]);
       } while ((st >> 62) == 0);
     }
-    const uint64_t pmask = __ballot(idx >= 0 && (st >> 62) == 2);
+    const uint64_t pmask = __ballot(idx >= lo && (st >> 62) == 2);
     if (pmask) {
-      const int first = __builtin_ctzll(pmask);
-      uint64_t v = (lane <= first && idx >= 0) ? (st & kValMask) : 0;
+      const int nearest = __builtin_ctzll(pmask);
+      uint64_t v = (lane <= nearest && idx >= lo) ? (st & kValMask) : 0;
       excl += wave_sum64(v);
       break;
     }
-    excl += wave_sum64(idx >= 0 ? (st & kValMask) : 0);
+    excl += wave_sum64(idx >= lo ? (st & kValMask) : 0);
     end -= 64;
   }
   if (lane == 0) atomic_store_u64(&status[tile], kFlagP | (excl + agg));
@@ -121,7 +141,10 @@ __device__ inline void store_value(void *base, int type, uint64_t pos, uint64_t 
   else ((uint64_t *)base)[pos] = v;
 }
 
+template <int NS, int V, int kGroups>
 __global__ __launch_bounds__(kBlock) void filter_project_kernel(const Params p) {
+  constexpr int kTile = kBlock * V * kGroups;
+  constexpr int kTileWords = kTile / 32 + 2;  // output validity words a tile can touch
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wtot[kGroups][kWaves];
   __shared__ uint32_t s_woff[kGroups][kWaves];
@@ -135,24 +158,43 @@ __global__ __launch_bounds__(kBlock) void filter_project_kernel(const Params p) 
   if (tid == 0) s_tile = atomicAdd(p.counter, 1u);
   for (int i = tid; i < kMaxOuts * kTileWords; i += kBlock) (&s_bits[0][0])[i] = 0;
   __syncthreads();
-  const uint64_t tile = s_tile;
-  const uint64_t tile_base = tile * (uint64_t)kTile;
+  const uint64_t tile = s_tile;  // ticket order: a batch's tiles in row order
+  const uint64_t batch = tile / p.tiles_per_batch;
+  const uint64_t lt = tile - batch * p.tiles_per_batch;
+  const uint64_t batch_row0 = batch * p.batch_rows;
+  const uint64_t row_end = min(p.n, batch_row0 + p.batch_rows);
+  const uint64_t tile_base = batch_row0 + lt * (uint64_t)kTile;
   bool div0 = false;
 
-  // 1. predicate
+  // 1. every column of every row group in flight at once
+  Slots<V, NS> S[kGroups];
+#pragma unroll
+  for (int j = 0; j < kGroups; j++) {
+    const uint64_t row0 = tile_base + ((uint64_t)(j * kBlock + tid)) * V;
+    load_slots<V, NS>(p.cols, p.slot_cols, p.nslots, row0, row_end, S[j]);
+  }
+  // 2. predicate
   uint32_t sel[kGroups];
   uint32_t excl[kGroups];
 #pragma unroll
   for (int j = 0; j < kGroups; j++) {
     const uint64_t row0 = tile_base + ((uint64_t)(j * kBlock + tid)) * V;
-    sel[j] = eval_predicate<V>(p.leaves, p.bprog, p.nb, p.ins, p.cols, row0, p.n, &div0);
-    const uint32_t cnt = __builtin_popcount(sel[j]);
-    const uint32_t incl = wave_incl_scan(cnt, lane);
-    excl[j] = incl - cnt;
-    if (lane == 63) s_wtot[j][wave] = incl;
+    sel[j] = eval_predicate<V, NS>(p.leaves, p.bprog, p.nb, p.ins, p.cols, S[j], p.magic, row0,
+                                   row_end, &div0);
+    if constexpr (V == 1) {
+      // one row per lane: ranks straight from the wave's ballot
+      const uint64_t b = __ballot(sel[j] != 0);
+      excl[j] = (uint32_t)__popcll(b & ((1ull << lane) - 1));
+      if (lane == 0) s_wtot[j][wave] = (uint32_t)__popcll(b);
+    } else {
+      const uint32_t cnt = __builtin_popcount(sel[j]);
+      const uint32_t incl = wave_incl_scan(cnt, lane);
+      excl[j] = incl - cnt;
+      if (lane == 63) s_wtot[j][wave] = incl;
+    }
   }
   __syncthreads();
-  // 2. block scan over (group, wave) in row order
+  // 3. block scan over (group, wave) in row order
   if (tid == 0) {
     uint32_t run = 0;
 #pragma unroll
@@ -165,29 +207,33 @@ __global__ __launch_bounds__(kBlock) void filter_project_kernel(const Params p) 
     s_total = run;
   }
   __syncthreads();
-  // 3. look-back
+  // 4. look-back
   if (wave == 0) {
-    const uint64_t pre = lookback(p.status, tile, s_total, lane);
+    const uint64_t pre = lookback(p.status, tile, tile - lt, s_total, lane);
     if (lane == 0) {
-      s_prefix = pre;
-      if (tile == p.ntiles - 1) p.result[0] = pre + s_total;
+      // a batch's output starts where its input does: batches need no
+      // offsets from one another
+      s_prefix = batch_row0 + pre;
+      if (lt == p.tiles_per_batch - 1) p.counts[batch] = pre + s_total;
     }
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
   const uint64_t word0 = prefix >> 5;
 
-  // 4. projections
+  // 5. projections
   for (int o = 0; o < p.nouts; o++) {
     const Out out = p.outs[o];
+    const int type = out.vp.type;
 #pragma unroll
     for (int j = 0; j < kGroups; j++) {
       const uint64_t row0 = tile_base + ((uint64_t)(j * kBlock + tid)) * V;
       uint64_t pos = prefix + s_woff[j][wave] + excl[j];
-      if (out.type == OXR_F32_VECTOR) {
+      if (type == OXR_F32_VECTOR) {
         // whole-wave copy of each selected row's dim floats
         const int dim = out.dim;
-        const float *src = (const float *)p.cols[out.col].values;
+        const DCol src_col = p.cols[out.vp.col];
+        const float *src = (const float *)src_col.values;
         float *dst = (float *)out.values;
         uint64_t lanes = __ballot(sel[j] != 0);
         while (lanes) {
@@ -204,45 +250,33 @@ __global__ __launch_bounds__(kBlock) void filter_project_kernel(const Params p) 
             lpos++;
           }
         }
-        if (out.valid) {
-          uint64_t x[V];
-          uint32_t vm;
-          load_rows<V>(p.cols[out.col], row0, p.n, x, vm);
-          (void)x;
+        if (out.valid && src_col.valid) {
           uint64_t at = pos;
           for (int q = 0; q < V; q++) {
             if (!((sel[j] >> q) & 1)) continue;
-            if ((vm >> q) & 1) atomicOr(&s_bits[o][(at >> 5) - word0], 1u << (at & 31));
+            const uint64_t r = row0 + q;
+            if ((src_col.valid[r >> 5] >> (r & 31)) & 1)
+              atomicOr(&s_bits[o][(at >> 5) - word0], 1u << (at & 31));
             at++;
           }
         }
         continue;
       }
-      if (!sel[j] && !out.valid) continue;
+      if (!sel[j]) continue;
       uint64_t v[V];
       uint32_t vm = 0;
-      if (out.col >= 0) {
-        load_rows<V>(p.cols[out.col], row0, p.n, v, vm);
-      } else {
-        uint64_t r[kRegs][V];
-        uint32_t m[kRegs] = {0, 0, 0, 0};
-        run_program<V>(p.ins, out.pstart, out.plen, p.cols, row0, p.n, r, m, &div0);
-#pragma unroll
-        for (int q = 0; q < V; q++) v[q] = r[0][q];
-        vm = m[0];
-      }
+      eval_value<V, NS>(out.vp, p.ins, S[j], p.magic, v, vm, &div0);
 #pragma unroll
       for (int q = 0; q < V; q++) {
         if (!((sel[j] >> q) & 1)) continue;
-        store_value(out.values, out.type, pos, v[q]);
-        if (out.valid && ((vm >> q) & 1))
-          atomicOr(&s_bits[o][(pos >> 5) - word0], 1u << (pos & 31));
+        store_value(out.values, type, pos, v[q]);
+        if (out.valid && ((vm >> q) & 1)) atomicOr(&s_bits[o][(pos >> 5) - word0], 1u << (pos & 31));
         pos++;
       }
     }
   }
-  if (div0) atomicOr((unsigned long long *)&p.result[1], 1ull);
-  // 5. validity words
+  if (div0) atomicOr((unsigned long long *)&p.result[0], 1ull);
+  // 6. validity words
   const uint32_t total = s_total;
   bool any_valid = false;
   for (int o = 0; o < p.nouts; o++) any_valid |= p.outs[o].valid != nullptr;
@@ -264,17 +298,40 @@ __global__ __launch_bounds__(kBlock) void filter_project_kernel(const Params p) 
   }
 }
 
+template <int NS>
+void launch_ns(const FilterShape &sh, dim3 grid, hipStream_t st, const Params &P) {
+  const dim3 block(kBlock);
+  if (sh.v == 1 && sh.g == 1) hipLaunchKernelGGL((filter_project_kernel<NS, 1, 1>), grid, block, 0, st, P);
+  else if (sh.v == 1 && sh.g == 4) hipLaunchKernelGGL((filter_project_kernel<NS, 1, 4>), grid, block, 0, st, P);
+  else if (sh.v == 2 && sh.g == 1) hipLaunchKernelGGL((filter_project_kernel<NS, 2, 1>), grid, block, 0, st, P);
+  else if (sh.v == 2 && sh.g == 2) hipLaunchKernelGGL((filter_project_kernel<NS, 2, 2>), grid, block, 0, st, P);
+  else if (sh.v == 4 && sh.g == 1) hipLaunchKernelGGL((filter_project_kernel<NS, 4, 1>), grid, block, 0, st, P);
+  else hipLaunchKernelGGL((filter_project_kernel<NS, 1, 2>), grid, block, 0, st, P);
+}
+
+void launch_filter(int ns, const FilterShape &sh, dim3 grid, hipStream_t st, const Params &P) {
+  switch (ns) {
+    case 2: launch_ns<2>(sh, grid, st, P); break;
+    case 4: launch_ns<4>(sh, grid, st, P); break;
+    default: launch_ns<8>(sh, grid, st, P); break;
+  }
+}
+
 }  // namespace
 }  // namespace oxr
 
 using namespace oxr;
 
-extern "C" int oxr_filter_project(oxr_stream *s, const oxr_batch *input, const oxr_expr *predicate,
-                                  const oxr_expr *projection, int32_t nproj,
-                                  const char *const *names, oxr_batch **out) {
-  clear_error();
-  if (!s || !input || !out || (nproj > 0 && !projection))
-    return fail(OXR_EPLAN, "rocm.filter", "null argument");
+namespace oxr {
+namespace {
+
+// Runs the filter over `input` cut into batches of `batch_rows` rows, all in
+// one launch. Output column buffers have the input's capacity; batch i's
+// output rows start at row i * batch_rows of them. counts[i] = its rows.
+int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const oxr_expr *predicate,
+               const oxr_expr *projection, int32_t nproj, const char *const *names,
+               std::vector<DevCol> *out_cols, std::vector<std::string> *out_names,
+               std::vector<uint64_t> *counts) {
   if (nproj < 0 || nproj > kMaxOuts)
     return fail(OXR_EUNSUPPORTED, "rocm.filter",
                 "at most " + std::to_string(kMaxOuts) + " projections per launch");
@@ -287,20 +344,25 @@ extern "C" int oxr_filter_project(oxr_stream *s, const oxr_batch *input, const o
   for (int i = 0; i < nproj; i++) OXR_TRY(compile_value(&projection[i], input->cols, &prog, &vps[i]));
 
   const uint64_t n = input->rows;
-  Params *pp = new Params();
-  std::unique_ptr<Params> guard(pp);
-  Params &P = *pp;
+  const FilterShape shape = filter_shape();
+  const uint64_t tile_rows = (uint64_t)kBlock * shape.v * shape.g;
+  const uint64_t nbatches = n ? (n + batch_rows - 1) / batch_rows : 0;
+  std::unique_ptr<Params> guard(new Params());
+  Params &P = *guard;
   std::memset(&P, 0, sizeof P);
   P.n = n;
-  P.ntiles = (n + kTile - 1) / kTile;
+  P.batch_rows = batch_rows;
+  P.tiles_per_batch = (batch_rows + tile_rows - 1) / tile_rows;
+  P.ntiles = nbatches * P.tiles_per_batch;
+  if (P.ntiles >= UINT32_MAX)
+    return fail(OXR_EUNSUPPORTED, "rocm.filter", "too many tiles for one launch");
   for (size_t i = 0; i < input->cols.size(); i++) {
     const DevCol &c = input->cols[i];
     P.cols[i].values = c.values;
     P.cols[i].valid = (const uint32_t *)c.validity;
     P.cols[i].type = c.type;
     P.cols[i].dim = c.dim;
-    const uintptr_t a = (uintptr_t)c.values;
-    if (n && (a % 16) != 0)
+    if (n && ((uintptr_t)c.values % 16) != 0)
       return fail(OXR_EUNSUPPORTED, "rocm.filter", "column values must be 16-byte aligned");
     if (c.validity && ((uintptr_t)c.validity % 4) != 0)
       return fail(OXR_EUNSUPPORTED, "rocm.filter", "validity bitmaps must be 4-byte aligned");
@@ -308,16 +370,18 @@ extern "C" int oxr_filter_project(oxr_stream *s, const oxr_batch *input, const o
   std::copy(prog.ins.begin(), prog.ins.end(), P.ins);
   std::copy(prog.leaves.begin(), prog.leaves.end(), P.leaves);
   std::copy(prog.bprog.begin(), prog.bprog.end(), P.bprog);
+  std::copy(prog.magic.begin(), prog.magic.end(), P.magic);
+  for (size_t i = 0; i < prog.slot_cols.size(); i++) P.slot_cols[i] = (int8_t)prog.slot_cols[i];
+  P.nslots = (int32_t)prog.slot_cols.size();
   P.nb = (int32_t)prog.bprog.size();
   P.nouts = nproj;
 
-  std::unique_ptr<oxr_batch> ob(new_batch(s, 0));
   auto release_outputs = [&]() {
-    for (auto &c : ob->cols) {
+    for (auto &c : *out_cols) {
       stream_release(s, c.values);
       stream_release(s, c.validity);
     }
-    ob->cols.clear();
+    out_cols->clear();
   };
   for (int i = 0; i < nproj; i++) {
     const ValueProg &v = vps[i];
@@ -328,22 +392,20 @@ extern "C" int oxr_filter_project(oxr_stream *s, const oxr_batch *input, const o
       release_outputs();
       return rc;
     }
-    ob->cols.push_back(c);
-    ob->names.push_back(names && names[i] ? names[i] : "p" + std::to_string(i));
+    out_cols->push_back(c);
+    out_names->push_back(names && names[i] ? names[i] : "p" + std::to_string(i));
     Out &o = P.outs[i];
     o.values = c.values;
     o.valid = (uint32_t *)c.validity;
-    o.type = v.type;
+    o.vp = v;
     o.dim = dim;
-    o.col = v.col;
-    o.pstart = v.start;
-    o.plen = v.len;
   }
 
-  // scratch: tile status words, ticket, result
+  // scratch: tile status words | error flags | ticket | per-batch counts
   const size_t status_bytes = (size_t)std::max<uint64_t>(P.ntiles, 1) * 8;
+  const size_t count_bytes = (size_t)std::max<uint64_t>(nbatches, 1) * 8;
   void *scratch = nullptr;
-  int rc = stream_alloc(s, status_bytes + 64, &scratch);
+  int rc = stream_alloc(s, status_bytes + 64 + count_bytes, &scratch);
   if (rc != OXR_OK) {
     release_outputs();
     return rc;
@@ -351,30 +413,99 @@ extern "C" int oxr_filter_project(oxr_stream *s, const oxr_batch *input, const o
   P.status = (uint64_t *)scratch;
   P.result = (uint64_t *)((char *)scratch + status_bytes);
   P.counter = (uint32_t *)((char *)scratch + status_bytes + 16);
-  rc = fill_zero(s, scratch, status_bytes + 64);
-  uint64_t host_result[2] = {0, 0};
+  P.counts = (uint64_t *)((char *)scratch + status_bytes + 64);
+  rc = fill_zero(s, scratch, status_bytes + 64 + count_bytes);
   if (rc == OXR_OK && P.ntiles) {
     ProfScope prof(s->s, "filter_project");
-    hipLaunchKernelGGL(filter_project_kernel, dim3((unsigned)P.ntiles), dim3(kBlock), 0, s->s, P);
+    launch_filter(slot_bucket(P.nslots), shape, dim3((unsigned)P.ntiles), s->s, P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail_hip(e, "filter_project_kernel launch");
   }
+  uint64_t err = 0;
+  counts->assign(nbatches, 0);
   if (rc == OXR_OK) {
-    hipError_t e = hipMemcpyAsync(host_result, P.result, 16, hipMemcpyDeviceToHost, s->s);
+    hipError_t e = hipMemcpyAsync(&err, P.result, 8, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess && nbatches)
+      e = hipMemcpyAsync(counts->data(), P.counts, nbatches * 8, hipMemcpyDeviceToHost, s->s);
     if (e == hipSuccess) e = hipStreamSynchronize(s->s);
     if (e != hipSuccess) rc = fail_hip(e, "filter_project result");
   }
   stream_release(s, scratch);
-  if (rc != OXR_OK) {
-    release_outputs();
-    return rc;
+  if (rc == OXR_OK && err) rc = fail(OXR_EEXECUTION, "rocm.filter", "Divide by zero error");
+  if (rc != OXR_OK) release_outputs();
+  return rc;
+}
+
+// Output buffers shared by the views of a multi-batch result.
+struct SharedCols {
+  oxr_device *dev = nullptr;
+  StreamCore *core = nullptr;
+  std::vector<DevCol> cols;
+  ~SharedCols() {
+    for (auto &c : cols) {
+      dev->arena->release(c.values, core->s);
+      dev->arena->release(c.validity, core->s);
+    }
+    release(core);
   }
-  if (host_result[1]) {
-    release_outputs();
-    return fail(OXR_EEXECUTION, "rocm.filter", "Divide by zero error");
-  }
-  ob->rows = host_result[0];
+};
+
+}  // namespace
+}  // namespace oxr
+
+extern "C" int oxr_filter_project(oxr_stream *s, const oxr_batch *input, const oxr_expr *predicate,
+                                  const oxr_expr *projection, int32_t nproj,
+                                  const char *const *names, oxr_batch **out) {
+  clear_error();
+  if (!s || !input || !out || (nproj > 0 && !projection))
+    return fail(OXR_EPLAN, "rocm.filter", "null argument");
+  std::unique_ptr<oxr_batch> ob(new_batch(s, 0));
+  std::vector<uint64_t> counts;
+  OXR_TRY(run_filter(s, input, std::max<uint64_t>(input->rows, 1), predicate, projection, nproj, names,
+                     &ob->cols, &ob->names, &counts));
+  ob->rows = counts.empty() ? 0 : counts[0];
   for (auto &c : ob->cols) c.len = ob->rows;
   *out = ob.release();
+  return OXR_OK;
+}
+
+extern "C" int oxr_filter_project_batches(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows,
+                                          const oxr_expr *predicate, const oxr_expr *projection,
+                                          int32_t nproj, const char *const *names, oxr_batch **outs,
+                                          uint64_t max_batches, uint64_t *nbatches) {
+  clear_error();
+  if (!s || !input || !outs || !nbatches || (nproj > 0 && !projection))
+    return fail(OXR_EPLAN, "rocm.filter", "null argument");
+  if (batch_rows == 0 || batch_rows % 32 != 0)
+    return fail(OXR_EPLAN, "rocm.filter", "batch_rows must be a non-zero multiple of 32");
+  const uint64_t nb = input->rows ? (input->rows + batch_rows - 1) / batch_rows : 0;
+  *nbatches = nb;
+  if (nb > max_batches)
+    return fail(OXR_EPLAN, "rocm.filter",
+                "the result has " + std::to_string(nb) + " batches but room for " +
+                    std::to_string(max_batches));
+  auto shared = std::make_shared<SharedCols>();
+  std::vector<std::string> out_names;
+  std::vector<uint64_t> counts;
+  OXR_TRY(run_filter(s, input, batch_rows, predicate, projection, nproj, names, &shared->cols,
+                     &out_names, &counts));
+  shared->dev = s->dev;
+  shared->core = retain(s->core);
+  for (uint64_t i = 0; i < nb; i++) {
+    oxr_batch *b = new oxr_batch();
+    b->dev = s->dev;
+    b->rows = counts[i];
+    b->holder = shared;
+    b->names = out_names;
+    for (const DevCol &c : shared->cols) {
+      DevCol v = c;
+      v.owned = false;
+      v.len = counts[i];
+      v.values = (char *)c.values + i * batch_rows * type_width(c.type, c.dim);
+      v.validity = c.validity ? c.validity + i * batch_rows / 8 : nullptr;
+      b->cols.push_back(v);
+    }
+    outs[i] = b;
+  }
   return OXR_OK;
 }

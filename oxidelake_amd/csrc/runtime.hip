@@ -234,6 +234,18 @@ int alloc_column(oxr_stream *s, int32_t type, int32_t dim, uint64_t rows, bool n
   return OXR_OK;
 }
 
+// -------------------------------------------------------------------- tuning
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, int64_t> g_tune;
+}  // namespace
+
+int64_t tune_get(const char *knob, int64_t dflt) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find(knob);
+  return it == g_tune.end() ? dflt : it->second;
+}
+
 // ----------------------------------------------------------------- profiling
 namespace {
 struct ProfRec {
@@ -841,6 +853,22 @@ int oxr_batch_column(const oxr_batch *b, int32_t i, oxr_column *out) {
 const char *oxr_batch_name(const oxr_batch *b, int32_t i) {
   if (!b || i < 0 || i >= (int32_t)b->names.size()) return "";
   return b->names[i].c_str();
+}
+
+int oxr_tune_set(const char *knob, int64_t value) {
+  if (!knob) return fail(OXR_EPLAN, "rocm.tune", "null knob");
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  if (value < 0) g_tune.erase(knob);
+  else g_tune[knob] = value;
+  return OXR_OK;
+}
+
+int oxr_tune_get(const char *knob, int64_t *value) {
+  if (!knob || !value) return fail(OXR_EPLAN, "rocm.tune", "null argument");
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find(knob);
+  *value = it == g_tune.end() ? -1 : it->second;
+  return OXR_OK;
 }
 
 void oxr_profile_enable(int on) { g_prof_on.store(on != 0); }

@@ -184,3 +184,54 @@ def test_batches_outlive_their_stream(backend, oracle):
     del out, dev
     gc.collect()
     s2.synchronize()
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (4, 1)])
+def test_every_launch_shape_is_bit_exact(backend, stream, oracle, shape):
+    # the autotuner may pick any instantiated shape: each must give the same rows
+    from oxidelake_amd import autotune
+
+    autotune.set_knob("filter.v", shape[0])
+    autotune.set_knob("filter.g", shape[1])
+    try:
+        rng = np.random.default_rng(77)
+        for n in (1, 1000, 9001):
+            batch = datagen.random_batch(rng, n)
+            for pred, proj in [(PREDICATES[10], PROJECTIONS[2]), (PREDICATES[5], PROJECTIONS[1]),
+                               (PREDICATES[12], PROJECTIONS[3])]:
+                got, want = run_both(backend, stream, oracle, batch, pred, proj)
+                assert_batches_equal(got, want, names=False)
+        dev = datagen.table(backend, stream, datagen.config2_columns(seed=5), 100_003)
+        host = backend.download(stream, dev)
+        pred, proj = datagen.config2_query(50)
+        got = backend.download(stream, backend.filter_project(stream, dev, pred, proj))
+        assert_batches_equal(got, oracle.filter_project(host, pred, proj), names=False)
+    finally:
+        autotune.set_knob("filter.v", -1)
+        autotune.set_knob("filter.g", -1)
+
+
+@pytest.mark.parametrize("batch_rows", [32, 4096, 65_536])
+def test_many_batches_in_one_launch(backend, stream, oracle, batch_rows):
+    # each batch of the input compacted on its own, as the reference filters
+    # batch by batch (exec/filter.rs:182-210)
+    n = 250_003
+    dev = datagen.table(backend, stream, datagen.config2_columns(seed=9), n)
+    host = backend.download(stream, dev)
+    pred, proj = datagen.config2_query(500)
+    outs = backend.filter_project_batches(stream, dev, batch_rows, pred, proj, ["a", "b2", "cd"])
+    assert len(outs) == -(-n // batch_rows)
+    for i, o in enumerate(outs):
+        got = backend.download(stream, o)
+        want = oracle.filter_project(host.slice(i * batch_rows, batch_rows), pred, proj, ["a", "b2", "cd"])
+        assert_batches_equal(got, want)
+    # views share buffers: freeing some leaves the others intact
+    keep = outs[-1]
+    del outs
+    import gc
+
+    gc.collect()
+    got = backend.download(stream, keep)
+    last = (n - 1) // batch_rows
+    assert_batches_equal(got, oracle.filter_project(host.slice(last * batch_rows, batch_rows), pred, proj,
+                                                    ["a", "b2", "cd"]))

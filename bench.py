@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--b-threshold", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="also time b thresholds 500 and 1000")
+    ap.add_argument("--filter-rows", type=int, default=0,
+                    help="rows per lane of the generated kernel (default: autotuned)")
     return ap.parse_args()
 
 
@@ -128,7 +130,11 @@ def main():
     # launch shape picked by the project autotuner on this device and table
     from oxidelake_amd import autotune
 
-    tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
+    if args.filter_rows:
+        autotune.set_knob("filter.rows", args.filter_rows)
+        tuned = {"knobs": {"filter.rows": args.filter_rows}, "ms": None}
+    else:
+        tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
 
     lib.oxr_profile_enable(0)
     for _ in range(args.warmup):
@@ -146,7 +152,10 @@ def main():
     lib.oxr_profile_enable(0)
     kms, launches = C.c_double(), C.c_uint64()
     lib.oxr_profile_get(b"filter_project", C.byref(kms), C.byref(launches))
-    kernel_ms = kms.value / max(1, launches.value)
+    if launches.value != args.steps:
+        raise RuntimeError(f"expected {args.steps} launches of the generated filter kernel, "
+                           f"saw {launches.value}")
+    kernel_ms = kms.value / launches.value
     lib.oxr_profile_reset()
 
     ms_per_step = elapsed * 1000 / args.steps

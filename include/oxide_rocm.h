@@ -250,6 +250,18 @@ int oxr_filter_project_batches(oxr_stream *s, const oxr_batch *input, uint64_t b
                                int32_t nproj, const char *const *names, oxr_batch **outs,
                                uint64_t max_batches, uint64_t *nbatches);
 
+/* The kernel source oxr_filter_project generates for this query shape (an
+ * EXPLAIN of the device plan; no counterpart in the reference, whose CUDA
+ * kernels are fixed per operator, cuda/ops.rs:143-373). Only the type, dim
+ * and validity (NULL or not) of each of the `ncols` columns are read; no
+ * device is needed. With `compile` set it is also compiled for gfx950.
+ * Writes the NUL-terminated source to buf (truncated to cap-1 bytes) and its
+ * full length to *len. OXR_EUNSUPPORTED: the query runs on the interpreter
+ * kernel instead of a generated one. */
+int oxr_filter_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_expr *predicate,
+                             const oxr_expr *projection, int32_t nproj, int32_t compile,
+                             char *buf, size_t cap, size_t *len);
+
 /* -------------------------------------------------------------- hash join */
 
 /* `GpuBackend::prepare_join_build` (backend.rs:129-142, args.rs:21-31): the

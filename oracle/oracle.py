@@ -196,14 +196,18 @@ def eval_expr(batch: pa.RecordBatch, e: ExprT) -> pa.Array:
 
 def filter_project(batch: pa.RecordBatch, predicate: ExprT | None, projection: Sequence,
                    names: Optional[Sequence[str]] = None) -> pa.RecordBatch:
-    """kernels.rs:144-152: mask, then `filter_record_batch` of the projection."""
+    """kernels.rs:144-152: mask, then `filter_record_batch` of the projection.
+    Arithmetic projections (beyond the reference's column-index projections)
+    are evaluated over the kept rows only, as DataFusion's ProjectionExec
+    above a FilterExec: a zero divisor in a dropped row is no error."""
     exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
     mask = predicate_mask(batch, predicate)
     idx = np.nonzero(mask)[0]
+    kept = batch.take(pa.array(idx, type=pa.int64()))
     cols, fields = [], []
     for i, e in enumerate(exprs):
-        arr = eval_expr(batch, e)
-        cols.append(arr.take(pa.array(idx, type=pa.int64())))
+        arr = eval_expr(kept, e)
+        cols.append(arr)
         nm = names[i] if names else (batch.schema.field(e.index).name if isinstance(e, plan.Col) else f"p{i}")
         fields.append(pa.field(nm, arr.type))
     return pa.RecordBatch.from_arrays(cols, schema=pa.schema(fields))

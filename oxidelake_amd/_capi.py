@@ -130,6 +130,11 @@ SIGNATURES: dict[str, tuple] = {
         C.c_int,
         [P, P, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(C.c_char_p), PP],
     ),
+    "oxr_filter_kernel_source": (
+        C.c_int,
+        [C.POINTER(Column), I32, C.POINTER(Expr), C.POINTER(Expr), I32, I32, C.c_char_p,
+         C.c_size_t, C.POINTER(C.c_size_t)],
+    ),
     "oxr_filter_project_batches": (
         C.c_int,
         [P, P, U64, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(C.c_char_p), PP, U64,

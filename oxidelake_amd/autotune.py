@@ -18,7 +18,8 @@ from ._capi import lib
 
 CACHE = Path(os.environ.get("OXR_TUNE_CACHE", Path.home() / ".cache" / "oxidelake_amd" / "tune.json"))
 
-FILTER_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (4, 1)]
+FILTER_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (4, 1)]  # interpreter kernel
+FILTER_ROWS = [4, 8, 16]  # generated kernels: rows per lane
 AGG_GRIDS = [2, 4, 8, 16]
 
 
@@ -48,6 +49,13 @@ def kernel_ms(kernel: str, run: Callable[[], object], reps: int) -> float:
     return ms.value / max(1, n.value)
 
 
+def launches(kernel: str) -> int:
+    """Launches of `kernel` recorded by the profiler since its last reset."""
+    ms, n = C.c_double(), C.c_uint64()
+    lib.oxr_profile_get(kernel.encode(), C.byref(ms), C.byref(n))
+    return int(n.value)
+
+
 def sweep(kernel: str, knobs: Sequence[str], candidates: Iterable[Sequence[int]],
           run: Callable[[], object], reps: int = 3) -> dict:
     """Times every candidate setting of `knobs`, applies the fastest."""
@@ -63,7 +71,13 @@ def sweep(kernel: str, knobs: Sequence[str], candidates: Iterable[Sequence[int]]
 
 
 def tune_filter(run: Callable[[], object], reps: int = 3) -> dict:
-    return sweep("filter_project", ["filter.v", "filter.g"], FILTER_SHAPES, run, reps)
+    """Tiles of the generated filter kernel (the path queries take)."""
+    return sweep("filter_project", ["filter.rows"], [(r,) for r in FILTER_ROWS], run, reps)
+
+
+def tune_filter_interpreter(run: Callable[[], object], reps: int = 3) -> dict:
+    """Launch shape of the interpreter kernel (filter.jit = 0)."""
+    return sweep("filter_project_vm", ["filter.v", "filter.g"], FILTER_SHAPES, run, reps)
 
 
 def tune_aggregate(run: Callable[[], object], reps: int = 3) -> dict:

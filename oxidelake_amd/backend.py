@@ -188,6 +188,30 @@ class MemoryInfo:
         return max(0, self.total_bytes - self.free_bytes)
 
 
+def filter_kernel_source(columns: Sequence[tuple], predicate: ExprT | None, projection: Sequence,
+                         compile: bool = False) -> str:
+    """The kernel source the filter generates for this query shape over
+    columns given as (type, nullable) or (type, dim, nullable); with
+    `compile`, it is also compiled for gfx950. No device is needed.
+    Raises Unsupported when the interpreter kernel would run the query."""
+    cols = (Column * max(1, len(columns)))()
+    for i, c in enumerate(columns):
+        typ, dim, nullable = (c[0], 0, c[1]) if len(c) == 2 else c
+        cols[i].type = typ
+        cols[i].dim = dim
+        cols[i].validity = 1 if nullable else None  # only NULL or not is read
+    exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
+    pred = Encoded(predicate)
+    parr, penc = plan.encode_many(exprs)
+    n = C.c_size_t()
+    check(lib.oxr_filter_kernel_source(cols, len(columns), pred.ref(), parr, len(exprs), 0, None, 0,
+                                       C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib.oxr_filter_kernel_source(cols, len(columns), pred.ref(), parr, len(exprs),
+                                       1 if compile else 0, buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
 def _cstrs(names: Optional[Sequence[str]], n: int):
     if names is None:
         return None, None

@@ -28,7 +28,12 @@ SOURCES = [
     "vector.hip",
     "datagen.hip",
     "pending.hip",
+    "jit.hip",
 ]
+
+# Device headers the generated kernels include; embedded into the library
+# as strings (jit_sources.inc) for hipRTC.
+JIT_HEADERS = ["divlit.h", "scalar.h", "compact.h", "filter_jit.h"]
 
 # Sources whose float arithmetic must not be contracted into FMA: the
 # filter and aggregate evaluate SQL expressions, where a*b+c is two roundings.
@@ -63,9 +68,32 @@ def common_flags() -> list[str]:
     ]
 
 
+def write_jit_sources() -> Path:
+    """Writes _build/jit_sources.inc: the JIT headers as raw string literals."""
+    out = OBJ_DIR / "jit_sources.inc"
+    delim = "OXRJIT"
+    parts = [
+        "// generated by oxidelake_amd/build.py from " + ", ".join(JIT_HEADERS),
+        f"static const int kJitHeaderCount = {len(JIT_HEADERS)};",
+        "static const char *const kJitHeaderNames[] = {"
+        + ", ".join(f'"{h}"' for h in JIT_HEADERS) + "};",
+        "static const char *const kJitHeaderText[] = {",
+    ]
+    for h in JIT_HEADERS:
+        text = (CSRC / h).read_text()
+        if f"){delim}\"" in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter")
+        parts.append(f'R"{delim}({text}){delim}",')
+    parts.append("};")
+    body = "\n".join(parts) + "\n"
+    if not out.exists() or out.read_text() != body:
+        out.write_text(body)
+    return out
+
+
 def _compile(src: str) -> Path:
     obj = OBJ_DIR / (src + ".o")
-    flags = common_flags()
+    flags = common_flags() + [f"-I{OBJ_DIR}"]
     if src in NO_CONTRACT:
         flags.append("-ffp-contract=off")
     if src in FP_ATOMICS:
@@ -83,12 +111,14 @@ def _compile(src: str) -> Path:
 def build(verbose: bool = False) -> Path:
     """Compiles every HIP source and links liboxide_rocm.so; returns its path."""
     OBJ_DIR.mkdir(exist_ok=True)
+    write_jit_sources()
     srcs = [s for s in SOURCES if (CSRC / s).exists()]
     workers = min(len(srcs), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           "-L/opt/rocm/lib", "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

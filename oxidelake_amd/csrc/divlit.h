@@ -7,7 +7,9 @@
 // i64::MIN / -1 overflows and is reported by the caller, not here.
 #pragma once
 
+#if !defined(__HIPCC_RTC__)
 #include <cstdint>
+#endif
 
 #if !defined(__HIPCC__)
 // plain C++ (the host unit test of this header)

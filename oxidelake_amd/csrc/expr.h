@@ -30,6 +30,7 @@
 
 #include "common.h"
 #include "divlit.h"
+#include "scalar.h"
 
 namespace oxr {
 
@@ -110,6 +111,9 @@ struct Program {
 int compile_predicate(const oxr_expr *e, const std::vector<DevCol> &cols, Program *p);
 // Compiles a value expression; appends instructions to p->ins.
 int compile_value(const oxr_expr *e, const std::vector<DevCol> &cols, Program *p, ValueProg *out);
+// Where the right operand of the binary node ending at end-1 begins (nodes
+// [begin, end) form one subtree).
+int split_operands(const oxr_expr *e, int begin, int end);
 // Whether a predicate is in this backend's grammar (no failure recorded).
 bool predicate_supported(const oxr_expr *e, std::string *why);
 
@@ -122,44 +126,6 @@ inline int32_t coerce(int32_t a, int32_t b) {
 }
 
 // ---------------------------------------------------------------- device
-__host__ __device__ inline int64_t order_key64(double x) {
-  // -0.0 read as +0.0, then IEEE totalOrder as a signed integer
-  // (kernels.rs:62-69, 825-830).
-  if (x == 0.0) x = 0.0;
-  int64_t b;
-  __builtin_memcpy(&b, &x, 8);
-  return b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1);
-}
-
-__host__ __device__ inline int32_t order_key32(float x) {
-  if (x == 0.0f) x = 0.0f;
-  int32_t b;
-  __builtin_memcpy(&b, &x, 4);
-  return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
-}
-
-__device__ inline double as_f64(uint64_t v) {
-  double d;
-  __builtin_memcpy(&d, &v, 8);
-  return d;
-}
-__device__ inline uint64_t f64_bits(double d) {
-  uint64_t v;
-  __builtin_memcpy(&v, &d, 8);
-  return v;
-}
-__device__ inline float as_f32(uint64_t v) {
-  uint32_t u = (uint32_t)v;
-  float f;
-  __builtin_memcpy(&f, &u, 4);
-  return f;
-}
-__device__ inline uint64_t f32_bits(float f) {
-  uint32_t u;
-  __builtin_memcpy(&u, &f, 4);
-  return (uint64_t)u;
-}
-
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
 typedef uint64_t v2u __attribute__((ext_vector_type(2)));
 
@@ -251,63 +217,6 @@ __device__ inline uint64_t cast_value(uint64_t v, int from, int to) {
       return v;
   }
   return v;
-}
-
-// Integer division of widened values in type t (32- or 64-bit), truncating;
-// a zero divisor, or the overflowing MIN / -1, in a valid row sets *div0.
-__device__ inline uint64_t idiv(bool mod, bool i32, uint64_t a, uint64_t b, bool valid, bool *div0) {
-  const int64_t x = (int64_t)a, y = (int64_t)b;
-  if (y == 0) {
-    if (valid) *div0 = true;
-    return 0;
-  }
-  const int64_t lo = i32 ? (int64_t)INT32_MIN : INT64_MIN;
-  if (x == lo && y == -1) {
-    if (!mod && valid) *div0 = true;
-    return 0;
-  }
-  uint64_t z = (uint64_t)(mod ? x % y : x / y);
-  return i32 ? (uint64_t)(int64_t)(int32_t)(uint32_t)z : z;
-}
-
-// x / lit or x % lit for a non-zero integer literal divisor.
-__device__ inline uint64_t idiv_lit(bool mod, bool i32, uint64_t a, const DivMagic &g, bool valid,
-                                    bool *div0) {
-  const int64_t x = (int64_t)a;
-  const int64_t lo = i32 ? (int64_t)INT32_MIN : INT64_MIN;
-  if (g.neg && g.d == 1 && x == lo) {
-    if (!mod && valid) *div0 = true;
-    return 0;
-  }
-  int64_t q, r;
-  sdivmod_magic(x, g, &q, &r);
-  uint64_t z = (uint64_t)(mod ? r : q);
-  return i32 ? (uint64_t)(int64_t)(int32_t)(uint32_t)z : z;
-}
-
-__device__ inline uint64_t wrap32(uint64_t z) { return (uint64_t)(int64_t)(int32_t)(uint32_t)z; }
-
-// NaN results as x86-64 SSE produces them (the platform the reference's CPU
-// path and its conformance suite run on): a NaN operand propagates, quieted,
-// the first operand's first; an invalid operation on non-NaN operands
-// (0 * inf, inf - inf, 0 / 0) gives the negative default NaN. The GPU's own
-// default NaN is positive, and the sign matters to totalOrder comparisons.
-__device__ inline uint64_t x86_nan64(uint64_t z, uint64_t a, uint64_t b) {
-  const uint64_t exp = 0x7FF0000000000000ull, man = 0x000FFFFFFFFFFFFFull;
-  const auto isnan = [&](uint64_t v) { return (v & exp) == exp && (v & man) != 0; };
-  if (!isnan(z)) return z;
-  if (isnan(a)) return a | 0x0008000000000000ull;
-  if (isnan(b)) return b | 0x0008000000000000ull;
-  return 0xFFF8000000000000ull;
-}
-
-__device__ inline uint64_t x86_nan32(uint64_t z, uint64_t a, uint64_t b) {
-  const uint32_t exp = 0x7F800000u, man = 0x007FFFFFu;
-  const auto isnan = [&](uint32_t v) { return (v & exp) == exp && (v & man) != 0; };
-  if (!isnan((uint32_t)z)) return z;
-  if (isnan((uint32_t)a)) return (uint64_t)((uint32_t)a | 0x00400000u);
-  if (isnan((uint32_t)b)) return (uint64_t)((uint32_t)b | 0x00400000u);
-  return 0xFFC00000ull;
 }
 
 template <int V>

@@ -353,6 +353,12 @@ int compile_bool(const oxr_expr *e, const std::vector<DevCol> &cols, Program *p,
 
 }  // namespace
 
+int split_operands(const oxr_expr *e, int begin, int end) {
+  int mid;
+  split(e, begin, end, &mid);
+  return mid;
+}
+
 int compile_predicate(const oxr_expr *e, const std::vector<DevCol> &cols, Program *p) {
   if (!e || e->count == 0) return OXR_OK;
   if (!e->nodes) return fail(OXR_EPLAN, "rocm.filter", "predicate without nodes");

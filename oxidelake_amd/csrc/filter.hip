@@ -23,7 +23,9 @@
 #include <memory>
 
 #include "common.h"
+#include "compact.h"
 #include "expr.h"
+#include "jit.h"
 
 namespace oxr {
 namespace {
@@ -47,10 +49,6 @@ FilterShape filter_shape() {
     if (k.v == v && k.g == g) return k;
   return {1, 2};
 }
-
-constexpr uint64_t kFlagA = 1ull << 62;  // tile aggregate available
-constexpr uint64_t kFlagP = 2ull << 62;  // inclusive prefix available
-constexpr uint64_t kValMask = (1ull << 62) - 1;
 
 struct Out {
   void *values;
@@ -77,64 +75,6 @@ struct Params {
   uint64_t *counts;   // selected rows per batch
   uint64_t batch_rows, tiles_per_batch;
 };
-
-__device__ inline uint64_t atomic_load_u64(uint64_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void atomic_store_u64(uint64_t *p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline uint32_t wave_incl_scan(uint32_t x, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
-}
-
-__device__ inline uint64_t wave_sum64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
-}
-
-// Decoupled look-back, run by one whole wave. Publishes `agg` for `tile`
-// and returns the exclusive prefix of the tiles of its batch before it
-// (`first` is the batch's first tile: every batch is compacted on its own).
-__device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg, int lane) {
-  if (tile == first) {
-    if (lane == 0) atomic_store_u64(&status[tile], kFlagP | agg);
-    return 0;
-  }
-  if (lane == 0) atomic_store_u64(&status[tile], kFlagA | agg);
-  uint64_t excl = 0;
-  int64_t end = (int64_t)tile - 1;  // nearest predecessor in this window
-  const int64_t lo = (int64_t)first;
-  while (true) {
-    const int64_t idx = end - lane;
-    uint64_t st = 0;
-    if (idx >= lo) {
-      // every predecessor took its ticket before this tile did, so it is
-      // running and will publish: the wait ends
-      do {
-        st = atomic_load_u64(&status[idx]);
-      } while ((st >> 62) == 0);
-    }
-    const uint64_t pmask = __ballot(idx >= lo && (st >> 62) == 2);
-    if (pmask) {
-      const int nearest = __builtin_ctzll(pmask);
-      uint64_t v = (lane <= nearest && idx >= lo) ? (st & kValMask) : 0;
-      excl += wave_sum64(v);
-      break;
-    }
-    excl += wave_sum64(idx >= lo ? (st & kValMask) : 0);
-    end -= 64;
-  }
-  if (lane == 0) atomic_store_u64(&status[tile], kFlagP | (excl + agg));
-  return excl;
-}
 
 __device__ inline void store_value(void *base, int type, uint64_t pos, uint64_t v) {
   if (type == OXR_INT32 || type == OXR_FLOAT32) ((uint32_t *)base)[pos] = (uint32_t)v;
@@ -221,7 +161,12 @@ __global__ __launch_bounds__(kBlock) void filter_project_kernel(const Params p) 
   const uint64_t prefix = s_prefix;
   const uint64_t word0 = prefix >> 5;
 
-  // 5. projections
+  // 5. projections, of the kept rows only: a lane's other rows must not
+  // raise errors (a zero divisor in a dropped row is no error)
+#pragma unroll
+  for (int j = 0; j < kGroups; j++)
+#pragma unroll
+    for (int k = 0; k < NS; k++) S[j].m[k] &= sel[j];
   for (int o = 0; o < p.nouts; o++) {
     const Out out = p.outs[o];
     const int type = out.vp.type;
@@ -343,9 +288,19 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   std::vector<ValueProg> vps(nproj);
   for (int i = 0; i < nproj; i++) OXR_TRY(compile_value(&projection[i], input->cols, &prog, &vps[i]));
 
+  // the query's own generated kernel, else the interpreter
+  JitFilter jf;
+  bool use_jit = false;
+  if (tune_get("filter.jit", 1) != 0 && input->rows > 0) {
+    const int jrc = jit_filter(s->dev, input->cols, predicate, projection, nproj, vps, &jf);
+    if (jrc == OXR_OK) use_jit = true;
+    else if (jrc != OXR_EUNSUPPORTED) return jrc;
+  }
+
   const uint64_t n = input->rows;
   const FilterShape shape = filter_shape();
-  const uint64_t tile_rows = (uint64_t)kBlock * shape.v * shape.g;
+  const uint64_t tile_rows =
+      use_jit ? (uint64_t)kJitBlock * jf.rows_per_lane : (uint64_t)kBlock * shape.v * shape.g;
   const uint64_t nbatches = n ? (n + batch_rows - 1) / batch_rows : 0;
   std::unique_ptr<Params> guard(new Params());
   Params &P = *guard;
@@ -415,8 +370,31 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   P.counter = (uint32_t *)((char *)scratch + status_bytes + 16);
   P.counts = (uint64_t *)((char *)scratch + status_bytes + 64);
   rc = fill_zero(s, scratch, status_bytes + 64 + count_bytes);
-  if (rc == OXR_OK && P.ntiles) {
+  if (rc == OXR_OK && P.ntiles && use_jit) {
+    JitArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.n = n;
+    A.batch_rows = batch_rows;
+    A.tiles_per_batch = P.tiles_per_batch;
+    A.status = P.status;
+    A.counter = P.counter;
+    A.result = P.result;
+    A.counts = P.counts;
+    for (size_t i = 0; i < input->cols.size(); i++) {
+      A.col[i] = input->cols[i].values;
+      A.valid[i] = (const uint32_t *)input->cols[i].validity;
+    }
+    for (int i = 0; i < nproj; i++) {
+      A.out[i] = (*out_cols)[i].values;
+      A.outv[i] = (uint32_t *)(*out_cols)[i].validity;
+      A.vslot[i] = jf.vslot[i];
+      A.dim[i] = (*out_cols)[i].dim;
+    }
+    A.nouts = nproj;
     ProfScope prof(s->s, "filter_project");
+    rc = jit_launch(jf.fn, (unsigned)P.ntiles, s->s, A);
+  } else if (rc == OXR_OK && P.ntiles) {
+    ProfScope prof(s->s, "filter_project_vm");
     launch_filter(slot_bucket(P.nslots), shape, dim3((unsigned)P.ntiles), s->s, P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail_hip(e, "filter_project_kernel launch");

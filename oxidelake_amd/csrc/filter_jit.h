@@ -1,0 +1,190 @@
+// Skeleton of the filter + projection kernels that jit.hip generates per
+// query shape and compiles at run time with hipRTC. The generated part is a
+// struct Q holding the query's straight-line code:
+//   struct In, ins(p, base)        the input columns of the tile's rows
+//                                  (pointers advanced to its first row, so
+//                                  rows are 32-bit offsets j from uniform
+//                                  bases: no 64-bit address per lane)
+//   struct Row, load(i, j, in, x)  the values (and validity words) one row
+//                                  reads (in: the row lies inside its batch)
+//   pred(x, j, in, err)            the predicate: true keeps the row
+//   struct Out, outs(p, base, at)  the outputs, advanced to the tile's first
+//                                  output row `at`; read from the arguments
+//                                  once per tile (stores into them could
+//                                  otherwise force a reload of every pointer
+//                                  at every row step)
+//   project(o, x, j, q, bitpos, bits, stride, err)
+//                                  stores a kept row's projections at output
+//                                  row q of the tile and sets its output
+//                                  validity bits in LDS
+//   vectors(o, sel, j, q, lane)    whole-wave copies of vector projections
+//   kValidOuts, kVecOuts           outputs with a validity bitmap / vectors
+// Semantics are those of the interpreter kernel in filter.hip
+// (crates/oxidelake-device/src/cpu/kernels.rs:99-152); this skeleton does
+// the compaction: a tile is 256 threads x S rows, row s*256 + tid of the tile
+// on thread tid (every load instruction of a wave reads one contiguous run),
+// wave ballots rank the kept rows, and a decoupled look-back (compact.h)
+// places the tile inside its batch. Compiles under hipcc and hipRTC.
+#pragma once
+
+#include "compact.h"
+
+namespace oxr {
+
+constexpr int kJitBlock = 256;
+constexpr int kJitCols = 16;
+constexpr int kJitOuts = 8;
+
+// Kernel arguments of every generated filter kernel (identical layout on
+// the host, which fills it, and in the generated code).
+struct JitArgs {
+  uint64_t n;                  // input rows
+  uint64_t batch_rows;         // rows per batch (each compacted on its own)
+  uint64_t tiles_per_batch;
+  uint64_t *status;            // one look-back word per tile, zeroed
+  uint32_t *counter;           // tile ticket, zeroed
+  uint64_t *result;            // [0]: error flags (1 = division by zero)
+  uint64_t *counts;            // kept rows per batch
+  const void *col[kJitCols];   // input column values
+  const uint32_t *valid[kJitCols];
+  void *out[kJitOuts];         // output column values
+  uint32_t *outv[kJitOuts];    // output validity words (NULL: none)
+  int32_t vslot[kJitOuts];     // LDS bit row of output k, -1: none
+  int32_t dim[kJitOuts];       // floats per row of a vector output
+  int32_t nouts, pad;
+};
+
+// Validity bit of tile row j (tiles start on 32-row boundaries).
+__device__ inline bool row_bit(uint32_t w, uint32_t j) { return (w >> (j & 31)) & 1u; }
+
+__device__ inline void set_out_bit(uint32_t *bits, uint32_t bitpos) {
+  atomicOr(&bits[bitpos >> 5], 1u << (bitpos & 31));
+}
+
+// Copies the dim floats of every kept row of the wave, one row at a time
+// with the whole wave; src and dst are advanced to the tile's first input
+// and output rows.
+__device__ inline void copy_vector_rows(const float *src, float *dst, int dim, bool sel, uint32_t j,
+                                        uint32_t q, int lane) {
+  uint64_t lanes = __ballot(sel);
+  while (lanes) {
+    const int l = __builtin_ctzll(lanes);
+    lanes &= lanes - 1;
+    const uint32_t lj = __shfl(j, l, 64);
+    const uint32_t lq = __shfl(q, l, 64);
+    const float *s = src + (uint64_t)lj * (uint64_t)dim;
+    float *d = dst + (uint64_t)lq * (uint64_t)dim;
+    for (int k = lane; k < dim; k += 64) d[k] = s[k];
+  }
+}
+
+template <class Q, int S>
+__device__ __forceinline__ void filter_tile(const JitArgs &p) {
+  constexpr int kTile = kJitBlock * S;
+  constexpr int kWords = kTile / 32 + 2;  // output validity words a tile can touch
+  constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_total;
+  __shared__ uint32_t s_cnt[S * 4];
+  __shared__ uint32_t s_off[S * 4];
+  __shared__ uint64_t s_mask[S * 4];  // each (step, wave)'s ballot of kept rows
+  __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_bits[kNV * kWords];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(p.counter, 1u);
+  if (Q::kValidOuts > 0)
+    for (int i = tid; i < kNV * kWords; i += kJitBlock) s_bits[i] = 0;
+  __syncthreads();
+  const uint64_t tile = s_tile;  // ticket order: a batch's tiles in row order
+  const uint64_t batch = tile / p.tiles_per_batch;
+  const uint64_t lt = tile - batch * p.tiles_per_batch;
+  const uint64_t b0 = batch * p.batch_rows;
+  const uint64_t bend = b0 + p.batch_rows;
+  const uint64_t end = p.n < bend ? p.n : bend;
+  const uint64_t base = b0 + lt * (uint64_t)kTile;
+  // rows j < lim of the tile exist (a short last batch may leave none)
+  const uint32_t lim = end > base ? (uint32_t)(end - base < (uint64_t)kTile ? end - base : kTile) : 0u;
+
+  // 1. every column of every row of the tile in flight at once
+  const typename Q::In in_cols = Q::ins(p, base);
+  typename Q::Row x[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+    Q::load(in_cols, j, j < lim, x[s]);
+  }
+  // 2. predicate and ranks. The kept flags live as bits of one register and
+  // the ballots in LDS until the stores: as per-lane booleans and 64-bit
+  // ballots they would take two SGPR pairs per row step.
+  bool err = false;
+  uint32_t selbits = 0;
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+    const bool in = j < lim;
+    const bool k = Q::pred(x[s], j, in, err) && in;
+    selbits |= (uint32_t)k << s;
+    const uint64_t mk = __ballot(k);
+    if (lane == 0) {
+      s_cnt[s * 4 + wave] = (uint32_t)__popcll(mk);
+      s_mask[s * 4 + wave] = mk;
+    }
+  }
+  __syncthreads();
+  // 3. tile scan over (step, wave) in row order, then the look-back
+  if (wave == 0) {
+    const uint32_t v = lane < S * 4 ? s_cnt[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(v, lane);
+    if (lane < S * 4) s_off[lane] = incl - v;
+    const uint32_t tot = __shfl(incl, 63, 64);
+    const uint64_t pre = lookback(p.status, tile, tile - lt, tot, lane);
+    if (lane == 0) {
+      s_total = tot;
+      // a batch's output starts where its input does
+      s_prefix = b0 + pre;
+      if (lt == p.tiles_per_batch - 1) p.counts[batch] = pre + tot;
+    }
+  }
+  __syncthreads();
+  const uint64_t prefix = s_prefix;
+  const uint64_t w0 = prefix >> 5;
+  const uint32_t bit0 = (uint32_t)(prefix & 31);
+  const uint64_t below = (1ull << lane) - 1;
+  // 4. projections of the kept rows, stored at their compacted positions
+  const typename Q::Out o = Q::outs(p, base, prefix);
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+    const bool k = (selbits >> s) & 1u;
+    const uint32_t q = s_off[s * 4 + wave] + (uint32_t)__popcll(s_mask[s * 4 + wave] & below);
+    if (k) Q::project(o, x[s], j, q, bit0 + q, s_bits, kWords, err);
+    if (Q::kVecOuts > 0) Q::vectors(o, k, j, q, lane);
+  }
+  if (err) atomicOr((unsigned long long *)&p.result[0], 1ull);
+  // 5. output validity words: interior words stored, the two boundary
+  // words (shared with the neighbouring tiles) OR-ed in
+  if (Q::kValidOuts == 0) return;
+  const uint32_t tot = s_total;
+  if (tot == 0) return;
+  __syncthreads();
+  const uint64_t last = (prefix + tot - 1) >> 5;
+  const int nw = (int)(last - w0 + 1);
+  for (int k = 0; k < p.nouts; k++) {
+    const int v = p.vslot[k];
+    if (v < 0) continue;
+    uint32_t *g = p.outv[k];
+    for (int w = tid; w < nw; w += kJitBlock) {
+      const uint32_t bits = s_bits[v * kWords + w];
+      if (w == 0 || w == nw - 1) {
+        if (bits) atomicOr(&g[w0 + w], bits);
+      } else {
+        g[w0 + w] = bits;
+      }
+    }
+  }
+}
+
+}  // namespace oxr

@@ -1,0 +1,40 @@
+// Kernels generated per query shape and compiled at run time with hipRTC
+// (host side). The reference ships its CUDA kernels as source and compiles
+// them with NVRTC on first use (crates/oxidelake-device/src/cuda/mod.rs:
+// 381-431, docs/decisions/ADR-0003-cudarc-dynamic-loading-nvrtc.md); here
+// the generated source is specialised to the query's expression structure,
+// column types and nullability, so the predicate and projections run as
+// straight-line code with its literals as constants (held as kernel
+// arguments they would keep ~30 more registers live across the tile's
+// look-back: 2.5 ms instead of 1.45 ms for 200M rows). A kernel is compiled
+// once per process for each distinct source.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "expr.h"
+#include "filter_jit.h"
+
+namespace oxr {
+
+struct JitFilter {
+  hipFunction_t fn = nullptr;
+  int rows_per_lane = 0;       // S: a tile is 256 * S rows
+  std::vector<int32_t> vslot;  // JitArgs::vslot per projection
+  std::string source;
+};
+
+// Finds or builds the filter kernel of this query shape for `dev`. `vps`
+// are the interpreter's compiled projections (their types and nullability
+// fix the output columns). OXR_EUNSUPPORTED (no error recorded) means the
+// query is outside what generated kernels cover and the interpreter runs it.
+int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr *pred,
+               const oxr_expr *proj, int32_t nproj, const std::vector<ValueProg> &vps,
+               JitFilter *out);
+
+// Launches a generated kernel: `grid` blocks of 256 threads.
+int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args);
+
+}  // namespace oxr

@@ -1,0 +1,454 @@
+// Generation, compilation (hipRTC) and caching of per-query filter kernels.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "jit.h"
+// kJitHeaderCount, kJitHeaderNames, kJitHeaderText: the device headers the
+// generated source includes (divlit.h, scalar.h, compact.h, filter_jit.h),
+// embedded at build time.
+#include "jit_sources.inc"
+
+namespace oxr {
+namespace {
+
+bool is_int_type(int32_t t) { return t == OXR_INT32 || t == OXR_INT64; }
+
+const char *cname(int32_t t) {
+  switch (t) {
+    case OXR_INT32: return "int32_t";
+    case OXR_INT64: return "int64_t";
+    case OXR_FLOAT32: return "float";
+    default: return "double";
+  }
+}
+
+const char *tprefix(int32_t t) {
+  switch (t) {
+    case OXR_INT32: return "i32";
+    case OXR_INT64: return "i64";
+    case OXR_FLOAT32: return "f32";
+    default: return "f64";
+  }
+}
+
+const char *cmp_op(int op) {
+  switch (op) {
+    case OXR_EQ: return "==";
+    case OXR_LT: return "<";
+    case OXR_LTEQ: return "<=";
+    case OXR_GT: return ">";
+    case OXR_GTEQ: return ">=";
+    default: return "!=";
+  }
+}
+
+// A generated value: C++ expression (a temporary or a row field), its SQL
+// type, and a C++ boolean expression that is true where it is not NULL.
+struct Val {
+  std::string v;
+  int32_t t;
+  std::string ok;
+  bool nullable;
+};
+
+std::string conj(const std::string &a, const std::string &b) {
+  if (a == "true") return b;
+  if (b == "true") return a;
+  if (a == b) return a;
+  return "(" + a + " && " + b + ")";
+}
+
+// Emits straight-line C++ for value and boolean subtrees of one expression.
+// The typing and coercion rules are those of the interpreter's compiler
+// (expr.hip): DataFusion's numeric coercion and widening casts only.
+struct Gen {
+  const std::vector<DevCol> &cols;
+  JitFilter *jf;
+  std::vector<char> &val_used;    // column values the row struct holds
+  std::vector<char> &valid_used;  // column validity words it holds
+  const oxr_expr *e = nullptr;
+  std::string code;
+  int tmp = 0;
+  bool unsupported = false;
+
+  std::string fresh() { return "t" + std::to_string(tmp++); }
+  void stmt(const std::string &s) { code += "    " + s + "\n"; }
+
+  int32_t type_of(int begin, int end) {
+    const oxr_expr_node &nd = e->nodes[end - 1];
+    if (nd.kind == OXR_X_COLUMN) return cols[nd.column].type;
+    if (nd.kind == OXR_X_LITERAL) return nd.type;
+    const int mid = split_operands(e, begin, end);
+    return coerce(type_of(begin, mid), type_of(mid, end - 1));
+  }
+
+  std::string name_ok(const std::string &ok) {
+    if (ok == "true" || ok == "in" || ok.find("&&") == std::string::npos) return ok;
+    const std::string n = fresh();
+    stmt("const bool " + n + " = " + ok + ";");
+    return n;
+  }
+
+  Val cast(const Val &a, int32_t to) {
+    if (a.t == to) return a;
+    Val r = a;
+    r.t = to;
+    if (!(to == OXR_INT64 && a.t == OXR_INT32) && !(to == OXR_FLOAT32 && is_int_type(a.t)) &&
+        !(to == OXR_FLOAT64 && (is_int_type(a.t) || a.t == OXR_FLOAT32))) {
+      unsupported = true;  // never narrows; the interpreter reports it
+      return r;
+    }
+    const std::string n = fresh();
+    stmt(std::string("const ") + cname(to) + " " + n + " = (" + cname(to) + ")" + a.v + ";");
+    r.v = n;
+    return r;
+  }
+
+  // Literals are compile-time constants of the generated code (their bit
+  // patterns, so every value, NaN payloads included, is exact): the
+  // compiler folds them into the comparisons and arithmetic.
+  Val literal(const oxr_expr_node &nd) {
+    char buf[64];
+    if (nd.type == OXR_INT32) {
+      std::snprintf(buf, sizeof buf, "((int32_t)0x%08xu)", (uint32_t)(int32_t)nd.lit_i);
+    } else if (nd.type == OXR_INT64) {
+      std::snprintf(buf, sizeof buf, "((int64_t)0x%016llxull)", (unsigned long long)nd.lit_i);
+    } else if (nd.type == OXR_FLOAT32) {
+      const float f = (float)nd.lit_f;
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      std::snprintf(buf, sizeof buf, "as_f32(0x%08xull)", u);
+    } else {
+      uint64_t u;
+      std::memcpy(&u, &nd.lit_f, 8);
+      std::snprintf(buf, sizeof buf, "as_f64(0x%016llxull)", (unsigned long long)u);
+    }
+    return {buf, nd.type, "true", false};
+  }
+
+  Val value(int begin, int end) {
+    const oxr_expr_node &nd = e->nodes[end - 1];
+    if (nd.kind == OXR_X_COLUMN) {
+      const int c = nd.column;
+      const DevCol &col = cols[c];
+      val_used[c] = 1;
+      const bool nb = col.validity != nullptr;
+      if (nb) valid_used[c] = 1;
+      const std::string cs = std::to_string(c);
+      return {"x.c" + cs, col.type, nb ? "(in && row_bit(x.w" + cs + ", j))" : "in", nb};
+    }
+    if (nd.kind == OXR_X_LITERAL) return literal(nd);
+    const int mid = split_operands(e, begin, end);
+    const oxr_expr_node &rhs = e->nodes[mid];
+    const bool divmod = nd.kind == OXR_X_DIV || nd.kind == OXR_X_MOD;
+    const char *mod = nd.kind == OXR_X_MOD ? "true" : "false";
+    const int64_t dlit = rhs.type == OXR_INT32 ? (int64_t)(int32_t)rhs.lit_i : rhs.lit_i;
+    if (divmod && end - 1 - mid == 1 && rhs.kind == OXR_X_LITERAL && is_int_type(rhs.type) &&
+        dlit != 0 && dlit != -1 && is_int_type(type_of(begin, mid))) {
+      // integer division by a literal other than 0 and -1 can neither fail
+      // nor overflow: C's truncating / and % are the SQL result, and the
+      // compiler turns the constant divisor into a multiply-high
+      const Val a = value(begin, mid);
+      const int32_t t = coerce(a.t, rhs.type);
+      const Val ca = cast(a, t);
+      const Val d = cast(literal(rhs), t);
+      const std::string n = fresh();
+      stmt(std::string("const ") + cname(t) + " " + n + " = (" + cname(t) + ")(" + ca.v +
+           (nd.kind == OXR_X_DIV ? " / " : " % ") + d.v + ");");
+      return {n, t, ca.ok, ca.nullable};
+    }
+    const Val a = value(begin, mid);
+    const Val b = value(mid, end - 1);
+    const int32_t t = coerce(a.t, b.t);
+    const Val ca = cast(a, t), cb = cast(b, t);
+    const std::string ok = name_ok(conj(ca.ok, cb.ok));
+    std::string expr;
+    const std::string args = ca.v + ", " + cb.v;
+    switch (nd.kind) {
+      case OXR_X_ADD: expr = std::string(tprefix(t)) + "_add(" + args + ")"; break;
+      case OXR_X_SUB: expr = std::string(tprefix(t)) + "_sub(" + args + ")"; break;
+      case OXR_X_MUL: expr = std::string(tprefix(t)) + "_mul(" + args + ")"; break;
+      case OXR_X_DIV:
+      case OXR_X_MOD:
+        if (is_int_type(t))
+          expr = std::string(tprefix(t)) + "_divmod(" + mod + ", " + args + ", " + ok + ", &err)";
+        else
+          expr = std::string(tprefix(t)) + (nd.kind == OXR_X_DIV ? "_div(" : "_mod(") + args + ")";
+        break;
+      default:
+        unsupported = true;
+        return {"0", t, "true", false};
+    }
+    const std::string n = fresh();
+    stmt(std::string("const ") + cname(t) + " " + n + " = " + expr + ";");
+    return {n, t, ok, ca.nullable || cb.nullable};
+  }
+
+  // Kleene AND/OR over comparisons with NULL read as false keep exactly the
+  // rows the three-valued result keeps (there is no NOT in the grammar).
+  std::string boolean(int begin, int end) {
+    const oxr_expr_node &nd = e->nodes[end - 1];
+    const int mid = split_operands(e, begin, end);
+    const std::string n = fresh();
+    if (nd.kind == OXR_X_CMP) {
+      const Val a = value(begin, mid);
+      const Val b = value(mid, end - 1);
+      const int32_t t = coerce(a.t, b.t);
+      const Val ca = cast(a, t), cb = cast(b, t);
+      std::string cmp;
+      if (t == OXR_FLOAT64)
+        cmp = "order_key64(" + ca.v + ") " + cmp_op(nd.op) + " order_key64(" + cb.v + ")";
+      else if (t == OXR_FLOAT32)
+        cmp = "order_key32(" + ca.v + ") " + cmp_op(nd.op) + " order_key32(" + cb.v + ")";
+      else
+        cmp = ca.v + " " + cmp_op(nd.op) + " " + cb.v;
+      const std::string ok = conj(ca.ok, cb.ok);
+      stmt("const bool " + n + " = " + (ok == "true" ? cmp : ok + " && (" + cmp + ")") + ";");
+      return n;
+    }
+    const std::string l = boolean(begin, mid);
+    const std::string r = boolean(mid, end - 1);
+    stmt("const bool " + n + " = " + l + (nd.kind == OXR_X_AND ? " & " : " | ") + r + ";");
+    return n;
+  }
+};
+
+// S (rows per lane) for a row of `vgprs` 32-bit registers: the largest power
+// of two up to 16 keeping the tile's registers near 128, unless the
+// "filter.rows" knob says otherwise.
+int pick_rows(int vgprs) {
+  const int64_t knob = tune_get("filter.rows", 0);
+  if (knob == 1 || knob == 2 || knob == 4 || knob == 8 || knob == 16) return (int)knob;
+  int s = 16;
+  while (s > 2 && s * vgprs > 128) s /= 2;
+  return s;
+}
+
+int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_expr *proj,
+             int32_t nproj, const std::vector<ValueProg> &vps, JitFilter *jf) {
+  if (cols.size() > (size_t)kJitCols || nproj > kJitOuts) return OXR_EUNSUPPORTED;
+  std::vector<char> val_used(cols.size(), 0), valid_used(cols.size(), 0);
+  Gen g{cols, jf, val_used, valid_used};
+  // predicate
+  std::string pred_code = "    return true;\n";
+  if (pred && pred->count > 0) {
+    g.e = pred;
+    g.code.clear();
+    g.tmp = 0;
+    const std::string res = g.boolean(0, pred->count);
+    pred_code = g.code + "    return " + res + ";\n";
+  }
+  // projections
+  std::string proj_code = "    const bool in = true;\n    (void)in;\n";
+  std::string vec_code, out_fields, out_init;
+  int nvalid = 0, nvec = 0;
+  jf->vslot.assign(nproj, -1);
+  for (int k = 0; k < nproj; k++) {
+    const oxr_expr &pe = proj[k];
+    const std::string ks = std::to_string(k);
+    if (vps[k].nullable) jf->vslot[k] = nvalid++;
+    const std::string setbit =
+        "set_out_bit(bits + " + std::to_string(jf->vslot[k]) + " * stride, bitpos);";
+    if (pe.count == 1 && pe.nodes[0].kind == OXR_X_COLUMN && cols[pe.nodes[0].column].type == OXR_F32_VECTOR) {
+      const int c = pe.nodes[0].column;
+      const std::string cs = std::to_string(c);
+      out_fields += "    const float *v" + ks + ";\n    float *o" + ks + ";\n    int d" + ks + ";\n";
+      out_init += "    o.d" + ks + " = p.dim[" + ks + "];\n    o.v" + ks + " = (const float *)p.col[" + cs +
+                  "] + base * (uint64_t)o.d" + ks + ";\n    o.o" + ks + " = (float *)p.out[" + ks +
+                  "] + at * (uint64_t)o.d" + ks + ";\n";
+      vec_code += "    copy_vector_rows(o.v" + ks + ", o.o" + ks + ", o.d" + ks + ", sel, j, q, lane);\n";
+      nvec++;
+      if (vps[k].nullable) {
+        valid_used[c] = 1;
+        proj_code += "    if (row_bit(x.w" + cs + ", j)) " + setbit + "\n";
+      }
+      continue;
+    }
+    g.e = &pe;
+    g.code.clear();
+    const Val v = g.value(0, pe.count);
+    if (v.t != vps[k].type) return OXR_EUNSUPPORTED;
+    proj_code += "    {\n" + g.code;
+    out_fields += std::string("    ") + cname(v.t) + " *o" + ks + ";\n";
+    out_init += std::string("    o.o") + ks + " = (" + cname(v.t) + " *)p.out[" + ks + "] + at;\n";
+    proj_code += "    o.o" + ks + "[q] = " + v.v + ";\n";
+    if (vps[k].nullable) proj_code += "    if (" + v.ok + ") " + setbit + "\n";
+    proj_code += "    }\n";
+  }
+  if (g.unsupported) return OXR_EUNSUPPORTED;
+  // the input struct, the row struct and its loads
+  std::string in_fields, in_init, row = "  struct Row {\n", load;
+  int vgprs = 0;
+  for (size_t c = 0; c < cols.size(); c++) {
+    const std::string cs = std::to_string(c);
+    if (val_used[c]) {
+      const int32_t t = cols[c].type;
+      in_fields += std::string("    const ") + cname(t) + " *c" + cs + ";\n";
+      in_init += std::string("    i.c") + cs + " = (const " + cname(t) + " *)p.col[" + cs + "] + base;\n";
+      row += std::string("    ") + cname(t) + " c" + cs + ";\n";
+      load += std::string("    x.c") + cs + " = in ? i.c" + cs + "[j] : (" + cname(t) + ")0;\n";
+      vgprs += (t == OXR_INT64 || t == OXR_FLOAT64) ? 2 : 1;
+    }
+    if (valid_used[c]) {
+      in_fields += "    const uint32_t *w" + cs + ";\n";
+      in_init += "    i.w" + cs + " = p.valid[" + cs + "] + (base >> 5);\n";
+      row += "    uint32_t w" + cs + ";\n";
+      load += "    x.w" + cs + " = in ? i.w" + cs + "[j >> 5] : 0u;\n";
+      vgprs += 1;
+    }
+  }
+  row += "  };\n";
+  jf->rows_per_lane = pick_rows(std::max(vgprs, 1));
+  std::string src;
+  src += "// generated filter + projection kernel\n";
+  src += "#include \"filter_jit.h\"\nusing namespace oxr;\nnamespace {\nstruct Q {\n";
+  src += "  static constexpr int kValidOuts = " + std::to_string(nvalid) + ";\n";
+  src += "  static constexpr int kVecOuts = " + std::to_string(nvec) + ";\n";
+  src += "  struct In {\n" + in_fields + "  };\n";
+  src += "  __device__ static inline In ins(const JitArgs &p, uint64_t base) {\n    In i;\n" + in_init +
+         "    return i;\n  }\n";
+  src += "  struct Out {\n" + out_fields + "  };\n";
+  src += "  __device__ static inline Out outs(const JitArgs &p, uint64_t base, uint64_t at) {\n    Out o;\n" +
+         out_init + "    return o;\n  }\n";
+  src += row;
+  src += "  __device__ static inline void load(const In &i, uint32_t j, bool in, Row &x) {\n" + load +
+         "  }\n";
+  src += "  __device__ static inline bool pred(const Row &x, uint32_t j, bool in, bool &err) {\n" + pred_code +
+         "  }\n";
+  src += "  __device__ static inline void project(const Out &o, const Row &x, uint32_t j, uint32_t q, "
+         "uint32_t bitpos, uint32_t *bits, int stride, bool &err) {\n" +
+         proj_code + "  }\n";
+  src += "  __device__ static inline void vectors(const Out &o, bool sel, uint32_t j, uint32_t q, "
+         "int lane) {\n" +
+         vec_code + "  }\n";
+  src += "};\n}  // namespace\n";
+  src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_filter_jit(const JitArgs p) {\n";
+  src += "  filter_tile<Q, " + std::to_string(jf->rows_per_lane) + ">(p);\n}\n";
+  jf->source = src;
+  return OXR_OK;
+}
+
+struct Entry {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+std::mutex g_jit_mu;
+std::unordered_map<std::string, Entry> g_jit_cache;
+
+// Compiles generated source for `arch` into a code object.
+int rtc_compile(const std::string &arch, const std::string &src, std::vector<char> *code) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "oxr_filter_jit.hip", kJitHeaderCount, kJitHeaderText,
+                          kJitHeaderNames) != HIPRTC_SUCCESS)
+    return fail(OXR_EDEVICE, "rocm.jit", "hiprtcCreateProgram failed");
+  const std::string arch_opt = "--offload-arch=" + arch;
+  const char *opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    return fail(OXR_EEXECUTION, "rocm.jit",
+                std::string("generated kernel did not compile (") + hiprtcGetErrorString(rc) + "):\n" + log +
+                    "\nsource:\n" + src);
+  }
+  size_t size = 0;
+  hiprtcGetCodeSize(prog, &size);
+  code->assign(size, 0);
+  hiprtcGetCode(prog, code->data());
+  hiprtcDestroyProgram(&prog);
+  return OXR_OK;
+}
+
+// The code object for `src` on `dev`, compiled once per process and device.
+int compile(oxr_device *dev, const std::string &src, hipFunction_t *fn) {
+  const std::string arch = dev->arch.substr(0, dev->arch.find(':'));
+  const std::string key = arch + "/" + std::to_string(dev->ordinal) + "\n" + src;
+  std::lock_guard<std::mutex> lock(g_jit_mu);
+  auto it = g_jit_cache.find(key);
+  if (it != g_jit_cache.end()) {
+    *fn = it->second.fn;
+    return OXR_OK;
+  }
+  std::vector<char> code;
+  OXR_TRY(rtc_compile(arch, src, &code));
+  int prev = 0;
+  OXR_HIP(hipGetDevice(&prev));
+  if (prev != dev->ordinal) OXR_HIP(hipSetDevice(dev->ordinal));
+  Entry ent;
+  hipError_t e = hipModuleLoadData(&ent.mod, code.data());
+  if (e == hipSuccess) e = hipModuleGetFunction(&ent.fn, ent.mod, "oxr_filter_jit");
+  if (prev != dev->ordinal) hipSetDevice(prev);
+  if (e != hipSuccess) return fail_hip(e, "loading a generated kernel");
+  g_jit_cache.emplace(key, ent);
+  *fn = ent.fn;
+  return OXR_OK;
+}
+
+}  // namespace
+
+int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr *pred,
+               const oxr_expr *proj, int32_t nproj, const std::vector<ValueProg> &vps,
+               JitFilter *out) {
+  JitFilter jf;
+  const int rc = generate(cols, pred, proj, nproj, vps, &jf);
+  if (rc != OXR_OK) return rc;
+  OXR_TRY(compile(dev, jf.source, &jf.fn));
+  *out = std::move(jf);
+  return OXR_OK;
+}
+
+int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args) {
+  JitArgs a = args;
+  size_t size = sizeof a;
+  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                 HIP_LAUNCH_PARAM_END};
+  OXR_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, kJitBlock, 1, 1, 0, s, nullptr, cfg));
+  return OXR_OK;
+}
+
+}  // namespace oxr
+
+using namespace oxr;
+
+extern "C" int oxr_filter_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_expr *predicate,
+                                        const oxr_expr *projection, int32_t nproj, int32_t compile,
+                                        char *buf, size_t cap, size_t *len) {
+  clear_error();
+  if ((ncols > 0 && !cols) || ncols < 0 || (nproj > 0 && !projection) || nproj < 0)
+    return fail(OXR_EPLAN, "rocm.jit", "null argument");
+  std::vector<DevCol> dc(ncols);
+  for (int i = 0; i < ncols; i++) {
+    dc[i].type = cols[i].type;
+    dc[i].dim = cols[i].dim;
+    dc[i].len = cols[i].len;
+    dc[i].validity = cols[i].validity;  // only NULL or not matters
+  }
+  Program prog;
+  OXR_TRY(compile_predicate(predicate, dc, &prog));
+  std::vector<ValueProg> vps(nproj);
+  for (int i = 0; i < nproj; i++) OXR_TRY(compile_value(&projection[i], dc, &prog, &vps[i]));
+  JitFilter jf;
+  const int rc = generate(dc, predicate, projection, nproj, vps, &jf);
+  if (rc != OXR_OK)
+    return fail(OXR_EUNSUPPORTED, "rocm.jit", "outside what generated kernels cover (the interpreter runs it)");
+  if (compile) {
+    std::vector<char> code;
+    OXR_TRY(rtc_compile("gfx950", jf.source, &code));
+  }
+  if (len) *len = jf.source.size();
+  if (buf && cap) {
+    const size_t n = std::min(cap - 1, jf.source.size());
+    std::memcpy(buf, jf.source.data(), n);
+    buf[n] = 0;
+  }
+  return OXR_OK;
+}

@@ -2,19 +2,37 @@
 
 Bit-exact: the selected rows, their order, every projected value (bit
 pattern) and every NULL must equal the oracle's restatement of
-crates/oxidelake-device/src/cpu/kernels.rs:99-152.
+crates/oxidelake-device/src/cpu/kernels.rs:99-152. Both device paths are
+checked: the kernel generated per query shape ("generated") and the
+expression interpreter kernel ("interpreter").
 """
 import numpy as np
 import pyarrow as pa
 import pytest
 
-from oxidelake_amd import datagen, plan
+from oxidelake_amd import autotune, datagen, plan
 from oxidelake_amd.backend import ExecutionError, Unsupported
 from oxidelake_amd.plan import Comparison as Cmp
 from oxidelake_amd.plan import Lit, Predicate, col
 from tests.util import assert_batches_equal
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["generated", "interpreter"])
+def path(request):
+    autotune.set_knob("filter.jit", 1 if request.param == "generated" else 0)
+    yield request.param
+    autotune.set_knob("filter.jit", -1)
+
+
+@pytest.fixture
+def small_tiles():
+    # 4 rows per lane: the random grid compiles one kernel per query shape,
+    # and a 16-row kernel takes twice as long to compile
+    autotune.set_knob("filter.rows", 4)
+    yield
+    autotune.set_knob("filter.rows", -1)
 
 
 def run_both(backend, stream, oracle, batch, pred, proj, names=None):
@@ -33,7 +51,7 @@ def ref_batch():
     })
 
 
-def test_reference_known_answer(backend, stream, oracle):
+def test_reference_known_answer(backend, stream, oracle, path):
     # kernels.rs:976-988: output k = [2, 5]
     pred = Predicate.and_(Predicate.compare(0, Cmp.GtEq, Lit.int64(2)),
                           Predicate.compare(1, Cmp.Lt, Lit.float64(4.0)))
@@ -42,7 +60,7 @@ def test_reference_known_answer(backend, stream, oracle):
     assert_batches_equal(got, want)
 
 
-def test_float_order_as_datafusion(backend, stream, oracle):
+def test_float_order_as_datafusion(backend, stream, oracle, path):
     # kernels.rs:990-1021
     vals = [float("nan"), -float("nan"), float("inf"), -float("inf"), 0.0, -0.0, 4.0]
     b = pa.record_batch({"v": pa.array(vals, pa.float64())})
@@ -93,7 +111,7 @@ PROJECTIONS = [
 
 @pytest.mark.parametrize("n", [0, 1, 7, 64, 1000, 4097, 50_000])
 @pytest.mark.parametrize("seed", [1, 42])
-def test_random_parity(backend, stream, oracle, n, seed):
+def test_random_parity(backend, stream, oracle, n, seed, path, small_tiles):
     rng = np.random.default_rng(seed * 1000 + n)
     batch = datagen.random_batch(rng, n)
     for pi, pred in enumerate(PREDICATES):
@@ -104,7 +122,7 @@ def test_random_parity(backend, stream, oracle, n, seed):
             assert_batches_equal(got, want, names=False)
 
 
-def test_no_predicate_keeps_every_row(backend, stream, oracle):
+def test_no_predicate_keeps_every_row(backend, stream, oracle, path):
     rng = np.random.default_rng(3)
     batch = datagen.random_batch(rng, 3000)
     got, want = run_both(backend, stream, oracle, batch, None, [0, 1, 2, 3, 4])
@@ -112,7 +130,7 @@ def test_no_predicate_keeps_every_row(backend, stream, oracle):
     assert_batches_equal(got, want)
 
 
-def test_sliced_input(backend, stream, oracle):
+def test_sliced_input(backend, stream, oracle, path):
     rng = np.random.default_rng(5)
     batch = datagen.random_batch(rng, 999).slice(13, 700)
     pred = Predicate.compare(2, Cmp.Gt, Lit.float64(0.0))
@@ -120,7 +138,7 @@ def test_sliced_input(backend, stream, oracle):
     assert_batches_equal(got, want)
 
 
-def test_divide_by_zero_is_an_error(backend, stream):
+def test_divide_by_zero_is_an_error(backend, stream, path):
     b = pa.record_batch({"x": pa.array([4, 5, 6], pa.int64()), "y": pa.array([1, 0, 2], pa.int64())})
     dev = backend.upload(stream, b)
     with pytest.raises(ExecutionError):
@@ -155,7 +173,7 @@ def test_vector_null_element_is_declined(backend, stream):
 
 
 @pytest.mark.parametrize("threshold", [50, 500])
-def test_config2_query_parity(backend, stream, oracle, threshold):
+def test_config2_query_parity(backend, stream, oracle, path, threshold):
     # SELECT a, b*2+1, c*d FROM t WHERE b < T AND c > 0.5 OR a % 7 = 3
     n = 300_001
     dev = datagen.table(backend, stream, datagen.config2_columns(seed=7), n)
@@ -193,6 +211,7 @@ def test_every_launch_shape_is_bit_exact(backend, stream, oracle, shape):
 
     autotune.set_knob("filter.v", shape[0])
     autotune.set_knob("filter.g", shape[1])
+    autotune.set_knob("filter.jit", 0)
     try:
         rng = np.random.default_rng(77)
         for n in (1, 1000, 9001):
@@ -209,10 +228,71 @@ def test_every_launch_shape_is_bit_exact(backend, stream, oracle, shape):
     finally:
         autotune.set_knob("filter.v", -1)
         autotune.set_knob("filter.g", -1)
+        autotune.set_knob("filter.jit", -1)
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4, 8, 16])
+def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows):
+    # generated kernels with S rows per lane (tiles of 256 * S rows)
+    autotune.set_knob("filter.rows", rows)
+    try:
+        rng = np.random.default_rng(78)
+        for n in (1, 1000, 9001):
+            batch = datagen.random_batch(rng, n)
+            for pred, proj in [(PREDICATES[10], PROJECTIONS[2]), (PREDICATES[12], PROJECTIONS[3])]:
+                got, want = run_both(backend, stream, oracle, batch, pred, proj)
+                assert_batches_equal(got, want, names=False)
+        dev = datagen.table(backend, stream, datagen.config2_columns(seed=5), 100_003)
+        host = backend.download(stream, dev)
+        pred, proj = datagen.config2_query(50)
+        outs = backend.filter_project_batches(stream, dev, 8192, pred, proj)
+        for i, o in enumerate(outs):
+            want = oracle.filter_project(host.slice(i * 8192, 8192), pred, proj)
+            assert_batches_equal(backend.download(stream, o), want, names=False)
+    finally:
+        autotune.set_knob("filter.rows", -1)
+
+
+def test_generated_kernel_is_the_one_that_runs(backend, stream):
+    # the per-query kernel, not the interpreter, runs a query in its grammar
+    from oxidelake_amd._capi import lib
+
+    dev = datagen.table(backend, stream, datagen.config2_columns(seed=3), 10_000)
+    pred, proj = datagen.config2_query(50)
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        backend.filter_project(stream, dev, pred, proj)
+        assert autotune.launches("filter_project") == 1
+        assert autotune.launches("filter_project_vm") == 0
+        autotune.set_knob("filter.jit", 0)
+        backend.filter_project(stream, dev, pred, proj)
+        assert autotune.launches("filter_project_vm") == 1
+    finally:
+        autotune.set_knob("filter.jit", -1)
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
+
+
+def test_zero_divisor_in_a_dropped_row_is_no_error(backend, stream, oracle, path):
+    # projections are evaluated over the kept rows only
+    x = np.arange(64, dtype=np.int64)
+    y = np.where(x % 4 == 1, 0, 3).astype(np.int64)
+    b = pa.record_batch({"x": pa.array(x), "y": pa.array(y)})
+    pred = Predicate.compare(1, Cmp.NotEq, Lit.int64(0))
+    for shape in [(1, 1), (4, 1), (2, 2)]:
+        autotune.set_knob("filter.v", shape[0])
+        autotune.set_knob("filter.g", shape[1])
+        try:
+            got, want = run_both(backend, stream, oracle, b, pred, [plan.div(col(0), col(1))])
+        finally:
+            autotune.set_knob("filter.v", -1)
+            autotune.set_knob("filter.g", -1)
+        assert_batches_equal(got, want, names=False)
 
 
 @pytest.mark.parametrize("batch_rows", [32, 4096, 65_536])
-def test_many_batches_in_one_launch(backend, stream, oracle, batch_rows):
+def test_many_batches_in_one_launch(backend, stream, oracle, path, batch_rows):
     # each batch of the input compacted on its own, as the reference filters
     # batch by batch (exec/filter.rs:182-210)
     n = 250_003

@@ -1,0 +1,91 @@
+"""Generated filter kernels, on the CPU: hipRTC compiles the kernel of every
+query shape the GPU parity tests run (no device is needed to compile), and
+the generated source has the structure the interpreter's compiler implies.
+"""
+import pytest
+
+from oxidelake_amd import autotune, backend, datagen, plan
+from oxidelake_amd.backend import PlanError, Unsupported
+from oxidelake_amd.plan import Comparison as Cmp
+from oxidelake_amd.plan import Lit, Predicate, col
+from tests.test_filter_gpu import PREDICATES, PROJECTIONS
+
+# random_batch's schema: i Int64, j Int32, f Float32, g Float64, e vector<3>
+RANDOM = [(plan.INT64, True), (plan.INT32, True), (plan.FLOAT32, True), (plan.FLOAT64, True),
+          (plan.F32_VECTOR, 3, True)]
+CONFIG2 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT32, True), (plan.FLOAT64, False)]
+
+
+@pytest.fixture(autouse=True)
+def four_rows():
+    autotune.set_knob("filter.rows", 4)
+    yield
+    autotune.set_knob("filter.rows", -1)
+
+
+@pytest.mark.parametrize("pi", range(len(PREDICATES)))
+def test_every_test_predicate_compiles(pi):
+    src = backend.filter_kernel_source(RANDOM, PREDICATES[pi], [0], compile=True)
+    assert "oxr_filter_jit" in src
+
+
+@pytest.mark.parametrize("qi", range(len(PROJECTIONS)))
+def test_every_test_projection_compiles(qi):
+    backend.filter_kernel_source(RANDOM, None, PROJECTIONS[qi], compile=True)
+
+
+def test_config2_kernel_shape():
+    autotune.set_knob("filter.rows", -1)
+    pred, proj = datagen.config2_query()
+    src = backend.filter_kernel_source(CONFIG2, pred, proj, compile=True)
+    # four 1-2 register columns plus one validity word: 16 rows per lane
+    assert "filter_tile<Q, 16>" in src
+    # a % 7 by a constant divisor (the compiler's multiply-high), no error path
+    assert "(x.c0 % ((int64_t)0x0000000000000007ull))" in src
+    # literals are constants: b < 50, then c > 0.5 by its bit pattern
+    assert "t3 < ((int64_t)0x0000000000000032ull)" in src
+    assert "order_key64(as_f64(0x3fe0000000000000ull))" in src
+    pred2, _ = datagen.config2_query(500)
+    src2 = backend.filter_kernel_source(CONFIG2, pred2, proj)
+    assert src2 == src.replace("0x0000000000000032ull", "0x00000000000001f4ull")
+    # only c*d can be NULL
+    assert "kValidOuts = 1;" in src
+
+
+def test_division_by_zero_or_minus_one_keeps_its_error_path():
+    c = [(plan.INT64, False)]
+    for d, fn in [(0, "i64_divmod(false"), (-1, "i64_divmod(false"), (3, " / ((int64_t)")]:
+        src = backend.filter_kernel_source(c, None, [plan.div(col(0), Lit.int64(d))], compile=d == -1)
+        assert fn in src, d
+    src = backend.filter_kernel_source([(plan.INT32, True)], None, [plan.mod(col(0), Lit.int32(-1))])
+    assert "i32_divmod(true" in src
+
+
+def test_nullability_changes_the_kernel():
+    pred, proj = datagen.config2_query()
+    all_valid = [(t, False) for t, _ in CONFIG2]
+    a = backend.filter_kernel_source(all_valid, pred, proj)
+    b = backend.filter_kernel_source(CONFIG2, pred, proj)
+    assert a != b and "row_bit" not in a
+
+
+def test_vector_projection_is_a_wave_copy():
+    src = backend.filter_kernel_source(RANDOM, PREDICATES[0], [4, 0], compile=True)
+    assert "copy_vector_rows" in src and "kVecOuts = 1;" in src
+
+
+@pytest.mark.parametrize("rows", [1, 2, 8, 16])
+def test_every_rows_per_lane_compiles(rows):
+    autotune.set_knob("filter.rows", rows)
+    pred, proj = datagen.config2_query()
+    src = backend.filter_kernel_source(CONFIG2, pred, proj, compile=True)
+    assert f"filter_tile<Q, {rows}>" in src
+
+
+def test_plan_errors_match_the_interpreter():
+    with pytest.raises(PlanError):
+        backend.filter_kernel_source(CONFIG2, Predicate.compare(9, Cmp.Eq, Lit.int64(1)), [0])
+    with pytest.raises(Unsupported):
+        backend.filter_kernel_source(RANDOM, Predicate.compare(4, Cmp.Eq, Lit.int64(1)), [0])
+    with pytest.raises(Unsupported):
+        backend.filter_kernel_source(CONFIG2, None, [plan.cmp(col(0), Cmp.Eq, col(1))])

@@ -46,6 +46,18 @@ def test_filter_drops_null_predicates_and_projects(oracle):
     assert out.column(0).to_pylist() == [2, 5]
 
 
+def test_projections_see_only_the_kept_rows(oracle):
+    # a projection above a filter (DataFusion's ProjectionExec over
+    # FilterExec): a zero divisor in a dropped row is no error, in a kept
+    # row it is
+    b = pa.record_batch({"x": pa.array([4, 5, 6], pa.int64()), "y": pa.array([2, 0, 3], pa.int64())})
+    keep = Predicate.compare(1, Cmp.NotEq, Lit.int64(0))
+    out = oracle.filter_project(b, keep, [plan.div(col(0), col(1))])
+    assert out.column(0).to_pylist() == [2, 2]
+    with pytest.raises(ZeroDivisionError):
+        oracle.filter_project(b, None, [plan.div(col(0), col(1))])
+
+
 def test_float_comparisons_order_as_datafusion_does(oracle):
     # kernels.rs:990-1021
     vals = [NAN, -NAN, INF, -INF, 0.0, -0.0, 4.0]

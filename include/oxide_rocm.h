@@ -363,6 +363,27 @@ int oxr_vector_topk(oxr_stream *s, const oxr_batch *input, int32_t column,
                     int32_t metric, int32_t k, int64_t *out_rows,
                     float *out_dist);
 
+/* A search index over a vector column (new, like top-k): row norms and the
+ * rows in the scan's precision, built once on the device and kept resident.
+ * OXR_VEC_FP32 scans the FP32 rows themselves; OXR_VEC_BF16 a BF16 copy;
+ * OXR_VEC_FP8 an e4m3 copy with one scale per row (max |x| / 448). The index
+ * keeps the batch's buffers alive; the batch may be freed first. */
+enum { OXR_VEC_FP32 = 1, OXR_VEC_BF16 = 2, OXR_VEC_FP8 = 3 };
+int oxr_vector_index_create(oxr_stream *s, const oxr_batch *input, int32_t column,
+                            int32_t metric, int32_t precision,
+                            oxr_vector_index **out);
+/* Top-k of `nq` queries (row-major nq x dim, host memory): the scan keeps
+ * the `candidates` best rows per query (at least k, at most 1024) by the
+ * index's precision, then ranks them by exact FP32 distance (ties: smaller
+ * row first). out_rows/out_dist are nq*k host arrays; rows -1 where fewer
+ * than k rows qualify. NULL rows and rows whose distance is NaN never
+ * qualify. With OXR_VEC_FP32 and candidates = k this is the exact path. */
+int oxr_vector_index_search(oxr_stream *s, const oxr_vector_index *index,
+                            const float *queries, int32_t nq, int32_t k,
+                            int32_t candidates, int64_t *out_rows,
+                            float *out_dist);
+void oxr_vector_index_free(oxr_vector_index *index);
+
 /* --------------------------------------------------------- synthetic tables */
 /* Seeded synthetic columns written straight into HBM, for benchmarks and
  * tests (the reference's counterpart is its `gen-data` table generator,

@@ -161,6 +161,12 @@ SIGNATURES: dict[str, tuple] = {
         [P, P, I32, C.POINTER(C.c_float), I32, I32, I32, I32, C.POINTER(C.c_int64),
          C.POINTER(C.c_float)],
     ),
+    "oxr_vector_index_create": (C.c_int, [P, P, I32, I32, I32, PP]),
+    "oxr_vector_index_search": (
+        C.c_int,
+        [P, P, C.POINTER(C.c_float), I32, I32, I32, C.POINTER(C.c_int64), C.POINTER(C.c_float)],
+    ),
+    "oxr_vector_index_free": (None, [P]),
     "oxr_generate": (C.c_int, [P, C.POINTER(GenSpec), C.POINTER(C.c_char_p), I32, U64, PP]),
     "oxr_tune_set": (C.c_int, [C.c_char_p, C.c_int64]),
     "oxr_tune_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),

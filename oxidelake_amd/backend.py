@@ -419,6 +419,49 @@ class RocmBackend:
         return rows, dist
 
 
+class VectorIndex:
+    """A resident search index over a vector column (oxr_vector_index_*):
+    precision VEC_FP32 (exact), VEC_BF16 or VEC_FP8 (reduced-precision scan
+    with an FP32 rerank of `candidates` rows per query)."""
+
+    def __init__(self, stream: Stream, batch: DeviceBatch, column: int, metric: DistanceMetric,
+                 precision: int):
+        out = C.c_void_p()
+        check(lib.oxr_vector_index_create(stream.handle, batch.handle, column, int(metric),
+                                          int(precision), C.byref(out)))
+        self.handle = out.value
+        self.precision = precision
+
+    def search(self, stream: Stream, queries: np.ndarray, k: int,
+               candidates: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        q = np.ascontiguousarray(np.asarray(queries, dtype=np.float32))
+        if q.ndim == 1:
+            q = q[None, :]
+        nq = q.shape[0]
+        rows = np.empty((nq, k), dtype=np.int64)
+        dist = np.empty((nq, k), dtype=np.float32)
+        check(lib.oxr_vector_index_search(stream.handle, self.handle,
+                                          q.ctypes.data_as(C.POINTER(C.c_float)), nq, k,
+                                          max(candidates, k),
+                                          rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          dist.ctypes.data_as(C.POINTER(C.c_float))))
+        return rows, dist
+
+    def free(self) -> None:
+        if self.handle:
+            lib.oxr_vector_index_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+VEC_FP32, VEC_BF16, VEC_FP8 = 1, 2, 3
+
+
 class AggState:
     """Streaming aggregation state resident in HBM (oxr_agg_*)."""
 

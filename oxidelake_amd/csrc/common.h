@@ -155,10 +155,25 @@ struct oxr_batch {
   uint64_t rows = 0;
   std::vector<oxr::DevCol> cols;
   std::vector<std::string> names;
-  std::shared_ptr<void> holder;  // buffers shared with other views of one result
+  // buffers this batch's columns point into but does not own (views of
+  // another result, or of an input whose buffers it shares)
+  std::vector<std::shared_ptr<void>> holders;
 };
 
 namespace oxr {
+// Arena buffers freed together, on the stream that allocated them, when the
+// last batch (or index) sharing them lets go.
+struct SharedBuffers {
+  oxr_device *dev = nullptr;
+  StreamCore *core = nullptr;
+  std::vector<void *> ptrs;
+  ~SharedBuffers();
+};
+// Keeps every buffer b's columns point to alive for as long as the returned
+// holder lives: b's owned columns move into a SharedBuffers (b still frees
+// nothing twice), so views of b may outlive b.
+std::shared_ptr<void> share_columns(const oxr_batch *b);
+
 // A batch whose columns are allocated by the library on stream s.
 oxr_batch *new_batch(oxr_stream *s, uint64_t rows);
 // Allocates a column of `rows` rows (values, and a zeroed validity bitmap

@@ -414,20 +414,6 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   return rc;
 }
 
-// Output buffers shared by the views of a multi-batch result.
-struct SharedCols {
-  oxr_device *dev = nullptr;
-  StreamCore *core = nullptr;
-  std::vector<DevCol> cols;
-  ~SharedCols() {
-    for (auto &c : cols) {
-      dev->arena->release(c.values, core->s);
-      dev->arena->release(c.validity, core->s);
-    }
-    release(core);
-  }
-};
-
 }  // namespace
 }  // namespace oxr
 
@@ -462,20 +448,25 @@ extern "C" int oxr_filter_project_batches(oxr_stream *s, const oxr_batch *input,
     return fail(OXR_EPLAN, "rocm.filter",
                 "the result has " + std::to_string(nb) + " batches but room for " +
                     std::to_string(max_batches));
-  auto shared = std::make_shared<SharedCols>();
+  std::vector<DevCol> cols;
   std::vector<std::string> out_names;
   std::vector<uint64_t> counts;
-  OXR_TRY(run_filter(s, input, batch_rows, predicate, projection, nproj, names, &shared->cols,
-                     &out_names, &counts));
+  OXR_TRY(run_filter(s, input, batch_rows, predicate, projection, nproj, names, &cols, &out_names,
+                     &counts));
+  auto shared = std::make_shared<SharedBuffers>();
   shared->dev = s->dev;
   shared->core = retain(s->core);
+  for (const DevCol &c : cols) {
+    shared->ptrs.push_back(c.values);
+    shared->ptrs.push_back(c.validity);
+  }
   for (uint64_t i = 0; i < nb; i++) {
     oxr_batch *b = new oxr_batch();
     b->dev = s->dev;
     b->rows = counts[i];
-    b->holder = shared;
+    b->holders.push_back(shared);
     b->names = out_names;
-    for (const DevCol &c : shared->cols) {
+    for (const DevCol &c : cols) {
       DevCol v = c;
       v.owned = false;
       v.len = counts[i];

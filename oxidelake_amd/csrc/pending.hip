@@ -11,8 +11,4 @@ using namespace oxr;
 extern "C" {
 int oxr_agg_update_join(oxr_stream *, oxr_agg_state *, const oxr_join_table *, const oxr_batch *,
                         int32_t, const oxr_batch *, int32_t) { DECLINE("rocm.aggregate"); }
-int oxr_vector_distance(oxr_stream *, const oxr_batch *, int32_t, const float *, int32_t, int32_t,
-                        const char *, oxr_batch **) { DECLINE("rocm.vector"); }
-int oxr_vector_topk(oxr_stream *, const oxr_batch *, int32_t, const float *, int32_t, int32_t,
-                    int32_t, int32_t, int64_t *, float *) { DECLINE("rocm.vector"); }
 }

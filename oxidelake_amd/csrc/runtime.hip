@@ -202,6 +202,33 @@ void release(StreamCore *c) {
   delete c;
 }
 
+SharedBuffers::~SharedBuffers() {
+  for (void *p : ptrs)
+    if (p) dev->arena->release(p, core->s);
+  release(core);
+}
+
+std::shared_ptr<void> share_columns(const oxr_batch *cb) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  oxr_batch *b = const_cast<oxr_batch *>(cb);  // ownership moves; contents do not change
+  bool owned = false;
+  for (const DevCol &c : b->cols) owned |= c.owned;
+  if (owned) {
+    auto sb = std::make_shared<SharedBuffers>();
+    sb->dev = b->dev;
+    sb->core = retain(b->owner);
+    for (DevCol &c : b->cols) {
+      if (!c.owned) continue;
+      sb->ptrs.push_back(c.values);
+      sb->ptrs.push_back(c.validity);
+      c.owned = false;
+    }
+    b->holders.push_back(sb);
+  }
+  return std::make_shared<std::vector<std::shared_ptr<void>>>(b->holders);
+}
+
 oxr_batch *new_batch(oxr_stream *s, uint64_t rows) {
   oxr_batch *b = new oxr_batch();
   b->dev = s->dev;

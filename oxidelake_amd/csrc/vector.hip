@@ -1,0 +1,937 @@
+// Vector distance (L2 / cosine) and top-k search.
+//
+// `oxr_vector_distance` replaces `ops::vector_distance` (crates/oxidelake-
+// device/src/cuda/ops.rs:1163-1249, kernels/cuda/vector_distance.cu): one
+// wave per row, the query in LDS, 16-byte loads, a wave reduction; the
+// distance column is appended to views of the input's columns (no copy).
+//
+// Top-k is new (the reference sorts distances on the CPU, DataFusion's
+// SortExec with a fetch: crates/oxidelake-runtime/tests/embedded.rs:24-25).
+// Queries run in batches through four kernels:
+//   1. scan: rows x queries on the matrix cores — FP32 (32x32x2), BF16
+//      (32x32x16) or FP8 e4m3 with per-row scales (32x32x64) — with FP32
+//      accumulation; the epilogue folds row norms and scales into a score
+//      that orders rows like the distance (L2: 2 x.q - |x|^2, cosine:
+//      x.q / |x|) and writes a tile of scores per query;
+//   2. select: a streaming top-C per (segment, query) with a threshold and
+//      an LDS candidate buffer, then a merge per query;
+//   3. rerank: exact FP32 distances of the C candidates from the FP32 rows,
+//      sorted by (distance, row); the first k are the answer.
+// With FP32 and C = k this is the plain exact path; BF16/FP8 with C > k is
+// the reduced-precision scan with an FP32 rerank.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "common.h"
+
+struct oxr_vector_index {
+  oxr_device *dev = nullptr;
+  oxr::StreamCore *owner = nullptr;
+  uint64_t n = 0;
+  int32_t dim = 0, dimp = 0, metric = 0, precision = 0;
+  const float *f32 = nullptr;      // the FP32 rows (the input batch's column)
+  const uint32_t *valid = nullptr; // their validity (NULL: all valid)
+  std::shared_ptr<void> holder;    // keeps the input's buffers alive
+  void *scan = nullptr;            // rows as the scan reads them: [n][dimp]
+  bool scan_owned = false;
+  float *rowinfo = nullptr;        // L2: |x|^2; cosine: 1/|x| (0: zero norm)
+  float *rscale = nullptr;         // FP8: per-row scale
+};
+
+namespace oxr {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+enum { P_FP32 = OXR_VEC_FP32, P_BF16 = OXR_VEC_BF16, P_FP8 = OXR_VEC_FP8 };
+constexpr float kFp8Max = 448.0f;  // OCP e4m3 (gfx950)
+constexpr int kBlock = 256;
+constexpr int kMaxDim = 8192;      // the query in LDS (with the rerank keys: < 64 KB)
+constexpr int kMaxCandidates = 1024;
+constexpr int kMaxScanQueries = 256;  // queries per scan pass (8 tiles of 32)
+
+__device__ inline float wave_sum(float x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+__device__ inline float wave_max(float x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x = fmaxf(x, __shfl_xor(x, d, 64));
+  return x;
+}
+
+__device__ inline bool valid_row(const uint32_t *valid, uint64_t r) {
+  return !valid || ((valid[r >> 5] >> (r & 31)) & 1u);
+}
+
+// ------------------------------------------------------- one-query distance
+struct DistArgs {
+  const float *x;
+  const uint32_t *valid;
+  uint64_t n;
+  int32_t dim, metric;
+  const float *q;
+  float *out;
+};
+
+// Per-lane partial sums over a row: L2 (sum of squared differences) or
+// cosine (dot, |x|^2). 16-byte loads when rows are 16-byte aligned.
+__device__ inline void row_partials(const float *x, const float *q, int dim, int metric, int lane,
+                                    float *p0, float *p1) {
+  float s0 = 0.f, s1 = 0.f;
+  if ((dim & 3) == 0) {
+    const f32x4 *x4 = (const f32x4 *)x;
+    const f32x4 *q4 = (const f32x4 *)q;
+    for (int c = lane; c < dim / 4; c += 64) {
+      const f32x4 a = x4[c], b = q4[c];
+      if (metric == OXR_COSINE) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          s0 = __fmaf_rn(a[e], b[e], s0);
+          s1 = __fmaf_rn(a[e], a[e], s1);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const float d = a[e] - b[e];
+          s0 = __fmaf_rn(d, d, s0);
+        }
+      }
+    }
+  } else {
+    for (int c = lane; c < dim; c += 64) {
+      const float a = x[c], b = q[c];
+      if (metric == OXR_COSINE) {
+        s0 = __fmaf_rn(a, b, s0);
+        s1 = __fmaf_rn(a, a, s1);
+      } else {
+        const float d = a - b;
+        s0 = __fmaf_rn(d, d, s0);
+      }
+    }
+  }
+  *p0 = s0;
+  *p1 = s1;
+}
+
+// L2 = sqrt(sum (a-b)^2); cosine = 1 - dot/(|a||b|), NaN when a norm is 0
+// (crates/oxidelake-device/src/cpu/kernels.rs:908-927).
+__device__ inline float finish_distance(int metric, float s0, float s1, float qnorm2) {
+  if (metric == OXR_COSINE) {
+    const float na = sqrtf(s1), nb = sqrtf(qnorm2);
+    if (na == 0.f || nb == 0.f) return __builtin_nanf("");
+    return 1.0f - s0 / (na * nb);
+  }
+  return sqrtf(s0);
+}
+
+__global__ __launch_bounds__(kBlock) void distance_kernel(const DistArgs a) {
+  extern __shared__ float s_q[];
+  for (int i = threadIdx.x; i < a.dim; i += kBlock) s_q[i] = a.q[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  float qn = 0.f;
+  for (int i = lane; i < a.dim; i += 64) qn = __fmaf_rn(s_q[i], s_q[i], qn);
+  const float qnorm2 = wave_sum(qn);
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t r = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < a.n; r += waves) {
+    if (!valid_row(a.valid, r)) {
+      if (lane == 0) a.out[r] = 0.f;
+      continue;
+    }
+    float s0, s1;
+    row_partials(a.x + r * (uint64_t)a.dim, s_q, a.dim, a.metric, lane, &s0, &s1);
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (lane == 0) a.out[r] = finish_distance(a.metric, s0, s1, qnorm2);
+  }
+}
+
+// ------------------------------------------------------------- index build
+// One wave per row: |x|^2 (and 1/|x|), and the row in the scan's precision
+// (BF16 round-to-nearest-even, FP8 e4m3 with scale max|x| / 448), padded.
+struct BuildArgs {
+  const float *x;
+  uint64_t n;
+  int32_t dim, dimp, metric, precision;
+  void *scan;
+  float *rowinfo, *rscale;
+};
+
+__global__ __launch_bounds__(kBlock) void build_kernel(const BuildArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+  for (uint64_t r = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); r < a.n; r += waves) {
+    const float *x = a.x + r * (uint64_t)a.dim;
+    float ss = 0.f, mx = 0.f;
+    for (int c = lane; c < a.dim; c += 64) {
+      const float v = x[c];
+      ss = __fmaf_rn(v, v, ss);
+      mx = fmaxf(mx, fabsf(v));
+    }
+    ss = wave_sum(ss);
+    mx = wave_max(mx);
+    if (lane == 0) {
+      if (a.metric == OXR_COSINE) a.rowinfo[r] = ss > 0.f ? 1.0f / sqrtf(ss) : 0.f;
+      else a.rowinfo[r] = ss;
+    }
+    if (a.precision == P_BF16) {
+      __bf16 *o = (__bf16 *)a.scan + r * (uint64_t)a.dimp;
+      for (int c = lane; c < a.dimp; c += 64) o[c] = (__bf16)(c < a.dim ? x[c] : 0.f);
+    } else if (a.precision == P_FP8) {
+      const float s = mx > 0.f ? mx / kFp8Max : 1.0f;
+      if (lane == 0) a.rscale[r] = s;
+      int *o = (int *)((uint8_t *)a.scan + r * (uint64_t)a.dimp);
+      for (int c4 = lane; c4 < a.dimp / 4; c4 += 64) {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int c = c4 * 4 + e;
+          v[e] = c < a.dim ? fminf(fmaxf(x[c] / s, -kFp8Max), kFp8Max) : 0.f;
+        }
+        const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+        o[c4] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+      }
+    } else if (a.precision == P_FP32 && a.scan) {  // padded FP32 copy
+      float *o = (float *)a.scan + r * (uint64_t)a.dimp;
+      for (int c = lane; c < a.dimp; c += 64) o[c] = c < a.dim ? x[c] : 0.f;
+    }
+  }
+}
+
+// Queries in the scan's precision: [qpad][dimp]; FP8 queries get a scale
+// each. One block per query row; rows past nq are zero.
+struct PrepArgs {
+  const float *q;  // [nq][dim]
+  int32_t nq, qpad, dim, dimp, precision;
+  void *out;
+  float *qscale;
+};
+
+__global__ __launch_bounds__(kBlock) void prep_kernel(const PrepArgs a) {
+  __shared__ float s_max[kBlock / 64];
+  const int qi = blockIdx.x;
+  const bool real = qi < a.nq;
+  const float *q = a.q + (uint64_t)(real ? qi : 0) * a.dim;
+  float mx = 0.f;
+  if (real)
+    for (int c = threadIdx.x; c < a.dim; c += kBlock) mx = fmaxf(mx, fabsf(q[c]));
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+  if (a.precision == P_FP32) {
+    float *o = (float *)a.out + (uint64_t)qi * a.dimp;
+    for (int c = threadIdx.x; c < a.dimp; c += kBlock) o[c] = real && c < a.dim ? q[c] : 0.f;
+  } else if (a.precision == P_BF16) {
+    __bf16 *o = (__bf16 *)a.out + (uint64_t)qi * a.dimp;
+    for (int c = threadIdx.x; c < a.dimp; c += kBlock) o[c] = (__bf16)(real && c < a.dim ? q[c] : 0.f);
+  } else {
+    const float s = mx > 0.f ? mx / kFp8Max : 1.0f;
+    if (threadIdx.x == 0) a.qscale[qi] = s;
+    int *o = (int *)((uint8_t *)a.out + (uint64_t)qi * a.dimp);
+    for (int c4 = threadIdx.x; c4 < a.dimp / 4; c4 += kBlock) {
+      float v[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int c = c4 * 4 + e;
+        v[e] = real && c < a.dim ? fminf(fmaxf(q[c] / s, -kFp8Max), kFp8Max) : 0.f;
+      }
+      const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+      o[c4] = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+    }
+  }
+}
+
+// -------------------------------------------------------------------- scan
+// Rows x queries on the matrix cores. A block is 4 waves; wave w computes
+// rows [32*RT*w, 32*RT*(w+1)) of the block's row range against NT tiles of
+// 32 queries. The k axis is walked in groups: per group each lane loads LB
+// bytes of its row (lane l: row l%32, the half l/32 of the group) straight
+// from HBM, and the group's slice of every query is staged through LDS
+// (double-buffered). Verified lane layouts (32x32 instructions): A row and
+// B column = lane%32, both at the same k set for lane half l/32; result
+// element r of lane l is (row 8*(r/4) + 4*(l/32) + r%4, query l%32).
+// The scan writes raw dot products; the select kernel makes them scores.
+struct ScanArgs {
+  const void *x;      // [n][dimp] in the scan precision
+  const void *qm;     // [qpad][dimp]
+  uint64_t n, tile0, tile_rows, nblocks;
+  int32_t dimp, groups, nq, pad;
+  float *scores;      // [nq][tile_rows]: x.q (FP8: of the scaled rows)
+};
+
+template <int P>
+struct Prec;
+template <>
+struct Prec<P_FP32> {
+  static constexpr int EB = 4, KG = 8, LB = 16;  // 4 steps of 32x32x2 per group
+};
+template <>
+struct Prec<P_BF16> {
+  static constexpr int EB = 2, KG = 16, LB = 16;  // one 32x32x16
+};
+template <>
+struct Prec<P_FP8> {
+  static constexpr int EB = 1, KG = 64, LB = 32;  // one 32x32x64
+};
+
+template <int P>
+struct Frag {
+  uint4 v[Prec<P>::LB / 16];
+};
+
+template <int P>
+__device__ inline void mfma_group(f32x16 &acc, const Frag<P> &a, const Frag<P> &b) {
+  if constexpr (P == P_FP32) {
+    f32x4 x, y;
+    __builtin_memcpy(&x, &a.v[0], 16);
+    __builtin_memcpy(&y, &b.v[0], 16);
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[j], y[j], acc, 0, 0, 0);
+  } else if constexpr (P == P_BF16) {
+    bf16x8 x, y;
+    __builtin_memcpy(&x, &a.v[0], 16);
+    __builtin_memcpy(&y, &b.v[0], 16);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc, 0, 0, 0);
+  } else {
+    i32x8 x, y;
+    __builtin_memcpy(&x, &a.v[0], 32);
+    __builtin_memcpy(&y, &b.v[0], 32);
+    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x, y, acc, 0, 0, 0, 127, 0, 127);
+  }
+}
+
+template <int P, int NT, int RT>
+__global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
+  constexpr int LB = Prec<P>::LB, KG = Prec<P>::KG, EB = Prec<P>::EB;
+  constexpr int U = LB / 16;                 // 16-byte units per lane per group
+  constexpr int QROW = 2 * U;                // units per query per group
+  constexpr int BUNITS = NT * 32 * QROW;     // units of one group's query slice
+  constexpr int PER_T = (BUNITS + kBlock - 1) / kBlock;
+  __shared__ uint4 s_b[2][BUNITS];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 31, h = lane >> 5;
+  const uint64_t rstride = (uint64_t)a.dimp * EB;  // bytes per row
+  const uint8_t *xb = (const uint8_t *)a.x;
+  const uint8_t *qb = (const uint8_t *)a.qm;
+  const uint64_t tile_end = min(a.n, a.tile0 + a.tile_rows);
+
+  for (uint64_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
+    const uint64_t rowbase = a.tile0 + blk * (uint64_t)(4 * RT * 32) + (uint64_t)wave * RT * 32;
+    f32x16 acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[r][t] = f32x16{};
+
+    auto load_a = [&](int g, Frag<P> (&fa)[RT]) {
+#pragma unroll
+      for (int r = 0; r < RT; r++) {
+        const uint64_t row = rowbase + r * 32 + i;
+        const bool in = row < tile_end;
+        // rows past the tile read row tile0 (always present) and are zeroed
+        const uint64_t arow = in ? row : a.tile0;
+        const uint4 *src = (const uint4 *)(xb + arow * rstride + (uint64_t)(g * KG + h * (KG / 2)) * EB);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const uint4 v = src[u];
+          fa[r].v[u] = in ? v : make_uint4(0, 0, 0, 0);
+        }
+      }
+    };
+    auto load_b = [&](int g, uint4 (&rb)[PER_T]) {
+#pragma unroll
+      for (int p = 0; p < PER_T; p++) {
+        const int unit = p * kBlock + tid;
+        if (unit < BUNITS) {
+          const int qrow = unit / QROW, part = unit % QROW;
+          rb[p] = *(const uint4 *)(qb + (uint64_t)qrow * rstride + (uint64_t)g * KG * EB + part * 16);
+        }
+      }
+    };
+    auto store_b = [&](int buf, const uint4 (&rb)[PER_T]) {
+#pragma unroll
+      for (int p = 0; p < PER_T; p++) {
+        const int unit = p * kBlock + tid;
+        if (unit < BUNITS) s_b[buf][unit] = rb[p];
+      }
+    };
+
+    Frag<P> fa[RT], fn[RT];
+    uint4 rb[PER_T];
+    load_a(0, fa);
+    load_b(0, rb);
+    store_b(0, rb);
+    __syncthreads();
+    for (int g = 0; g < a.groups; g++) {
+      const bool more = g + 1 < a.groups;
+      if (more) {
+        load_a(g + 1, fn);
+        load_b(g + 1, rb);
+      }
+      const int buf = g & 1;
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        Frag<P> fb;
+#pragma unroll
+        for (int u = 0; u < U; u++) fb.v[u] = s_b[buf][(t * 32 + i) * QROW + h * U + u];
+#pragma unroll
+        for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], fa[r], fb);
+      }
+      if (more) {
+        store_b(buf ^ 1, rb);
+#pragma unroll
+        for (int r = 0; r < RT; r++) fa[r] = fn[r];
+      }
+      __syncthreads();
+    }
+
+    // epilogue: raw dot products (the select kernel turns them into scores,
+    // so no row data is live across the k loop)
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int q = t * 32 + i;
+        if (q >= a.nq) continue;
+        float *out = a.scores + (uint64_t)q * a.tile_rows + (rowbase + r * 32 - a.tile0);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          const int m = 8 * (e / 4) + 4 * h + (e % 4);
+          if (rowbase + r * 32 + m < tile_end) out[m] = acc[r][t][e];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ select
+// Keys that order candidates: larger is better. The score's bits made
+// order-preserving as an unsigned integer, then the row (smaller row wins a
+// tie). 0 = no candidate.
+__device__ inline uint32_t ordered(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline uint64_t score_key(float score, uint64_t row) {
+  return ((uint64_t)ordered(score) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)row);
+}
+__device__ inline uint64_t key_row(uint64_t key) { return 0xFFFFFFFFu - (uint32_t)key; }
+
+constexpr int kBuf = 4096;   // LDS candidate buffer (keys)
+constexpr int kChunk = 2048; // keys read per round (8 per thread)
+
+// Bitonic sort of s[0..kBuf) in descending order by the whole block.
+__device__ void sort_desc(uint64_t *s) {
+  for (int size = 2; size <= kBuf; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < kBuf / 2; t += kBlock) {
+        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int hi = lo + stride;
+        const bool desc = (lo & size) == 0;
+        const uint64_t x = s[lo], y = s[hi];
+        if ((x < y) == desc) {
+          s[lo] = y;
+          s[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Keeps the best C keys of the buffer: sorts, truncates, sets the threshold.
+__device__ void compact(uint64_t *buf, int *count, uint64_t *tau, int C) {
+  const int n = *count;
+  for (int t = threadIdx.x; t < kBuf; t += kBlock)
+    if (t >= n) buf[t] = 0;
+  __syncthreads();
+  sort_desc(buf);
+  if (threadIdx.x == 0) {
+    const int keep = n < C ? n : C;
+    *count = keep;
+    *tau = keep == C ? buf[C - 1] : 0;
+  }
+  __syncthreads();
+}
+
+// Streams `len` keys from key_at(i) through the buffer; leaves the best C
+// (sorted, descending) in buf[0..C) with zeros after them.
+template <class F>
+__device__ void top_c(F key_at, uint64_t len, int C, uint64_t *buf, int *count, uint64_t *tau) {
+  if (threadIdx.x == 0) {
+    *count = 0;
+    *tau = 0;
+  }
+  __syncthreads();
+  for (uint64_t base = 0; base < len; base += kChunk) {
+    const uint64_t th = *tau;
+#pragma unroll
+    for (int j = 0; j < kChunk / kBlock; j++) {
+      const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
+      if (idx < len) {
+        const uint64_t k = key_at(idx);
+        if (k > th) buf[atomicAdd(count, 1)] = k;
+      }
+    }
+    __syncthreads();
+    const int cnt = *count;
+    __syncthreads();  // every thread has read the count before more keys arrive
+    if (cnt > kBuf - kChunk) compact(buf, count, tau, C);
+  }
+  compact(buf, count, tau, C);
+}
+
+struct SelectArgs {
+  const float *dots;    // [nq][tile_rows]
+  uint64_t tile0, tile_len, tile_rows, seg_rows;
+  int32_t nq, C, seg_first, segs_total;
+  int32_t metric, precision;
+  const float *rowinfo, *rscale, *qscale;
+  const uint32_t *valid;
+  uint64_t *cand;       // [nq][segs_total][C]
+};
+
+// The score of a row for a query, ordered like the distance (larger is
+// nearer): L2 2 x.q - |x|^2, cosine x.q / |x|; -inf for rows that cannot
+// qualify (NULL, zero norm).
+__device__ inline float row_score(const SelectArgs &a, float dot, uint64_t row, float qs) {
+  if (!valid_row(a.valid, row)) return -INFINITY;
+  if (a.precision == P_FP8) dot = dot * a.rscale[row] * qs;
+  const float info = a.rowinfo[row];
+  if (a.metric == OXR_COSINE) return info > 0.f ? dot * info : -INFINITY;
+  return 2.0f * dot - info;
+}
+
+// grid (segments of this tile, queries)
+__global__ __launch_bounds__(kBlock) void select_tile_kernel(const SelectArgs a) {
+  __shared__ uint64_t s_buf[kBuf];
+  __shared__ int s_count;
+  __shared__ uint64_t s_tau;
+  const int q = blockIdx.y;
+  const uint64_t lo = (uint64_t)blockIdx.x * a.seg_rows;
+  if (lo >= a.tile_len) return;
+  const uint64_t len = min(a.seg_rows, a.tile_len - lo);
+  const float *dp = a.dots + (uint64_t)q * a.tile_rows + lo;
+  const uint64_t row0 = a.tile0 + lo;
+  const float qs = a.precision == P_FP8 ? a.qscale[q] : 1.0f;
+  top_c(
+      [&](uint64_t idx) -> uint64_t {
+        const float s = row_score(a, dp[idx], row0 + idx, qs);
+        return s > -INFINITY ? score_key(s, row0 + idx) : 0;
+      },
+      len, a.C, s_buf, &s_count, &s_tau);
+  uint64_t *out = a.cand + ((uint64_t)q * a.segs_total + a.seg_first + blockIdx.x) * a.C;
+  for (int j = threadIdx.x; j < a.C; j += kBlock) out[j] = s_buf[j];
+}
+
+// grid (queries): the best C of every segment's C
+__global__ __launch_bounds__(kBlock) void merge_kernel(uint64_t *cand, int segs_total, int C,
+                                                       uint64_t *best) {
+  __shared__ uint64_t s_buf[kBuf];
+  __shared__ int s_count;
+  __shared__ uint64_t s_tau;
+  const int q = blockIdx.x;
+  const uint64_t *in = cand + (uint64_t)q * segs_total * C;
+  top_c([&](uint64_t idx) -> uint64_t { return in[idx]; }, (uint64_t)segs_total * C, C, s_buf,
+        &s_count, &s_tau);
+  for (int j = threadIdx.x; j < C; j += kBlock) best[(uint64_t)q * C + j] = s_buf[j];
+}
+
+// grid (queries): exact FP32 distances of the C candidates, sorted by
+// (distance, row); the first k out.
+struct RerankArgs {
+  const float *x;
+  int32_t dim, metric, C, Cpad, k;
+  const float *queries;   // [nq][dim] FP32
+  const uint64_t *best;   // [nq][C]
+  int64_t *out_rows;      // [nq][k]
+  float *out_dist;
+};
+
+__global__ __launch_bounds__(kBlock) void rerank_kernel(const RerankArgs a) {
+  extern __shared__ uint64_t s_dyn[];
+  uint64_t *s_key = s_dyn;                      // [Cpad]
+  float *s_q = (float *)(s_dyn + a.Cpad);       // [dim]
+  const int q = blockIdx.x;
+  const float *qv = a.queries + (uint64_t)q * a.dim;
+  for (int c = threadIdx.x; c < a.dim; c += kBlock) s_q[c] = qv[c];
+  for (int j = threadIdx.x; j < a.Cpad; j += kBlock) s_key[j] = ~0ull;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float qn = 0.f;
+  for (int c = lane; c < a.dim; c += 64) qn = __fmaf_rn(s_q[c], s_q[c], qn);
+  const float qnorm2 = wave_sum(qn);
+  for (int j = wave; j < a.C; j += kBlock / 64) {
+    const uint64_t key = a.best[(uint64_t)q * a.C + j];
+    if (key == 0) continue;
+    const uint64_t row = key_row(key);
+    float s0, s1;
+    row_partials(a.x + row * (uint64_t)a.dim, s_q, a.dim, a.metric, lane, &s0, &s1);
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    const float d = finish_distance(a.metric, s0, s1, qnorm2);
+    if (lane == 0 && !__builtin_isnan(d)) s_key[j] = ((uint64_t)ordered(d) << 32) | row;
+  }
+  __syncthreads();
+  // ascending bitonic sort of Cpad keys (a power of two)
+  for (int size = 2; size <= a.Cpad; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < a.Cpad / 2; t += kBlock) {
+        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int hi = lo + stride;
+        const bool asc = (lo & size) == 0;
+        const uint64_t x = s_key[lo], y = s_key[hi];
+        if ((x > y) == asc) {
+          s_key[lo] = y;
+          s_key[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = threadIdx.x; j < a.k; j += kBlock) {
+    const uint64_t key = s_key[j];
+    int64_t row = -1;
+    float d = __builtin_nanf("");
+    if (key != ~0ull) {
+      row = (int64_t)(uint32_t)key;
+      const uint32_t u = (uint32_t)(key >> 32);
+      const uint32_t bits = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+      __builtin_memcpy(&d, &bits, 4);
+    }
+    a.out_rows[(uint64_t)q * a.k + j] = row;
+    a.out_dist[(uint64_t)q * a.k + j] = d;
+  }
+}
+
+// ------------------------------------------------------------------- host
+int grid_for(oxr_device *d, uint64_t units, int per_cu) {
+  const uint64_t cap = (uint64_t)std::max(1, d->cu_count) * per_cu;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(units, cap));
+}
+
+template <int P, int NT>
+void launch_scan_nt(dim3 grid, hipStream_t s, const ScanArgs &a) {
+  constexpr int RT = NT == 1 ? 4 : NT == 2 ? 2 : 1;
+  hipLaunchKernelGGL((scan_kernel<P, NT, RT>), grid, dim3(kBlock), 0, s, a);
+}
+
+int rows_per_block(int nt) { return 4 * 32 * (nt == 1 ? 4 : nt == 2 ? 2 : 1); }
+
+template <int P>
+void launch_scan_p(int nt, dim3 grid, hipStream_t s, const ScanArgs &a) {
+  switch (nt) {
+    case 1: launch_scan_nt<P, 1>(grid, s, a); break;
+    case 2: launch_scan_nt<P, 2>(grid, s, a); break;
+    case 4: launch_scan_nt<P, 4>(grid, s, a); break;
+    default: launch_scan_nt<P, 8>(grid, s, a); break;
+  }
+}
+
+void launch_scan(int prec, int nt, dim3 grid, hipStream_t s, const ScanArgs &a) {
+  if (prec == P_FP32) launch_scan_p<P_FP32>(nt, grid, s, a);
+  else if (prec == P_BF16) launch_scan_p<P_BF16>(nt, grid, s, a);
+  else launch_scan_p<P_FP8>(nt, grid, s, a);
+}
+
+int pad_dim(int dim, int prec) {
+  const int m = prec == P_FP8 ? 64 : prec == P_BF16 ? 16 : 8;
+  return (dim + m - 1) / m * m;
+}
+
+// Scratch freed on the stream when the scope ends.
+struct Scratch {
+  oxr_stream *s;
+  std::vector<void *> ptrs;
+  explicit Scratch(oxr_stream *st) : s(st) {}
+  ~Scratch() {
+    for (void *p : ptrs) stream_release(s, p);
+  }
+  template <class T>
+  int get(size_t bytes, T **out) {
+    void *p = nullptr;
+    OXR_TRY(stream_alloc(s, std::max<size_t>(bytes, 256), &p));
+    ptrs.push_back(p);
+    *out = (T *)p;
+    return OXR_OK;
+  }
+};
+
+int check_vector_column(const oxr_batch *input, int32_t column, const char *feature) {
+  if (!input) return fail(OXR_EPLAN, feature, "null batch");
+  if (column < 0 || column >= (int)input->cols.size())
+    return fail(OXR_EPLAN, feature, "column index out of range");
+  const DevCol &c = input->cols[column];
+  if (c.type != OXR_F32_VECTOR)
+    return fail(OXR_EPLAN, feature, "column is not a FixedSizeList<Float32> vector");
+  if (c.dim <= 0 || c.dim > kMaxDim)
+    return fail(OXR_EUNSUPPORTED, feature, "vector size " + std::to_string(c.dim) + " is above " +
+                                               std::to_string(kMaxDim));
+  return OXR_OK;
+}
+
+int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int32_t nq, int32_t k,
+           int32_t C, int64_t *out_rows, float *out_dist) {
+  if (nq <= 0) return OXR_OK;
+  if (!queries || !out_rows || !out_dist) return fail(OXR_EPLAN, "rocm.vector", "null argument");
+  if (k <= 0) return fail(OXR_EPLAN, "rocm.vector", "k must be positive");
+  if (C < k) C = k;
+  if (C > kMaxCandidates)
+    return fail(OXR_EUNSUPPORTED, "rocm.vector",
+                "at most " + std::to_string(kMaxCandidates) + " candidates per query");
+  if (ix->n == 0) {  // no rows: every answer is missing
+    for (int64_t j = 0; j < (int64_t)nq * k; j++) {
+      out_rows[j] = -1;
+      out_dist[j] = NAN;
+    }
+    return OXR_OK;
+  }
+  hipStream_t st = s->s;
+  Scratch sc(s);
+  const size_t qbytes = (size_t)nq * ix->dim * 4;
+  float *dq;
+  OXR_TRY(sc.get(qbytes, &dq));
+  OXR_HIP(hipMemcpyAsync(dq, queries, qbytes, hipMemcpyHostToDevice, st));
+  int Cpad = 64;  // a power of two; >= 2 keeps the query after the keys 16-byte aligned
+  while (Cpad < C) Cpad <<= 1;
+  int64_t *drows;
+  float *ddist;
+  OXR_TRY(sc.get((size_t)nq * k * 8, &drows));
+  OXR_TRY(sc.get((size_t)nq * k * 4, &ddist));
+  const int eb = ix->precision == P_FP8 ? 1 : ix->precision == P_BF16 ? 2 : 4;
+  const void *xscan = ix->scan ? ix->scan : (const void *)ix->f32;
+
+  for (int q0 = 0; q0 < nq; q0 += kMaxScanQueries) {
+    const int nqc = std::min(kMaxScanQueries, nq - q0);
+    const int nt = nqc <= 32 ? 1 : nqc <= 64 ? 2 : nqc <= 128 ? 4 : 8;
+    const int qpad = nt * 32;
+    void *qm;
+    float *qscale;
+    OXR_TRY(sc.get((size_t)qpad * ix->dimp * eb, &qm));
+    OXR_TRY(sc.get((size_t)qpad * 4, &qscale));
+    PrepArgs pa{dq + (size_t)q0 * ix->dim, nqc, qpad, ix->dim, ix->dimp, ix->precision, qm, qscale};
+    hipLaunchKernelGGL(prep_kernel, dim3(qpad), dim3(kBlock), 0, st, pa);
+    OXR_HIP(hipGetLastError());
+
+    // score tiles of at most ~2 GB
+    const uint64_t rpb = (uint64_t)rows_per_block(nt);
+    uint64_t tile = std::max<uint64_t>(rpb, ((2ull << 30) / (4ull * nqc)) / rpb * rpb);
+    if (tile > ix->n) tile = (ix->n + rpb - 1) / rpb * rpb;
+    const uint64_t ntiles = (ix->n + tile - 1) / tile;
+    // segments: about 2048 select blocks per tile, each at least 8K rows
+    const uint64_t want = std::max<uint64_t>(1, 2048 / (uint64_t)nqc);
+    uint64_t seg_rows = std::max<uint64_t>(8192, (tile + want - 1) / want);
+    const uint64_t segs_per_tile = (tile + seg_rows - 1) / seg_rows;
+    const uint64_t segs_total = segs_per_tile * ntiles;
+    float *scores;
+    uint64_t *cand, *best;
+    OXR_TRY(sc.get((size_t)nqc * tile * 4, &scores));
+    OXR_TRY(sc.get((size_t)nqc * segs_total * C * 8, &cand));
+    OXR_TRY(sc.get((size_t)nqc * C * 8, &best));
+    // segments past the end of the last tile stay empty
+    OXR_TRY(fill_zero(s, cand, (size_t)nqc * segs_total * C * 8));
+    for (uint64_t ti = 0; ti < ntiles; ti++) {
+      const uint64_t t0 = ti * tile;
+      const uint64_t tlen = std::min(tile, ix->n - t0);
+      ScanArgs a{};
+      a.x = xscan;
+      a.qm = qm;
+      a.n = ix->n;
+      a.tile0 = t0;
+      a.tile_rows = tile;
+      a.nblocks = (tlen + rpb - 1) / rpb;
+      a.dimp = ix->dimp;
+      a.groups = ix->dimp / (ix->precision == P_FP8 ? 64 : ix->precision == P_BF16 ? 16 : 8);
+      a.nq = nqc;
+      a.scores = scores;
+      {
+        ProfScope prof(st, "vector_scan");
+        launch_scan(ix->precision, nt, dim3(grid_for(ix->dev, a.nblocks, nt >= 4 ? 2 : 8)), st, a);
+        OXR_HIP(hipGetLastError());
+      }
+      SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total,
+                    ix->metric, ix->precision, ix->rowinfo, ix->rscale, qscale, ix->valid, cand};
+      const uint64_t segs_here = (tlen + seg_rows - 1) / seg_rows;
+      {
+        ProfScope prof(st, "vector_select");
+        hipLaunchKernelGGL(select_tile_kernel, dim3((unsigned)segs_here, nqc), dim3(kBlock), 0, st, sa);
+        OXR_HIP(hipGetLastError());
+      }
+    }
+    {
+      ProfScope prof(st, "vector_merge");
+      hipLaunchKernelGGL(merge_kernel, dim3(nqc), dim3(kBlock), 0, st, cand, (int)segs_total, C, best);
+      OXR_HIP(hipGetLastError());
+    }
+    RerankArgs ra{ix->f32, ix->dim, ix->metric, C, Cpad, k, dq + (size_t)q0 * ix->dim, best,
+                  drows + (size_t)q0 * k, ddist + (size_t)q0 * k};
+    const size_t shm = (size_t)Cpad * 8 + (size_t)ix->dim * 4;
+    {
+      ProfScope prof(st, "vector_rerank");
+      hipLaunchKernelGGL(rerank_kernel, dim3(nqc), dim3(kBlock), shm, st, ra);
+      OXR_HIP(hipGetLastError());
+    }
+  }
+  OXR_HIP(hipMemcpyAsync(out_rows, drows, (size_t)nq * k * 8, hipMemcpyDeviceToHost, st));
+  OXR_HIP(hipMemcpyAsync(out_dist, ddist, (size_t)nq * k * 4, hipMemcpyDeviceToHost, st));
+  OXR_HIP(hipStreamSynchronize(st));
+  return OXR_OK;
+}
+
+int make_index(oxr_stream *s, const oxr_batch *input, int32_t column, int32_t metric, int32_t precision,
+               oxr_vector_index **out) {
+  OXR_TRY(check_vector_column(input, column, "rocm.vector"));
+  if (metric != OXR_L2 && metric != OXR_COSINE) return fail(OXR_EPLAN, "rocm.vector", "unknown metric");
+  if (precision != P_FP32 && precision != P_BF16 && precision != P_FP8)
+    return fail(OXR_EPLAN, "rocm.vector", "unknown precision");
+  if (input->rows >= 0xFFFFFFFFull)
+    return fail(OXR_EUNSUPPORTED, "rocm.vector", "at most 2^32 - 2 rows per index");
+  const DevCol &c = input->cols[column];
+  std::unique_ptr<oxr_vector_index> ix(new oxr_vector_index());
+  ix->dev = s->dev;
+  ix->n = input->rows;
+  ix->dim = c.dim;
+  ix->dimp = pad_dim(c.dim, precision);
+  ix->metric = metric;
+  ix->precision = precision;
+  ix->f32 = (const float *)c.values;
+  ix->valid = (const uint32_t *)c.validity;
+  ix->holder = share_columns(input);
+  ix->owner = retain(s->core);
+  auto cleanup = [&]() {
+    stream_release(s, ix->scan_owned ? ix->scan : nullptr);
+    stream_release(s, ix->rowinfo);
+    stream_release(s, ix->rscale);
+  };
+  const uint64_t n = std::max<uint64_t>(ix->n, 1);
+  int rc = stream_alloc(s, n * 4, (void **)&ix->rowinfo);
+  if (rc == OXR_OK && precision == P_FP8) rc = stream_alloc(s, n * 4, (void **)&ix->rscale);
+  // FP32 rows are scanned in place unless they need padding
+  const bool copy = precision != P_FP32 || ix->dimp != ix->dim;
+  if (rc == OXR_OK && copy) {
+    const int eb = precision == P_FP8 ? 1 : precision == P_BF16 ? 2 : 4;
+    rc = stream_alloc(s, n * (uint64_t)ix->dimp * eb, &ix->scan);
+    ix->scan_owned = rc == OXR_OK;
+  }
+  if (rc == OXR_OK && ix->n) {
+    BuildArgs b{ix->f32, ix->n, ix->dim, ix->dimp, metric, precision, copy ? ix->scan : nullptr, ix->rowinfo,
+                ix->rscale};
+    ProfScope prof(s->s, "vector_build");
+    hipLaunchKernelGGL(build_kernel, dim3(grid_for(s->dev, (ix->n + 3) / 4, 16)), dim3(kBlock), 0, s->s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail_hip(e, "vector build kernel");
+  }
+  if (rc != OXR_OK) {
+    cleanup();
+    release(ix->owner);
+    return rc;
+  }
+  *out = ix.release();
+  return OXR_OK;
+}
+
+}  // namespace
+}  // namespace oxr
+
+using namespace oxr;
+
+extern "C" int oxr_vector_distance(oxr_stream *s, const oxr_batch *input, int32_t column,
+                                   const float *query, int32_t dim, int32_t metric,
+                                   const char *output_name, oxr_batch **out) {
+  clear_error();
+  if (!s || !out || !query) return fail(OXR_EPLAN, "rocm.vector", "null argument");
+  OXR_TRY(check_vector_column(input, column, "rocm.vector"));
+  const DevCol &c = input->cols[column];
+  if (dim != c.dim)
+    return fail(OXR_EPLAN, "rocm.vector",
+                "query has " + std::to_string(dim) + " elements, the column " + std::to_string(c.dim));
+  if (metric != OXR_L2 && metric != OXR_COSINE) return fail(OXR_EPLAN, "rocm.vector", "unknown metric");
+  const uint64_t n = input->rows;
+  std::unique_ptr<oxr_batch> ob(new_batch(s, n));
+  ob->holders.push_back(share_columns(input));
+  for (size_t i = 0; i < input->cols.size(); i++) {
+    DevCol v = input->cols[i];
+    v.owned = false;
+    ob->cols.push_back(v);
+    ob->names.push_back(input->names[i]);
+  }
+  DevCol d;
+  OXR_TRY(alloc_column(s, OXR_FLOAT32, 0, n, c.validity != nullptr, &d));
+  ob->cols.push_back(d);
+  ob->names.push_back(output_name ? output_name : "distance");
+  if (n == 0) {
+    *out = ob.release();
+    return OXR_OK;
+  }
+  float *dq = nullptr;
+  OXR_TRY(stream_alloc(s, (size_t)dim * 4, (void **)&dq));
+  hipError_t e = hipMemcpyAsync(dq, query, (size_t)dim * 4, hipMemcpyHostToDevice, s->s);
+  if (e == hipSuccess && c.validity)
+    e = hipMemcpyAsync(d.validity, c.validity, validity_bytes(n), hipMemcpyDeviceToDevice, s->s);
+  if (e == hipSuccess) {
+    DistArgs a{(const float *)c.values, (const uint32_t *)c.validity, n, dim, metric, dq, (float *)d.values};
+    ProfScope prof(s->s, "vector_distance");
+    hipLaunchKernelGGL(distance_kernel, dim3(grid_for(s->dev, (n + 3) / 4, 16)), dim3(kBlock),
+                       (size_t)dim * 4, s->s, a);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->s);  // the host query may go away
+  stream_release(s, dq);
+  if (e != hipSuccess) {
+    oxr_batch_free(ob.release());
+    return fail_hip(e, "vector distance");
+  }
+  *out = ob.release();
+  return OXR_OK;
+}
+
+extern "C" int oxr_vector_index_create(oxr_stream *s, const oxr_batch *input, int32_t column,
+                                       int32_t metric, int32_t precision, oxr_vector_index **out) {
+  clear_error();
+  if (!s || !out) return fail(OXR_EPLAN, "rocm.vector", "null argument");
+  return make_index(s, input, column, metric, precision, out);
+}
+
+extern "C" int oxr_vector_index_search(oxr_stream *s, const oxr_vector_index *index,
+                                       const float *queries, int32_t nq, int32_t k, int32_t candidates,
+                                       int64_t *out_rows, float *out_dist) {
+  clear_error();
+  if (!s || !index) return fail(OXR_EPLAN, "rocm.vector", "null argument");
+  if (index->dev != s->dev) return fail(OXR_EDEVICE, "rocm.vector", "index belongs to another device");
+  return search(s, index, queries, nq, k, candidates, out_rows, out_dist);
+}
+
+extern "C" void oxr_vector_index_free(oxr_vector_index *index) {
+  if (!index) return;
+  hipStream_t st = index->owner ? index->owner->s : nullptr;
+  if (index->scan_owned) index->dev->arena->release(index->scan, st);
+  index->dev->arena->release(index->rowinfo, st);
+  index->dev->arena->release(index->rscale, st);
+  release(index->owner);
+  delete index;
+}
+
+extern "C" int oxr_vector_topk(oxr_stream *s, const oxr_batch *input, int32_t column, const float *queries,
+                               int32_t nq, int32_t dim, int32_t metric, int32_t k, int64_t *out_rows,
+                               float *out_dist) {
+  clear_error();
+  if (!s) return fail(OXR_EPLAN, "rocm.vector", "null argument");
+  OXR_TRY(check_vector_column(input, column, "rocm.vector"));
+  if (dim != input->cols[column].dim) return fail(OXR_EPLAN, "rocm.vector", "query size differs from the column's");
+  oxr_vector_index *ix = nullptr;
+  OXR_TRY(make_index(s, input, column, metric, P_FP32, &ix));
+  const int rc = search(s, ix, queries, nq, k, k, out_rows, out_dist);
+  oxr_vector_index_free(ix);
+  return rc;
+}

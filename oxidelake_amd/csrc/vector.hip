@@ -259,13 +259,14 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(const PrepArgs a) {
 // (double-buffered). Verified lane layouts (32x32 instructions): A row and
 // B column = lane%32, both at the same k set for lane half l/32; result
 // element r of lane l is (row 8*(r/4) + 4*(l/32) + r%4, query l%32).
-// The scan writes raw dot products; the select kernel makes them scores.
 struct ScanArgs {
   const void *x;      // [n][dimp] in the scan precision
   const void *qm;     // [qpad][dimp]
   uint64_t n, tile0, tile_rows, nblocks;
-  int32_t dimp, groups, nq, pad;
-  float *scores;      // [nq][tile_rows]: x.q (FP8: of the scaled rows)
+  int32_t dimp, groups, nq, metric;
+  const float *rowinfo, *rscale, *qscale;
+  const uint32_t *valid;
+  float *scores;      // [nq][tile_rows]: scores ordered like the distance
 };
 
 template <int P>
@@ -316,7 +317,12 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   constexpr int QROW = 2 * U;                // units per query per group
   constexpr int BUNITS = NT * 32 * QROW;     // units of one group's query slice
   constexpr int PER_T = (BUNITS + kBlock - 1) / kBlock;
+  constexpr int BROWS = 4 * RT * 32;         // rows per block
   __shared__ uint4 s_b[2][BUNITS];
+  // the block's rows' norms, scales and validity, read once per block into
+  // LDS so the epilogue holds no row data in registers across the k loop
+  __shared__ float s_info[BROWS];
+  __shared__ float s_scale[BROWS];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 31, h = lane >> 5;
@@ -326,7 +332,20 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   const uint64_t tile_end = min(a.n, a.tile0 + a.tile_rows);
 
   for (uint64_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
-    const uint64_t rowbase = a.tile0 + blk * (uint64_t)(4 * RT * 32) + (uint64_t)wave * RT * 32;
+    const uint64_t blockrow = a.tile0 + blk * (uint64_t)BROWS;
+    const uint64_t rowbase = blockrow + (uint64_t)wave * RT * 32;
+    for (int t = tid; t < BROWS; t += kBlock) {
+      const uint64_t row = blockrow + t;
+      float info = -INFINITY, scl = 1.0f;  // -inf: the row cannot qualify
+      if (row < tile_end && valid_row(a.valid, row)) {
+        const float ri = a.rowinfo[row];
+        if (a.metric == OXR_COSINE) info = ri > 0.f ? ri : -INFINITY;
+        else info = ri;
+        if (P == P_FP8) scl = a.rscale[row];
+      }
+      s_info[t] = info;
+      s_scale[t] = scl;
+    }
     f32x16 acc[RT][NT];
 #pragma unroll
     for (int r = 0; r < RT; r++)
@@ -395,20 +414,35 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
       __syncthreads();
     }
 
-    // epilogue: raw dot products (the select kernel turns them into scores,
-    // so no row data is live across the k loop)
+    // epilogue: scores ordered like the distance (larger is nearer):
+    // L2 2 x.q - |x|^2, cosine x.q / |x|; -inf where a row cannot qualify
+    // (NULL, zero norm). Row data comes from LDS (filled before the k loop;
+    // the loop's barriers order it).
 #pragma unroll
     for (int r = 0; r < RT; r++) {
 #pragma unroll
       for (int t = 0; t < NT; t++) {
         const int q = t * 32 + i;
         if (q >= a.nq) continue;
+        const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
         float *out = a.scores + (uint64_t)q * a.tile_rows + (rowbase + r * 32 - a.tile0);
 #pragma unroll
         for (int e = 0; e < 16; e++) {
           const int m = 8 * (e / 4) + 4 * h + (e % 4);
-          if (rowbase + r * 32 + m < tile_end) out[m] = acc[r][t][e];
+          if (rowbase + r * 32 + m >= tile_end) continue;
+          const int br = wave * RT * 32 + r * 32 + m;
+          const float info = s_info[br];
+          float dot = acc[r][t][e];
+          if (P == P_FP8) dot = dot * s_scale[br] * qs;
+          float score;
+          if (info == -INFINITY) score = -INFINITY;  // the row cannot qualify
+          else if (a.metric == OXR_COSINE) score = dot * info;
+          else score = 2.0f * dot - info;
+          out[m] = score;
         }
+        // one (row tile, query tile) at a time: the scheduler may not batch
+        // every tile's LDS reads ahead (that would cost occupancy)
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
@@ -474,15 +508,26 @@ __device__ void top_c(F key_at, uint64_t len, int C, uint64_t *buf, int *count, 
     *tau = 0;
   }
   __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1;
   for (uint64_t base = 0; base < len; base += kChunk) {
     const uint64_t th = *tau;
+    uint64_t keys[kChunk / kBlock];
 #pragma unroll
     for (int j = 0; j < kChunk / kBlock; j++) {
       const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
-      if (idx < len) {
-        const uint64_t k = key_at(idx);
-        if (k > th) buf[atomicAdd(count, 1)] = k;
-      }
+      keys[j] = idx < len ? key_at(idx) : 0;
+    }
+    // wave-aggregated insertion: one LDS atomic per wave and step
+#pragma unroll
+    for (int j = 0; j < kChunk / kBlock; j++) {
+      const bool pass = keys[j] > th;
+      const uint64_t m = __ballot(pass);
+      if (m == 0) continue;
+      int at = 0;
+      if (lane == 0) at = atomicAdd(count, __popcll(m));
+      at = __shfl(at, 0, 64);
+      if (pass) buf[at + __popcll(m & below)] = keys[j];
     }
     __syncthreads();
     const int cnt = *count;
@@ -493,25 +538,13 @@ __device__ void top_c(F key_at, uint64_t len, int C, uint64_t *buf, int *count, 
 }
 
 struct SelectArgs {
-  const float *dots;    // [nq][tile_rows]
+  const float *scores;  // [nq][tile_rows]
   uint64_t tile0, tile_len, tile_rows, seg_rows;
   int32_t nq, C, seg_first, segs_total;
-  int32_t metric, precision;
-  const float *rowinfo, *rscale, *qscale;
-  const uint32_t *valid;
   uint64_t *cand;       // [nq][segs_total][C]
 };
 
-// The score of a row for a query, ordered like the distance (larger is
-// nearer): L2 2 x.q - |x|^2, cosine x.q / |x|; -inf for rows that cannot
-// qualify (NULL, zero norm).
-__device__ inline float row_score(const SelectArgs &a, float dot, uint64_t row, float qs) {
-  if (!valid_row(a.valid, row)) return -INFINITY;
-  if (a.precision == P_FP8) dot = dot * a.rscale[row] * qs;
-  const float info = a.rowinfo[row];
-  if (a.metric == OXR_COSINE) return info > 0.f ? dot * info : -INFINITY;
-  return 2.0f * dot - info;
-}
+
 
 // grid (segments of this tile, queries)
 __global__ __launch_bounds__(kBlock) void select_tile_kernel(const SelectArgs a) {
@@ -522,12 +555,11 @@ __global__ __launch_bounds__(kBlock) void select_tile_kernel(const SelectArgs a)
   const uint64_t lo = (uint64_t)blockIdx.x * a.seg_rows;
   if (lo >= a.tile_len) return;
   const uint64_t len = min(a.seg_rows, a.tile_len - lo);
-  const float *dp = a.dots + (uint64_t)q * a.tile_rows + lo;
+  const float *sp = a.scores + (uint64_t)q * a.tile_rows + lo;
   const uint64_t row0 = a.tile0 + lo;
-  const float qs = a.precision == P_FP8 ? a.qscale[q] : 1.0f;
   top_c(
       [&](uint64_t idx) -> uint64_t {
-        const float s = row_score(a, dp[idx], row0 + idx, qs);
+        const float s = sp[idx];
         return s > -INFINITY ? score_key(s, row0 + idx) : 0;
       },
       len, a.C, s_buf, &s_count, &s_tau);
@@ -754,14 +786,18 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
       a.dimp = ix->dimp;
       a.groups = ix->dimp / (ix->precision == P_FP8 ? 64 : ix->precision == P_BF16 ? 16 : 8);
       a.nq = nqc;
+      a.metric = ix->metric;
+      a.rowinfo = ix->rowinfo;
+      a.rscale = ix->rscale;
+      a.qscale = qscale;
+      a.valid = ix->valid;
       a.scores = scores;
       {
         ProfScope prof(st, "vector_scan");
         launch_scan(ix->precision, nt, dim3(grid_for(ix->dev, a.nblocks, nt >= 4 ? 2 : 8)), st, a);
         OXR_HIP(hipGetLastError());
       }
-      SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total,
-                    ix->metric, ix->precision, ix->rowinfo, ix->rscale, qscale, ix->valid, cand};
+      SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand};
       const uint64_t segs_here = (tlen + seg_rows - 1) / seg_rows;
       {
         ProfScope prof(st, "vector_select");

@@ -27,7 +27,6 @@ SOURCES = [
     "join.hip",
     "vector.hip",
     "datagen.hip",
-    "pending.hip",
     "jit.hip",
 ]
 

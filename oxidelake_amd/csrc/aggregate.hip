@@ -25,6 +25,7 @@
 
 #include "common.h"
 #include "expr.h"
+#include "join.h"
 
 namespace oxr {
 namespace {
@@ -107,6 +108,12 @@ struct Params {
   AggDesc aggs[kMaxAggs];
   Table t;
   uint32_t *error;   // division by zero
+  // fused inner join: the group key of a probe row is the build side's
+  // group column at each build row its key matches
+  int32_t join;           // 0: the key is key_slot of the row itself
+  int32_t probe_slot;     // join: slot of the probe key column
+  JTable jt;
+  DCol build_group;
 };
 
 __device__ inline void acc_apply_global(uint8_t kind, int64_t *addr, int64_t v) {
@@ -268,101 +275,111 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
     load_slots<V, NS>(p.cols, p.slot_cols, p.nslots, row0, p.n, S);
     uint32_t sel = eval_predicate<V, NS>(p.leaves, p.bprog, p.nb, p.ins, p.cols, S, p.magic, row0,
                                          p.n, &div0);
-    uint64_t key[V];
-    uint32_t kvalid = 0;  // one global group: the slot a NULL key would use
-#pragma unroll
-    for (int q = 0; q < V; q++) key[q] = 0;
-    if (p.key_slot >= 0) {
-      // a NULL key is a group of its own: validity only picks the slot
+    static_assert(V == 1, "one row per lane");
+    // the row's matches: itself, or the build rows its key joins
+    uint32_t nmatch = sel & 1u;
+    uint32_t jstart = 0;
+    if (p.join && nmatch) {
+      nmatch = 0;
 #pragma unroll
       for (int k = 0; k < NS; k++) {
-        if (p.key_slot == k) {
-#pragma unroll
-          for (int q = 0; q < V; q++) key[q] = S.v[k][q];
-          kvalid = S.m[k];
+        if (p.probe_slot == k && (S.m[k] & 1u)) {
+          const uint64_t js = join_find(p.jt, (int64_t)S.v[k][0]);
+          if (js != UINT64_MAX) {
+            jstart = p.jt.start[js];
+            nmatch = p.jt.count[js];
+          }
         }
       }
     }
-    if (sel) {
-      // LDS slots of this lane's selected rows (-1: go global)
-      int lslot[V];
+    // the key of a plain row (a NULL key is a group of its own: validity
+    // only picks the slot; one global group takes the NULL slot)
+    int64_t rkey = 0;
+    bool rkvalid = false;
+    if (!p.join && p.key_slot >= 0) {
 #pragma unroll
-      for (int q = 0; q < V; q++) {
-        lslot[q] = -1;
-        if (!((sel >> q) & 1)) continue;
-        if (la == 0) continue;
-        const bool kv = (kvalid >> q) & 1;
-        const int64_t k = (int64_t)key[q];
-        if (!kv) {
-          lslot[q] = kLdsSlots;
-          s_keys[kLdsSlots] = 0;  // marks the NULL-key slot used
-          continue;
-        }
-        if (k == kEmpty) {
-          lslot[q] = kLdsSlots + 1;
-          s_keys[kLdsSlots + 1] = 0;
-          continue;
-        }
-        uint32_t h = (uint32_t)fmix64((uint64_t)k) & (kLdsSlots - 1);
-        for (int pr = 0; pr < kLdsProbe; pr++) {
-          int64_t cur = s_keys[h];
-          if (cur == k) {
-            lslot[q] = (int)h;
-            break;
-          }
-          if (cur == kEmpty) {
-            unsigned long long old = atomicCAS((unsigned long long *)&s_keys[h],
-                                               (unsigned long long)kEmpty, (unsigned long long)k);
-            if ((int64_t)old == kEmpty) {
-              atomicAdd(&s_used, 1u);
-              lslot[q] = (int)h;
-              break;
-            }
-            if ((int64_t)old == k) {
-              lslot[q] = (int)h;
-              break;
-            }
-          }
-          h = (h + 1) & (kLdsSlots - 1);
+      for (int k = 0; k < NS; k++) {
+        if (p.key_slot == k) {
+          rkey = (int64_t)S.v[k][0];
+          rkvalid = S.m[k] & 1u;
         }
       }
-      // global slots for rows the block table could not take
-      uint64_t gslot[V];
-      bool need_global = false;
+    }
+    // aggregate inputs, once per row
+    uint64_t av[kMaxAggs];
+    uint32_t avm = 0;
+    if (nmatch) {
 #pragma unroll
-      for (int q = 0; q < V; q++) need_global |= ((sel >> q) & 1) && lslot[q] < 0;
-      const bool all_global = la < p.naggs;
-#pragma unroll
-      for (int q = 0; q < V; q++) {
-        gslot[q] = 0;
-        if (((sel >> q) & 1) && (lslot[q] < 0 || all_global))
-          gslot[q] = slot_of(p.t, (kvalid >> q) & 1, (int64_t)key[q], &ok);
-      }
-      (void)need_global;
-      for (int a = 0; a < p.naggs; a++) {
+      for (int a = 0; a < kMaxAggs; a++) {
+        if (a >= p.naggs) break;
         const AggDesc d = p.aggs[a];
-        uint64_t v[V];
-        uint32_t vm = (1u << V) - 1;
-        if (d.kind == A_COUNT_STAR) {
-#pragma unroll
-          for (int q = 0; q < V; q++) v[q] = 0;
+        uint64_t v[V] = {0};
+        uint32_t vm = 1;
+        if (d.kind != A_COUNT_STAR) eval_value<V, NS>(d.vp, p.ins, S, p.magic, v, vm, &div0);
+        av[a] = v[0];
+        avm |= (vm & 1u) << a;
+      }
+    }
+    for (uint32_t mi = 0; mi < nmatch; mi++) {
+      int64_t k = rkey;
+      bool kv = rkvalid;
+      if (p.join) {
+        const uint64_t b = p.jt.rows[jstart + mi];
+        const DCol &g = p.build_group;
+        kv = !g.valid || ((g.valid[b >> 5] >> (b & 31)) & 1u);
+        k = g.type == OXR_INT32 ? (int64_t)((const int32_t *)g.values)[b] : ((const int64_t *)g.values)[b];
+      }
+      // the block table's slot (-1: go global)
+      int lslot = -1;
+      if (la) {
+        if (!kv) {
+          lslot = kLdsSlots;
+          s_keys[kLdsSlots] = 0;  // marks the NULL-key slot used
+        } else if (k == kEmpty) {
+          lslot = kLdsSlots + 1;
+          s_keys[kLdsSlots + 1] = 0;
         } else {
-          eval_value<V, NS>(d.vp, p.ins, S, p.magic, v, vm, &div0);
-        }
-#pragma unroll
-        for (int q = 0; q < V; q++) {
-          if (!((sel >> q) & 1)) continue;
-          const bool valid = (vm >> q) & 1;
-          if (!valid && d.kind != A_COUNT_STAR) continue;
-          const int64_t c = contribution(d.kind, d.in_type, v[q]);
-          if (a < la && lslot[q] >= 0) {
-            acc_apply_lds(d.kind, &s_acc[a][lslot[q]], c);
-            if (d.nullable) s_has[a][lslot[q]] = 1;
-          } else {
-            if (!ok) continue;
-            acc_apply_global(d.kind, &p.t.acc[a][gslot[q]], c);
-            if (p.t.has[a]) p.t.has[a][gslot[q]] = 1;
+          uint32_t h = (uint32_t)fmix64((uint64_t)k) & (kLdsSlots - 1);
+          for (int pr = 0; pr < kLdsProbe; pr++) {
+            const int64_t cur = s_keys[h];
+            if (cur == k) {
+              lslot = (int)h;
+              break;
+            }
+            if (cur == kEmpty) {
+              const unsigned long long old = atomicCAS((unsigned long long *)&s_keys[h],
+                                                       (unsigned long long)kEmpty, (unsigned long long)k);
+              if ((int64_t)old == kEmpty) {
+                atomicAdd(&s_used, 1u);
+                lslot = (int)h;
+                break;
+              }
+              if ((int64_t)old == k) {
+                lslot = (int)h;
+                break;
+              }
+            }
+            h = (h + 1) & (kLdsSlots - 1);
           }
+        }
+      }
+      // the HBM slot, for rows the block table could not take (or for
+      // aggregates past the ones it holds)
+      uint64_t gslot = 0;
+      if (lslot < 0 || la < p.naggs) gslot = slot_of(p.t, kv, k, &ok);
+#pragma unroll
+      for (int a = 0; a < kMaxAggs; a++) {
+        if (a >= p.naggs) break;
+        const AggDesc d = p.aggs[a];
+        if (!((avm >> a) & 1u) && d.kind != A_COUNT_STAR) continue;
+        const int64_t c = contribution(d.kind, d.in_type, av[a]);
+        if (a < la && lslot >= 0) {
+          acc_apply_lds(d.kind, &s_acc[a][lslot], c);
+          if (d.nullable) s_has[a][lslot] = 1;
+        } else {
+          if (!ok) continue;
+          acc_apply_global(d.kind, &p.t.acc[a][gslot], c);
+          if (p.t.has[a]) p.t.has[a][gslot] = 1;
         }
       }
     }
@@ -511,7 +528,7 @@ using namespace oxr;
 struct oxr_agg_state {
   oxr_device *dev = nullptr;
   oxr::StreamCore *owner = nullptr;
-  int32_t key_col = -1;
+  int32_t key_col = -1;  // -1: one global group; -2: keys from a join's build side
   int32_t key_slot = -1;
   int32_t key_type = OXR_INT64;
   std::string key_name;
@@ -677,7 +694,15 @@ int compile_spec(const oxr_batch *like, const oxr_agg_spec *spec, oxr_agg_state 
   return OXR_OK;
 }
 
-int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in) {
+// The join an update folds through (oxr_agg_update_join), or none.
+struct JoinSide {
+  const oxr_join_table *table = nullptr;
+  const oxr_batch *build = nullptr;
+  int32_t build_group = -1;
+  int32_t probe_slot = -1;
+};
+
+int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const JoinSide *js = nullptr) {
   if (in->cols.size() != st->col_types.size())
     return fail(OXR_EPLAN, "rocm.aggregate", "batch does not match the aggregation's schema");
   for (size_t i = 0; i < in->cols.size(); i++)
@@ -686,7 +711,8 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in) {
   const uint64_t n = in->rows;
   if (n == 0) return OXR_OK;
   // keep the load at most one half for every key this batch might add
-  uint64_t more = st->key_col < 0 ? 1 : n;
+  uint64_t more = st->key_col == -1 ? 1 : n;
+  if (js) more = std::min<uint64_t>(more, std::max<uint64_t>(js->table->nrows, 1));
   if (st->hint) more = std::min<uint64_t>(more, st->hint > st->groups ? st->hint - st->groups : 16);
   OXR_TRY(ensure_capacity(s, st, more));
 
@@ -715,6 +741,15 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in) {
   P.lds_aggs = P.naggs <= kLdsAggs ? P.naggs : 0;
   for (int a = 0; a < P.naggs; a++) P.aggs[a] = st->aggs[a];
   P.t = st->t;
+  if (js) {
+    const DevCol &g = js->build->cols[js->build_group];
+    P.join = 1;
+    P.probe_slot = js->probe_slot;
+    P.jt = js->table->t;
+    P.build_group.values = g.values;
+    P.build_group.valid = (const uint32_t *)g.validity;
+    P.build_group.type = g.type;
+  }
   void *err = nullptr;
   OXR_TRY(stream_alloc(s, 64, &err));
   P.error = (uint32_t *)err;
@@ -799,7 +834,7 @@ int run_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {
     hipError_t e = hipStreamSynchronize(s->s);
     if (e != hipSuccess) rc = fail_hip(e, "aggregate finish");
   }
-  const bool global_only = st->key_col < 0;
+  const bool global_only = st->key_col == -1;
   const int has_null = seen[0] ? 1 : 0;
   const int has_min = seen[1] ? 1 : 0;
   uint64_t total = g + has_null + has_min;
@@ -910,7 +945,46 @@ int oxr_agg_create(oxr_stream *s, const oxr_batch *like, const oxr_agg_spec *spe
 int oxr_agg_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *input) {
   clear_error();
   if (!s || !st || !input) return fail(OXR_EPLAN, "rocm.aggregate", "null argument");
+  if (st->key_col == -2)
+    return fail(OXR_EPLAN, "rocm.aggregate", "a state takes either plain or joined updates");
   return run_update(s, st, input);
+}
+
+int oxr_agg_update_join(oxr_stream *s, oxr_agg_state *st, const oxr_join_table *table,
+                        const oxr_batch *build, int32_t build_group, const oxr_batch *probe,
+                        int32_t probe_key) {
+  clear_error();
+  if (!s || !st || !table || !build || !probe) return fail(OXR_EPLAN, "rocm.aggregate", "null argument");
+  if (table->dev != s->dev) return fail(OXR_EDEVICE, "rocm.aggregate", "join table of another device");
+  if (table->build_rows != build->rows)
+    return fail(OXR_EPLAN, "rocm.aggregate", "the build batch is not the one the table was built over");
+  if (build_group < 0 || build_group >= (int)build->cols.size())
+    return fail(OXR_EPLAN, "rocm.aggregate", "build group column out of range");
+  const int32_t gt = build->cols[build_group].type;
+  if (gt != OXR_INT64 && gt != OXR_INT32)
+    return fail(OXR_EUNSUPPORTED, "cpu.int64_column", "group key column has a type other than Int64/Int32");
+  if (probe_key < 0 || probe_key >= (int)probe->cols.size())
+    return fail(OXR_EPLAN, "rocm.aggregate", "probe key column out of range");
+  const int32_t pt = probe->cols[probe_key].type;
+  if (pt != OXR_INT64 && pt != OXR_INT32)
+    return fail(OXR_EUNSUPPORTED, "cpu.int64_column", "join key column has a type other than Int64/Int32");
+  // the groups are the build side's: the state's own key (if any) is unused
+  if (st->groups == 0 && st->key_col != -2) {
+    st->key_type = gt;
+    st->key_name = build->names[build_group];
+  } else if (st->key_col != -2 || st->key_type != gt) {
+    return fail(OXR_EPLAN, "rocm.aggregate", "a state takes either plain or joined updates, of one key type");
+  }
+  st->key_col = -2;  // joined: the key column comes from the build side
+  JoinSide js;
+  js.table = table;
+  js.build = build;
+  js.build_group = build_group;
+  js.probe_slot = st->prog.slot_of(probe_key);
+  if (js.probe_slot < 0)
+    return fail(OXR_EUNSUPPORTED, "rocm.aggregate",
+                "an operator reads at most " + std::to_string(kMaxSlots) + " distinct columns");
+  return run_update(s, st, probe, &js);
 }
 
 int oxr_agg_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {

@@ -343,6 +343,16 @@ int oxr_agg_update_join(oxr_stream *s, oxr_agg_state *st,
 int oxr_agg_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out);
 void oxr_agg_free(oxr_agg_state *st);
 
+/* The aggregation kernel source generated for this spec over columns of the
+ * given types and nullability (no device needed; no counterpart in the
+ * reference, whose aggregation kernels are fixed, cuda/ops.rs:623-1161).
+ * probe_key >= 0: the kernel of oxr_agg_update_join with that probe key
+ * column (a unique-key join). OXR_EUNSUPPORTED: the interpreter kernel runs
+ * this shape. Output as oxr_filter_kernel_source. */
+int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec,
+                                int32_t probe_key, int32_t compile, char *buf, size_t cap,
+                                size_t *len);
+
 /* --------------------------------------------------------- vector distance */
 enum { OXR_L2 = 1, OXR_COSINE = 2 }; /* DistanceMetric, params.rs:151-157 */
 

@@ -152,6 +152,11 @@ SIGNATURES: dict[str, tuple] = {
     "oxr_agg_update_join": (C.c_int, [P, P, P, P, I32, P, I32]),
     "oxr_agg_finish": (C.c_int, [P, P, PP]),
     "oxr_agg_free": (None, [P]),
+    "oxr_aggregate_kernel_source": (
+        C.c_int,
+        [C.POINTER(Column), I32, C.POINTER(AggSpec), I32, I32, C.c_char_p, C.c_size_t,
+         C.POINTER(C.c_size_t)],
+    ),
     "oxr_vector_distance": (
         C.c_int,
         [P, P, I32, C.POINTER(C.c_float), I32, I32, C.c_char_p, PP],

@@ -212,6 +212,27 @@ def filter_kernel_source(columns: Sequence[tuple], predicate: ExprT | None, proj
     return buf.value.decode()
 
 
+def aggregate_kernel_source(columns: Sequence[tuple], spec: AggregateSpec, probe_key: int = -1,
+                            compile: bool = False) -> str:
+    """The aggregation kernel generated for `spec` over columns given as
+    (type, nullable) (probe_key >= 0: the fused join's kernel). No device is
+    needed. Raises Unsupported when the interpreter kernel runs the shape."""
+    cols = (Column * max(1, len(columns)))()
+    for i, c in enumerate(columns):
+        typ, dim, nullable = (c[0], 0, c[1]) if len(c) == 2 else c
+        cols[i].type = typ
+        cols[i].dim = dim
+        cols[i].validity = 1 if nullable else None
+    cspec, keep = _agg_spec(spec)
+    n = C.c_size_t()
+    check(lib.oxr_aggregate_kernel_source(cols, len(columns), C.byref(cspec), probe_key, 0, None, 0,
+                                          C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib.oxr_aggregate_kernel_source(cols, len(columns), C.byref(cspec), probe_key,
+                                          1 if compile else 0, buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
 def _cstrs(names: Optional[Sequence[str]], n: int):
     if names is None:
         return None, None

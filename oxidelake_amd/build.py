@@ -32,7 +32,7 @@ SOURCES = [
 
 # Device headers the generated kernels include; embedded into the library
 # as strings (jit_sources.inc) for hipRTC.
-JIT_HEADERS = ["divlit.h", "scalar.h", "compact.h", "filter_jit.h"]
+JIT_HEADERS = ["divlit.h", "scalar.h", "compact.h", "filter_jit.h", "jtable.h", "aggregate_jit.h"]
 
 # Sources whose float arithmetic must not be contracted into FMA: the
 # filter and aggregate evaluate SQL expressions, where a*b+c is two roundings.

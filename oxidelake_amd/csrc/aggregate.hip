@@ -26,6 +26,8 @@
 #include "common.h"
 #include "expr.h"
 #include "join.h"
+#include "aggregate_jit.h"
+#include "jit.h"
 
 namespace oxr {
 namespace {
@@ -33,46 +35,6 @@ namespace {
 constexpr int V = 1;  // rows per lane (register use grows with it)
 constexpr int kBlock = 256;
 constexpr int kTileRows = kBlock * V;
-constexpr int kMaxAggs = 8;
-constexpr int kLdsSlots = 1024;       // block table slots (power of two)
-constexpr int kLdsAggs = 4;           // aggregates pre-aggregated in LDS
-constexpr int kLdsProbe = 8;          // LDS probe limit before going global
-constexpr int64_t kEmpty = INT64_MIN; // empty-slot marker
-
-enum : uint8_t {
-  A_COUNT = 1,
-  A_SUM_I64,
-  A_SUM_F64,
-  A_MIN_I64,
-  A_MAX_I64,
-  A_MIN_F64,  // over totalOrder keys of f64
-  A_MAX_F64,
-  A_MIN_F32,  // over totalOrder keys of f32 (sign-extended)
-  A_MAX_F32,
-  A_COUNT_STAR
-};
-
-__host__ __device__ inline int64_t acc_init(uint8_t kind) {
-  switch (kind) {
-    case A_MIN_I64:
-    case A_MIN_F64:
-    case A_MIN_F32: return INT64_MAX;
-    case A_MAX_I64:
-    case A_MAX_F64:
-    case A_MAX_F32: return INT64_MIN;
-  }
-  return 0;
-}
-
-__host__ __device__ inline uint64_t fmix64(uint64_t h) {
-  // MurmurHash3 finalizer
-  h ^= h >> 33;
-  h *= 0xff51afd7ed558ccdull;
-  h ^= h >> 33;
-  h *= 0xc4ceb9fe1a85ec53ull;
-  h ^= h >> 33;
-  return h;
-}
 
 struct AggDesc {
   uint8_t kind;
@@ -80,16 +42,6 @@ struct AggDesc {
   uint8_t nullable;  // input can be NULL (then `has` flags are kept)
   uint8_t pad;
   ValueProg vp;      // the input: a slot or a program
-};
-
-struct Table {
-  uint64_t cap;       // power of two; slots cap (NULL key) and cap+1 (INT64_MIN key) follow
-  int64_t *keys;      // cap+2
-  int64_t *acc[kMaxAggs];
-  uint32_t *has[kMaxAggs];  // cap+2 each, or NULL
-  uint32_t *seen;     // [0] NULL-key group, [1] INT64_MIN group
-  uint64_t *distinct; // new keys claimed
-  uint32_t *overflow; // a key found no slot
 };
 
 struct Params {
@@ -115,105 +67,6 @@ struct Params {
   JTable jt;
   DCol build_group;
 };
-
-__device__ inline void acc_apply_global(uint8_t kind, int64_t *addr, int64_t v) {
-  switch (kind) {
-    case A_COUNT:
-    case A_COUNT_STAR:
-    case A_SUM_I64:
-      atomicAdd((unsigned long long *)addr, (unsigned long long)v);
-      break;
-    case A_SUM_F64:
-      unsafeAtomicAdd((double *)addr, as_f64((uint64_t)v));
-      break;
-    case A_MIN_I64:
-    case A_MIN_F64:
-    case A_MIN_F32:
-      atomicMin((long long *)addr, (long long)v);
-      break;
-    default:
-      atomicMax((long long *)addr, (long long)v);
-      break;
-  }
-}
-
-__device__ inline void acc_apply_lds(uint8_t kind, int64_t *addr, int64_t v) {
-  switch (kind) {
-    case A_COUNT:
-    case A_COUNT_STAR:
-    case A_SUM_I64:
-      atomicAdd((unsigned long long *)addr, (unsigned long long)v);
-      break;
-    case A_SUM_F64:
-      unsafeAtomicAdd((double *)addr, as_f64((uint64_t)v));
-      break;
-    case A_MIN_I64:
-    case A_MIN_F64:
-    case A_MIN_F32:
-      atomicMin((long long *)addr, (long long)v);
-      break;
-    default:
-      atomicMax((long long *)addr, (long long)v);
-      break;
-  }
-}
-
-// The per-row contribution of a value of `in_type` to an accumulator.
-__device__ inline int64_t contribution(uint8_t kind, uint8_t in_type, uint64_t v) {
-  switch (kind) {
-    case A_COUNT:
-    case A_COUNT_STAR: return 1;
-    case A_SUM_I64:
-    case A_MIN_I64:
-    case A_MAX_I64: return (int64_t)v;  // ints arrive sign-extended
-    case A_SUM_F64:
-      return (int64_t)(in_type == OXR_FLOAT32 ? f64_bits((double)as_f32(v)) : v);
-    case A_MIN_F64:
-    case A_MAX_F64: {
-      // totalOrder without zero normalisation (kernels.rs:527-538)
-      int64_t b = (int64_t)v;
-      return b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1);
-    }
-    default: {
-      int32_t b = (int32_t)(uint32_t)v;
-      return (int64_t)(b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1));
-    }
-  }
-}
-
-// Finds (or claims) the HBM slot of `key` (not NULL, not INT64_MIN).
-__device__ inline uint64_t global_slot(const Table &t, int64_t key, bool *ok) {
-  const uint64_t mask = t.cap - 1;
-  uint64_t h = fmix64((uint64_t)key) & mask;
-  for (uint64_t probe = 0; probe < t.cap; probe++) {
-    int64_t cur = __hip_atomic_load(&t.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == key) return h;
-    if (cur == kEmpty) {
-      unsigned long long old = atomicCAS((unsigned long long *)&t.keys[h],
-                                         (unsigned long long)kEmpty, (unsigned long long)key);
-      if ((int64_t)old == kEmpty) {
-        atomicAdd((unsigned long long *)t.distinct, 1ull);
-        return h;
-      }
-      if ((int64_t)old == key) return h;
-    }
-    h = (h + 1) & mask;
-  }
-  *ok = false;
-  return 0;
-}
-
-__device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, bool *ok) {
-  if (!key_valid) {
-    t.seen[0] = 1;
-    return t.cap;
-  }
-  if (key == kEmpty) {
-    t.seen[1] = 1;
-    return t.cap + 1;
-  }
-  return global_slot(t, key, ok);
-}
 
 template <int NS>
 __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
@@ -541,6 +394,15 @@ struct oxr_agg_state {
   Table t{};
   void *mem = nullptr;  // one arena block holding the table
   uint64_t groups = 0;  // distinct ordinary keys seen (after the last update)
+  // the spec as expressions, for the generated kernels
+  std::vector<oxr_expr_node> filter_nodes;
+  std::vector<std::vector<oxr_expr_node>> input_nodes;
+  std::vector<int32_t> funcs;
+  // the group of every slot of the join table last folded through
+  void *gos = nullptr;
+  bool gos32 = true;
+  const void *gos_table = nullptr, *gos_build = nullptr;
+  int32_t gos_col = -1;
 };
 
 namespace oxr {
@@ -650,6 +512,16 @@ int compile_spec(const oxr_batch *like, const oxr_agg_spec *spec, oxr_agg_state 
     st->key_slot = st->prog.slot_of(spec->group_by);
   }
   OXR_TRY(compile_predicate(spec->filter, like->cols, &st->prog));
+  if (spec->filter && spec->filter->count > 0 && spec->filter->nodes)
+    st->filter_nodes.assign(spec->filter->nodes, spec->filter->nodes + spec->filter->count);
+  for (int i = 0; i < spec->naggs; i++) {
+    const oxr_agg &g = spec->aggs[i];
+    st->funcs.push_back(g.func);
+    if (g.func != OXR_COUNT_STAR && g.input.nodes && g.input.count > 0)
+      st->input_nodes.emplace_back(g.input.nodes, g.input.nodes + g.input.count);
+    else
+      st->input_nodes.emplace_back();
+  }
   for (int i = 0; i < spec->naggs; i++) {
     const oxr_agg &g = spec->aggs[i];
     AggDesc d{};
@@ -694,13 +566,114 @@ int compile_spec(const oxr_batch *like, const oxr_agg_spec *spec, oxr_agg_state 
   return OXR_OK;
 }
 
+// The group of every slot of a unique-key join table: Int32 groups as
+// (state << 32) | group, Int64 groups as {group, state}; state 0 = no build
+// row, 1 = a group, 2 = the NULL group.
+__global__ void gos_kernel(JTable t, DCol grp, uint64_t nslots, uint64_t *gos32, uint64_t *gos64) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  uint64_t state = 0;
+  int64_t g = 0;
+  if (t.count[i]) {
+    const uint64_t b = t.rows[t.start[i]];
+    const bool v = !grp.valid || ((grp.valid[b >> 5] >> (b & 31)) & 1u);
+    g = grp.type == OXR_INT32 ? (int64_t)((const int32_t *)grp.values)[b] : ((const int64_t *)grp.values)[b];
+    state = v ? 1 : 2;
+    if (!v) g = 0;
+  }
+  if (gos32) gos32[i] = (state << 32) | (uint32_t)g;
+  else {
+    gos64[2 * i] = (uint64_t)g;
+    gos64[2 * i + 1] = state;
+  }
+}
+
 // The join an update folds through (oxr_agg_update_join), or none.
 struct JoinSide {
   const oxr_join_table *table = nullptr;
   const oxr_batch *build = nullptr;
   int32_t build_group = -1;
   int32_t probe_slot = -1;
+  int32_t probe_key = -1;
 };
+
+// The generator's view of an aggregation state: with probe_key >= 0 the key
+// comes through a join on that probe column, with -2 from the state's own key.
+struct ShapeOf {
+  AggJitShape shape;
+  oxr_expr fexpr;
+  std::vector<oxr_expr> inputs;
+  ShapeOf(const oxr_agg_state *st, int32_t probe_key) : inputs(st->aggs.size()) {
+    fexpr = oxr_expr{st->filter_nodes.data(), (int32_t)st->filter_nodes.size()};
+    shape.filter = st->filter_nodes.empty() ? nullptr : &fexpr;
+    shape.join = probe_key >= 0;
+    shape.key_col = shape.join ? probe_key : st->key_col;
+    for (size_t a = 0; a < st->aggs.size(); a++) {
+      shape.kinds.push_back(st->aggs[a].kind);
+      shape.nullable.push_back(st->aggs[a].nullable);
+      inputs[a] = oxr_expr{st->input_nodes[a].data(), (int32_t)st->input_nodes[a].size()};
+      shape.inputs.push_back(st->aggs[a].kind == A_COUNT_STAR ? nullptr : &inputs[a]);
+    }
+  }
+};
+
+// The update as the kernel generated for this aggregation's shape, when the
+// generator covers it (a plain key or one global group, or a join on unique
+// build keys). OXR_EUNSUPPORTED: the interpreter kernel runs it.
+int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const JoinSide *js,
+                         uint32_t *err) {
+  if (tune_get("agg.jit", 1) == 0 || st->aggs.empty() || (js && !js->table->t.unique))
+    return OXR_EUNSUPPORTED;
+  ShapeOf so(st, js ? js->probe_key : -2);
+  JitAgg ja;
+  const AggJitShape &sh = so.shape;
+  const int jrc = jit_aggregate(s->dev, in->cols, sh, &ja);
+  if (jrc != OXR_OK) return jrc;
+  AggJitArgs A;
+  std::memset(&A, 0, sizeof A);
+  A.n = in->rows;
+  for (size_t i = 0; i < in->cols.size(); i++) {
+    A.col[i] = in->cols[i].values;
+    A.valid[i] = (const uint32_t *)in->cols[i].validity;
+  }
+  A.t = st->t;
+  A.error = err;
+  void *gos = nullptr;
+  if (js) {
+    // the group of every slot of the join table
+    const JTable &t = js->table->t;
+    const DevCol &g = js->build->cols[js->build_group];
+    const uint64_t nslots = t.slots + 1;
+    const bool g32 = g.type == OXR_INT32;
+    OXR_TRY(stream_alloc(s, nslots * (g32 ? 8 : 16), &gos));
+    DCol gc;
+    gc.values = g.values;
+    gc.valid = (const uint32_t *)g.validity;
+    gc.type = g.type;
+    gc.dim = 0;
+    hipLaunchKernelGGL(gos_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s->s, t, gc, nslots,
+                       g32 ? (uint64_t *)gos : nullptr, g32 ? nullptr : (uint64_t *)gos);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      stream_release(s, gos);
+      return fail_hip(e, "gos_kernel launch");
+    }
+    A.jt = t;
+    A.gos = g32 ? (const uint64_t *)gos : nullptr;
+    A.gos64 = g32 ? nullptr : (const uint64_t *)gos;
+  }
+  const uint64_t tile = 256ull * ja.rows_per_lane;
+  const uint64_t ntiles = (in->rows + tile - 1) / tile;
+  const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.jit_grid", 4));
+  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
+  int rc;
+  {
+    ProfScope prof(s->s, "aggregate");
+    rc = jit_launch_raw(ja.fn, (unsigned)grid, s->s, &A, sizeof A);
+  }
+  stream_release(s, gos);
+  return rc;
+}
 
 int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const JoinSide *js = nullptr) {
   if (in->cols.size() != st->col_types.size())
@@ -757,8 +730,14 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   const uint64_t ntiles = (n + kTileRows - 1) / kTileRows;
   const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.grid", 8));
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
+  bool generated = false;
   if (rc == OXR_OK) {
-    ProfScope prof(s->s, "aggregate");
+    const int grc = run_update_generated(s, st, in, js, (uint32_t *)err);
+    if (grc == OXR_OK) generated = true;
+    else if (grc != OXR_EUNSUPPORTED) rc = grc;
+  }
+  if (rc == OXR_OK && !generated) {
+    ProfScope prof(s->s, "aggregate_vm");
     switch (slot_bucket(P.nslots)) {
       case 2: hipLaunchKernelGGL(aggregate_kernel<2>, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P); break;
       case 4: hipLaunchKernelGGL(aggregate_kernel<4>, dim3((unsigned)grid), dim3(kBlock), 0, s->s, P); break;
@@ -981,6 +960,7 @@ int oxr_agg_update_join(oxr_stream *s, oxr_agg_state *st, const oxr_join_table *
   js.build = build;
   js.build_group = build_group;
   js.probe_slot = st->prog.slot_of(probe_key);
+  js.probe_key = probe_key;
   if (js.probe_slot < 0)
     return fail(OXR_EUNSUPPORTED, "rocm.aggregate",
                 "an operator reads at most " + std::to_string(kMaxSlots) + " distinct columns");
@@ -1012,3 +992,36 @@ int oxr_aggregate(oxr_stream *s, const oxr_batch *input, const oxr_agg_spec *spe
 }
 
 }  // extern "C"
+
+extern "C" int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec,
+                                           int32_t probe_key, int32_t compile, char *buf, size_t cap,
+                                           size_t *len) {
+  clear_error();
+  if (!spec || ncols < 0 || (ncols > 0 && !cols) || (spec->naggs > 0 && !spec->aggs))
+    return fail(OXR_EPLAN, "rocm.jit", "null argument");
+  oxr_batch like;
+  for (int i = 0; i < ncols; i++) {
+    DevCol c;
+    c.type = cols[i].type;
+    c.dim = cols[i].dim;
+    c.validity = cols[i].validity;  // only NULL or not matters
+    like.cols.push_back(c);
+    like.names.push_back("c" + std::to_string(i));
+  }
+  oxr_agg_state st;
+  OXR_TRY(compile_spec(&like, spec, &st));
+  if (probe_key >= ncols) return fail(OXR_EPLAN, "rocm.jit", "probe key column out of range");
+  ShapeOf so(&st, probe_key >= 0 ? probe_key : -2);
+  std::string src;
+  const int rc = jit_aggregate_source(like.cols, so.shape, compile != 0, &src);
+  if (rc == OXR_EUNSUPPORTED)
+    return fail(OXR_EUNSUPPORTED, "rocm.jit", "outside what generated kernels cover (the interpreter runs it)");
+  OXR_TRY(rc);
+  if (len) *len = src.size();
+  if (buf && cap) {
+    const size_t n = std::min(cap - 1, src.size());
+    std::memcpy(buf, src.data(), n);
+    buf[n] = 0;
+  }
+  return OXR_OK;
+}

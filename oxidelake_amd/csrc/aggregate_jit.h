@@ -1,0 +1,388 @@
+// Grouped aggregation on the device: the HBM group table, the accumulators,
+// and the skeleton of the aggregation kernels that jit.hip generates per
+// aggregation shape and compiles at run time with hipRTC. Shared with the
+// interpreter kernel in aggregate.hip; compiles under hipcc and hipRTC.
+//
+// Semantics (crates/oxidelake-device/src/cpu/kernels.rs:478-818): a NULL key
+// is a group of its own; SUM/MIN/MAX skip NULL inputs; COUNT counts non-NULL
+// inputs; SUM over integers wraps in Int64; SUM over floats is an FP64 sum
+// from +0.0; MIN/MAX of floats compare totalOrder keys (kernels.rs:820-838).
+#pragma once
+
+#include "jtable.h"
+
+namespace oxr {
+
+constexpr int kMaxAggs = 8;
+constexpr int kLdsSlots = 1024;       // block table slots (power of two)
+constexpr int kLdsAggs = 4;           // aggregates pre-aggregated in LDS
+constexpr int kLdsProbe = 8;          // LDS probe limit before going global
+constexpr int64_t kEmpty = kI64Min;   // empty-slot marker
+constexpr int64_t kI64Max = (int64_t)(~(uint64_t)0 >> 1);
+
+enum : uint8_t {
+  A_COUNT = 1,
+  A_SUM_I64,
+  A_SUM_F64,
+  A_MIN_I64,
+  A_MAX_I64,
+  A_MIN_F64,  // over totalOrder keys of f64
+  A_MAX_F64,
+  A_MIN_F32,  // over totalOrder keys of f32 (sign-extended)
+  A_MAX_F32,
+  A_COUNT_STAR
+};
+
+__host__ __device__ inline int64_t acc_init(uint8_t kind) {
+  switch (kind) {
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32: return kI64Max;
+    case A_MAX_I64:
+    case A_MAX_F64:
+    case A_MAX_F32: return kI64Min;
+  }
+  return 0;
+}
+
+__host__ __device__ inline uint64_t fmix64(uint64_t h) { return join_hash(h); }
+
+// The HBM group table: open addressing, one 64-bit CAS claims a key.
+struct Table {
+  uint64_t cap;       // power of two; slots cap (NULL key) and cap+1 (INT64_MIN key) follow
+  int64_t *keys;      // cap+2
+  int64_t *acc[kMaxAggs];
+  uint32_t *has[kMaxAggs];  // cap+2 each, or NULL
+  uint32_t *seen;     // [0] NULL-key group, [1] INT64_MIN group
+  uint64_t *distinct; // new keys claimed
+  uint32_t *overflow; // a key found no slot
+};
+
+__device__ inline void acc_apply_global(uint8_t kind, int64_t *addr, int64_t v) {
+  switch (kind) {
+    case A_COUNT:
+    case A_COUNT_STAR:
+    case A_SUM_I64:
+      atomicAdd((unsigned long long *)addr, (unsigned long long)v);
+      break;
+    case A_SUM_F64:
+      unsafeAtomicAdd((double *)addr, as_f64((uint64_t)v));
+      break;
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32:
+      atomicMin((long long *)addr, (long long)v);
+      break;
+    default:
+      atomicMax((long long *)addr, (long long)v);
+      break;
+  }
+}
+
+__device__ inline void acc_apply_lds(uint8_t kind, int64_t *addr, int64_t v) {
+  switch (kind) {
+    case A_COUNT:
+    case A_COUNT_STAR:
+    case A_SUM_I64:
+      atomicAdd((unsigned long long *)addr, (unsigned long long)v);
+      break;
+    case A_SUM_F64:
+      unsafeAtomicAdd((double *)addr, as_f64((uint64_t)v));
+      break;
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32:
+      atomicMin((long long *)addr, (long long)v);
+      break;
+    default:
+      atomicMax((long long *)addr, (long long)v);
+      break;
+  }
+}
+
+// The per-row contribution of a value of `in_type` (widened to 64 bits:
+// ints sign-extended, floats as their bits) to an accumulator.
+__device__ inline int64_t contribution(uint8_t kind, uint8_t in_type, uint64_t v) {
+  switch (kind) {
+    case A_COUNT:
+    case A_COUNT_STAR: return 1;
+    case A_SUM_I64:
+    case A_MIN_I64:
+    case A_MAX_I64: return (int64_t)v;
+    case A_SUM_F64:
+      return (int64_t)(in_type == 3 /* OXR_FLOAT32 */ ? f64_bits((double)as_f32(v)) : v);
+    case A_MIN_F64:
+    case A_MAX_F64: {
+      // totalOrder without zero normalisation (kernels.rs:527-538)
+      const int64_t b = (int64_t)v;
+      return b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1);
+    }
+    default: {
+      const int32_t b = (int32_t)(uint32_t)v;
+      return (int64_t)(b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1));
+    }
+  }
+}
+
+// MIN/MAX of floats compare these keys (totalOrder, no zero normalisation).
+__device__ inline int64_t total_order64(double x) {
+  const int64_t b = (int64_t)f64_bits(x);
+  return b ^ (int64_t)(((uint64_t)(b >> 63)) >> 1);
+}
+__device__ inline int64_t total_order32(float x) {
+  const int32_t b = (int32_t)(uint32_t)f32_bits(x);
+  return (int64_t)(b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1));
+}
+
+// Finds (or claims) the HBM slot of `key` (not NULL, not INT64_MIN).
+__device__ inline uint64_t global_slot(const Table &t, int64_t key, bool *ok) {
+  const uint64_t mask = t.cap - 1;
+  uint64_t h = fmix64((uint64_t)key) & mask;
+  for (uint64_t probe = 0; probe < t.cap; probe++) {
+    const int64_t cur = __hip_atomic_load(&t.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == key) return h;
+    if (cur == kEmpty) {
+      const unsigned long long old =
+          atomicCAS((unsigned long long *)&t.keys[h], (unsigned long long)kEmpty, (unsigned long long)key);
+      if ((int64_t)old == kEmpty) {
+        atomicAdd((unsigned long long *)t.distinct, 1ull);
+        return h;
+      }
+      if ((int64_t)old == key) return h;
+    }
+    h = (h + 1) & mask;
+  }
+  *ok = false;
+  return 0;
+}
+
+__device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, bool *ok) {
+  if (!key_valid) {
+    t.seen[0] = 1;
+    return t.cap;
+  }
+  if (key == kEmpty) {
+    t.seen[1] = 1;
+    return t.cap + 1;
+  }
+  return global_slot(t, key, ok);
+}
+
+// ------------------------------------------------- generated-kernel skeleton
+// The generated part is a struct Q:
+//   kAggs, kLdsAggs, kKind[kAggs], kNullable[kAggs], kAnyNullable
+//   struct In, ins(p, base); struct Row, load(i, j, in, x)   (as for filters)
+//   pred(x, j, in, err)                 the fused filter (true: keep)
+//   key(p, x, j, in, key, kv)           the group key (false: no group, i.e.
+//                                       no join match); kv false = NULL key
+//   contrib(x, j, sel, c, cm, err)      each aggregate's contribution; bit a
+//                                       of cm set where input a is not NULL
+constexpr int kAggCols = 16;
+
+struct AggJitArgs {
+  uint64_t n;                   // input (probe) rows
+  const void *col[kAggCols];
+  const uint32_t *valid[kAggCols];
+  Table t;
+  uint32_t *error;              // [0] division by zero
+  // join: the group of each table slot, built once per join (unique keys)
+  JTable jt;
+  const uint64_t *gos;          // Int32 groups: (state << 32) | group
+  const uint64_t *gos64;        // Int64 groups: {group, state} pairs
+};
+
+// group of a probe key through the join: false when nothing matches
+__device__ inline bool join_group(const AggJitArgs &p, int64_t pk, int64_t &g, bool &gv) {
+  uint64_t slot;
+  if (p.jt.dense) {
+    // every slot of a dense table has an entry (state 0: no build row)
+    slot = (uint64_t)pk - (uint64_t)p.jt.min_key;
+    if (pk < p.jt.min_key || slot >= p.jt.slots) return false;
+  } else {
+    slot = join_find(p.jt, pk);
+    if (slot == kNoSlot) return false;
+  }
+  uint32_t state;
+  if (p.gos) {
+    const uint64_t e = p.gos[slot];
+    state = (uint32_t)(e >> 32);
+    g = (int64_t)(int32_t)(uint32_t)e;
+  } else {
+    g = (int64_t)p.gos64[2 * slot];
+    state = (uint32_t)p.gos64[2 * slot + 1];
+  }
+  gv = state == 1;  // 2: the build row's group is NULL
+  return state != 0;
+}
+
+template <uint8_t K>
+__device__ inline int64_t combine(int64_t a, int64_t b) {
+  if (K == A_SUM_F64) return (int64_t)f64_bits(as_f64((uint64_t)a) + as_f64((uint64_t)b));
+  if (K == A_MIN_I64 || K == A_MIN_F64 || K == A_MIN_F32) return a < b ? a : b;
+  if (K == A_MAX_I64 || K == A_MAX_F64 || K == A_MAX_F32) return a > b ? a : b;
+  return (int64_t)((uint64_t)a + (uint64_t)b);
+}
+
+template <uint8_t K>
+__device__ inline int64_t wave_combine(int64_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = combine<K>(v, (int64_t)__shfl_xor((long long)v, d, 64));
+  return v;
+}
+
+template <class Q, int S>
+__device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
+  constexpr int NA = Q::kAggs;
+  constexpr int LA = Q::kLdsAggs;
+  constexpr int LAS = LA > 0 ? LA : 1;
+  constexpr int kTile = 256 * S;
+  constexpr int HS = Q::kAnyNullable ? kLdsSlots + 2 : 1;  // validity flags only when needed
+  __shared__ int64_t s_keys[kLdsSlots + 2];
+  __shared__ int64_t s_acc[LAS][kLdsSlots + 2];
+  __shared__ uint32_t s_has[LAS][HS];
+  __shared__ uint32_t s_used;
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  for (int i = tid; i < kLdsSlots + 2; i += 256) {
+    s_keys[i] = kEmpty;
+#pragma unroll
+    for (int a = 0; a < LA; a++) {
+      s_acc[a][i] = acc_init(Q::kKind[a]);
+      if (HS > 1) s_has[a][i] = 0;
+    }
+  }
+  if (tid == 0) s_used = 0;
+  __syncthreads();
+
+  bool err = false, ok = true;
+  const uint64_t ntiles = (p.n + kTile - 1) / kTile;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    const typename Q::In in_cols = Q::ins(p, base);
+    typename Q::Row x[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+    // the block table takes new keys while it is at most half full; after
+    // that it only folds the keys it holds (hot keys arrive early and often)
+    const bool fill = s_used < kLdsSlots / 2;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      const bool in = j < lim;
+      bool sel = in && Q::pred(x[s], j, in, err);
+      int64_t key = 0;
+      bool kv = false;
+      if (sel) sel = Q::key(p, x[s], j, in, key, kv);
+      if (!kv) key = 0;  // every NULL key is the same group
+      int64_t c[NA];
+      uint32_t cm = 0;
+      Q::contrib(x[s], j, sel, c, cm, err);
+      // one matching round: the lanes sharing the first selected lane's key
+      // fold into it, so a hot key costs one update per wave
+      const uint64_t act = __ballot(sel);
+      if (act) {
+        const int lead = __builtin_ctzll(act);
+        const int64_t lk = (int64_t)__shfl((long long)key, lead, 64);
+        const int lkv = __shfl((int)kv, lead, 64);
+        const bool same = sel && key == lk && (int)kv == lkv;
+        const uint64_t sm = __ballot(same);
+        if (__popcll(sm) > 1) {
+          uint32_t cmc = 0;
+#pragma unroll
+          for (int a = 0; a < NA; a++) {
+            const bool use = same && ((cm >> a) & 1u);
+            int64_t v = use ? c[a] : acc_init(Q::kKind[a]);
+            switch (Q::kKind[a]) {
+              case A_SUM_F64: v = wave_combine<A_SUM_F64>(v); break;
+              case A_MIN_I64: case A_MIN_F64: case A_MIN_F32: v = wave_combine<A_MIN_I64>(v); break;
+              case A_MAX_I64: case A_MAX_F64: case A_MAX_F32: v = wave_combine<A_MAX_I64>(v); break;
+              default: v = wave_combine<A_SUM_I64>(v); break;
+            }
+            if (lane == lead) c[a] = v;  // the other lanes keep their own
+            cmc |= (uint32_t)(__ballot(use) != 0) << a;
+          }
+          if (same) {
+            if (lane == lead) cm = cmc;
+            else sel = false;
+          }
+        }
+      }
+      if (!sel) continue;
+      // the block table's slot (-1: go global)
+      int lslot = -1;
+      if (LA > 0) {
+        if (!kv) {
+          lslot = kLdsSlots;
+          s_keys[kLdsSlots] = 0;  // marks the NULL-key slot used
+        } else if (key == kEmpty) {
+          lslot = kLdsSlots + 1;
+          s_keys[kLdsSlots + 1] = 0;
+        } else {
+          uint32_t h = (uint32_t)fmix64((uint64_t)key) & (kLdsSlots - 1);
+          for (int pr = 0; pr < kLdsProbe; pr++) {
+            const int64_t cur = s_keys[h];
+            if (cur == key) {
+              lslot = (int)h;
+              break;
+            }
+            if (cur == kEmpty) {
+              if (!fill) break;
+              const unsigned long long old = atomicCAS((unsigned long long *)&s_keys[h],
+                                                       (unsigned long long)kEmpty, (unsigned long long)key);
+              if ((int64_t)old == kEmpty) {
+                atomicAdd(&s_used, 1u);
+                lslot = (int)h;
+                break;
+              }
+              if ((int64_t)old == key) {
+                lslot = (int)h;
+                break;
+              }
+            }
+            h = (h + 1) & (kLdsSlots - 1);
+          }
+        }
+      }
+      uint64_t gslot = 0;
+      if (lslot < 0 || LA < NA) gslot = slot_of(p.t, kv, key, &ok);
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (!((cm >> a) & 1u)) continue;
+        if (a < LA && lslot >= 0) {
+          acc_apply_lds(Q::kKind[a], &s_acc[a < LA ? a : 0][lslot], c[a]);
+          if (Q::kNullable[a] && HS > 1) s_has[a < LA ? a : 0][lslot] = 1;
+        } else if (ok) {
+          acc_apply_global(Q::kKind[a], &p.t.acc[a][gslot], c[a]);
+          if (p.t.has[a]) p.t.has[a][gslot] = 1;
+        }
+      }
+    }
+    __syncthreads();  // `fill` of the next tile reads s_used
+  }
+  // the block table into the HBM table: one update per (group, aggregate)
+  if (LA > 0) {
+    for (int i = tid; i < kLdsSlots + 2; i += 256) {
+      const int64_t k = s_keys[i];
+      if (k == kEmpty) continue;
+      uint64_t gs;
+      if (i == kLdsSlots) gs = slot_of(p.t, false, 0, &ok);
+      else if (i == kLdsSlots + 1) gs = slot_of(p.t, true, kEmpty, &ok);
+      else gs = global_slot(p.t, k, &ok);
+      if (!ok) continue;
+#pragma unroll
+      for (int a = 0; a < LA; a++) {
+        const int64_t v = s_acc[a][i];
+        if (v != acc_init(Q::kKind[a]) || Q::kKind[a] == A_SUM_F64) acc_apply_global(Q::kKind[a], &p.t.acc[a][gs], v);
+        if (Q::kNullable[a] && HS > 1 && p.t.has[a] && s_has[a][i]) p.t.has[a][gs] = 1;
+      }
+    }
+  }
+  if (err) atomicOr(p.error, 1u);
+  if (!ok) atomicOr(p.t.overflow, 1u);
+}
+
+}  // namespace oxr

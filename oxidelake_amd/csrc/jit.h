@@ -16,6 +16,7 @@
 #include "common.h"
 #include "expr.h"
 #include "filter_jit.h"
+#include "aggregate_jit.h"
 
 namespace oxr {
 
@@ -36,5 +37,32 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
 
 // Launches a generated kernel: `grid` blocks of 256 threads.
 int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args);
+int jit_launch_raw(hipFunction_t fn, unsigned grid, hipStream_t s, void *args, size_t size);
+
+// An aggregation shape: the fused filter, the key (a column of the input,
+// -1 for one global group, or with `join` the probe key column whose match
+// gives the group), and each aggregate's accumulator kind, input expression
+// (NULL for COUNT(*)) and whether that input can be NULL.
+struct AggJitShape {
+  const oxr_expr *filter = nullptr;
+  int32_t key_col = -1;
+  bool join = false;
+  std::vector<uint8_t> kinds;
+  std::vector<const oxr_expr *> inputs;
+  std::vector<uint8_t> nullable;
+};
+
+struct JitAgg {
+  hipFunction_t fn = nullptr;
+  int rows_per_lane = 0;
+  std::string source;
+};
+
+// Finds or builds the aggregation kernel of this shape for `dev`;
+// OXR_EUNSUPPORTED (no error recorded): the interpreter kernel runs it.
+int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJitShape &shape, JitAgg *out);
+// The generated source (compiled for gfx950 when asked), no device needed.
+int jit_aggregate_source(const std::vector<DevCol> &cols, const AggJitShape &shape, bool do_compile,
+                         std::string *src);
 
 }  // namespace oxr

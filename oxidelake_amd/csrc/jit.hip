@@ -217,11 +217,38 @@ struct Gen {
   }
 };
 
+// The input struct (column pointers advanced to the tile's first row), the
+// row struct and its loads, for the columns the generated code reads.
+void row_code(const std::vector<DevCol> &cols, const std::vector<char> &val_used,
+              const std::vector<char> &valid_used, std::string *in_fields, std::string *in_init,
+              std::string *row, std::string *load, int *vgprs) {
+  *row = "  struct Row {\n";
+  for (size_t c = 0; c < cols.size(); c++) {
+    const std::string cs = std::to_string(c);
+    if (val_used[c]) {
+      const int32_t t = cols[c].type;
+      *in_fields += std::string("    const ") + cname(t) + " *c" + cs + ";\n";
+      *in_init += std::string("    i.c") + cs + " = (const " + cname(t) + " *)p.col[" + cs + "] + base;\n";
+      *row += std::string("    ") + cname(t) + " c" + cs + ";\n";
+      *load += std::string("    x.c") + cs + " = in ? i.c" + cs + "[j] : (" + cname(t) + ")0;\n";
+      *vgprs += (t == OXR_INT64 || t == OXR_FLOAT64) ? 2 : 1;
+    }
+    if (valid_used[c]) {
+      *in_fields += "    const uint32_t *w" + cs + ";\n";
+      *in_init += "    i.w" + cs + " = p.valid[" + cs + "] + (base >> 5);\n";
+      *row += "    uint32_t w" + cs + ";\n";
+      *load += "    x.w" + cs + " = in ? i.w" + cs + "[j >> 5] : 0u;\n";
+      *vgprs += 1;
+    }
+  }
+  *row += "  };\n";
+}
+
 // S (rows per lane) for a row of `vgprs` 32-bit registers: the largest power
 // of two up to 16 keeping the tile's registers near 128, unless the
 // "filter.rows" knob says otherwise.
-int pick_rows(int vgprs) {
-  const int64_t knob = tune_get("filter.rows", 0);
+int pick_rows(int vgprs, const char *knob_name = "filter.rows") {
+  const int64_t knob = tune_get(knob_name, 0);
   if (knob == 1 || knob == 2 || knob == 4 || knob == 8 || knob == 16) return (int)knob;
   int s = 16;
   while (s > 2 && s * vgprs > 128) s /= 2;
@@ -280,28 +307,9 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
     proj_code += "    }\n";
   }
   if (g.unsupported) return OXR_EUNSUPPORTED;
-  // the input struct, the row struct and its loads
-  std::string in_fields, in_init, row = "  struct Row {\n", load;
+  std::string in_fields, in_init, row, load;
   int vgprs = 0;
-  for (size_t c = 0; c < cols.size(); c++) {
-    const std::string cs = std::to_string(c);
-    if (val_used[c]) {
-      const int32_t t = cols[c].type;
-      in_fields += std::string("    const ") + cname(t) + " *c" + cs + ";\n";
-      in_init += std::string("    i.c") + cs + " = (const " + cname(t) + " *)p.col[" + cs + "] + base;\n";
-      row += std::string("    ") + cname(t) + " c" + cs + ";\n";
-      load += std::string("    x.c") + cs + " = in ? i.c" + cs + "[j] : (" + cname(t) + ")0;\n";
-      vgprs += (t == OXR_INT64 || t == OXR_FLOAT64) ? 2 : 1;
-    }
-    if (valid_used[c]) {
-      in_fields += "    const uint32_t *w" + cs + ";\n";
-      in_init += "    i.w" + cs + " = p.valid[" + cs + "] + (base >> 5);\n";
-      row += "    uint32_t w" + cs + ";\n";
-      load += "    x.w" + cs + " = in ? i.w" + cs + "[j >> 5] : 0u;\n";
-      vgprs += 1;
-    }
-  }
-  row += "  };\n";
+  row_code(cols, val_used, valid_used, &in_fields, &in_init, &row, &load, &vgprs);
   jf->rows_per_lane = pick_rows(std::max(vgprs, 1));
   std::string src;
   src += "// generated filter + projection kernel\n";
@@ -369,7 +377,7 @@ int rtc_compile(const std::string &arch, const std::string &src, std::vector<cha
 }
 
 // The code object for `src` on `dev`, compiled once per process and device.
-int compile(oxr_device *dev, const std::string &src, hipFunction_t *fn) {
+int compile(oxr_device *dev, const std::string &src, const char *kernel, hipFunction_t *fn) {
   const std::string arch = dev->arch.substr(0, dev->arch.find(':'));
   const std::string key = arch + "/" + std::to_string(dev->ordinal) + "\n" + src;
   std::lock_guard<std::mutex> lock(g_jit_mu);
@@ -385,7 +393,7 @@ int compile(oxr_device *dev, const std::string &src, hipFunction_t *fn) {
   if (prev != dev->ordinal) OXR_HIP(hipSetDevice(dev->ordinal));
   Entry ent;
   hipError_t e = hipModuleLoadData(&ent.mod, code.data());
-  if (e == hipSuccess) e = hipModuleGetFunction(&ent.fn, ent.mod, "oxr_filter_jit");
+  if (e == hipSuccess) e = hipModuleGetFunction(&ent.fn, ent.mod, kernel);
   if (prev != dev->ordinal) hipSetDevice(prev);
   if (e != hipSuccess) return fail_hip(e, "loading a generated kernel");
   g_jit_cache.emplace(key, ent);
@@ -393,6 +401,130 @@ int compile(oxr_device *dev, const std::string &src, hipFunction_t *fn) {
   return OXR_OK;
 }
 
+// The aggregation kernel of one shape: filter, key (a column, one global
+// group, or the group a join gives the probe key) and each aggregate's
+// contribution, over the aggregate_tiles skeleton (aggregate_jit.h).
+int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, JitAgg *ja) {
+  const int na = (int)sh.kinds.size();
+  if (na < 1 || na > kMaxAggs || cols.size() > (size_t)kAggCols) return OXR_EUNSUPPORTED;
+  std::vector<char> val_used(cols.size(), 0), valid_used(cols.size(), 0);
+  JitFilter scratch;
+  Gen g{cols, &scratch, val_used, valid_used};
+  std::string pred_code = "    return true;\n";
+  if (sh.filter && sh.filter->count > 0) {
+    g.e = sh.filter;
+    g.code.clear();
+    const std::string res = g.boolean(0, sh.filter->count);
+    pred_code = g.code + "    return " + res + ";\n";
+  }
+  // the key
+  std::string key_code;
+  auto column_ok = [&](int c) -> std::string {
+    val_used[c] = 1;
+    if (!cols[c].validity) return "in";
+    valid_used[c] = 1;
+    return "(in && row_bit(x.w" + std::to_string(c) + ", j))";
+  };
+  if (sh.join) {
+    const int c = sh.key_col;  // the probe key
+    const std::string ok = column_ok(c);
+    key_code = "    if (!" + ok + ") return false;\n    return join_group(p, (int64_t)x.c" + std::to_string(c) +
+               ", key, kv);\n";
+  } else if (sh.key_col >= 0) {
+    const int c = sh.key_col;
+    const std::string ok = column_ok(c);
+    key_code = "    key = (int64_t)x.c" + std::to_string(c) + ";\n    kv = " + ok + ";\n    return true;\n";
+  } else {
+    key_code = "    key = 0;\n    kv = false;  // one global group: the NULL-key slot\n    return true;\n";
+  }
+  // the contributions
+  std::string contrib = "    const bool in = sel;\n    (void)in;\n";
+  std::string kinds, nulls;
+  for (int a = 0; a < na; a++) {
+    const std::string as = std::to_string(a);
+    const uint8_t k = sh.kinds[a];
+    kinds += (a ? ", " : "") + std::to_string(k);
+    nulls += (a ? ", " : "") + std::string(sh.nullable[a] ? "1" : "0");
+    if (k == A_COUNT_STAR) {
+      contrib += "    c[" + as + "] = 1;\n    cm |= (uint32_t)sel << " + as + ";\n";
+      continue;
+    }
+    g.e = sh.inputs[a];
+    g.code.clear();
+    const Val v = g.value(0, sh.inputs[a]->count);
+    std::string cv;
+    switch (k) {
+      case A_COUNT: cv = "1"; break;
+      case A_SUM_I64:
+      case A_MIN_I64:
+      case A_MAX_I64: cv = "(int64_t)" + v.v; break;
+      case A_SUM_F64: cv = "(int64_t)f64_bits((double)" + v.v + ")"; break;
+      case A_MIN_F64:
+      case A_MAX_F64: cv = "total_order64(" + v.v + ")"; break;
+      default: cv = "total_order32(" + v.v + ")"; break;
+    }
+    contrib += "    {\n" + g.code + "    c[" + as + "] = " + cv + ";\n    cm |= (uint32_t)(sel && " + v.ok +
+               ") << " + as + ";\n    }\n";
+  }
+  if (g.unsupported) return OXR_EUNSUPPORTED;
+  std::string in_fields, in_init, row, load;
+  int vgprs = 0;
+  row_code(cols, val_used, valid_used, &in_fields, &in_init, &row, &load, &vgprs);
+  ja->rows_per_lane = pick_rows(std::max(vgprs + 2 * na, 1), "agg.rows");
+  const int la = na <= kLdsAggs ? na : 0;
+  std::string src;
+  src += "// generated aggregation kernel\n";
+  src += "#include \"filter_jit.h\"\n#include \"aggregate_jit.h\"\nusing namespace oxr;\nnamespace {\nstruct Q {\n";
+  src += "  static constexpr int kAggs = " + std::to_string(na) + ";\n";
+  src += "  static constexpr int kLdsAggs = " + std::to_string(la) + ";\n";
+  src += "  static constexpr uint8_t kKind[kAggs] = {" + kinds + "};\n";
+  src += "  static constexpr uint8_t kNullable[kAggs] = {" + nulls + "};\n";
+  bool any_null = false;
+  for (uint8_t v : sh.nullable) any_null |= v != 0;
+  src += std::string("  static constexpr bool kAnyNullable = ") + (any_null ? "true" : "false") + ";\n";
+  src += "  struct In {\n" + in_fields + "  };\n";
+  src += "  __device__ static inline In ins(const AggJitArgs &p, uint64_t base) {\n    In i;\n" + in_init +
+         "    return i;\n  }\n";
+  src += row;
+  src += "  __device__ static inline void load(const In &i, uint32_t j, bool in, Row &x) {\n" + load + "  }\n";
+  src += "  __device__ static inline bool pred(const Row &x, uint32_t j, bool in, bool &err) {\n" + pred_code +
+         "  }\n";
+  src += "  __device__ static inline bool key(const AggJitArgs &p, const Row &x, uint32_t j, bool in, "
+         "int64_t &key, bool &kv) {\n" + key_code + "  }\n";
+  src += "  __device__ static inline void contrib(const Row &x, uint32_t j, bool sel, int64_t (&c)[kAggs], "
+         "uint32_t &cm, bool &err) {\n" + contrib + "  }\n";
+  src += "};\n}  // namespace\n";
+  src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_jit(const AggJitArgs p) {\n";
+  src += "  aggregate_tiles<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";
+  ja->source = src;
+  return OXR_OK;
+}
+
+}  // namespace
+
+int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJitShape &shape, JitAgg *out) {
+  JitAgg ja;
+  const int rc = generate_aggregate(cols, shape, &ja);
+  if (rc != OXR_OK) return rc;
+  OXR_TRY(compile(dev, ja.source, "oxr_aggregate_jit", &ja.fn));
+  *out = std::move(ja);
+  return OXR_OK;
+}
+
+int jit_aggregate_source(const std::vector<DevCol> &cols, const AggJitShape &shape, bool do_compile,
+                         std::string *src) {
+  JitAgg ja;
+  const int rc = generate_aggregate(cols, shape, &ja);
+  if (rc != OXR_OK) return rc;
+  if (do_compile) {
+    std::vector<char> code;
+    OXR_TRY(rtc_compile("gfx950", ja.source, &code));
+  }
+  *src = ja.source;
+  return OXR_OK;
+}
+
+namespace {
 }  // namespace
 
 int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr *pred,
@@ -401,18 +533,21 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
   JitFilter jf;
   const int rc = generate(cols, pred, proj, nproj, vps, &jf);
   if (rc != OXR_OK) return rc;
-  OXR_TRY(compile(dev, jf.source, &jf.fn));
+  OXR_TRY(compile(dev, jf.source, "oxr_filter_jit", &jf.fn));
   *out = std::move(jf);
+  return OXR_OK;
+}
+
+int jit_launch_raw(hipFunction_t fn, unsigned grid, hipStream_t s, void *args, size_t size) {
+  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                 HIP_LAUNCH_PARAM_END};
+  OXR_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, kJitBlock, 1, 1, 0, s, nullptr, cfg));
   return OXR_OK;
 }
 
 int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args) {
   JitArgs a = args;
-  size_t size = sizeof a;
-  void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &a, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
-                 HIP_LAUNCH_PARAM_END};
-  OXR_HIP(hipModuleLaunchKernel(fn, grid, 1, 1, kJitBlock, 1, 1, 0, s, nullptr, cfg));
-  return OXR_OK;
+  return jit_launch_raw(fn, grid, s, &a, sizeof a);
 }
 
 }  // namespace oxr

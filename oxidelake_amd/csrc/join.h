@@ -1,66 +1,12 @@
-// Device-side join table shared by join.hip (build, probe, gather) and
-// aggregate.hip (fused join + aggregation).
-//
-// The table indexes the build side's rows grouped by key, build order within
-// a key: each distinct non-NULL key has a run [start, start+count) in `rows`
-// (the layout of the reference's JoinTable, crates/oxidelake-device/src/cpu/
-// kernels.rs:181-266). Two forms, picked at build time from the key range:
-//  - dense: keys in [min, min+range) address their slot directly (no hashing,
-//    one read per probe) when the range is at most twice the build rows;
-//  - hashed: open addressing, capacity a power of two at least twice the
-//    build rows, 64-bit keys claimed by one CAS (INT64_MIN marks an empty
-//    slot; the key INT64_MIN itself has the extra slot `cap`).
+// Join handles shared by join.hip (build, probe, gather) and aggregate.hip
+// (fused join + aggregation); the device view is in jtable.h.
 #pragma once
 
 #include <cstdint>
 
 #include "common.h"
+#include "jtable.h"
 
-namespace oxr {
-
-constexpr int64_t kJoinEmpty = INT64_MIN;
-
-struct JTable {
-  int32_t dense;
-  int32_t unique;     // every key has exactly one build row
-  int64_t min_key;    // dense: smallest key
-  uint64_t slots;     // dense: range; hashed: cap (power of two); +1 slot follows
-  int64_t *keys;      // hashed only: slots+1
-  uint32_t *start;    // slots+1
-  uint32_t *count;    // slots+1
-  uint32_t *rows;     // build rows with a non-NULL key, grouped by slot
-};
-
-__host__ __device__ inline uint64_t join_hash(uint64_t h) {
-  // MurmurHash3 finalizer
-  h ^= h >> 33;
-  h *= 0xff51afd7ed558ccdull;
-  h ^= h >> 33;
-  h *= 0xc4ceb9fe1a85ec53ull;
-  h ^= h >> 33;
-  return h;
-}
-
-// Slot of `key`, or UINT64_MAX when the table does not hold it.
-__device__ inline uint64_t join_find(const JTable &t, int64_t key) {
-  if (t.dense) {
-    const uint64_t d = (uint64_t)key - (uint64_t)t.min_key;
-    if (key < t.min_key || d >= t.slots) return UINT64_MAX;
-    return t.count[d] ? d : UINT64_MAX;
-  }
-  if (key == kJoinEmpty) return t.count[t.slots] ? t.slots : UINT64_MAX;
-  const uint64_t mask = t.slots - 1;
-  uint64_t h = join_hash((uint64_t)key) & mask;
-  for (uint64_t p = 0; p < t.slots; p++) {
-    const int64_t k = t.keys[h];
-    if (k == key) return h;
-    if (k == kJoinEmpty) return UINT64_MAX;
-    h = (h + 1) & mask;
-  }
-  return UINT64_MAX;
-}
-
-}  // namespace oxr
 
 struct oxr_join_table {
   oxr_device *dev = nullptr;

@@ -25,6 +25,17 @@ pytestmark = pytest.mark.gpu
 SUM_REL = 1e-12
 
 
+@pytest.fixture(autouse=True, params=["generated", "interpreter"])
+def path(request):
+    # every test runs on the kernel generated per aggregation shape and on
+    # the interpreter kernel
+    from oxidelake_amd import autotune
+
+    autotune.set_knob("agg.jit", 1 if request.param == "generated" else 0)
+    yield request.param
+    autotune.set_knob("agg.jit", -1)
+
+
 def fbits(v):
     return struct.unpack("<Q", struct.pack("<d", v))[0]
 
@@ -186,3 +197,49 @@ def test_global_aggregate_over_no_rows_is_one_row(backend, stream, oracle):
     got = device_aggregate(backend, stream, b, spec)
     assert [c.to_pylist() for c in got.columns] == [[None], [0], [None], [0]]
     assert_agg_equal(got, oracle.aggregate(b, spec), spec)
+
+
+def test_hot_keys_fold_across_the_wave(backend, stream, oracle):
+    # one key on most rows (the matching round folds it per wave), NULL keys
+    # and NULL inputs mixed in, a cold tail of distinct keys
+    rng = np.random.default_rng(21)
+    n = 300_000
+    k = np.where(rng.random(n) < 0.7, 7, rng.integers(0, 100_000, n)).astype(np.int64)
+    x = rng.integers(-1000, 1000, n).astype(np.int64)
+    v = rng.normal(size=n)
+    b = pa.record_batch({"k": pa.array(k, mask=rng.random(n) < 0.05),
+                         "x": pa.array(x, mask=rng.random(n) < 0.1), "v": pa.array(v)})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.Sum, col(2)), Aggregate(F.Min, col(2)),
+                             Aggregate(F.Max, col(1)), Aggregate(F.Count, col(1))])
+    assert_agg_equal(device_aggregate(backend, stream, b, spec), oracle.aggregate(b, spec, sum_mode=1), spec)
+
+
+@pytest.mark.parametrize("rows", [1, 4, 16])
+def test_every_rows_per_lane(backend, stream, oracle, rows, path):
+    from oxidelake_amd import autotune
+
+    autotune.set_knob("agg.rows", rows)
+    try:
+        rng = np.random.default_rng(31)
+        batch = datagen.random_batch(rng, 20_001).select([0, 1, 2, 3])
+        for spec in (SPECS[0], SPECS[3], SPECS[5]):
+            got = device_aggregate(backend, stream, batch, spec)
+            assert_agg_equal(got, oracle.aggregate(batch, spec, sum_mode=1), spec)
+    finally:
+        autotune.set_knob("agg.rows", -1)
+
+
+def test_generated_kernel_is_the_one_that_runs(backend, stream, path):
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    dev = datagen.table(backend, stream, datagen.config1_columns(seed=3), 10_000)
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        backend.aggregate(stream, dev, datagen.config1_spec())
+        want = (1, 0) if path == "generated" else (0, 1)
+        assert (autotune.launches("aggregate"), autotune.launches("aggregate_vm")) == want
+    finally:
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()

@@ -89,3 +89,34 @@ def test_plan_errors_match_the_interpreter():
         backend.filter_kernel_source(RANDOM, Predicate.compare(4, Cmp.Eq, Lit.int64(1)), [0])
     with pytest.raises(Unsupported):
         backend.filter_kernel_source(CONFIG2, None, [plan.cmp(col(0), Cmp.Eq, col(1))])
+
+
+# ------------------------------------------------------------- aggregation
+from tests.test_aggregate_gpu import SPECS as AGG_SPECS  # noqa: E402
+
+AGG_COLS = [(plan.INT64, True), (plan.INT32, True), (plan.FLOAT32, True), (plan.FLOAT64, True)]
+
+
+@pytest.mark.parametrize("si", range(len(AGG_SPECS)))
+def test_every_test_aggregation_compiles(si):
+    src = backend.aggregate_kernel_source(AGG_COLS, AGG_SPECS[si], compile=True)
+    assert "oxr_aggregate_jit" in src
+
+
+def test_config1_and_config3_kernels():
+    autotune.set_knob("agg.rows", -1)
+    c1 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False), (plan.F32_VECTOR, 8, False)]
+    src = backend.aggregate_kernel_source(c1, datagen.config1_spec(), compile=True)
+    # k and v only (the id and vector columns are never loaded), no NULLs
+    assert "x.c1" in src and "x.c2" in src and "x.c0" not in src and "x.c3" not in src
+    assert "kAnyNullable = false" in src and "aggregate_tiles<Q, 16>" in src
+    c3 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False)]
+    src = backend.aggregate_kernel_source(c3, datagen.config3_spec(), probe_key=0, compile=True)
+    assert "join_group(p, (int64_t)x.c0, key, kv)" in src
+
+
+def test_aggregation_without_aggregates_uses_the_interpreter():
+    from oxidelake_amd.backend import Unsupported
+
+    with pytest.raises(Unsupported):
+        backend.aggregate_kernel_source(AGG_COLS, plan.AggregateSpec(0, []))

@@ -17,6 +17,17 @@ from tests.test_aggregate_gpu import assert_agg_equal
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=["generated", "interpreter"])
+def path(request):
+    # unique-key joins run on the generated kernel, others on the interpreter
+    from oxidelake_amd import autotune
+
+    autotune.set_knob("agg.jit", 1 if request.param == "generated" else 0)
+    yield request.param
+    autotune.set_knob("agg.jit", -1)
+
+
 # fact: fk, qty, price; dim: pk, grp
 AGGS = [Aggregate(F.Sum, col(1)), Aggregate(F.Sum, plan.mul(col(1), col(2))), Aggregate(F.CountStar, None),
         Aggregate(F.Min, col(2)), Aggregate(F.Max, col(1)), Aggregate(F.Count, col(1))]

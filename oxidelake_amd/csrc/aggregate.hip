@@ -638,6 +638,7 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   }
   A.t = st->t;
   A.error = err;
+  std::copy(ja.lits.begin(), ja.lits.end(), A.lit);
   void *gos = nullptr;
   if (js) {
     // the group of every slot of the join table

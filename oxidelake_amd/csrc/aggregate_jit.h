@@ -172,10 +172,11 @@ __device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, 
 // The generated part is a struct Q:
 //   kAggs, kLdsAggs, kKind[kAggs], kNullable[kAggs], kAnyNullable
 //   struct In, ins(p, base); struct Row, load(i, j, in, x)   (as for filters)
-//   pred(x, j, in, err)                 the fused filter (true: keep)
+//   pred(L, x, j, in, err)              the fused filter (true: keep; L: the
+//                                       literals, in prepared mode)
 //   key(p, x, j, in, key, kv)           the group key (false: no group, i.e.
 //                                       no join match); kv false = NULL key
-//   contrib(x, j, sel, c, cm, err)      each aggregate's contribution; bit a
+//   contrib(L, x, j, sel, c, cm, err)   each aggregate's contribution; bit a
 //                                       of cm set where input a is not NULL
 constexpr int kAggCols = 16;
 
@@ -189,6 +190,7 @@ struct AggJitArgs {
   JTable jt;
   const uint64_t *gos;          // Int32 groups: (state << 32) | group
   const uint64_t *gos64;        // Int64 groups: {group, state} pairs
+  uint64_t lit[32];             // prepared mode: literal bits
 };
 
 // group of a probe key through the join: false when nothing matches
@@ -273,14 +275,14 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
     for (int s = 0; s < S; s++) {
       const uint32_t j = (uint32_t)(s * 256 + tid);
       const bool in = j < lim;
-      bool sel = in && Q::pred(x[s], j, in, err);
+      bool sel = in && Q::pred(p.lit, x[s], j, in, err);
       int64_t key = 0;
       bool kv = false;
       if (sel) sel = Q::key(p, x[s], j, in, key, kv);
       if (!kv) key = 0;  // every NULL key is the same group
       int64_t c[NA];
       uint32_t cm = 0;
-      Q::contrib(x[s], j, sel, c, cm, err);
+      Q::contrib(p.lit, x[s], j, sel, c, cm, err);
       // one matching round: the lanes sharing the first selected lane's key
       // fold into it, so a hot key costs one update per wave
       const uint64_t act = __ballot(sel);

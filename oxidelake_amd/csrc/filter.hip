@@ -391,6 +391,7 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
       A.dim[i] = (*out_cols)[i].dim;
     }
     A.nouts = nproj;
+    std::copy(jf.lits.begin(), jf.lits.end(), A.lit);
     ProfScope prof(s->s, "filter_project");
     rc = jit_launch(jf.fn, (unsigned)P.ntiles, s->s, A);
   } else if (rc == OXR_OK && P.ntiles) {

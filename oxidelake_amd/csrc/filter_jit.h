@@ -7,13 +7,14 @@
 //                                  bases: no 64-bit address per lane)
 //   struct Row, load(i, j, in, x)  the values (and validity words) one row
 //                                  reads (in: the row lies inside its batch)
-//   pred(x, j, in, err)            the predicate: true keeps the row
+//   pred(L, x, j, in, err)         the predicate: true keeps the row (L: the
+//                                  literals, in prepared mode)
 //   struct Out, outs(p, base, at)  the outputs, advanced to the tile's first
 //                                  output row `at`; read from the arguments
 //                                  once per tile (stores into them could
 //                                  otherwise force a reload of every pointer
 //                                  at every row step)
-//   project(o, x, j, q, bitpos, bits, stride, err)
+//   project(L, o, x, j, q, bitpos, bits, stride, err)
 //                                  stores a kept row's projections at output
 //                                  row q of the tile and sets its output
 //                                  validity bits in LDS
@@ -34,6 +35,7 @@ namespace oxr {
 constexpr int kJitBlock = 256;
 constexpr int kJitCols = 16;
 constexpr int kJitOuts = 8;
+constexpr int kJitLits = 32;
 
 // Kernel arguments of every generated filter kernel (identical layout on
 // the host, which fills it, and in the generated code).
@@ -52,6 +54,7 @@ struct JitArgs {
   int32_t vslot[kJitOuts];     // LDS bit row of output k, -1: none
   int32_t dim[kJitOuts];       // floats per row of a vector output
   int32_t nouts, pad;
+  uint64_t lit[kJitLits];      // prepared mode: literal bits, in each literal's type
 };
 
 // Validity bit of tile row j (tiles start on 32-row boundaries).
@@ -125,7 +128,7 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   for (int s = 0; s < S; s++) {
     const uint32_t j = (uint32_t)(s * kJitBlock + tid);
     const bool in = j < lim;
-    const bool k = Q::pred(x[s], j, in, err) && in;
+    const bool k = Q::pred(p.lit, x[s], j, in, err) && in;
     selbits |= (uint32_t)k << s;
     const uint64_t mk = __ballot(k);
     if (lane == 0) {
@@ -160,7 +163,7 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
     const uint32_t j = (uint32_t)(s * kJitBlock + tid);
     const bool k = (selbits >> s) & 1u;
     const uint32_t q = s_off[s * 4 + wave] + (uint32_t)__popcll(s_mask[s * 4 + wave] & below);
-    if (k) Q::project(o, x[s], j, q, bit0 + q, s_bits, kWords, err);
+    if (k) Q::project(p.lit, o, x[s], j, q, bit0 + q, s_bits, kWords, err);
     if (Q::kVecOuts > 0) Q::vectors(o, k, j, q, lane);
   }
   if (err) atomicOr((unsigned long long *)&p.result[0], 1ull);

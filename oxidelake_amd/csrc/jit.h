@@ -24,6 +24,7 @@ struct JitFilter {
   hipFunction_t fn = nullptr;
   int rows_per_lane = 0;       // S: a tile is 256 * S rows
   std::vector<int32_t> vslot;  // JitArgs::vslot per projection
+  std::vector<uint64_t> lits;  // JitArgs::lit (prepared mode, knob jit.params)
   std::string source;
 };
 
@@ -55,6 +56,7 @@ struct AggJitShape {
 struct JitAgg {
   hipFunction_t fn = nullptr;
   int rows_per_lane = 0;
+  std::vector<uint64_t> lits;  // AggJitArgs::lit (prepared mode)
   std::string source;
 };
 

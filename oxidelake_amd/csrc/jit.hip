@@ -74,6 +74,9 @@ struct Gen {
   std::string code;
   int tmp = 0;
   bool unsupported = false;
+  // prepared mode: literals are kernel arguments L[i] (their bits), so one
+  // binary serves every value; else they are constants of the code
+  std::vector<uint64_t> *params = nullptr;
 
   std::string fresh() { return "t" + std::to_string(tmp++); }
   void stmt(const std::string &s) { code += "    " + s + "\n"; }
@@ -113,6 +116,33 @@ struct Gen {
   // compiler folds them into the comparisons and arithmetic.
   Val literal(const oxr_expr_node &nd) {
     char buf[64];
+    if (params) {
+      if ((int)params->size() >= kJitLits) {
+        unsupported = true;
+        return {"0", nd.type, "true", false};
+      }
+      uint64_t bits = 0;
+      const std::string ref = "L[" + std::to_string(params->size()) + "]";
+      std::string v;
+      if (nd.type == OXR_INT32) {
+        bits = (uint64_t)(int64_t)(int32_t)nd.lit_i;
+        v = "((int32_t)(int64_t)" + ref + ")";
+      } else if (nd.type == OXR_INT64) {
+        bits = (uint64_t)nd.lit_i;
+        v = "((int64_t)" + ref + ")";
+      } else if (nd.type == OXR_FLOAT32) {
+        const float f = (float)nd.lit_f;
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        bits = u;
+        v = "as_f32(" + ref + ")";
+      } else {
+        std::memcpy(&bits, &nd.lit_f, 8);
+        v = "as_f64(" + ref + ")";
+      }
+      params->push_back(bits);
+      return {v, nd.type, "true", false};
+    }
     if (nd.type == OXR_INT32) {
       std::snprintf(buf, sizeof buf, "((int32_t)0x%08xu)", (uint32_t)(int32_t)nd.lit_i);
     } else if (nd.type == OXR_INT64) {
@@ -148,7 +178,7 @@ struct Gen {
     const char *mod = nd.kind == OXR_X_MOD ? "true" : "false";
     const int64_t dlit = rhs.type == OXR_INT32 ? (int64_t)(int32_t)rhs.lit_i : rhs.lit_i;
     if (divmod && end - 1 - mid == 1 && rhs.kind == OXR_X_LITERAL && is_int_type(rhs.type) &&
-        dlit != 0 && dlit != -1 && is_int_type(type_of(begin, mid))) {
+        dlit != 0 && dlit != -1 && is_int_type(type_of(begin, mid)) && !params) {
       // integer division by a literal other than 0 and -1 can neither fail
       // nor overflow: C's truncating / and % are the SQL result, and the
       // compiler turns the constant divisor into a multiply-high
@@ -260,6 +290,7 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   if (cols.size() > (size_t)kJitCols || nproj > kJitOuts) return OXR_EUNSUPPORTED;
   std::vector<char> val_used(cols.size(), 0), valid_used(cols.size(), 0);
   Gen g{cols, jf, val_used, valid_used};
+  if (tune_get("jit.params", 0) > 0) g.params = &jf->lits;
   // predicate
   std::string pred_code = "    return true;\n";
   if (pred && pred->count > 0) {
@@ -325,10 +356,10 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   src += row;
   src += "  __device__ static inline void load(const In &i, uint32_t j, bool in, Row &x) {\n" + load +
          "  }\n";
-  src += "  __device__ static inline bool pred(const Row &x, uint32_t j, bool in, bool &err) {\n" + pred_code +
-         "  }\n";
-  src += "  __device__ static inline void project(const Out &o, const Row &x, uint32_t j, uint32_t q, "
-         "uint32_t bitpos, uint32_t *bits, int stride, bool &err) {\n" +
+  src += "  __device__ static inline bool pred(const uint64_t *L, const Row &x, uint32_t j, bool in, "
+         "bool &err) {\n" + pred_code + "  }\n";
+  src += "  __device__ static inline void project(const uint64_t *L, const Out &o, const Row &x, uint32_t j, "
+         "uint32_t q, uint32_t bitpos, uint32_t *bits, int stride, bool &err) {\n" +
          proj_code + "  }\n";
   src += "  __device__ static inline void vectors(const Out &o, bool sel, uint32_t j, uint32_t q, "
          "int lane) {\n" +
@@ -410,6 +441,7 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   std::vector<char> val_used(cols.size(), 0), valid_used(cols.size(), 0);
   JitFilter scratch;
   Gen g{cols, &scratch, val_used, valid_used};
+  if (tune_get("jit.params", 0) > 0) g.params = &ja->lits;
   std::string pred_code = "    return true;\n";
   if (sh.filter && sh.filter->count > 0) {
     g.e = sh.filter;
@@ -487,12 +519,12 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
          "    return i;\n  }\n";
   src += row;
   src += "  __device__ static inline void load(const In &i, uint32_t j, bool in, Row &x) {\n" + load + "  }\n";
-  src += "  __device__ static inline bool pred(const Row &x, uint32_t j, bool in, bool &err) {\n" + pred_code +
-         "  }\n";
+  src += "  __device__ static inline bool pred(const uint64_t *L, const Row &x, uint32_t j, bool in, "
+         "bool &err) {\n" + pred_code + "  }\n";
   src += "  __device__ static inline bool key(const AggJitArgs &p, const Row &x, uint32_t j, bool in, "
          "int64_t &key, bool &kv) {\n" + key_code + "  }\n";
-  src += "  __device__ static inline void contrib(const Row &x, uint32_t j, bool sel, int64_t (&c)[kAggs], "
-         "uint32_t &cm, bool &err) {\n" + contrib + "  }\n";
+  src += "  __device__ static inline void contrib(const uint64_t *L, const Row &x, uint32_t j, bool sel, "
+         "int64_t (&c)[kAggs], uint32_t &cm, bool &err) {\n" + contrib + "  }\n";
   src += "};\n}  // namespace\n";
   src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_jit(const AggJitArgs p) {\n";
   src += "  aggregate_tiles<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";

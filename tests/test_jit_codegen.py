@@ -120,3 +120,21 @@ def test_aggregation_without_aggregates_uses_the_interpreter():
 
     with pytest.raises(Unsupported):
         backend.aggregate_kernel_source(AGG_COLS, plan.AggregateSpec(0, []))
+
+
+def test_prepared_mode_shares_one_kernel_across_literals():
+    # knob jit.params: literals become kernel arguments (one binary per shape)
+    autotune.set_knob("jit.params", 1)
+    try:
+        p50, proj = datagen.config2_query(50)
+        p500, _ = datagen.config2_query(500)
+        a = backend.filter_kernel_source(CONFIG2, p50, proj, compile=True)
+        assert a == backend.filter_kernel_source(CONFIG2, p500, proj) and "L[0]" in a
+        # no literal-divisor shortcut: the checked division path
+        assert "i64_divmod(true" in a
+        spec = lambda t: plan.AggregateSpec(1, [plan.Aggregate(plan.AggregateFunction.Sum, col(0))],
+                                            filter=Predicate.compare(1, Cmp.Lt, Lit.int64(t)))
+        x = backend.aggregate_kernel_source(CONFIG2, spec(10), compile=True)
+        assert x == backend.aggregate_kernel_source(CONFIG2, spec(400)) and "L[0]" in x
+    finally:
+        autotune.set_knob("jit.params", -1)

@@ -1,0 +1,314 @@
+"""A serving loop over HBM-resident tables: mixed requests, continuous
+batching, admission by device-memory footprint, overlap on the device.
+
+Three request kinds, over tables that stay resident for the server's life:
+  - "vector": top-k of 1..16 queries over a vector index (BF16 scan + FP32
+    rerank by default);
+  - "filter_agg": COUNT(*), SUM(a), SUM(c*d) FROM t WHERE b < T over the
+    200M-row filter table (prepared: one compiled kernel for every T);
+  - "join_agg": SUM(qty), SUM(qty*price), COUNT(*) FROM fact JOIN dim ON
+    fk = pk WHERE fk in [lo, hi) GROUP BY grp.
+
+Every kind has its own worker thread and HIP stream, so requests of
+different kinds run at the same time on the device. Requests are admitted
+continuously: a vector worker takes every vector request waiting when it
+becomes free (up to 256 queries) into one scan, so requests that arrive
+while a scan runs join the next one instead of queueing one by one. A
+memory gate admits a batch only when its estimated device footprint fits in
+the budget beside everything in flight. The one-at-a-time path
+(`run_one_at_a_time`) runs the same requests in arrival order, each alone,
+on one stream: the plain drop-in path.
+
+Nothing here is a second implementation of an operator: every request runs
+through RocmBackend's entry points (the library's HIP kernels).
+"""
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+import pyarrow as pa
+
+from . import autotune, plan
+from .backend import AggState, JoinTable, RocmBackend, VectorIndex
+from .plan import Aggregate, AggregateSpec
+from .plan import AggregateFunction as F
+from .plan import Comparison as Cmp
+
+
+@dataclass
+class Request:
+    id: int
+    kind: str                 # "vector" | "filter_agg" | "join_agg"
+    arrival: float            # seconds after the run starts
+    payload: object           # vector: (q, dim) float32; filter_agg: T; join_agg: (lo, hi)
+    result: object = None
+    start: float = 0.0        # when its batch was admitted to the device
+    done: float = 0.0
+    batch: int = 1            # requests in the batch that served it
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+@dataclass
+class Tables:
+    """The resident state a server answers from."""
+    filter_table: object      # DeviceBatch: a Int64, b Int32, c Float32 (NULLs), d Float64
+    fact: object              # DeviceBatch: fk Int64, qty Int32, price Float64
+    dim: object               # DeviceBatch: pk Int64, grp Int32
+    join: JoinTable           # built once over dim.pk
+    index: VectorIndex
+    vec_rows: int             # rows the index holds
+    candidates: int = 100     # rerank depth of the vector index
+    k: int = 10
+    keep: list = field(default_factory=list)  # anything the tables view into
+
+
+def filter_agg_spec(threshold: int) -> AggregateSpec:
+    return AggregateSpec(-1, [Aggregate(F.CountStar, None, "COUNT(*)"), Aggregate(F.Sum, plan.col(0), "SUM(a)"),
+                              Aggregate(F.Sum, plan.mul(plan.col(2), plan.col(3)), "SUM(c*d)")],
+                         filter=plan.cmp(plan.col(1), Cmp.Lt, plan.Lit.int64(threshold)))
+
+
+def join_agg_spec(lo: int, hi: int) -> AggregateSpec:
+    return AggregateSpec(-1, [Aggregate(F.Sum, plan.col(1), "SUM(qty)"),
+                              Aggregate(F.Sum, plan.mul(plan.col(1), plan.col(2)), "SUM(qty*price)"),
+                              Aggregate(F.CountStar, None, "COUNT(*)")],
+                         filter=plan.and_(plan.cmp(plan.col(0), Cmp.GtEq, plan.Lit.int64(lo)),
+                                          plan.cmp(plan.col(0), Cmp.Lt, plan.Lit.int64(hi))))
+
+
+class MemoryGate:
+    """Admission by footprint: acquire(n) waits until n more bytes fit under
+    the budget beside everything admitted (one request always fits alone)."""
+
+    def __init__(self, budget: int):
+        self.budget = budget
+        self.used = 0
+        self.peak = 0
+        self.cv = threading.Condition()
+
+    def acquire(self, n: int) -> None:
+        with self.cv:
+            while self.used > 0 and self.used + n > self.budget:
+                self.cv.wait()
+            self.used += n
+            self.peak = max(self.peak, self.used)
+
+    def release(self, n: int) -> None:
+        with self.cv:
+            self.used -= n
+            self.cv.notify_all()
+
+
+def footprint(tables: Tables, kind: str, reqs: list[Request]) -> int:
+    """Estimated device bytes a batch holds while it runs (scratch and
+    results; the resident tables are not counted)."""
+    if kind == "vector":
+        # scores of one scan tile (at most 2 GiB) and the per-block candidates
+        nq = sum(len(r.payload) for r in reqs)
+        return min(2 << 30, nq * tables.vec_rows * 4) + nq * tables.candidates * 8 * 4096
+    if kind == "join_agg":
+        # a slot-to-group entry per dimension row and one group state per row
+        return tables.dim.num_rows * (8 + 64)
+    return 1 << 20
+
+
+def execute(be: RocmBackend, tables: Tables, stream, kind: str, reqs: list[Request]) -> list:
+    """Runs one batch on `stream`; returns one result per request."""
+    if kind == "vector":
+        qs = np.concatenate([r.payload for r in reqs])
+        rows, dist = tables.index.search(stream, qs, tables.k, tables.candidates)
+        out, at = [], 0
+        for r in reqs:
+            q = len(r.payload)
+            out.append((rows[at:at + q], dist[at:at + q]))
+            at += q
+        return out
+    out = []
+    for r in reqs:
+        if kind == "filter_agg":
+            st = AggState(be, stream, tables.filter_table, filter_agg_spec(int(r.payload)))
+            st.update(stream, tables.filter_table)
+        else:
+            lo, hi = r.payload
+            st = AggState(be, stream, tables.fact, join_agg_spec(lo, hi))
+            st.update_join(stream, tables.join, tables.dim, 1, tables.fact, 0)
+        out.append(be.download(stream, st.finish(stream)))
+    return out
+
+
+class Server:
+    """Continuous batching over resident tables; see the module docstring."""
+
+    KINDS = ("vector", "filter_agg", "join_agg")
+
+    def __init__(self, be: RocmBackend, tables: Tables, budget: Optional[int] = None,
+                 max_vector_queries: int = 256):
+        self.be = be
+        self.tables = tables
+        # default: half of the device memory the resident tables left free
+        self.gate = MemoryGate(budget if budget is not None else be.memory_info().free_bytes // 2)
+        self.max_q = max_vector_queries
+        self.queues = {k: queue.Queue() for k in self.KINDS}
+        self.held: dict[str, Request] = {}  # a request that did not fit its kind's last batch
+        self.streams = {k: be.create_stream() for k in self.KINDS}
+        self.t0 = 0.0
+        self.errors: list = []
+
+    def _take(self, kind: str) -> Optional[list[Request]]:
+        q = self.queues[kind]
+        first = self.held.pop(kind, None)
+        if first is None:
+            first = q.get()
+        if first is None:
+            return None
+        batch = [first]
+        if kind == "vector":
+            nq = len(first.payload)
+            while True:  # every vector request waiting now, up to max_q queries
+                try:
+                    nxt = q.get_nowait()
+                except queue.Empty:
+                    break
+                if nxt is None:  # the end marker stays for the next take
+                    q.put(None)
+                    break
+                if nq + len(nxt.payload) > self.max_q:
+                    self.held[kind] = nxt  # heads the next batch: order within a kind is kept
+                    break
+                batch.append(nxt)
+                nq += len(nxt.payload)
+        return batch
+
+    def _worker(self, kind: str) -> None:
+        stream = self.streams[kind]
+        while True:
+            batch = self._take(kind)
+            if batch is None:
+                return
+            n = footprint(self.tables, kind, batch)
+            self.gate.acquire(n)
+            start = time.perf_counter() - self.t0
+            try:
+                results = execute(self.be, self.tables, stream, kind, batch)
+            except Exception as e:  # a failed request fails alone
+                self.errors.append(e)
+                results = [e] * len(batch)
+            finally:
+                self.gate.release(n)
+            done = time.perf_counter() - self.t0
+            for r, res in zip(batch, results):
+                r.result, r.start, r.done, r.batch = res, start, done, len(batch)
+
+    def run(self, requests: list[Request]) -> dict:
+        """Feeds `requests` at their arrival times and serves them; returns
+        latency and throughput statistics."""
+        autotune.set_knob("jit.params", 1)  # one kernel per query shape
+        workers = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in self.KINDS]
+        self.t0 = time.perf_counter()
+        for w in workers:
+            w.start()
+        for r in sorted(requests, key=lambda r: r.arrival):
+            wait = r.arrival - (time.perf_counter() - self.t0)
+            if wait > 0:
+                time.sleep(wait)
+            self.queues[r.kind].put(r)
+        for k in self.KINDS:
+            self.queues[k].put(None)
+        for w in workers:
+            w.join()
+        return stats(requests, time.perf_counter() - self.t0)
+
+
+def run_one_at_a_time(be: RocmBackend, tables: Tables, requests: list[Request]) -> dict:
+    """The plain path: one stream, arrival order, every request alone."""
+    autotune.set_knob("jit.params", 1)
+    stream = be.create_stream()
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda r: r.arrival):
+        wait = r.arrival - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        r.start = time.perf_counter() - t0
+        r.result = execute(be, tables, stream, r.kind, [r])[0]
+        r.done = time.perf_counter() - t0
+        r.batch = 1
+    return stats(requests, time.perf_counter() - t0)
+
+
+def stats(requests: list[Request], elapsed: float) -> dict:
+    lat = np.array([r.latency for r in requests]) * 1000
+    out = {"requests": len(requests), "elapsed_s": elapsed, "throughput_rps": len(requests) / elapsed,
+           "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99))}
+    for k in Server.KINDS:
+        kl = np.array([r.latency for r in requests if r.kind == k]) * 1000
+        if len(kl):
+            out[k] = {"requests": int(len(kl)), "latency_ms_p50": float(np.percentile(kl, 50)),
+                      "latency_ms_p99": float(np.percentile(kl, 99)),
+                      "mean_batch": float(np.mean([r.batch for r in requests if r.kind == k]))}
+    return out
+
+
+def poisson_mix(rate: float, n: int, query_pool: np.ndarray, dim_rows: int, seed: int = 7) -> list[Request]:
+    """n requests arriving as a Poisson process at `rate` per second: 60%
+    vector (1-16 queries from the pool), 30% filter_agg (T in [1, 500]: 0.1%
+    to 50% of the rows), 10% join_agg (an fk range of 1-10% of the keys)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        u = rng.random()
+        if u < 0.6:
+            q = int(rng.integers(1, 17))
+            at = rng.integers(0, len(query_pool), q)
+            out.append(Request(i, "vector", float(t[i]), query_pool[at]))
+        elif u < 0.9:
+            out.append(Request(i, "filter_agg", float(t[i]), int(rng.integers(1, 501))))
+        else:
+            w = int(dim_rows * rng.uniform(0.01, 0.1))
+            lo = int(rng.integers(0, max(1, dim_rows - w)))
+            out.append(Request(i, "join_agg", float(t[i]), (lo, lo + w)))
+    return out
+
+
+def same_result(a: Request, b: Request, rel: float = 1e-12, dist_tol: float = 1e-6) -> bool:
+    """Whether two runs of one request agree: top-k rows equal except where
+    distances tie within `dist_tol` (then the distances must agree);
+    aggregate integers exact, floats within `rel` relative (a float sum's
+    order of additions depends on scheduling)."""
+    if isinstance(a.result, Exception) or isinstance(b.result, Exception):
+        return False
+    if a.kind == "vector":
+        (ra, da), (rb, db) = a.result, b.result
+        if ra.shape != rb.shape or not np.allclose(da, db, rtol=0, atol=dist_tol):
+            return False
+        for qa, qb, dq in zip(ra, rb, da):
+            for i in np.nonzero(qa != qb)[0]:
+                if not (np.abs(dq - dq[i]) <= dist_tol).sum() > 1:  # a row only swaps with a tie
+                    return False
+        return True
+    ta, tb = a.result, b.result
+    if ta.num_rows != tb.num_rows or ta.schema != tb.schema:
+        return False
+    for x, y in zip(ta.columns, tb.columns):
+        if not np.array_equal(x.is_null().to_numpy(zero_copy_only=False),
+                              y.is_null().to_numpy(zero_copy_only=False)):
+            return False
+        zero = 0.0 if pa.types.is_floating(x.type) else 0
+        xv = x.fill_null(zero).to_numpy(zero_copy_only=False)
+        yv = y.fill_null(zero).to_numpy(zero_copy_only=False)
+        if xv.dtype.kind == "f":
+            ok = np.isnan(xv) == np.isnan(yv)
+            scale = np.maximum(np.abs(np.nan_to_num(xv)), 1.0)
+            if not (ok.all() and np.all(np.abs(np.nan_to_num(xv) - np.nan_to_num(yv)) <= rel * scale)):
+                return False
+        elif not np.array_equal(xv, yv):
+            return False
+    return True

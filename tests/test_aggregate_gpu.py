@@ -25,15 +25,18 @@ pytestmark = pytest.mark.gpu
 SUM_REL = 1e-12
 
 
-@pytest.fixture(autouse=True, params=["generated", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "prepared", "interpreter"])
 def path(request):
-    # every test runs on the kernel generated per aggregation shape and on
-    # the interpreter kernel
+    # every test runs on the kernel generated per aggregation shape (with
+    # literals as constants, and prepared: literals as kernel arguments) and
+    # on the interpreter kernel
     from oxidelake_amd import autotune
 
-    autotune.set_knob("agg.jit", 1 if request.param == "generated" else 0)
+    autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
+    autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
     yield request.param
     autotune.set_knob("agg.jit", -1)
+    autotune.set_knob("jit.params", -1)
 
 
 def fbits(v):
@@ -238,7 +241,7 @@ def test_generated_kernel_is_the_one_that_runs(backend, stream, path):
     lib.oxr_profile_reset()
     try:
         backend.aggregate(stream, dev, datagen.config1_spec())
-        want = (1, 0) if path == "generated" else (0, 1)
+        want = (0, 1) if path == "interpreter" else (1, 0)
         assert (autotune.launches("aggregate"), autotune.launches("aggregate_vm")) == want
     finally:
         lib.oxr_profile_enable(0)

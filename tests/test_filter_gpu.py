@@ -2,8 +2,9 @@
 
 Bit-exact: the selected rows, their order, every projected value (bit
 pattern) and every NULL must equal the oracle's restatement of
-crates/oxidelake-device/src/cpu/kernels.rs:99-152. Both device paths are
-checked: the kernel generated per query shape ("generated") and the
+crates/oxidelake-device/src/cpu/kernels.rs:99-152. Three device paths are
+checked: the kernel generated per query shape, with literals compiled in
+("generated") or passed as kernel arguments ("prepared"), and the
 expression interpreter kernel ("interpreter").
 """
 import numpy as np
@@ -19,11 +20,14 @@ from tests.util import assert_batches_equal
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["generated", "interpreter"])
+@pytest.fixture(params=["generated", "prepared", "interpreter"])
 def path(request):
-    autotune.set_knob("filter.jit", 1 if request.param == "generated" else 0)
+    # prepared: the generated kernel with literals passed as arguments
+    autotune.set_knob("filter.jit", 0 if request.param == "interpreter" else 1)
+    autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
     yield request.param
     autotune.set_knob("filter.jit", -1)
+    autotune.set_knob("jit.params", -1)
 
 
 @pytest.fixture

@@ -45,7 +45,7 @@ def parse():
                     help="rows per lane of the generated kernel (default: autotuned)")
     ap.add_argument("--workload", choices=["filter", "vector", "aggregate", "join", "serving"], default="filter",
                     help="filter: the 200M-row filter+projection (default); vector: 20M x 768 top-10; "
-                         "aggregate: the config-1 query over 100M rows; join: 500M x 10M join + aggregate")
+                         "aggregate: the k/v GROUP BY query over 100M rows; join: 500M x 10M join + aggregate")
     ap.add_argument("--agg-rows", type=int, default=100_000_000)
     ap.add_argument("--fact-rows", type=int, default=500_000_000)
     ap.add_argument("--dim-rows", type=int, default=10_000_000)
@@ -67,7 +67,7 @@ def cpu_baseline(host_sample, threshold: int):
     from oracle import oracle as O
     from oxidelake_amd import datagen
 
-    pred, proj = datagen.config2_query(threshold)
+    pred, proj = datagen.filter_query(threshold)
     O.filter_project(host_sample, pred, proj)  # warm
     best, reps, t0 = float("inf"), 0, time.perf_counter()
     while reps < 5 and time.perf_counter() - t0 < 20:
@@ -117,7 +117,7 @@ class _DevPtr:
 
 
 def bench_vector(args):
-    """Config 4: top-10 over 20M x 768 L2-normalised mixture rows (cosine),
+    """Embedding search: top-10 over 20M x 768 L2-normalised mixture rows (cosine),
     query batches of 1, 32 and 256; plain FP32, BF16 + FP32 rerank of 100,
     FP8 e4m3 (per-row scale) + FP32 rerank of 100. QPS and latency p50/p99
     per batch size; recall@10 and max distance error against an FP64 oracle
@@ -133,7 +133,7 @@ def bench_vector(args):
     n, dim, qmax, k = args.vec_rows, 768, 256, 10
     metric = plan.DistanceMetric.Cosine
     # rows [n, n + 256) of the same counter-based mixture are the queries
-    table = datagen.table(be, stream, datagen.config4_columns(seed=42, dim=dim), n + qmax)
+    table = datagen.table(be, stream, datagen.embedding_columns(seed=42, dim=dim), n + qmax)
     base_ptr = table.column(0).values
     rows_view = be.wrap([(_DevPtr(base_ptr, n, dim), None, plan.F32_VECTOR, dim)], ["emb"])
     q_view = be.wrap([(_DevPtr(base_ptr + n * dim * 4, qmax, dim), None, plan.F32_VECTOR, dim)], ["emb"])
@@ -304,8 +304,8 @@ def bench_aggregate(args):
     be = RocmBackend(0)
     stream = be.create_stream()
     n = args.agg_rows
-    dev = datagen.table(be, stream, datagen.config1_columns(seed=42), n)
-    spec = datagen.config1_spec()
+    dev = datagen.table(be, stream, datagen.groupby_table_columns(seed=42), n)
+    spec = datagen.groupby_query_spec()
 
     def step():
         st = AggState(be, stream, dev, spec)
@@ -355,7 +355,7 @@ def bench_aggregate(args):
 
 
 def bench_join(args):
-    """Config 3: fact (fk Int64 Zipf(1.1) over the dimension's keys, qty Int32,
+    """Star join: fact (fk Int64 Zipf(1.1) over the dimension's keys, qty Int32,
     price Float64) JOIN dim (pk Int64 a permutation, grp Int32 over 1M groups)
     ON fk = pk, SUM(qty), SUM(qty*price), COUNT(*) GROUP BY grp, all in HBM:
     the build once, then one fused probe + aggregate pass per step."""
@@ -368,10 +368,10 @@ def bench_join(args):
     be = RocmBackend(0)
     stream = be.create_stream()
     nf, nd, groups = args.fact_rows, args.dim_rows, args.groups
-    fact_cols, dim_cols = datagen.config3_columns(nf, nd, groups, seed=42)
+    fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=42)
     dfact = datagen.table(be, stream, fact_cols, nf)
     ddim = datagen.table(be, stream, dim_cols, nd)
-    spec = datagen.config3_spec()
+    spec = datagen.star_query_spec()
     lib.oxr_profile_reset()
     lib.oxr_profile_enable(1)
     t0 = time.perf_counter()
@@ -458,14 +458,14 @@ def bench_serving(args):
 
     be = RocmBackend(0)
     stream = be.create_stream()
-    ftab = datagen.table(be, stream, datagen.config2_columns(seed=42), args.rows)
+    ftab = datagen.table(be, stream, datagen.filter_table_columns(seed=42), args.rows)
     nf, nd, groups = args.serve_fact_rows, args.dim_rows, args.groups
-    fact_cols, dim_cols = datagen.config3_columns(nf, nd, groups, seed=42)
+    fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=42)
     fact = datagen.table(be, stream, fact_cols, nf)
     dim = datagen.table(be, stream, dim_cols, nd)
     join = be.prepare_join_build(stream, dim, 0)
     n, d, pool = args.vec_rows, 768, 4096
-    emb = datagen.table(be, stream, datagen.config4_columns(seed=42, dim=d), n + pool)
+    emb = datagen.table(be, stream, datagen.embedding_columns(seed=42, dim=d), n + pool)
     base = emb.column(0).values
     rows_view = be.wrap([(_DevPtr(base, n, d), None, plan.F32_VECTOR, d)], ["emb"])
     q_view = be.wrap([(_DevPtr(base + n * d * 4, pool, d), None, plan.F32_VECTOR, d)], ["emb"])
@@ -547,11 +547,11 @@ def main():
     be = RocmBackend(0)
     stream = be.create_stream()
     n = args.rows
-    dev = datagen.table(be, stream, datagen.config2_columns(seed=42), n)
+    dev = datagen.table(be, stream, datagen.filter_table_columns(seed=42), n)
 
     def run(threshold):
         # 1M-row batches, each filtered and compacted on its own, one launch
-        pred, proj = datagen.config2_query(threshold)
+        pred, proj = datagen.filter_query(threshold)
         return be.filter_project_batches(stream, dev, BATCH_ROWS, pred, proj, ["a", "b2", "cd"])
 
     def rows_of(outs):
@@ -568,7 +568,7 @@ def main():
     outs = out_host = None
     del host
     # the generator is counter-based: these are the full table's first 2M rows
-    sample = be.download(stream, datagen.table(be, stream, datagen.config2_columns(seed=42), 2_000_000))
+    sample = be.download(stream, datagen.table(be, stream, datagen.filter_table_columns(seed=42), 2_000_000))
 
     # launch shape picked by the project autotuner on this device and table
     from oxidelake_amd import autotune

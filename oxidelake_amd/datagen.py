@@ -44,7 +44,7 @@ class GenColumn:
 
 
 # ---------------------------------------------------------------- configs
-def config1_columns(seed: int = 42) -> list[GenColumn]:
+def groupby_table_columns(seed: int = 42) -> list[GenColumn]:
     """1M-row table: id Int64 0..N, k Int32 U[0,16), v Float64 U[0,8),
     emb FixedSizeList<Float32, 8> i.i.d. N(0,1); no NULLs."""
     return [GenColumn("id", plan.INT64, GEN_SEQUENCE, 0, 0, seed=seed),
@@ -53,7 +53,7 @@ def config1_columns(seed: int = 42) -> list[GenColumn]:
             GenColumn("emb", plan.F32_VECTOR, GEN_MIXTURE, 0, 1.0, dim=8, seed=seed)]
 
 
-def config1_spec() -> plan.AggregateSpec:
+def groupby_query_spec() -> plan.AggregateSpec:
     """SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k (k = col 1, v = col 2)."""
     C = plan.Comparison
     return plan.AggregateSpec(
@@ -62,7 +62,7 @@ def config1_spec() -> plan.AggregateSpec:
                          plan.cmp(plan.col(2), C.Lt, plan.Lit.float64(4.0))))
 
 
-def config2_columns(seed: int = 42, null_frac: float = 0.05) -> list[GenColumn]:
+def filter_table_columns(seed: int = 42, null_frac: float = 0.05) -> list[GenColumn]:
     """a Int64 U[0,2^40), b Int32 U[0,1000), c Float32 N(0,1) with 5% NULLs,
     d Float64 lognormal(0,1)."""
     return [GenColumn("a", plan.INT64, GEN_UNIFORM_INT, 0, float(1 << 40), seed=seed),
@@ -71,7 +71,7 @@ def config2_columns(seed: int = 42, null_frac: float = 0.05) -> list[GenColumn]:
             GenColumn("d", plan.FLOAT64, GEN_LOGNORMAL, 0.0, 1.0, seed=seed)]
 
 
-def config2_query(b_threshold: int = 50, c_threshold: float = 0.5, modulus: int = 7):
+def filter_query(b_threshold: int = 50, c_threshold: float = 0.5, modulus: int = 7):
     """SELECT a, b*2+1, c*d FROM t WHERE b < T AND c > 0.5 OR a % 7 = 3 over
     columns (a, b, c, d) = (0, 1, 2, 3). Returns (predicate, projection)."""
     C = plan.Comparison
@@ -86,7 +86,7 @@ def config2_query(b_threshold: int = 50, c_threshold: float = 0.5, modulus: int 
     return pred, proj
 
 
-def config2_selectivity(b_threshold: int, c_threshold: float = 0.5, null_frac: float = 0.05,
+def filter_selectivity(b_threshold: int, c_threshold: float = 0.5, null_frac: float = 0.05,
                         modulus: int = 7) -> float:
     """Expected selectivity: P(A) + 1/m - P(A)/m with P(A) = T/1000 * (1-nulls) * P(c > t)."""
     from math import erf, sqrt
@@ -96,7 +96,7 @@ def config2_selectivity(b_threshold: int, c_threshold: float = 0.5, null_frac: f
     return pa_ + 1 / modulus - pa_ / modulus
 
 
-def config3_columns(fact_rows: int, dim_rows: int, groups: int, seed: int = 42):
+def star_columns(fact_rows: int, dim_rows: int, groups: int, seed: int = 42):
     """fact: fk Int64 Zipf(1.1) over dim_rows keys, qty Int32 U[1,100],
     price Float64 U[1,1000]; dim: pk Int64 a permutation of 0..dim_rows-1,
     grp Int32 U[0, groups)."""
@@ -108,7 +108,7 @@ def config3_columns(fact_rows: int, dim_rows: int, groups: int, seed: int = 42):
     return fact, dim
 
 
-def config3_spec() -> plan.AggregateSpec:
+def star_query_spec() -> plan.AggregateSpec:
     """SUM(qty), SUM(qty*price), COUNT(*) over the fact columns (qty = 1,
     price = 2); the group key comes from the dimension side."""
     F = plan.AggregateFunction
@@ -119,7 +119,7 @@ def config3_spec() -> plan.AggregateSpec:
     ])
 
 
-def config4_columns(seed: int = 42, dim: int = 768, centroids: int = 1024, sigma: float = 0.3):
+def embedding_columns(seed: int = 42, dim: int = 768, centroids: int = 1024, sigma: float = 0.3):
     """Embeddings: a Gaussian mixture of N(0,1) centroids, sigma per point,
     L2-normalised."""
     return [GenColumn("emb", plan.F32_VECTOR, GEN_MIXTURE_NORMALIZED, centroids, sigma, dim=dim, seed=seed)]

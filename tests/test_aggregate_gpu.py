@@ -174,7 +174,7 @@ def test_many_groups_and_streaming_state(backend, stream, oracle):
     assert_agg_equal(got, want, spec)
 
 
-def test_config1_query(backend, stream, oracle):
+def test_groupby_query(backend, stream, oracle):
     # SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k over 1M rows
     rng = np.random.default_rng(42)
     n = 1_000_000
@@ -236,11 +236,11 @@ def test_generated_kernel_is_the_one_that_runs(backend, stream, path):
     from oxidelake_amd import autotune
     from oxidelake_amd._capi import lib
 
-    dev = datagen.table(backend, stream, datagen.config1_columns(seed=3), 10_000)
+    dev = datagen.table(backend, stream, datagen.groupby_table_columns(seed=3), 10_000)
     lib.oxr_profile_enable(1)
     lib.oxr_profile_reset()
     try:
-        backend.aggregate(stream, dev, datagen.config1_spec())
+        backend.aggregate(stream, dev, datagen.groupby_query_spec())
         want = (0, 1) if path == "interpreter" else (1, 0)
         assert (autotune.launches("aggregate"), autotune.launches("aggregate_vm")) == want
     finally:

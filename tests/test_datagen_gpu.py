@@ -10,15 +10,15 @@ pytestmark = pytest.mark.gpu
 
 
 def test_prefix_of_a_larger_table_is_the_smaller_table(backend, stream):
-    cols = datagen.config2_columns(seed=11)
+    cols = datagen.filter_table_columns(seed=11)
     small = backend.download(stream, datagen.table(backend, stream, cols, 1000))
     big = backend.download(stream, datagen.table(backend, stream, cols, 100_000))
     assert big.slice(0, 1000).equals(small)
 
 
-def test_config2_distributions(backend, stream):
+def test_filter_table_distributions(backend, stream):
     n = 1_000_000
-    t = backend.download(stream, datagen.table(backend, stream, datagen.config2_columns(seed=3), n))
+    t = backend.download(stream, datagen.table(backend, stream, datagen.filter_table_columns(seed=3), n))
     a, b, c, d = (t.column(i) for i in range(4))
     a_, b_ = a.to_numpy(), b.to_numpy()
     assert a_.min() >= 0 and a_.max() < 2**40 and abs(a_.mean() / 2**39 - 1) < 0.01
@@ -49,7 +49,7 @@ def test_permutation_and_zipf(backend, stream):
 
 
 def test_normalized_mixture_vectors(backend, stream):
-    cols = datagen.config4_columns(seed=1, dim=64, centroids=16)
+    cols = datagen.embedding_columns(seed=1, dim=64, centroids=16)
     t = backend.download(stream, datagen.table(backend, stream, cols, 5000))
     v = np.asarray(t.column(0).flatten().to_numpy()).reshape(5000, 64)
     assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-5)

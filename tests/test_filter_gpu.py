@@ -177,17 +177,17 @@ def test_vector_null_element_is_declined(backend, stream):
 
 
 @pytest.mark.parametrize("threshold", [50, 500])
-def test_config2_query_parity(backend, stream, oracle, path, threshold):
+def test_filter_query_parity(backend, stream, oracle, path, threshold):
     # SELECT a, b*2+1, c*d FROM t WHERE b < T AND c > 0.5 OR a % 7 = 3
     n = 300_001
-    dev = datagen.table(backend, stream, datagen.config2_columns(seed=7), n)
+    dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=7), n)
     host = backend.download(stream, dev)
     assert host.column(2).null_count > 0
-    pred, proj = datagen.config2_query(threshold)
+    pred, proj = datagen.filter_query(threshold)
     got = backend.download(stream, backend.filter_project(stream, dev, pred, proj, ["a", "b2", "cd"]))
     want = oracle.filter_project(host, pred, proj, ["a", "b2", "cd"])
     assert_batches_equal(got, want)
-    assert abs(got.num_rows / n - datagen.config2_selectivity(threshold)) < 0.01
+    assert abs(got.num_rows / n - datagen.filter_selectivity(threshold)) < 0.01
 
 
 def test_batches_outlive_their_stream(backend, oracle):
@@ -224,9 +224,9 @@ def test_every_launch_shape_is_bit_exact(backend, stream, oracle, shape):
                                (PREDICATES[12], PROJECTIONS[3])]:
                 got, want = run_both(backend, stream, oracle, batch, pred, proj)
                 assert_batches_equal(got, want, names=False)
-        dev = datagen.table(backend, stream, datagen.config2_columns(seed=5), 100_003)
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=5), 100_003)
         host = backend.download(stream, dev)
-        pred, proj = datagen.config2_query(50)
+        pred, proj = datagen.filter_query(50)
         got = backend.download(stream, backend.filter_project(stream, dev, pred, proj))
         assert_batches_equal(got, oracle.filter_project(host, pred, proj), names=False)
     finally:
@@ -246,9 +246,9 @@ def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows):
             for pred, proj in [(PREDICATES[10], PROJECTIONS[2]), (PREDICATES[12], PROJECTIONS[3])]:
                 got, want = run_both(backend, stream, oracle, batch, pred, proj)
                 assert_batches_equal(got, want, names=False)
-        dev = datagen.table(backend, stream, datagen.config2_columns(seed=5), 100_003)
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=5), 100_003)
         host = backend.download(stream, dev)
-        pred, proj = datagen.config2_query(50)
+        pred, proj = datagen.filter_query(50)
         outs = backend.filter_project_batches(stream, dev, 8192, pred, proj)
         for i, o in enumerate(outs):
             want = oracle.filter_project(host.slice(i * 8192, 8192), pred, proj)
@@ -261,8 +261,8 @@ def test_generated_kernel_is_the_one_that_runs(backend, stream):
     # the per-query kernel, not the interpreter, runs a query in its grammar
     from oxidelake_amd._capi import lib
 
-    dev = datagen.table(backend, stream, datagen.config2_columns(seed=3), 10_000)
-    pred, proj = datagen.config2_query(50)
+    dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=3), 10_000)
+    pred, proj = datagen.filter_query(50)
     lib.oxr_profile_enable(1)
     lib.oxr_profile_reset()
     try:
@@ -300,9 +300,9 @@ def test_many_batches_in_one_launch(backend, stream, oracle, path, batch_rows):
     # each batch of the input compacted on its own, as the reference filters
     # batch by batch (exec/filter.rs:182-210)
     n = 250_003
-    dev = datagen.table(backend, stream, datagen.config2_columns(seed=9), n)
+    dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=9), n)
     host = backend.download(stream, dev)
-    pred, proj = datagen.config2_query(500)
+    pred, proj = datagen.filter_query(500)
     outs = backend.filter_project_batches(stream, dev, batch_rows, pred, proj, ["a", "b2", "cd"])
     assert len(outs) == -(-n // batch_rows)
     for i, o in enumerate(outs):

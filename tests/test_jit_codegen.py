@@ -13,7 +13,7 @@ from tests.test_filter_gpu import PREDICATES, PROJECTIONS
 # random_batch's schema: i Int64, j Int32, f Float32, g Float64, e vector<3>
 RANDOM = [(plan.INT64, True), (plan.INT32, True), (plan.FLOAT32, True), (plan.FLOAT64, True),
           (plan.F32_VECTOR, 3, True)]
-CONFIG2 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT32, True), (plan.FLOAT64, False)]
+FILTER_TABLE = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT32, True), (plan.FLOAT64, False)]
 
 
 @pytest.fixture(autouse=True)
@@ -34,10 +34,10 @@ def test_every_test_projection_compiles(qi):
     backend.filter_kernel_source(RANDOM, None, PROJECTIONS[qi], compile=True)
 
 
-def test_config2_kernel_shape():
+def test_filter_query_kernel_shape():
     autotune.set_knob("filter.rows", -1)
-    pred, proj = datagen.config2_query()
-    src = backend.filter_kernel_source(CONFIG2, pred, proj, compile=True)
+    pred, proj = datagen.filter_query()
+    src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
     # four 1-2 register columns plus one validity word: 16 rows per lane
     assert "filter_tile<Q, 16>" in src
     # a % 7 by a constant divisor (the compiler's multiply-high), no error path
@@ -45,8 +45,8 @@ def test_config2_kernel_shape():
     # literals are constants: b < 50, then c > 0.5 by its bit pattern
     assert "t3 < ((int64_t)0x0000000000000032ull)" in src
     assert "order_key64(as_f64(0x3fe0000000000000ull))" in src
-    pred2, _ = datagen.config2_query(500)
-    src2 = backend.filter_kernel_source(CONFIG2, pred2, proj)
+    pred2, _ = datagen.filter_query(500)
+    src2 = backend.filter_kernel_source(FILTER_TABLE, pred2, proj)
     assert src2 == src.replace("0x0000000000000032ull", "0x00000000000001f4ull")
     # only c*d can be NULL
     assert "kValidOuts = 1;" in src
@@ -62,10 +62,10 @@ def test_division_by_zero_or_minus_one_keeps_its_error_path():
 
 
 def test_nullability_changes_the_kernel():
-    pred, proj = datagen.config2_query()
-    all_valid = [(t, False) for t, _ in CONFIG2]
+    pred, proj = datagen.filter_query()
+    all_valid = [(t, False) for t, _ in FILTER_TABLE]
     a = backend.filter_kernel_source(all_valid, pred, proj)
-    b = backend.filter_kernel_source(CONFIG2, pred, proj)
+    b = backend.filter_kernel_source(FILTER_TABLE, pred, proj)
     assert a != b and "row_bit" not in a
 
 
@@ -77,18 +77,18 @@ def test_vector_projection_is_a_wave_copy():
 @pytest.mark.parametrize("rows", [1, 2, 8, 16])
 def test_every_rows_per_lane_compiles(rows):
     autotune.set_knob("filter.rows", rows)
-    pred, proj = datagen.config2_query()
-    src = backend.filter_kernel_source(CONFIG2, pred, proj, compile=True)
+    pred, proj = datagen.filter_query()
+    src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
     assert f"filter_tile<Q, {rows}>" in src
 
 
 def test_plan_errors_match_the_interpreter():
     with pytest.raises(PlanError):
-        backend.filter_kernel_source(CONFIG2, Predicate.compare(9, Cmp.Eq, Lit.int64(1)), [0])
+        backend.filter_kernel_source(FILTER_TABLE, Predicate.compare(9, Cmp.Eq, Lit.int64(1)), [0])
     with pytest.raises(Unsupported):
         backend.filter_kernel_source(RANDOM, Predicate.compare(4, Cmp.Eq, Lit.int64(1)), [0])
     with pytest.raises(Unsupported):
-        backend.filter_kernel_source(CONFIG2, None, [plan.cmp(col(0), Cmp.Eq, col(1))])
+        backend.filter_kernel_source(FILTER_TABLE, None, [plan.cmp(col(0), Cmp.Eq, col(1))])
 
 
 # ------------------------------------------------------------- aggregation
@@ -103,15 +103,15 @@ def test_every_test_aggregation_compiles(si):
     assert "oxr_aggregate_jit" in src
 
 
-def test_config1_and_config3_kernels():
+def test_groupby_and_star_kernels():
     autotune.set_knob("agg.rows", -1)
     c1 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False), (plan.F32_VECTOR, 8, False)]
-    src = backend.aggregate_kernel_source(c1, datagen.config1_spec(), compile=True)
+    src = backend.aggregate_kernel_source(c1, datagen.groupby_query_spec(), compile=True)
     # k and v only (the id and vector columns are never loaded), no NULLs
     assert "x.c1" in src and "x.c2" in src and "x.c0" not in src and "x.c3" not in src
     assert "kAnyNullable = false" in src and "aggregate_tiles<Q, 16>" in src
     c3 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False)]
-    src = backend.aggregate_kernel_source(c3, datagen.config3_spec(), probe_key=0, compile=True)
+    src = backend.aggregate_kernel_source(c3, datagen.star_query_spec(), probe_key=0, compile=True)
     assert "join_group(p, (int64_t)x.c0, key, kv)" in src
 
 
@@ -126,15 +126,15 @@ def test_prepared_mode_shares_one_kernel_across_literals():
     # knob jit.params: literals become kernel arguments (one binary per shape)
     autotune.set_knob("jit.params", 1)
     try:
-        p50, proj = datagen.config2_query(50)
-        p500, _ = datagen.config2_query(500)
-        a = backend.filter_kernel_source(CONFIG2, p50, proj, compile=True)
-        assert a == backend.filter_kernel_source(CONFIG2, p500, proj) and "L[0]" in a
+        p50, proj = datagen.filter_query(50)
+        p500, _ = datagen.filter_query(500)
+        a = backend.filter_kernel_source(FILTER_TABLE, p50, proj, compile=True)
+        assert a == backend.filter_kernel_source(FILTER_TABLE, p500, proj) and "L[0]" in a
         # no literal-divisor shortcut: the checked division path
         assert "i64_divmod(true" in a
         spec = lambda t: plan.AggregateSpec(1, [plan.Aggregate(plan.AggregateFunction.Sum, col(0))],
                                             filter=Predicate.compare(1, Cmp.Lt, Lit.int64(t)))
-        x = backend.aggregate_kernel_source(CONFIG2, spec(10), compile=True)
-        assert x == backend.aggregate_kernel_source(CONFIG2, spec(400)) and "L[0]" in x
+        x = backend.aggregate_kernel_source(FILTER_TABLE, spec(10), compile=True)
+        assert x == backend.aggregate_kernel_source(FILTER_TABLE, spec(400)) and "L[0]" in x
     finally:
         autotune.set_knob("jit.params", -1)

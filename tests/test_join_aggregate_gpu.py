@@ -85,14 +85,14 @@ def test_filter_and_streaming_pieces(backend, stream, oracle):
     assert_agg_equal(got, want, spec)
 
 
-def test_config3_shape_with_zipf_skew(backend, stream, oracle):
+def test_star_query_with_zipf_skew(backend, stream, oracle):
     # SUM(qty), SUM(qty*price), COUNT(*) GROUP BY grp over fact JOIN dim ON fk = pk
     nf, nd, groups = 2_000_000, 100_000, 10_000
-    fact_cols, dim_cols = datagen.config3_columns(nf, nd, groups, seed=5)
+    fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=5)
     dfact = datagen.table(backend, stream, fact_cols, nf)
     ddim = datagen.table(backend, stream, dim_cols, nd)
     fact, dim = backend.download(stream, dfact), backend.download(stream, ddim)
-    aggs = datagen.config3_spec().aggregates
+    aggs = datagen.star_query_spec().aggregates
     table = backend.prepare_join_build(stream, ddim, 0)
     st = AggState(backend, stream, dfact, AggregateSpec(-1, aggs))
     st.update_join(stream, table, ddim, 1, dfact, 0)

@@ -141,7 +141,7 @@ def test_extended_grammar_matches_pyarrow(oracle):
     cv = rng.random(n) >= 0.05
     d = np.exp(rng.normal(size=n))
     batch = pa.record_batch({"a": pa.array(a), "b": pa.array(b_), "c": pa.array(c, mask=~cv), "d": pa.array(d)})
-    pred, proj = datagen.config2_query(50)
+    pred, proj = datagen.filter_query(50)
     got = oracle.filter_project(batch, pred, proj, ["a", "b2", "cd"])
     A, B, Cc, D = batch.columns
     keep = pc.or_kleene(pc.and_kleene(pc.less(B, 50), pc.greater(pc.cast(Cc, pa.float64()), 0.5)),

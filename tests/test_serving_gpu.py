@@ -30,12 +30,12 @@ class _DevPtr:
 @pytest.fixture(scope="module")
 def served(backend):
     s = backend.create_stream()
-    ftab = datagen.table(backend, s, datagen.config2_columns(seed=11), 1_000_000)
-    fact_cols, dim_cols = datagen.config3_columns(FACT, DIM_ROWS, GROUPS, seed=12)
+    ftab = datagen.table(backend, s, datagen.filter_table_columns(seed=11), 1_000_000)
+    fact_cols, dim_cols = datagen.star_columns(FACT, DIM_ROWS, GROUPS, seed=12)
     fact = datagen.table(backend, s, fact_cols, FACT)
     dim = datagen.table(backend, s, dim_cols, DIM_ROWS)
     join = backend.prepare_join_build(s, dim, 0)
-    emb = datagen.table(backend, s, datagen.config4_columns(seed=13, dim=DIM, centroids=64), VEC_ROWS + POOL)
+    emb = datagen.table(backend, s, datagen.embedding_columns(seed=13, dim=DIM, centroids=64), VEC_ROWS + POOL)
     base = emb.column(0).values
     rows = backend.wrap([(_DevPtr(base, VEC_ROWS, DIM), None, plan.F32_VECTOR, DIM)], ["emb"])
     qv = backend.wrap([(_DevPtr(base + VEC_ROWS * DIM * 4, POOL, DIM), None, plan.F32_VECTOR, DIM)], ["emb"])

@@ -43,9 +43,12 @@ def parse():
     ap.add_argument("--sweep", action="store_true", help="also time b thresholds 500 and 1000")
     ap.add_argument("--filter-rows", type=int, default=0,
                     help="rows per lane of the generated kernel (default: autotuned)")
-    ap.add_argument("--workload", choices=["filter", "vector", "aggregate", "join", "serving"], default="filter",
+    ap.add_argument("--workload", choices=["filter", "vector", "aggregate", "join", "serving", "copy"],
+                    default="filter",
                     help="filter: the 200M-row filter+projection (default); vector: 20M x 768 top-10; "
-                         "aggregate: the k/v GROUP BY query over 100M rows; join: 500M x 10M join + aggregate")
+                         "aggregate: the k/v GROUP BY query over 100M rows; join: 500M x 10M join + aggregate; "
+                         "serving: the mixed request loop; copy: a plain 1 GiB elementwise copy that "
+                         "calibrates the HBM byte counters")
     ap.add_argument("--agg-rows", type=int, default=100_000_000)
     ap.add_argument("--fact-rows", type=int, default=500_000_000)
     ap.add_argument("--dim-rows", type=int, default=10_000_000)
@@ -492,8 +495,11 @@ def bench_serving(args):
         if server.errors:
             raise server.errors[0]
         p_stats = serving.run_one_at_a_time(be, tables, alone)
-        bad = sum(not serving.same_result(a, b) for a, b in zip(reqs, alone))
+        diff = [serving.describe_difference(a, b) for a, b in zip(reqs, alone) if not serving.same_result(a, b)]
+        bad = len(diff)
         mismatches += bad
+        for line in diff[:5]:
+            print(f"# mismatch: {line}", file=sys.stderr, flush=True)
         same = same and bad == 0
         per_rate[str(int(rate))] = {"serving_loop": s_stats, "one_at_a_time": p_stats}
         print(f"# rate {rate:g}/s: loop {s_stats['throughput_rps']:.1f} req/s p99 "
@@ -522,6 +528,45 @@ def bench_serving(args):
     }), flush=True)
 
 
+def bench_copy(args):
+    """Calibration for the HBM byte counters: a plain elementwise kernel of
+    known traffic (y = -x over 2^28 float32: 1 GiB read, 1 GiB written per
+    launch). FETCH_SIZE and WRITE_SIZE of this kernel under rocprofv3 --pmc,
+    divided into 1 GiB, give the factor that turns the counters of another
+    kernel into bytes."""
+    import torch
+
+    n = 1 << 28
+    x = torch.randn(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    print(json.dumps({"workload": "copy: y = -x over 2^28 float32", "read_bytes": 4 * n, "write_bytes": 4 * n,
+                      "launches": args.steps + args.warmup, "ms_per_launch": ms,
+                      "gbs": 8 * n / (ms / 1000) / 1e9}), flush=True)
+
+
+def recorded_traffic(kernel: str, **workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc runs,
+    scaled by the copy calibration), when it was taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or any(rec.get("workload", {}).get(k) != v for k, v in workload.items()):
+        return None
+    return rec["bytes_per_launch"]
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -539,6 +584,9 @@ def main():
         return
     if args.workload == "serving":
         bench_serving(args)
+        return
+    if args.workload == "copy":
+        bench_copy(args)
         return
     from oxidelake_amd import datagen
     from oxidelake_amd._capi import lib
@@ -567,6 +615,7 @@ def main():
     ok = full_size_check(host, out_host, args.b_threshold)
     outs = out_host = None
     del host
+    print(f"# full-size parity checked: {ok}", file=sys.stderr, flush=True)
     # the generator is counter-based: these are the full table's first 2M rows
     sample = be.download(stream, datagen.table(be, stream, datagen.filter_table_columns(seed=42), 2_000_000))
 
@@ -578,6 +627,7 @@ def main():
         tuned = {"knobs": {"filter.rows": args.filter_rows}, "ms": None}
     else:
         tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
+    print(f"# launch shape: {tuned['knobs']}", file=sys.stderr, flush=True)
 
     lib.oxr_profile_enable(0)
     for _ in range(args.warmup):
@@ -634,7 +684,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": recorded_traffic("filter_project", rows=n, b_threshold=args.b_threshold),
         },
     }
     if args.sweep:

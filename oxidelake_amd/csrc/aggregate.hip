@@ -66,13 +66,15 @@ struct Params {
   int32_t probe_slot;     // join: slot of the probe key column
   JTable jt;
   DCol build_group;
+  int32_t exact_f64;      // compensated FP64 sums (aggregate_jit.h)
 };
 
 template <int NS>
 __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
   __shared__ int64_t s_keys[kLdsSlots + 2];
   __shared__ int64_t s_acc[kLdsAggs][kLdsSlots + 2];
-  __shared__ uint32_t s_has[kLdsAggs][kLdsSlots + 2];
+  __shared__ int64_t s_lo[kLdsAggs][kLdsSlots + 2];  // FP64 sums: rounding errors
+  __shared__ uint8_t s_has[kLdsAggs][kLdsSlots + 2];
   __shared__ uint32_t s_used;
   __shared__ uint32_t s_flush;
 
@@ -82,6 +84,7 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
     s_keys[i] = kEmpty;
     for (int a = 0; a < la; a++) {
       s_acc[a][i] = acc_init(p.aggs[a].kind);
+      s_lo[a][i] = 0;
       s_has[a][i] = 0;
     }
   }
@@ -108,7 +111,9 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
           for (int a = 0; a < la; a++) {
             const AggDesc d = p.aggs[a];
             const int64_t v = s_acc[a][i];
-            if (v != acc_init(d.kind) || d.kind == A_SUM_F64)
+            if (p.exact_f64 && d.kind == A_SUM_F64)
+              sum_f64(&p.t.acc[a][gs], &p.t.lo[a][gs], as_f64((uint64_t)v), as_f64((uint64_t)s_lo[a][i]));
+            else if (v != acc_init(d.kind) || d.kind == A_SUM_F64)
               acc_apply_global(d.kind, &p.t.acc[a][gs], v);
             if (p.t.has[a] && s_has[a][i]) p.t.has[a][gs] = 1;
           }
@@ -117,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
       s_keys[i] = kEmpty;
       for (int a = 0; a < la; a++) {
         s_acc[a][i] = acc_init(p.aggs[a].kind);
+        s_lo[a][i] = 0;
         s_has[a][i] = 0;
       }
     }
@@ -227,11 +233,13 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
         if (!((avm >> a) & 1u) && d.kind != A_COUNT_STAR) continue;
         const int64_t c = contribution(d.kind, d.in_type, av[a]);
         if (a < la && lslot >= 0) {
-          acc_apply_lds(d.kind, &s_acc[a][lslot], c);
+          if (p.exact_f64 && d.kind == A_SUM_F64) sum_f64(&s_acc[a][lslot], &s_lo[a][lslot], as_f64((uint64_t)c), 0.0);
+          else acc_apply_lds(d.kind, &s_acc[a][lslot], c);
           if (d.nullable) s_has[a][lslot] = 1;
         } else {
           if (!ok) continue;
-          acc_apply_global(d.kind, &p.t.acc[a][gslot], c);
+          if (p.exact_f64 && d.kind == A_SUM_F64) sum_f64(&p.t.acc[a][gslot], &p.t.lo[a][gslot], as_f64((uint64_t)c), 0.0);
+          else acc_apply_global(d.kind, &p.t.acc[a][gslot], c);
           if (p.t.has[a]) p.t.has[a][gslot] = 1;
         }
       }
@@ -302,7 +310,9 @@ __global__ void gather_kernel(Table t, const int64_t *keys_sorted, const uint64_
   }
   for (int a = 0; a < naggs; a++) {
     const OutCol o = outs[a];
-    const int64_t v = t.acc[a][slot];
+    int64_t v = t.acc[a][slot];
+    if (o.kind == A_SUM_F64 && t.lo[a])
+      v = (int64_t)f64_bits(sum_value(as_f64((uint64_t)v), as_f64((uint64_t)t.lo[a][slot])));
     bool valid;
     if (o.kind == A_COUNT || o.kind == A_COUNT_STAR) valid = true;
     else if (t.has[a]) valid = t.has[a][slot] != 0;
@@ -355,6 +365,7 @@ __global__ void rehash_kernel(Table from, Table to, int naggs) {
   }
   for (int a = 0; a < naggs; a++) {
     to.acc[a][dst] = from.acc[a][i];
+    if (from.lo[a]) to.lo[a][dst] = from.lo[a][i];
     if (from.has[a]) to.has[a][dst] = from.has[a][i];
   }
 }
@@ -412,8 +423,10 @@ int table_alloc(oxr_stream *s, oxr_agg_state *st, uint64_t cap, Table *out, void
   const int na = (int)st->aggs.size();
   const uint64_t slots = cap + 2;
   size_t bytes = slots * 8 * (1 + na) + 64;
-  for (int a = 0; a < na; a++)
+  for (int a = 0; a < na; a++) {
     if (st->aggs[a].nullable) bytes += slots * 4 + 16;
+    if (st->aggs[a].kind == A_SUM_F64) bytes += slots * 8;
+  }
   bytes += 64;  // seen, distinct, overflow
   void *m = nullptr;
   OXR_TRY(stream_alloc(s, bytes, &m));
@@ -425,6 +438,13 @@ int table_alloc(oxr_stream *s, oxr_agg_state *st, uint64_t cap, Table *out, void
   for (int a = 0; a < na; a++) {
     t.acc[a] = (int64_t *)p;
     p += slots * 8;
+  }
+  for (int a = 0; a < na; a++) {
+    t.lo[a] = nullptr;
+    if (st->aggs[a].kind == A_SUM_F64) {
+      t.lo[a] = (int64_t *)p;
+      p += slots * 8;
+    }
   }
   p = (char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
   for (int a = 0; a < na; a++) {
@@ -440,6 +460,8 @@ int table_alloc(oxr_stream *s, oxr_agg_state *st, uint64_t cap, Table *out, void
   t.overflow = (uint32_t *)(p + 24);
   int rc = fill64(s, t.keys, slots, kEmpty);
   for (int a = 0; a < na && rc == OXR_OK; a++) rc = fill64(s, t.acc[a], slots, acc_init(st->aggs[a].kind));
+  for (int a = 0; a < na && rc == OXR_OK; a++)
+    if (t.lo[a]) rc = fill64(s, t.lo[a], slots, 0);
   for (int a = 0; a < na && rc == OXR_OK; a++)
     if (t.has[a]) rc = fill_zero(s, t.has[a], slots * 4);
   if (rc == OXR_OK) rc = fill_zero(s, t.seen, 32);
@@ -713,6 +735,7 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   P.key_slot = st->key_col >= 0 ? st->key_slot : -1;
   P.naggs = (int32_t)st->aggs.size();
   P.lds_aggs = P.naggs <= kLdsAggs ? P.naggs : 0;
+  P.exact_f64 = tune_get("agg.exact_f64", 0) > 0 ? 1 : 0;
   for (int a = 0; a < P.naggs; a++) P.aggs[a] = st->aggs[a];
   P.t = st->t;
   if (js) {

@@ -7,6 +7,14 @@
 // is a group of its own; SUM/MIN/MAX skip NULL inputs; COUNT counts non-NULL
 // inputs; SUM over integers wraps in Int64; SUM over floats is an FP64 sum
 // from +0.0; MIN/MAX of floats compare totalOrder keys (kernels.rs:820-838).
+//
+// FP64 sums are plain atomic sums by default. In exact mode (knob
+// agg.exact_f64) they are compensated: every accumulator (lane pair, LDS
+// slot, HBM slot) is a pair (hi, lo) whose lo gathers the exact rounding
+// error of each addition into hi (TwoSum, Knuth TAOCP vol. 2, 4.2.2), so the
+// result hi + lo is the exact sum up to second-order terms, in whatever order
+// the atomics land: the same from run to run even where the sum cancels, at
+// the price of a second atomic per update.
 #pragma once
 
 #include "jtable.h"
@@ -52,6 +60,7 @@ struct Table {
   uint64_t cap;       // power of two; slots cap (NULL key) and cap+1 (INT64_MIN key) follow
   int64_t *keys;      // cap+2
   int64_t *acc[kMaxAggs];
+  int64_t *lo[kMaxAggs];    // FP64 SUM: the rounding errors of acc (doubles), else NULL
   uint32_t *has[kMaxAggs];  // cap+2 each, or NULL
   uint32_t *seen;     // [0] NULL-key group, [1] INT64_MIN group
   uint64_t *distinct; // new keys claimed
@@ -99,6 +108,41 @@ __device__ inline void acc_apply_lds(uint8_t kind, int64_t *addr, int64_t v) {
       break;
   }
 }
+
+// The exact error of s = fl(a + b) for a finite s (TwoSum; needs the adds
+// compiled without contraction).
+__device__ inline double two_sum_err(double a, double b, double s) {
+  const double bb = s - a;
+  return (a - (s - bb)) + (b - bb);
+}
+
+// Adds the compensated value (v, vlo) to the accumulator (hi, lo). The
+// atomic returns the value it added v to, so the rounding error of that
+// addition is recovered exactly and goes to lo with vlo. Works on LDS and
+// HBM addresses alike.
+__device__ inline void sum_f64(int64_t *hi, int64_t *lo, double v, double vlo) {
+  const double old = unsafeAtomicAdd((double *)hi, v);
+  const double s = old + v;
+  double e = vlo;
+  if (__builtin_isfinite(s)) e += two_sum_err(old, v, s);
+  if (e != 0.0) unsafeAtomicAdd((double *)lo, e);
+}
+
+// The compensated sum of (hi, lo) over the wave (every lane gets it).
+__device__ inline void wave_sum_f64(double &hi, double &lo) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const double oh = __shfl_xor(hi, d, 64);
+    const double ol = __shfl_xor(lo, d, 64);
+    const double s = hi + oh;
+    const double e = __builtin_isfinite(s) ? two_sum_err(hi, oh, s) : 0.0;
+    hi = s;
+    lo = (lo + ol) + e;
+  }
+}
+
+// hi + lo as the accumulator's value (lo == 0 keeps hi's sign of zero).
+__host__ __device__ inline double sum_value(double hi, double lo) { return lo != 0.0 ? hi + lo : hi; }
 
 // The per-row contribution of a value of `in_type` (widened to 64 bits:
 // ints sign-extended, floats as their bits) to an accumulator.
@@ -170,7 +214,8 @@ __device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, 
 
 // ------------------------------------------------- generated-kernel skeleton
 // The generated part is a struct Q:
-//   kAggs, kLdsAggs, kKind[kAggs], kNullable[kAggs], kAnyNullable
+//   kAggs, kLdsAggs, kKind[kAggs], kNullable[kAggs], kAnyNullable,
+//   kExactF64 (compensated FP64 sums)
 //   struct In, ins(p, base); struct Row, load(i, j, in, x)   (as for filters)
 //   pred(L, x, j, in, err)              the fused filter (true: keep; L: the
 //                                       literals, in prepared mode)
@@ -232,6 +277,14 @@ __device__ inline int64_t wave_combine(int64_t v) {
   return v;
 }
 
+// Index of aggregate a among the FP64 sums before it (its LDS lo array).
+template <class Q>
+__host__ __device__ constexpr int lo_index(int a) {
+  int n = 0;
+  for (int i = 0; i < a; i++) n += Q::kKind[i] == A_SUM_F64;
+  return n;
+}
+
 template <class Q, int S>
 __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
   constexpr int NA = Q::kAggs;
@@ -239,8 +292,10 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
   constexpr int LAS = LA > 0 ? LA : 1;
   constexpr int kTile = 256 * S;
   constexpr int HS = Q::kAnyNullable ? kLdsSlots + 2 : 1;  // validity flags only when needed
+  constexpr int NLO = Q::kExactF64 ? lo_index<Q>(LA) : 0;  // compensated FP64 sums in LDS
   __shared__ int64_t s_keys[kLdsSlots + 2];
   __shared__ int64_t s_acc[LAS][kLdsSlots + 2];
+  __shared__ int64_t s_lo[NLO > 0 ? NLO : 1][NLO > 0 ? kLdsSlots + 2 : 1];
   __shared__ uint32_t s_has[LAS][HS];
   __shared__ uint32_t s_used;
 
@@ -252,6 +307,8 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       s_acc[a][i] = acc_init(Q::kKind[a]);
       if (HS > 1) s_has[a][i] = 0;
     }
+#pragma unroll
+    for (int b = 0; b < NLO; b++) s_lo[b][i] = 0;
   }
   if (tid == 0) s_used = 0;
   __syncthreads();
@@ -281,8 +338,11 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       if (sel) sel = Q::key(p, x[s], j, in, key, kv);
       if (!kv) key = 0;  // every NULL key is the same group
       int64_t c[NA];
+      double cl[NA];  // FP64 sums: the rounding error carried with c
       uint32_t cm = 0;
       Q::contrib(p.lit, x[s], j, sel, c, cm, err);
+#pragma unroll
+      for (int a = 0; a < NA; a++) cl[a] = 0.0;
       // one matching round: the lanes sharing the first selected lane's key
       // fold into it, so a hot key costs one update per wave
       const uint64_t act = __ballot(sel);
@@ -298,13 +358,26 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
           for (int a = 0; a < NA; a++) {
             const bool use = same && ((cm >> a) & 1u);
             int64_t v = use ? c[a] : acc_init(Q::kKind[a]);
+            double vl = 0.0;
             switch (Q::kKind[a]) {
-              case A_SUM_F64: v = wave_combine<A_SUM_F64>(v); break;
+              case A_SUM_F64: {
+                if (Q::kExactF64) {
+                  double hi = as_f64((uint64_t)v);
+                  wave_sum_f64(hi, vl);
+                  v = (int64_t)f64_bits(hi);
+                } else {
+                  v = wave_combine<A_SUM_F64>(v);
+                }
+                break;
+              }
               case A_MIN_I64: case A_MIN_F64: case A_MIN_F32: v = wave_combine<A_MIN_I64>(v); break;
               case A_MAX_I64: case A_MAX_F64: case A_MAX_F32: v = wave_combine<A_MAX_I64>(v); break;
               default: v = wave_combine<A_SUM_I64>(v); break;
             }
-            if (lane == lead) c[a] = v;  // the other lanes keep their own
+            if (lane == lead) {  // the other lanes keep their own
+              c[a] = v;
+              cl[a] = vl;
+            }
             cmc |= (uint32_t)(__ballot(use) != 0) << a;
           }
           if (same) {
@@ -355,10 +428,17 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       for (int a = 0; a < NA; a++) {
         if (!((cm >> a) & 1u)) continue;
         if (a < LA && lslot >= 0) {
-          acc_apply_lds(Q::kKind[a], &s_acc[a < LA ? a : 0][lslot], c[a]);
+          if (Q::kExactF64 && Q::kKind[a] == A_SUM_F64)
+            sum_f64(&s_acc[a < LA ? a : 0][lslot], &s_lo[NLO > 0 ? lo_index<Q>(a < LA ? a : 0) : 0][NLO > 0 ? lslot : 0],
+                    as_f64((uint64_t)c[a]), cl[a]);
+          else
+            acc_apply_lds(Q::kKind[a], &s_acc[a < LA ? a : 0][lslot], c[a]);
           if (Q::kNullable[a] && HS > 1) s_has[a < LA ? a : 0][lslot] = 1;
         } else if (ok) {
-          acc_apply_global(Q::kKind[a], &p.t.acc[a][gslot], c[a]);
+          if (Q::kExactF64 && Q::kKind[a] == A_SUM_F64)
+            sum_f64(&p.t.acc[a][gslot], &p.t.lo[a][gslot], as_f64((uint64_t)c[a]), cl[a]);
+          else
+            acc_apply_global(Q::kKind[a], &p.t.acc[a][gslot], c[a]);
           if (p.t.has[a]) p.t.has[a][gslot] = 1;
         }
       }
@@ -378,7 +458,11 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
 #pragma unroll
       for (int a = 0; a < LA; a++) {
         const int64_t v = s_acc[a][i];
-        if (v != acc_init(Q::kKind[a]) || Q::kKind[a] == A_SUM_F64) acc_apply_global(Q::kKind[a], &p.t.acc[a][gs], v);
+        if (Q::kExactF64 && Q::kKind[a] == A_SUM_F64)
+          sum_f64(&p.t.acc[a][gs], &p.t.lo[a][gs], as_f64((uint64_t)v),
+                  as_f64((uint64_t)s_lo[NLO > 0 ? lo_index<Q>(a) : 0][NLO > 0 ? i : 0]));
+        else if (v != acc_init(Q::kKind[a]) || Q::kKind[a] == A_SUM_F64)
+          acc_apply_global(Q::kKind[a], &p.t.acc[a][gs], v);
         if (Q::kNullable[a] && HS > 1 && p.t.has[a] && s_has[a][i]) p.t.has[a][gs] = 1;
       }
     }

@@ -514,6 +514,8 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   bool any_null = false;
   for (uint8_t v : sh.nullable) any_null |= v != 0;
   src += std::string("  static constexpr bool kAnyNullable = ") + (any_null ? "true" : "false") + ";\n";
+  src += std::string("  static constexpr bool kExactF64 = ") + (tune_get("agg.exact_f64", 0) > 0 ? "true" : "false") +
+         ";\n";
   src += "  struct In {\n" + in_fields + "  };\n";
   src += "  __device__ static inline In ins(const AggJitArgs &p, uint64_t base) {\n    In i;\n" + in_init +
          "    return i;\n  }\n";

@@ -19,6 +19,10 @@ the budget beside everything in flight. The one-at-a-time path
 (`run_one_at_a_time`) runs the same requests in arrival order, each alone,
 on one stream: the plain drop-in path.
 
+Requests run as prepared statements (one compiled kernel per query shape,
+the literals passed as arguments) with compensated FP64 sums, so that a
+request's result does not depend on what runs beside it.
+
 Nothing here is a second implementation of an operator: every request runs
 through RocmBackend's entry points (the library's HIP kernels).
 """
@@ -82,6 +86,14 @@ def join_agg_spec(lo: int, hi: int) -> AggregateSpec:
                               Aggregate(F.CountStar, None, "COUNT(*)")],
                          filter=plan.and_(plan.cmp(plan.col(0), Cmp.GtEq, plan.Lit.int64(lo)),
                                           plan.cmp(plan.col(0), Cmp.Lt, plan.Lit.int64(hi))))
+
+
+def serving_knobs() -> None:
+    """Prepared statements (one compiled kernel per query shape, literals as
+    arguments) and compensated FP64 sums (a request's result does not depend
+    on what runs beside it)."""
+    autotune.set_knob("jit.params", 1)
+    autotune.set_knob("agg.exact_f64", 1)
 
 
 class MemoryGate:
@@ -210,7 +222,7 @@ class Server:
     def run(self, requests: list[Request]) -> dict:
         """Feeds `requests` at their arrival times and serves them; returns
         latency and throughput statistics."""
-        autotune.set_knob("jit.params", 1)  # one kernel per query shape
+        serving_knobs()
         workers = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in self.KINDS]
         self.t0 = time.perf_counter()
         for w in workers:
@@ -229,7 +241,7 @@ class Server:
 
 def run_one_at_a_time(be: RocmBackend, tables: Tables, requests: list[Request]) -> dict:
     """The plain path: one stream, arrival order, every request alone."""
-    autotune.set_knob("jit.params", 1)
+    serving_knobs()
     stream = be.create_stream()
     t0 = time.perf_counter()
     for r in sorted(requests, key=lambda r: r.arrival):
@@ -312,3 +324,28 @@ def same_result(a: Request, b: Request, rel: float = 1e-12, dist_tol: float = 1e
         elif not np.array_equal(xv, yv):
             return False
     return True
+
+
+def describe_difference(a: Request, b: Request) -> str:
+    """What differs between two runs of one request (for reports)."""
+    if isinstance(a.result, Exception) or isinstance(b.result, Exception):
+        return f"{a.kind}: error {a.result!r} / {b.result!r}"
+    if a.kind == "vector":
+        (ra, da), (rb, db) = a.result, b.result
+        rows = int((ra != rb).sum())
+        return (f"vector ({len(ra)} queries): {rows} rows differ, max distance difference "
+                f"{float(np.abs(da - db).max()):.3g}, distances {da[ra != rb][:4].tolist()} vs "
+                f"{db[ra != rb][:4].tolist()}")
+    parts = []
+    for name, x, y in zip(a.result.schema.names, a.result.columns, b.result.columns):
+        xv = np.asarray(x.to_numpy(zero_copy_only=False), np.float64)
+        yv = np.asarray(y.to_numpy(zero_copy_only=False), np.float64)
+        if xv.shape != yv.shape:
+            parts.append(f"{name}: {xv.shape} vs {yv.shape} rows")
+            continue
+        with np.errstate(all="ignore"):
+            rel = np.nan_to_num(np.abs(xv - yv) / np.maximum(np.abs(yv), 1e-300))
+        if rel.size and rel.max() > 0:
+            i = int(rel.argmax())
+            parts.append(f"{name}: max relative difference {rel[i]:.3g} ({xv[i]!r} vs {yv[i]!r})")
+    return f"{a.kind} {a.payload!r}: " + ("; ".join(parts) or "same values")

@@ -25,18 +25,20 @@ pytestmark = pytest.mark.gpu
 SUM_REL = 1e-12
 
 
-@pytest.fixture(autouse=True, params=["generated", "prepared", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "interpreter"])
 def path(request):
     # every test runs on the kernel generated per aggregation shape (with
-    # literals as constants, and prepared: literals as kernel arguments) and
-    # on the interpreter kernel
+    # literals as constants; prepared: literals as kernel arguments; exact:
+    # compensated FP64 sums) and on the interpreter kernel
     from oxidelake_amd import autotune
 
     autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
     autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
+    autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
     yield request.param
     autotune.set_knob("agg.jit", -1)
     autotune.set_knob("jit.params", -1)
+    autotune.set_knob("agg.exact_f64", -1)
 
 
 def fbits(v):
@@ -246,3 +248,42 @@ def test_generated_kernel_is_the_one_that_runs(backend, stream, path):
     finally:
         lib.oxr_profile_enable(0)
         lib.oxr_profile_reset()
+
+
+def _ulps(x, y):
+    return abs(x - y) / math.ulp(abs(y)) if y else abs(x)
+
+
+@pytest.mark.parametrize("group_by", [0, -1])
+def test_float_sums_are_compensated(backend, stream, group_by, path):
+    # 1e16 + 1 rounds back to 1e16 in FP64, so a plain sum of
+    # [1e16, 1 x 5000, -1e16] in any order that meets 1e16 first loses ones;
+    # the compensated device sum is exact. Key 1 holds a cancelling series
+    # (N(0,1) x lognormal): its sum stays within a few ulps of the exact
+    # (math.fsum) sum, in whatever order the atomics land.
+    rng = np.random.default_rng(8)
+    ones = 5000
+    v0 = np.concatenate([[1e16, -1e16], np.ones(ones)])
+    v1 = rng.normal(size=300_000) * rng.lognormal(size=300_000)
+    k = np.concatenate([np.zeros(len(v0), np.int64), np.ones(len(v1), np.int64)])
+    v = np.concatenate([v0, v1])
+    perm = rng.permutation(len(v))
+    k, v = k[perm], v[perm]
+    if group_by < 0:  # the global group over key 0's rows only
+        m = k == 0
+        k, v = k[m], v[m]
+    batch = pa.record_batch({"k": pa.array(k), "v": pa.array(v)})
+    spec = AggregateSpec(group_by, [Aggregate(F.Sum, col(1))])
+    from oxidelake_amd import autotune
+
+    autotune.set_knob("agg.exact_f64", 1)  # on every path (reset by the fixture)
+    got = device_aggregate(backend, stream, batch, spec)
+    sums = got.column(got.num_columns - 1).to_pylist()
+    assert sums[0] == float(ones)
+    if group_by == 0:
+        assert got.column(0).to_pylist() == [0, 1]
+        want = math.fsum(v1)
+        assert _ulps(sums[1], want) <= 4, (sums[1], want)
+    # the same twice: no run-to-run drift
+    again = device_aggregate(backend, stream, batch, spec)
+    assert again.column(again.num_columns - 1).to_pylist() == sums

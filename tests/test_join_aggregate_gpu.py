@@ -18,14 +18,17 @@ from tests.test_aggregate_gpu import assert_agg_equal
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["generated", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "exact", "interpreter"])
 def path(request):
-    # unique-key joins run on the generated kernel, others on the interpreter
+    # unique-key joins run on the generated kernel, others on the
+    # interpreter; exact: the generated kernel with compensated FP64 sums
     from oxidelake_amd import autotune
 
-    autotune.set_knob("agg.jit", 1 if request.param == "generated" else 0)
+    autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
+    autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
     yield request.param
     autotune.set_knob("agg.jit", -1)
+    autotune.set_knob("agg.exact_f64", -1)
 
 
 # fact: fk, qty, price; dim: pk, grp

@@ -263,15 +263,23 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
 }
 
 // ------------------------------------------------------------------ finish
-// Marks occupied slots and compacts (key, slot) of ordinary keys.
+// Compacts (key, slot) of the occupied ordinary slots; one counter atomic
+// per wave.
 __global__ void compact_kernel(Table t, int64_t *out_keys, uint64_t *out_slots, uint64_t *count) {
   const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i >= t.cap) return;
-  const int64_t k = t.keys[i];
-  if (k == kEmpty) return;
-  const uint64_t at = atomicAdd((unsigned long long *)count, 1ull);
-  out_keys[at] = k;
-  out_slots[at] = i;
+  const int64_t k = i < t.cap ? t.keys[i] : kEmpty;
+  const bool used = k != kEmpty;
+  const uint64_t m = __ballot(used);
+  if (!m) return;  // the whole wave
+  const int lane = threadIdx.x & 63, first = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == first) base = atomicAdd((unsigned long long *)count, (unsigned long long)__popcll(m));
+  base = (unsigned long long)__shfl((long long)base, first, 64);
+  if (used) {
+    const uint64_t at = base + __popcll(m & ((1ull << lane) - 1));
+    out_keys[at] = k;
+    out_slots[at] = i;
+  }
 }
 
 struct OutCol {
@@ -280,63 +288,77 @@ struct OutCol {
   uint8_t kind, in_type, out_type, nullable;
 };
 
+// A validity bit per lane of the wave, written as whole words: lane 0 writes
+// the word of lanes 0-31, lane 32 that of lanes 32-63 (j is 64-aligned per
+// wave, so each word has one writer).
+__device__ inline void put_valid_bits(uint32_t *valid, uint64_t j, uint64_t total, bool bit) {
+  const uint64_t m = __ballot(bit);
+  const int lane = threadIdx.x & 63;
+  if ((lane == 0 || lane == 32) && j < total) valid[j >> 5] = (uint32_t)(lane == 0 ? m : m >> 32);
+}
+
 // Row j of the output: group slots[j]; optional special rows appended.
-__global__ void gather_kernel(Table t, const int64_t *keys_sorted, const uint64_t *slots_sorted,
-                              uint64_t g_ordinary, int has_min_key, int has_null, int key_type,
-                              void *out_key, uint32_t *out_key_valid, OutCol *outs, int naggs,
-                              uint64_t total, int empty_group) {
+__global__ __launch_bounds__(256) void gather_kernel(Table t, const int64_t *keys_sorted,
+                                                     const uint64_t *slots_sorted, uint64_t g_ordinary,
+                                                     int has_min_key, int has_null, int key_type, void *out_key,
+                                                     uint32_t *out_key_valid, OutCol *outs, int naggs,
+                                                     uint64_t total, int empty_group) {
   const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (j >= total) return;
-  uint64_t slot;
-  int64_t key;
-  bool key_valid = true;
-  // order: the INT64_MIN key first (smallest), ordinary keys ascending, NULL last
-  if (has_min_key && j == 0) {
-    slot = t.cap + 1;
-    key = INT64_MIN;
-  } else if (has_null && j == total - 1) {
-    slot = t.cap;
-    key = 0;
-    key_valid = false;
-  } else {
-    const uint64_t r = j - (has_min_key ? 1 : 0);
-    slot = slots_sorted[r];
-    key = keys_sorted[r];
+  const bool in = j < total;
+  uint64_t slot = 0;
+  int64_t key = 0;
+  bool key_valid = in;
+  if (in) {
+    // order: the INT64_MIN key first (smallest), ordinary keys ascending, NULL last
+    if (has_min_key && j == 0) {
+      slot = t.cap + 1;
+      key = INT64_MIN;
+    } else if (has_null && j == total - 1) {
+      slot = t.cap;
+      key = 0;
+      key_valid = false;
+    } else {
+      const uint64_t r = j - (has_min_key ? 1 : 0);
+      slot = slots_sorted[r];
+      key = keys_sorted[r];
+    }
+    if (out_key) {
+      if (key_type == OXR_INT32) ((int32_t *)out_key)[j] = (int32_t)key;
+      else ((int64_t *)out_key)[j] = key;
+    }
   }
-  if (out_key) {
-    if (key_type == OXR_INT32) ((int32_t *)out_key)[j] = (int32_t)key;
-    else ((int64_t *)out_key)[j] = key;
-    if (key_valid) atomicOr(&out_key_valid[j >> 5], 1u << (j & 31));
-  }
+  if (out_key) put_valid_bits(out_key_valid, j, total, key_valid);
   for (int a = 0; a < naggs; a++) {
     const OutCol o = outs[a];
-    int64_t v = t.acc[a][slot];
-    if (o.kind == A_SUM_F64 && t.lo[a])
-      v = (int64_t)f64_bits(sum_value(as_f64((uint64_t)v), as_f64((uint64_t)t.lo[a][slot])));
-    bool valid;
-    if (o.kind == A_COUNT || o.kind == A_COUNT_STAR) valid = true;
-    else if (t.has[a]) valid = t.has[a][slot] != 0;
-    else valid = !empty_group;  // a non-nullable input: every row of a group holds a value
-    uint64_t outv;
-    switch (o.kind) {
-      case A_MIN_F64:
-      case A_MAX_F64: {
-        int64_t b = v ^ (int64_t)(((uint64_t)(v >> 63)) >> 1);
-        outv = (uint64_t)b;
-        break;
+    bool valid = false;
+    if (in) {
+      int64_t v = t.acc[a][slot];
+      if (o.kind == A_SUM_F64 && t.lo[a])
+        v = (int64_t)f64_bits(sum_value(as_f64((uint64_t)v), as_f64((uint64_t)t.lo[a][slot])));
+      if (o.kind == A_COUNT || o.kind == A_COUNT_STAR) valid = true;
+      else if (t.has[a]) valid = t.has[a][slot] != 0;
+      else valid = !empty_group;  // a non-nullable input: every row of a group holds a value
+      uint64_t outv;
+      switch (o.kind) {
+        case A_MIN_F64:
+        case A_MAX_F64: {
+          int64_t b = v ^ (int64_t)(((uint64_t)(v >> 63)) >> 1);
+          outv = (uint64_t)b;
+          break;
+        }
+        case A_MIN_F32:
+        case A_MAX_F32: {
+          int32_t k32 = (int32_t)v;
+          int32_t b = k32 ^ (int32_t)(((uint32_t)(k32 >> 31)) >> 1);
+          outv = (uint64_t)(uint32_t)b;
+          break;
+        }
+        default: outv = (uint64_t)v;
       }
-      case A_MIN_F32:
-      case A_MAX_F32: {
-        int32_t k32 = (int32_t)v;
-        int32_t b = k32 ^ (int32_t)(((uint32_t)(k32 >> 31)) >> 1);
-        outv = (uint64_t)(uint32_t)b;
-        break;
-      }
-      default: outv = (uint64_t)v;
+      if (o.out_type == OXR_INT32 || o.out_type == OXR_FLOAT32) ((uint32_t *)o.values)[j] = (uint32_t)outv;
+      else ((uint64_t *)o.values)[j] = outv;
     }
-    if (o.out_type == OXR_INT32 || o.out_type == OXR_FLOAT32) ((uint32_t *)o.values)[j] = (uint32_t)outv;
-    else ((uint64_t *)o.values)[j] = outv;
-    if (o.valid && valid) atomicOr(&o.valid[j >> 5], 1u << (j & 31));
+    if (o.valid) put_valid_bits(o.valid, j, total, valid);
   }
 }
 
@@ -377,6 +399,10 @@ int fill64(oxr_stream *s, int64_t *p, uint64_t n, int64_t v) {
   OXR_HIP(hipGetLastError());
   return OXR_OK;
 }
+
+// Key columns of more rows than this get their value range checked before
+// the table is sized (below it, sizing for a key per row costs less).
+constexpr uint64_t kRangeCheckRows = 1u << 16;
 
 uint64_t next_pow2(uint64_t x) {
   uint64_t c = 16;
@@ -471,6 +497,91 @@ int table_alloc(oxr_stream *s, oxr_agg_state *st, uint64_t cap, Table *out, void
   }
   *out = t;
   *mem = m;
+  return OXR_OK;
+}
+
+// min and max of the valid values of an Int32/Int64 column, into mm[0..1]
+// (set to INT64_MAX and INT64_MIN beforehand). Each thread reads 8
+// consecutive values (one validity byte) per step; one atomic pair per block.
+template <typename T, bool kValid>
+__global__ __launch_bounds__(256) void key_range_kernel(const T *v, const uint32_t *valid, uint64_t n,
+                                                        int64_t *mm) {
+  __shared__ int64_t s_lo[4], s_hi[4];
+  int64_t lo = kI64Max, hi = kI64Min;
+  const uint64_t chunks = n / 8;
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < chunks; c += (uint64_t)gridDim.x * blockDim.x) {
+    T x[8];
+    __builtin_memcpy(x, v + c * 8, sizeof x);
+    const uint32_t m = kValid ? (valid[c >> 2] >> ((c & 3) * 8)) & 0xFFu : 0xFFu;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      if (!((m >> e) & 1u)) continue;
+      const int64_t y = (int64_t)x[e];
+      lo = y < lo ? y : lo;
+      hi = y > hi ? y : hi;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - chunks * 8) {  // the last n % 8 values
+    const uint64_t i = chunks * 8 + threadIdx.x;
+    if (!kValid || ((valid[i >> 5] >> (i & 31)) & 1u)) {
+      const int64_t y = (int64_t)v[i];
+      lo = y < lo ? y : lo;
+      hi = y > hi ? y : hi;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const int64_t ol = (int64_t)__shfl_xor((long long)lo, d, 64);
+    const int64_t oh = (int64_t)__shfl_xor((long long)hi, d, 64);
+    lo = ol < lo ? ol : lo;
+    hi = oh > hi ? oh : hi;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[w] = lo;
+    s_hi[w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 4; i++) {
+      lo = s_lo[i] < lo ? s_lo[i] : lo;
+      hi = s_hi[i] > hi ? s_hi[i] : hi;
+    }
+    if (lo != kI64Max) atomicMin((long long *)&mm[0], (long long)lo);
+    if (hi != kI64Min) atomicMax((long long *)&mm[1], (long long)hi);
+  }
+}
+
+// How many distinct valid keys an Int32/Int64 column can hold: at most its
+// value range (0 when every value is NULL). One pass over the column.
+int key_range(oxr_stream *s, const DevCol &c, uint64_t n, uint64_t *out) {
+  void *m = nullptr;
+  OXR_TRY(stream_alloc(s, 16, &m));
+  int64_t *mm = (int64_t *)m;
+  int rc = fill64(s, mm, 1, kI64Max);
+  if (rc == OXR_OK) rc = fill64(s, mm + 1, 1, kI64Min);
+  int64_t host[2] = {kI64Max, kI64Min};
+  if (rc == OXR_OK) {
+    const uint32_t *valid = (const uint32_t *)c.validity;
+    const uint64_t grid = std::max<uint64_t>(
+        1, std::min<uint64_t>((n / 8 + 255) / 256, (uint64_t)std::max(1, s->dev->cu_count) * 2));
+    const dim3 g((unsigned)grid), b(256);
+    if (c.type == OXR_INT32) {
+      if (valid) hipLaunchKernelGGL((key_range_kernel<int32_t, true>), g, b, 0, s->s, (const int32_t *)c.values, valid, n, mm);
+      else hipLaunchKernelGGL((key_range_kernel<int32_t, false>), g, b, 0, s->s, (const int32_t *)c.values, valid, n, mm);
+    } else {
+      if (valid) hipLaunchKernelGGL((key_range_kernel<int64_t, true>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
+      else hipLaunchKernelGGL((key_range_kernel<int64_t, false>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, mm, 16, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+    if (e != hipSuccess) rc = fail_hip(e, "key range");
+  }
+  stream_release(s, m);
+  if (rc != OXR_OK) return rc;
+  *out = host[1] < host[0] ? 0 : (uint64_t)host[1] - (uint64_t)host[0] + 1;  // wraps to 0 only for the full range
+  if (*out == 0 && host[1] >= host[0]) *out = ~0ull;
   return OXR_OK;
 }
 
@@ -706,10 +817,18 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
       return fail(OXR_EPLAN, "rocm.aggregate", "batch column types differ from the spec's");
   const uint64_t n = in->rows;
   if (n == 0) return OXR_OK;
-  // keep the load at most one half for every key this batch might add
+  // keep the load at most one half for every key this batch might add: at
+  // most a key per row, per build row of a join, and per value of the key's
+  // range (one pass over the key column finds it, when no hint says more)
   uint64_t more = st->key_col == -1 ? 1 : n;
   if (js) more = std::min<uint64_t>(more, std::max<uint64_t>(js->table->nrows, 1));
   if (st->hint) more = std::min<uint64_t>(more, st->hint > st->groups ? st->hint - st->groups : 16);
+  if (!st->hint && more > kRangeCheckRows) {
+    uint64_t range = 0;
+    if (js) OXR_TRY(key_range(s, js->build->cols[js->build_group], js->build->rows, &range));
+    else OXR_TRY(key_range(s, in->cols[st->key_col], n, &range));
+    more = std::min<uint64_t>(more, std::max<uint64_t>(range, 1));
+  }
   OXR_TRY(ensure_capacity(s, st, more));
 
   std::unique_ptr<Params> pp(new Params());

@@ -400,9 +400,10 @@ int fill64(oxr_stream *s, int64_t *p, uint64_t n, int64_t v) {
   return OXR_OK;
 }
 
-// Key columns of more rows than this get their value range checked before
-// the table is sized (below it, sizing for a key per row costs less).
-constexpr uint64_t kRangeCheckRows = 1u << 16;
+// Key columns of more rows than this (knob agg.range_rows) get their value
+// range measured before the table is sized (below it, sizing for a key per
+// row costs less).
+constexpr int64_t kRangeCheckRows = 1 << 16;
 
 uint64_t next_pow2(uint64_t x) {
   uint64_t c = 16;
@@ -552,9 +553,16 @@ __global__ __launch_bounds__(256) void key_range_kernel(const T *v, const uint32
   }
 }
 
-// How many distinct valid keys an Int32/Int64 column can hold: at most its
-// value range (0 when every value is NULL). One pass over the column.
-int key_range(oxr_stream *s, const DevCol &c, uint64_t n, uint64_t *out) {
+// The value range of a key column: its smallest valid value and how many
+// distinct valid keys it can hold (max - min + 1; 0 when every value is
+// NULL). One pass over the column.
+struct KeyRange {
+  bool known = false;
+  int64_t min = 0;
+  uint64_t count = 0;
+};
+
+int key_range(oxr_stream *s, const DevCol &c, uint64_t n, KeyRange *out) {
   void *m = nullptr;
   OXR_TRY(stream_alloc(s, 16, &m));
   int64_t *mm = (int64_t *)m;
@@ -580,8 +588,10 @@ int key_range(oxr_stream *s, const DevCol &c, uint64_t n, uint64_t *out) {
   }
   stream_release(s, m);
   if (rc != OXR_OK) return rc;
-  *out = host[1] < host[0] ? 0 : (uint64_t)host[1] - (uint64_t)host[0] + 1;  // wraps to 0 only for the full range
-  if (*out == 0 && host[1] >= host[0]) *out = ~0ull;
+  out->known = true;
+  out->min = host[0];
+  out->count = host[1] < host[0] ? 0 : (uint64_t)host[1] - (uint64_t)host[0] + 1;  // 0 only for the full range
+  if (out->count == 0 && host[1] >= host[0]) out->count = ~0ull;
   return OXR_OK;
 }
 
@@ -754,7 +764,7 @@ struct ShapeOf {
 // generator covers it (a plain key or one global group, or a join on unique
 // build keys). OXR_EUNSUPPORTED: the interpreter kernel runs it.
 int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const JoinSide *js,
-                         uint32_t *err) {
+                         uint32_t *err, const KeyRange *kr) {
   if (tune_get("agg.jit", 1) == 0 || st->aggs.empty() || (js && !js->table->t.unique))
     return OXR_EUNSUPPORTED;
   ShapeOf so(st, js ? js->probe_key : -2);
@@ -796,14 +806,27 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     A.gos = g32 ? (const uint64_t *)gos : nullptr;
     A.gos64 = g32 ? nullptr : (const uint64_t *)gos;
   }
+  // a small measured key range: the dense kernel (a per-lane LDS column of
+  // accumulators for every (key, aggregate) pair, its lo parts in exact mode)
+  hipFunction_t fn = ja.fn;
+  if (ja.dense && kr && kr->known && kr->count > 0 && kr->count <= 32 && tune_get("agg.dense", 1) > 0) {
+    const uint64_t nkeys = kr->count + (in->cols[st->key_col].validity ? 1 : 0);
+    const uint64_t need = nkeys * st->aggs.size() * (tune_get("agg.exact_f64", 0) > 0 ? 2 : 1);
+    if (nkeys * st->aggs.size() <= 32 && need <= 32) {
+      fn = need <= 16 ? ja.dense16 : ja.dense32;
+      A.dense_min = kr->min;
+      A.dense_keys = (uint32_t)kr->count;
+      A.dense_null = in->cols[st->key_col].validity ? 1 : 0;
+    }
+  }
   const uint64_t tile = 256ull * ja.rows_per_lane;
   const uint64_t ntiles = (in->rows + tile - 1) / tile;
   const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.jit_grid", 4));
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   int rc;
   {
-    ProfScope prof(s->s, "aggregate");
-    rc = jit_launch_raw(ja.fn, (unsigned)grid, s->s, &A, sizeof A);
+    ProfScope prof(s->s, fn == ja.fn ? "aggregate" : "aggregate_dense");
+    rc = jit_launch_raw(fn, (unsigned)grid, s->s, &A, sizeof A);
   }
   stream_release(s, gos);
   return rc;
@@ -819,15 +842,15 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   if (n == 0) return OXR_OK;
   // keep the load at most one half for every key this batch might add: at
   // most a key per row, per build row of a join, and per value of the key's
-  // range (one pass over the key column finds it, when no hint says more)
+  // range (one pass over the key column measures it)
   uint64_t more = st->key_col == -1 ? 1 : n;
   if (js) more = std::min<uint64_t>(more, std::max<uint64_t>(js->table->nrows, 1));
   if (st->hint) more = std::min<uint64_t>(more, st->hint > st->groups ? st->hint - st->groups : 16);
-  if (!st->hint && more > kRangeCheckRows) {
-    uint64_t range = 0;
-    if (js) OXR_TRY(key_range(s, js->build->cols[js->build_group], js->build->rows, &range));
-    else OXR_TRY(key_range(s, in->cols[st->key_col], n, &range));
-    more = std::min<uint64_t>(more, std::max<uint64_t>(range, 1));
+  KeyRange kr;
+  if (st->key_col != -1 && n > (uint64_t)std::max<int64_t>(0, tune_get("agg.range_rows", kRangeCheckRows))) {
+    if (js) OXR_TRY(key_range(s, js->build->cols[js->build_group], js->build->rows, &kr));
+    else OXR_TRY(key_range(s, in->cols[st->key_col], n, &kr));
+    more = std::min<uint64_t>(more, std::max<uint64_t>(kr.count, 1));
   }
   OXR_TRY(ensure_capacity(s, st, more));
 
@@ -875,7 +898,7 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   bool generated = false;
   if (rc == OXR_OK) {
-    const int grc = run_update_generated(s, st, in, js, (uint32_t *)err);
+    const int grc = run_update_generated(s, st, in, js, (uint32_t *)err, js ? nullptr : &kr);
     if (grc == OXR_OK) generated = true;
     else if (grc != OXR_EUNSUPPORTED) rc = grc;
   }

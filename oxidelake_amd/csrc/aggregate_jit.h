@@ -236,6 +236,10 @@ struct AggJitArgs {
   const uint64_t *gos;          // Int32 groups: (state << 32) | group
   const uint64_t *gos64;        // Int64 groups: {group, state} pairs
   uint64_t lit[32];             // prepared mode: literal bits
+  // dense kernels: keys [dense_min, dense_min + dense_keys), plus the NULL
+  // key's slot at index dense_keys when dense_null
+  int64_t dense_min;
+  uint32_t dense_keys, dense_null;
 };
 
 // group of a probe key through the join: false when nothing matches
@@ -464,6 +468,167 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
         else if (v != acc_init(Q::kKind[a]) || Q::kKind[a] == A_SUM_F64)
           acc_apply_global(Q::kKind[a], &p.t.acc[a][gs], v);
         if (Q::kNullable[a] && HS > 1 && p.t.has[a] && s_has[a][i]) p.t.has[a][gs] = 1;
+      }
+    }
+  }
+  if (err) atomicOr(p.error, 1u);
+  if (!ok) atomicOr(p.t.overflow, 1u);
+}
+
+// -------------------------------------------------- dense small-range kernel
+// When the key column's value range is small (the host measured it), every
+// lane keeps its own accumulator for every (key, aggregate) pair in LDS:
+// plain read-modify-writes of the lane's own column, no atomics and no
+// probing. The block folds the 256 columns at the end (16 lanes per slot)
+// and adds one update per (key, aggregate) to the HBM table. DS slots of 256
+// lanes x 8 bytes (DS = 16: 32 KB, DS = 32: 64 KB); in exact mode the FP64
+// sums' lo parts take the second half.
+
+__device__ inline int64_t combine_kind(uint8_t kind, int64_t a, int64_t b) {
+  switch (kind) {
+    case A_SUM_F64: return (int64_t)f64_bits(as_f64((uint64_t)a) + as_f64((uint64_t)b));
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32: return a < b ? a : b;
+    case A_MAX_I64:
+    case A_MAX_F64:
+    case A_MAX_F32: return a > b ? a : b;
+    default: return (int64_t)((uint64_t)a + (uint64_t)b);
+  }
+}
+
+// (hi, lo) += (v, vl) with the exact error of hi + v going to lo.
+__device__ inline void add_f64_pair(double &hi, double &lo, double v, double vl) {
+  const double s = hi + v;
+  const double e = __builtin_isfinite(s) ? two_sum_err(hi, v, s) : 0.0;
+  hi = s;
+  lo = (lo + vl) + e;
+}
+
+template <class Q, int S, int DS>
+__device__ __forceinline__ void aggregate_dense(const AggJitArgs &p) {
+  constexpr int NA = Q::kAggs;
+  constexpr bool EX = Q::kExactF64;
+  constexpr int kTile = 256 * S;
+  __shared__ int64_t s_acc[DS * 256];
+  __shared__ uint32_t s_bits[2];  // keys used, non-NULL inputs seen
+
+  const int tid = threadIdx.x;
+  const uint32_t nkeys = p.dense_keys + p.dense_null;
+  const uint32_t nslot = nkeys * NA;  // host: nslot * (EX ? 2 : 1) <= DS, nslot <= 32
+  for (uint32_t sl = 0; sl < nslot; sl++) {
+    s_acc[sl * 256 + tid] = acc_init(Q::kKind[sl % NA]);
+    if (EX) s_acc[(nslot + sl) * 256 + tid] = 0;
+  }
+  if (tid < 2) s_bits[tid] = 0;
+  __syncthreads();  // s_bits before any wave's atomicOr
+  uint32_t used = 0, has = 0;
+  bool err = false, ok = true;
+  const uint64_t ntiles = (p.n + kTile - 1) / kTile;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    const typename Q::In in_cols = Q::ins(p, base);
+    typename Q::Row x[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      const bool in = j < lim;
+      bool sel = in && Q::pred(p.lit, x[s], j, in, err);
+      int64_t key = 0;
+      bool kv = false;
+      if (sel) sel = Q::key(p, x[s], j, in, key, kv);
+      if (!kv) key = 0;
+      int64_t c[NA];
+      uint32_t cm = 0;
+      Q::contrib(p.lit, x[s], j, sel, c, cm, err);
+      if (!sel) continue;
+      uint32_t r = 0xFFFFFFFFu;
+      if (!kv) {
+        if (p.dense_null) r = p.dense_keys;
+      } else if (key >= p.dense_min && (uint64_t)key - (uint64_t)p.dense_min < p.dense_keys) {
+        r = (uint32_t)((uint64_t)key - (uint64_t)p.dense_min);
+      }
+      if (r == 0xFFFFFFFFu) {  // outside the measured range: the HBM table directly
+        const uint64_t gs = slot_of(p.t, kv, key, &ok);
+        if (!ok) continue;
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+          if (!((cm >> a) & 1u)) continue;
+          if (EX && Q::kKind[a] == A_SUM_F64) sum_f64(&p.t.acc[a][gs], &p.t.lo[a][gs], as_f64((uint64_t)c[a]), 0.0);
+          else acc_apply_global(Q::kKind[a], &p.t.acc[a][gs], c[a]);
+          if (p.t.has[a]) p.t.has[a][gs] = 1;
+        }
+        continue;
+      }
+      used |= 1u << r;
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (!((cm >> a) & 1u)) continue;
+        const uint32_t sl = r * NA + a;
+        int64_t &acc = s_acc[sl * 256 + tid];
+        if (EX && Q::kKind[a] == A_SUM_F64) {
+          double hi = as_f64((uint64_t)acc), lo = as_f64((uint64_t)s_acc[(nslot + sl) * 256 + tid]);
+          add_f64_pair(hi, lo, as_f64((uint64_t)c[a]), 0.0);
+          acc = (int64_t)f64_bits(hi);
+          s_acc[(nslot + sl) * 256 + tid] = (int64_t)f64_bits(lo);
+        } else {
+          acc = combine_kind(Q::kKind[a], acc, c[a]);
+        }
+        if (Q::kNullable[a]) has |= 1u << sl;
+      }
+    }
+  }
+  // the block's key and has bits
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    used |= (uint32_t)__shfl_xor((int)used, d, 64);
+    has |= (uint32_t)__shfl_xor((int)has, d, 64);
+  }
+  if ((tid & 63) == 0) {
+    atomicOr(&s_bits[0], used);
+    atomicOr(&s_bits[1], has);
+  }
+  __syncthreads();
+  const uint32_t ub = s_bits[0], hb = s_bits[1];
+  // fold: 16 lanes per slot, each over 16 of the 256 columns, then across
+  // the 16 (xor distances 8..1 stay inside the group)
+  for (uint32_t b0 = 0; b0 < nslot; b0 += 16) {
+    const uint32_t sl = b0 + (uint32_t)(tid >> 4), part = (uint32_t)(tid & 15);
+    const bool live = sl < nslot;
+    const uint32_t a = live ? sl % NA : 0, r = live ? sl / NA : 0;
+    const uint8_t kind = Q::kKind[a];
+    int64_t v = acc_init(kind);
+    double vh = 0.0, vl = 0.0;
+    if (live) {
+      for (uint32_t l = part * 16; l < part * 16 + 16; l++) {
+        const int64_t y = s_acc[sl * 256 + l];
+        if (EX && kind == A_SUM_F64) add_f64_pair(vh, vl, as_f64((uint64_t)y), as_f64((uint64_t)s_acc[(nslot + sl) * 256 + l]));
+        else v = combine_kind(kind, v, y);
+      }
+    }
+#pragma unroll
+    for (int d = 8; d > 0; d >>= 1) {
+      if (EX && kind == A_SUM_F64) {
+        const double oh = __shfl_xor(vh, d, 64), ol = __shfl_xor(vl, d, 64);
+        add_f64_pair(vh, vl, oh, ol);
+      } else {
+        v = combine_kind(kind, v, (int64_t)__shfl_xor((long long)v, d, 64));
+      }
+    }
+    if (live && part == 0 && ((ub >> r) & 1u)) {
+      const bool null_slot = p.dense_null && r == p.dense_keys;
+      const int64_t key = null_slot ? 0 : (int64_t)((uint64_t)p.dense_min + r);
+      const uint64_t gs = slot_of(p.t, !null_slot, key, &ok);
+      if (ok) {
+        if (EX && kind == A_SUM_F64) sum_f64(&p.t.acc[a][gs], &p.t.lo[a][gs], vh, vl);
+        else if (v != acc_init(kind) || kind == A_SUM_F64) acc_apply_global(kind, &p.t.acc[a][gs], v);
+        if (p.t.has[a] && ((hb >> sl) & 1u)) p.t.has[a][gs] = 1;
       }
     }
   }

@@ -3,6 +3,7 @@
 #include <hip/hiprtc.h>
 
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 
@@ -373,7 +374,7 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
 
 struct Entry {
   hipModule_t mod = nullptr;
-  hipFunction_t fn = nullptr;
+  std::map<std::string, hipFunction_t> fns;  // kernels looked up so far
 };
 
 std::mutex g_jit_mu;
@@ -414,7 +415,16 @@ int compile(oxr_device *dev, const std::string &src, const char *kernel, hipFunc
   std::lock_guard<std::mutex> lock(g_jit_mu);
   auto it = g_jit_cache.find(key);
   if (it != g_jit_cache.end()) {
-    *fn = it->second.fn;
+    auto f = it->second.fns.find(kernel);
+    if (f != it->second.fns.end()) {
+      *fn = f->second;
+      return OXR_OK;
+    }
+    hipFunction_t h = nullptr;
+    const hipError_t e = hipModuleGetFunction(&h, it->second.mod, kernel);
+    if (e != hipSuccess) return fail_hip(e, "looking up a generated kernel");
+    it->second.fns.emplace(kernel, h);
+    *fn = h;
     return OXR_OK;
   }
   std::vector<char> code;
@@ -423,12 +433,14 @@ int compile(oxr_device *dev, const std::string &src, const char *kernel, hipFunc
   OXR_HIP(hipGetDevice(&prev));
   if (prev != dev->ordinal) OXR_HIP(hipSetDevice(dev->ordinal));
   Entry ent;
+  hipFunction_t h = nullptr;
   hipError_t e = hipModuleLoadData(&ent.mod, code.data());
-  if (e == hipSuccess) e = hipModuleGetFunction(&ent.fn, ent.mod, kernel);
+  if (e == hipSuccess) e = hipModuleGetFunction(&h, ent.mod, kernel);
   if (prev != dev->ordinal) hipSetDevice(prev);
   if (e != hipSuccess) return fail_hip(e, "loading a generated kernel");
+  ent.fns.emplace(kernel, h);
   g_jit_cache.emplace(key, ent);
-  *fn = ent.fn;
+  *fn = h;
   return OXR_OK;
 }
 
@@ -530,6 +542,14 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   src += "};\n}  // namespace\n";
   src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_jit(const AggJitArgs p) {\n";
   src += "  aggregate_tiles<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";
+  ja->dense = sh.key_col >= 0 && !sh.join;
+  if (ja->dense) {
+    // small key ranges: per-lane LDS accumulators (aggregate_dense)
+    for (int ds : {16, 32})
+      src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_dense" + std::to_string(ds) +
+             "(const AggJitArgs p) {\n  aggregate_dense<Q, " + std::to_string(ja->rows_per_lane) + ", " +
+             std::to_string(ds) + ">(p);\n}\n";
+  }
   ja->source = src;
   return OXR_OK;
 }
@@ -541,6 +561,10 @@ int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJit
   const int rc = generate_aggregate(cols, shape, &ja);
   if (rc != OXR_OK) return rc;
   OXR_TRY(compile(dev, ja.source, "oxr_aggregate_jit", &ja.fn));
+  if (ja.dense) {
+    OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense16", &ja.dense16));
+    OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense32", &ja.dense32));
+  }
   *out = std::move(ja);
   return OXR_OK;
 }

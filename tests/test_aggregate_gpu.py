@@ -25,20 +25,21 @@ pytestmark = pytest.mark.gpu
 SUM_REL = 1e-12
 
 
-@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "dense", "interpreter"])
 def path(request):
     # every test runs on the kernel generated per aggregation shape (with
     # literals as constants; prepared: literals as kernel arguments; exact:
-    # compensated FP64 sums) and on the interpreter kernel
+    # compensated FP64 sums; dense: every key range measured, so small ones
+    # take the per-lane LDS kernel) and on the interpreter kernel
     from oxidelake_amd import autotune
 
     autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
     autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
     autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
+    autotune.set_knob("agg.range_rows", 0 if request.param == "dense" else -1)
     yield request.param
-    autotune.set_knob("agg.jit", -1)
-    autotune.set_knob("jit.params", -1)
-    autotune.set_knob("agg.exact_f64", -1)
+    for k in ("agg.jit", "jit.params", "agg.exact_f64", "agg.range_rows"):
+        autotune.set_knob(k, -1)
 
 
 def fbits(v):
@@ -243,8 +244,9 @@ def test_generated_kernel_is_the_one_that_runs(backend, stream, path):
     lib.oxr_profile_reset()
     try:
         backend.aggregate(stream, dev, datagen.groupby_query_spec())
-        want = (0, 1) if path == "interpreter" else (1, 0)
-        assert (autotune.launches("aggregate"), autotune.launches("aggregate_vm")) == want
+        want = {"interpreter": (0, 0, 1), "dense": (0, 1, 0)}.get(path, (1, 0, 0))
+        got = (autotune.launches("aggregate"), autotune.launches("aggregate_dense"), autotune.launches("aggregate_vm"))
+        assert got == want
     finally:
         lib.oxr_profile_enable(0)
         lib.oxr_profile_reset()
@@ -287,3 +289,45 @@ def test_float_sums_are_compensated(backend, stream, group_by, path):
     # the same twice: no run-to-run drift
     again = device_aggregate(backend, stream, batch, spec)
     assert again.column(again.num_columns - 1).to_pylist() == sums
+
+
+@pytest.mark.parametrize("keys", ["int32", "int64_negative", "nullable", "wide"])
+def test_dense_small_key_ranges(backend, stream, oracle, path, keys):
+    # the per-lane LDS kernel: Int32 and negative Int64 keys, a NULL-key slot,
+    # and a range too wide for it (the hashed kernel takes that one)
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    rng = np.random.default_rng(17)
+    n = 150_000
+    if keys == "int32":
+        k = pa.array(rng.integers(5, 12, n).astype(np.int32))
+    elif keys == "int64_negative":
+        k = pa.array(rng.integers(-9, -1, n).astype(np.int64))
+    elif keys == "nullable":
+        k = pa.array(rng.integers(0, 6, n).astype(np.int64), mask=rng.random(n) < 0.1)
+    else:
+        k = pa.array(rng.integers(0, 40, n).astype(np.int64))
+    v = pa.array(rng.normal(size=n), mask=rng.random(n) < 0.05)
+    w = pa.array(rng.integers(-1000, 1000, n).astype(np.int32))
+    batch = pa.record_batch({"k": k, "v": v, "w": w})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.Count, col(1)), Aggregate(F.Min, col(2)),
+                             Aggregate(F.Max, col(1))],
+                         filter=plan.cmp(col(2), Cmp.Gt, Lit.int32(-900)))
+    autotune.set_knob("agg.range_rows", 0)  # measure the range on every path
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        got = device_aggregate(backend, stream, batch, spec)
+        dense = autotune.launches("aggregate_dense")
+    finally:
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
+    want = oracle.aggregate(batch, spec, sum_mode=1)
+    assert_agg_equal(got, want, spec)
+    if path == "interpreter":
+        assert dense == 0
+    else:
+        # 4 aggregates x (range + NULL slot) slots fit 32 unless the range is wide
+        # (twice that in exact mode: too many for every case here)
+        assert dense == (0 if keys == "wide" or path == "exact" else 1), dense

@@ -731,6 +731,96 @@ __global__ void gos_kernel(JTable t, DCol grp, uint64_t nslots, uint64_t *gos32,
   }
 }
 
+// ------------------------------------------------------------------- spill
+// Rows the generated kernel's block tables do not hold (the cold tail of a
+// high-cardinality key) are spilled as records (group index | cm << 32, then
+// one contribution per aggregate) into per-block regions, with a histogram
+// over partitions of 2^slog2 groups. Random HBM atomics run at ~24 G/s on
+// this device whatever their scope or footprint, so instead of one atomic
+// per row and aggregate, the records are partitioned (scatter) and each
+// partition is folded in LDS by one workgroup (fold), which adds one update
+// per (group, aggregate) to the HBM table.
+
+// Workgroup b moves its region's records to their partitions' ranges
+// (offs[part * B + b] = start of block b's records of that partition).
+__global__ __launch_bounds__(256) void spill_scatter_kernel(const int64_t *spill, uint64_t cap,
+                                                            const uint32_t *spill_n, const uint32_t *offs,
+                                                            uint32_t nparts, uint32_t slog2, uint32_t words,
+                                                            int64_t *out) {
+  __shared__ uint32_t s_cur[kMaxSpillParts];
+  const uint32_t b = blockIdx.x, B = gridDim.x;
+  for (uint32_t i = threadIdx.x; i < nparts; i += 256) s_cur[i] = offs[(uint64_t)i * B + b];
+  __syncthreads();
+  const uint64_t n = spill_n[b];
+  const int64_t *src = spill + (uint64_t)b * cap * words;
+  for (uint64_t r = threadIdx.x; r < n; r += 256) {
+    const int64_t *rec = src + r * words;
+    const int64_t w0 = rec[0];
+    const uint32_t pos = atomicAdd(&s_cur[(uint32_t)w0 >> slog2], 1u);
+    int64_t *dst = out + (uint64_t)pos * words;
+    dst[0] = w0;
+    for (uint32_t k = 1; k < words; k++) dst[k] = rec[k];
+  }
+}
+
+struct FoldKinds {
+  uint8_t kind[kMaxAggs];
+};
+
+// Workgroup p folds partition p's records into an LDS table of 2^slog2
+// groups (accumulators, non-NULL flags, used bits), then adds each used
+// group's accumulators to the HBM table.
+__global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, const uint32_t *offs, uint32_t B,
+                                                         uint32_t words, uint32_t slog2, int64_t smin, Table t,
+                                                         FoldKinds fk, int na) {
+  extern __shared__ int64_t s_dyn[];
+  const uint32_t G = 1u << slog2;
+  int64_t *s_acc = s_dyn;                                // [na][G]
+  uint32_t *s_used = (uint32_t *)(s_acc + (uint64_t)na * G);  // [G / 32]
+  uint8_t *s_has = (uint8_t *)(s_used + G / 32);         // [na][G]
+  for (uint32_t i = threadIdx.x; i < G; i += 256) {
+    for (int a = 0; a < na; a++) {
+      s_acc[a * G + i] = acc_init(fk.kind[a]);
+      s_has[a * G + i] = 0;
+    }
+    if ((i & 31) == 0) s_used[i >> 5] = 0;
+  }
+  __syncthreads();
+  const uint32_t part = blockIdx.x;
+  const uint64_t lo = offs[(uint64_t)part * B], hi = offs[(uint64_t)(part + 1) * B];
+  for (uint64_t r = lo + threadIdx.x; r < hi; r += 256) {
+    const int64_t *rec = recs + r * words;
+    const uint64_t w0 = (uint64_t)rec[0];
+    const uint32_t g = (uint32_t)w0 & (G - 1), cm = (uint32_t)(w0 >> 32);
+    atomicOr(&s_used[g >> 5], 1u << (g & 31));
+    for (int a = 0; a < na; a++) {
+      if (!((cm >> a) & 1u)) continue;
+      acc_apply_lds(fk.kind[a], &s_acc[a * G + g], rec[1 + a]);
+      s_has[a * G + g] = 1;
+    }
+  }
+  __syncthreads();
+  bool ok = true;
+  for (uint32_t i = threadIdx.x; i < G && ok; i += 256) {
+    if (!((s_used[i >> 5] >> (i & 31)) & 1u)) continue;
+    const int64_t key = (int64_t)((uint64_t)smin + (uint64_t)part * G + i);
+    const uint64_t gs = slot_of(t, true, key, &ok);
+    if (!ok) break;
+    for (int a = 0; a < na; a++) {
+      const int64_t v = s_acc[a * G + i];
+      if (v != acc_init(fk.kind[a]) || fk.kind[a] == A_SUM_F64) acc_apply_global(fk.kind[a], &t.acc[a][gs], v);
+      if (t.has[a] && s_has[a * G + i]) t.has[a][gs] = 1;
+    }
+  }
+  if (!ok) atomicOr(t.overflow, 1u);
+}
+
+// LDS bytes of the fold for 2^slog2 groups and na aggregates.
+size_t fold_lds(uint32_t slog2, int na) {
+  const size_t G = (size_t)1 << slog2;
+  return G * 8 * na + G / 8 + G * na;
+}
+
 // The join an update folds through (oxr_agg_update_join), or none.
 struct JoinSide {
   const oxr_join_table *table = nullptr;
@@ -809,7 +899,7 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   // a small measured key range: the dense kernel (a per-lane LDS column of
   // accumulators for every (key, aggregate) pair, its lo parts in exact mode)
   hipFunction_t fn = ja.fn;
-  if (ja.dense && kr && kr->known && kr->count > 0 && kr->count <= 32 && tune_get("agg.dense", 1) > 0) {
+  if (ja.dense && !js && kr && kr->known && kr->count > 0 && kr->count <= 32 && tune_get("agg.dense", 1) > 0) {
     const uint64_t nkeys = kr->count + (in->cols[st->key_col].validity ? 1 : 0);
     const uint64_t need = nkeys * st->aggs.size() * (tune_get("agg.exact_f64", 0) > 0 ? 2 : 1);
     if (nkeys * st->aggs.size() <= 32 && need <= 32) {
@@ -823,11 +913,95 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   const uint64_t ntiles = (in->rows + tile - 1) / tile;
   const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.jit_grid", 4));
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
+
+  // spill the cold tail of a high-cardinality key (see spill_fold_kernel):
+  // a measured key range that the fold's partitions cover, every aggregate
+  // in the block table, plain FP64 sums, and enough rows to pay for it
+  const int na = (int)st->aggs.size();
+  const bool exact = tune_get("agg.exact_f64", 0) > 0;
+  uint32_t slog2 = 0, nparts = 0;
+  if (fn == ja.fn && kr && kr->known && kr->count > 0 && !exact && na <= kLdsAggs &&
+      tune_get("agg.spill", 1) > 0 && in->rows >= (uint64_t)std::max<int64_t>(0, tune_get("agg.spill_rows", 1 << 20))) {
+    slog2 = 14;
+    while (slog2 > 5 && fold_lds(slog2, na) > 64 * 1024) slog2--;
+    const uint64_t parts = (kr->count + (1ull << slog2) - 1) >> slog2;
+    if (parts <= (uint64_t)kMaxSpillParts && kr->count < (1ull << 32)) nparts = (uint32_t)parts;
+  }
+  const uint32_t words = (uint32_t)na + 1;
+  void *spill_mem = nullptr;
+  int64_t *spill = nullptr, *sorted = nullptr;
+  uint32_t *spill_n = nullptr, *hist = nullptr, *offs = nullptr;
+  uint64_t cap = 0;
+  if (nparts) {
+    // each block's region holds a fraction (knob agg.spill_pct, percent) of
+    // its rows; rows past it take the HBM path
+    const uint64_t tiles_per_block = (ntiles + grid - 1) / grid;
+    const int64_t pct = std::min<int64_t>(100, std::max<int64_t>(1, tune_get("agg.spill_pct", 50)));
+    cap = std::max<uint64_t>(tile, tiles_per_block * tile * (uint64_t)pct / 100);
+    const uint64_t recs = grid * cap;
+    const uint64_t nh = (uint64_t)nparts * grid + 1;
+    const size_t bytes = recs * words * 8 * 2 + (grid + 2 * nh) * 4 + 1024;
+    if (stream_alloc(s, bytes, &spill_mem) == OXR_OK) {
+      char *q = (char *)spill_mem;
+      spill = (int64_t *)q;
+      q += recs * words * 8;
+      sorted = (int64_t *)q;
+      q += recs * words * 8;
+      spill_n = (uint32_t *)q;
+      q += grid * 4;
+      hist = (uint32_t *)q;
+      q += nh * 4;
+      offs = (uint32_t *)q;
+      const int zrc = fill_zero(s, hist + nh - 1, 4);  // the scan's trailing zero: offs[nh - 1] = total
+      if (zrc != OXR_OK) {
+        stream_release(s, spill_mem);
+        stream_release(s, gos);
+        return zrc;
+      }
+      A.spill = spill;
+      A.spill_cap = cap;
+      A.spill_n = spill_n;
+      A.hist = hist;
+      A.smin = kr->min;
+      A.srange = kr->count;
+      A.slog2 = slog2;
+      A.sparts = nparts;
+    } else {
+      clear_error();  // no room for the spill buffers: the HBM path takes every row
+      nparts = 0;
+    }
+  }
   int rc;
   {
     ProfScope prof(s->s, fn == ja.fn ? "aggregate" : "aggregate_dense");
     rc = jit_launch_raw(fn, (unsigned)grid, s->s, &A, sizeof A);
   }
+  if (rc == OXR_OK && nparts) {
+    ProfScope prof(s->s, "aggregate_spill");
+    const uint64_t nh = (uint64_t)nparts * grid + 1;
+    size_t tmp_bytes = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, hist, offs, (int)nh, s->s);
+    void *tmp = nullptr;
+    if (e == hipSuccess) rc = stream_alloc(s, tmp_bytes + 16, &tmp);
+    if (rc == OXR_OK && e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, hist, offs, (int)nh, s->s);
+    if (rc == OXR_OK && e == hipSuccess) {
+      ProfScope p2(s->s, "aggregate_scatter");
+      hipLaunchKernelGGL(spill_scatter_kernel, dim3((unsigned)grid), dim3(256), 0, s->s, spill, cap, spill_n, offs,
+                         nparts, slog2, words, sorted);
+      e = hipGetLastError();
+    }
+    if (rc == OXR_OK && e == hipSuccess) {
+      ProfScope p3(s->s, "aggregate_fold");
+      FoldKinds fk{};
+      for (int a = 0; a < na; a++) fk.kind[a] = st->aggs[a].kind;
+      hipLaunchKernelGGL(spill_fold_kernel, dim3(nparts), dim3(256), fold_lds(slog2, na), s->s, sorted, offs,
+                         (uint32_t)grid, words, slog2, kr->min, st->t, fk, na);
+      e = hipGetLastError();
+    }
+    if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "aggregate spill passes");
+    stream_release(s, tmp);
+  }
+  if (spill_mem) stream_release(s, spill_mem);
   stream_release(s, gos);
   return rc;
 }
@@ -898,7 +1072,7 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   bool generated = false;
   if (rc == OXR_OK) {
-    const int grc = run_update_generated(s, st, in, js, (uint32_t *)err, js ? nullptr : &kr);
+    const int grc = run_update_generated(s, st, in, js, (uint32_t *)err, &kr);
     if (grc == OXR_OK) generated = true;
     else if (grc != OXR_EUNSUPPORTED) rc = grc;
   }

@@ -22,6 +22,7 @@
 namespace oxr {
 
 constexpr int kMaxAggs = 8;
+constexpr int kMaxSpillParts = 1024;  // partitions of the spill range
 constexpr int kLdsSlots = 1024;       // block table slots (power of two)
 constexpr int kLdsAggs = 4;           // aggregates pre-aggregated in LDS
 constexpr int kLdsProbe = 8;          // LDS probe limit before going global
@@ -240,6 +241,18 @@ struct AggJitArgs {
   // key's slot at index dense_keys when dense_null
   int64_t dense_min;
   uint32_t dense_keys, dense_null;
+  // spill: rows the block table does not hold whose key is in [smin, smin +
+  // srange) go to the block's region of `spill` (records of 1 + kAggs words:
+  // group index | cm << 32, then the contributions) instead of HBM atomics;
+  // per block, its record count and a histogram over partitions of 2^slog2
+  // groups (hist[part * gridDim + block])
+  int64_t *spill;               // NULL: no spilling
+  uint64_t spill_cap;           // records per block region
+  uint32_t *spill_n;
+  uint32_t *hist;
+  int64_t smin;
+  uint64_t srange;
+  uint32_t slog2, sparts;
 };
 
 // group of a probe key through the join: false when nothing matches
@@ -297,6 +310,9 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
   constexpr int kTile = 256 * S;
   constexpr int HS = Q::kAnyNullable ? kLdsSlots + 2 : 1;  // validity flags only when needed
   constexpr int NLO = Q::kExactF64 ? lo_index<Q>(LA) : 0;  // compensated FP64 sums in LDS
+  constexpr bool SP = LA == NA && !Q::kExactF64;            // rows may spill (see AggJitArgs)
+  __shared__ uint32_t s_hist[SP ? kMaxSpillParts : 1];
+  __shared__ uint32_t s_spill;
   __shared__ int64_t s_keys[kLdsSlots + 2];
   __shared__ int64_t s_acc[LAS][kLdsSlots + 2];
   __shared__ int64_t s_lo[NLO > 0 ? NLO : 1][NLO > 0 ? kLdsSlots + 2 : 1];
@@ -314,7 +330,12 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
 #pragma unroll
     for (int b = 0; b < NLO; b++) s_lo[b][i] = 0;
   }
-  if (tid == 0) s_used = 0;
+  if (tid == 0) {
+    s_used = 0;
+    s_spill = 0;
+  }
+  if (SP && p.spill)
+    for (uint32_t i = tid; i < p.sparts; i += 256) s_hist[i] = 0;
   __syncthreads();
 
   bool err = false, ok = true;
@@ -390,10 +411,9 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
           }
         }
       }
-      if (!sel) continue;
-      // the block table's slot (-1: go global)
+      // the block table's slot (-1: not held)
       int lslot = -1;
-      if (LA > 0) {
+      if (sel && LA > 0) {
         if (!kv) {
           lslot = kLdsSlots;
           s_keys[kLdsSlots] = 0;  // marks the NULL-key slot used
@@ -426,6 +446,32 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
           }
         }
       }
+      // a row the block table does not hold goes to the spill buffer when its
+      // key is in the spill range (positions: one LDS atomic per wave), else
+      // to the HBM table
+      bool spilled = false;
+      if (SP) {
+        const bool want = p.spill && sel && lslot < 0 && kv && key >= p.smin &&
+                          (uint64_t)key - (uint64_t)p.smin < p.srange;
+        const uint64_t wm = __ballot(want);
+        if (wm) {
+          const int first = __builtin_ctzll(wm);
+          uint32_t at = 0;
+          if (lane == first) at = atomicAdd(&s_spill, (uint32_t)__popcll(wm));
+          at = (uint32_t)__shfl((int)at, first, 64);
+          const uint64_t pos = (uint64_t)at + __popcll(wm & ((1ull << lane) - 1));
+          if (want && pos < p.spill_cap) {
+            const uint32_t g = (uint32_t)((uint64_t)key - (uint64_t)p.smin);
+            int64_t *r = p.spill + ((uint64_t)blockIdx.x * p.spill_cap + pos) * (NA + 1);
+            r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
+#pragma unroll
+            for (int a = 0; a < NA; a++) r[1 + a] = c[a];
+            atomicAdd(&s_hist[SP ? (g >> p.slog2) : 0], 1u);
+            spilled = true;
+          }
+        }
+      }
+      if (!sel || spilled) continue;
       uint64_t gslot = 0;
       if (lslot < 0 || LA < NA) gslot = slot_of(p.t, kv, key, &ok);
 #pragma unroll
@@ -470,6 +516,11 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
         if (Q::kNullable[a] && HS > 1 && p.t.has[a] && s_has[a][i]) p.t.has[a][gs] = 1;
       }
     }
+  }
+  if (SP && p.spill) {
+    // s_hist and s_spill are final: every tile ended with a barrier
+    for (uint32_t i = tid; i < p.sparts; i += 256) p.hist[(uint64_t)i * gridDim.x + blockIdx.x] = s_hist[i];
+    if (tid == 0) p.spill_n[blockIdx.x] = s_spill < p.spill_cap ? s_spill : (uint32_t)p.spill_cap;
   }
   if (err) atomicOr(p.error, 1u);
   if (!ok) atomicOr(p.t.overflow, 1u);

@@ -25,20 +25,28 @@ pytestmark = pytest.mark.gpu
 SUM_REL = 1e-12
 
 
-@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "dense", "interpreter"])
+SPILL_KNOBS = {"agg.range_rows": 0, "agg.spill_rows": 0, "agg.dense": 0, "agg.spill_pct": 100}
+
+
+@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "dense", "spill", "interpreter"])
 def path(request):
     # every test runs on the kernel generated per aggregation shape (with
     # literals as constants; prepared: literals as kernel arguments; exact:
     # compensated FP64 sums; dense: every key range measured, so small ones
-    # take the per-lane LDS kernel) and on the interpreter kernel
+    # take the per-lane LDS kernel; spill: rows the block table does not
+    # hold are spilled, partitioned and folded in LDS) and on the
+    # interpreter kernel
     from oxidelake_amd import autotune
 
     autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
     autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
     autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
     autotune.set_knob("agg.range_rows", 0 if request.param == "dense" else -1)
+    if request.param == "spill":
+        for k, v in SPILL_KNOBS.items():
+            autotune.set_knob(k, v)
     yield request.param
-    for k in ("agg.jit", "jit.params", "agg.exact_f64", "agg.range_rows"):
+    for k in ("agg.jit", "jit.params", "agg.exact_f64", "agg.range_rows", *SPILL_KNOBS):
         autotune.set_knob(k, -1)
 
 
@@ -329,5 +337,39 @@ def test_dense_small_key_ranges(backend, stream, oracle, path, keys):
         assert dense == 0
     else:
         # 4 aggregates x (range + NULL slot) slots fit 32 unless the range is wide
-        # (twice that in exact mode: too many for every case here)
-        assert dense == (0 if keys == "wide" or path == "exact" else 1), dense
+        # (twice that in exact mode: too many for every case here; the spill
+        # path turns the dense kernel off)
+        assert dense == (0 if keys == "wide" or path in ("exact", "spill") else 1), dense
+
+
+@pytest.mark.parametrize("pct", [1, 100])
+def test_spill_many_groups(backend, stream, oracle, path, pct):
+    # 300K distinct keys over 1M rows: most rows miss the block tables and
+    # are spilled, partitioned and folded (pct 1: the spill regions fill up
+    # and the rest take the HBM atomics, both in one update)
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    rng = np.random.default_rng(23)
+    n = 1_000_000
+    k = pa.array(rng.integers(-150_000, 150_000, n).astype(np.int64), mask=rng.random(n) < 0.01)
+    # positive values: a sum of a few cancelling terms is ill-conditioned and
+    # any other order of additions leaves it more than 1e-12 relative apart
+    v = pa.array(rng.uniform(0.5, 5, n), mask=rng.random(n) < 0.05)
+    w = pa.array(rng.integers(0, 1000, n).astype(np.int32))
+    batch = pa.record_batch({"k": k, "v": v, "w": w})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.CountStar, None), Aggregate(F.Max, col(2)),
+                             Aggregate(F.Sum, col(2))], filter=plan.cmp(col(2), Cmp.Lt, Lit.int32(900)))
+    for kk, vv in SPILL_KNOBS.items():
+        autotune.set_knob(kk, vv)
+    autotune.set_knob("agg.spill_pct", pct)
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        got = device_aggregate(backend, stream, batch, spec)
+        spills = autotune.launches("aggregate_spill")
+    finally:
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
+    assert_agg_equal(got, oracle.aggregate(batch, spec, sum_mode=1), spec)
+    assert spills == (0 if path in ("interpreter", "exact") else 1)

@@ -18,17 +18,22 @@ from tests.test_aggregate_gpu import assert_agg_equal
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["generated", "exact", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "exact", "spill", "interpreter"])
 def path(request):
     # unique-key joins run on the generated kernel, others on the
-    # interpreter; exact: the generated kernel with compensated FP64 sums
+    # interpreter; exact: the generated kernel with compensated FP64 sums;
+    # spill: cold groups spilled, partitioned and folded in LDS
     from oxidelake_amd import autotune
+    from tests.test_aggregate_gpu import SPILL_KNOBS
 
     autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
     autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
+    if request.param == "spill":
+        for k, v in SPILL_KNOBS.items():
+            autotune.set_knob(k, v)
     yield request.param
-    autotune.set_knob("agg.jit", -1)
-    autotune.set_knob("agg.exact_f64", -1)
+    for k in ("agg.jit", "agg.exact_f64", *SPILL_KNOBS):
+        autotune.set_knob(k, -1)
 
 
 # fact: fk, qty, price; dim: pk, grp

@@ -734,29 +734,34 @@ __global__ void gos_kernel(JTable t, DCol grp, uint64_t nslots, uint64_t *gos32,
 // ------------------------------------------------------------------- spill
 // Rows the generated kernel's block tables do not hold (the cold tail of a
 // high-cardinality key) are spilled as records (group index | cm << 32, then
-// one contribution per aggregate) into per-block regions, with a histogram
-// over partitions of 2^slog2 groups. Random HBM atomics run at ~24 G/s on
-// this device whatever their scope or footprint, so instead of one atomic
-// per row and aggregate, the records are partitioned (scatter) and each
-// partition is folded in LDS by one workgroup (fold), which adds one update
-// per (group, aggregate) to the HBM table.
+// one contribution per aggregate), each block appending to 32 coarse
+// buckets (their tails stay in L2 while they fill, so the writes reach HBM
+// as whole lines), with a histogram over partitions of 2^slog2 groups.
+// Random HBM atomics run at ~24 G/s on this device whatever their scope or
+// footprint, so instead of one atomic per row and aggregate the records are
+// partitioned (scatter: one coarse bucket of one block per workgroup, a few
+// fine partitions each) and each partition is folded in LDS by one
+// workgroup (fold), which adds one update per (group, aggregate) to the HBM
+// table.
 
-// Workgroup b moves its region's records to their partitions' ranges
-// (offs[part * B + b] = start of block b's records of that partition).
+// Workgroup w = b * kSpillBuckets + c moves block b's bucket c to its fine
+// partitions' ranges (offs[part * B + b] = start of block b's records of
+// that partition; bucket c holds partitions [c * sper, (c + 1) * sper)).
 __global__ __launch_bounds__(256) void spill_scatter_kernel(const int64_t *spill, uint64_t cap,
                                                             const uint32_t *spill_n, const uint32_t *offs,
-                                                            uint32_t nparts, uint32_t slog2, uint32_t words,
-                                                            int64_t *out) {
+                                                            uint32_t B, uint32_t nparts, uint32_t sper,
+                                                            uint32_t slog2, uint32_t words, int64_t *out) {
   __shared__ uint32_t s_cur[kMaxSpillParts];
-  const uint32_t b = blockIdx.x, B = gridDim.x;
-  for (uint32_t i = threadIdx.x; i < nparts; i += 256) s_cur[i] = offs[(uint64_t)i * B + b];
+  const uint32_t w = blockIdx.x, b = w / kSpillBuckets, c = w % kSpillBuckets;
+  const uint32_t p0 = c * sper;
+  for (uint32_t f = threadIdx.x; f < sper && p0 + f < nparts; f += 256) s_cur[f] = offs[(uint64_t)(p0 + f) * B + b];
   __syncthreads();
-  const uint64_t n = spill_n[b];
-  const int64_t *src = spill + (uint64_t)b * cap * words;
+  const uint64_t n = spill_n[w];
+  const int64_t *src = spill + (uint64_t)w * cap * words;
   for (uint64_t r = threadIdx.x; r < n; r += 256) {
     const int64_t *rec = src + r * words;
     const int64_t w0 = rec[0];
-    const uint32_t pos = atomicAdd(&s_cur[(uint32_t)w0 >> slog2], 1u);
+    const uint32_t pos = atomicAdd(&s_cur[((uint32_t)w0 >> slog2) - p0], 1u);
     int64_t *dst = out + (uint64_t)pos * words;
     dst[0] = w0;
     for (uint32_t k = 1; k < words; k++) dst[k] = rec[k];
@@ -768,22 +773,22 @@ struct FoldKinds {
 };
 
 // Workgroup p folds partition p's records into an LDS table of 2^slog2
-// groups (accumulators, non-NULL flags, used bits), then adds each used
-// group's accumulators to the HBM table.
+// groups (accumulators, non-NULL flags, used flags: plain byte stores), then
+// adds each used group's accumulators to the HBM table.
 __global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, const uint32_t *offs, uint32_t B,
                                                          uint32_t words, uint32_t slog2, int64_t smin, Table t,
                                                          FoldKinds fk, int na) {
   extern __shared__ int64_t s_dyn[];
   const uint32_t G = 1u << slog2;
-  int64_t *s_acc = s_dyn;                                // [na][G]
-  uint32_t *s_used = (uint32_t *)(s_acc + (uint64_t)na * G);  // [G / 32]
-  uint8_t *s_has = (uint8_t *)(s_used + G / 32);         // [na][G]
+  int64_t *s_acc = s_dyn;                                  // [na][G]
+  uint8_t *s_used = (uint8_t *)(s_acc + (uint64_t)na * G);  // [G]
+  uint8_t *s_has = s_used + G;                             // [na][G]
   for (uint32_t i = threadIdx.x; i < G; i += 256) {
     for (int a = 0; a < na; a++) {
       s_acc[a * G + i] = acc_init(fk.kind[a]);
       s_has[a * G + i] = 0;
     }
-    if ((i & 31) == 0) s_used[i >> 5] = 0;
+    s_used[i] = 0;
   }
   __syncthreads();
   const uint32_t part = blockIdx.x;
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, co
     const int64_t *rec = recs + r * words;
     const uint64_t w0 = (uint64_t)rec[0];
     const uint32_t g = (uint32_t)w0 & (G - 1), cm = (uint32_t)(w0 >> 32);
-    atomicOr(&s_used[g >> 5], 1u << (g & 31));
+    s_used[g] = 1;
     for (int a = 0; a < na; a++) {
       if (!((cm >> a) & 1u)) continue;
       acc_apply_lds(fk.kind[a], &s_acc[a * G + g], rec[1 + a]);
@@ -802,7 +807,7 @@ __global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, co
   __syncthreads();
   bool ok = true;
   for (uint32_t i = threadIdx.x; i < G && ok; i += 256) {
-    if (!((s_used[i >> 5] >> (i & 31)) & 1u)) continue;
+    if (!s_used[i]) continue;
     const int64_t key = (int64_t)((uint64_t)smin + (uint64_t)part * G + i);
     const uint64_t gs = slot_of(t, true, key, &ok);
     if (!ok) break;
@@ -818,7 +823,7 @@ __global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, co
 // LDS bytes of the fold for 2^slog2 groups and na aggregates.
 size_t fold_lds(uint32_t slog2, int na) {
   const size_t G = (size_t)1 << slog2;
-  return G * 8 * na + G / 8 + G * na;
+  return G * 8 * na + G + G * na;
 }
 
 // The join an update folds through (oxr_agg_update_join), or none.
@@ -929,26 +934,25 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   }
   const uint32_t words = (uint32_t)na + 1;
   void *spill_mem = nullptr;
-  int64_t *spill = nullptr, *sorted = nullptr;
+  int64_t *spill = nullptr;
   uint32_t *spill_n = nullptr, *hist = nullptr, *offs = nullptr;
   uint64_t cap = 0;
+  const uint32_t sper = nparts ? (nparts + kSpillBuckets - 1) / kSpillBuckets : 1;
   if (nparts) {
-    // each block's region holds a fraction (knob agg.spill_pct, percent) of
-    // its rows; rows past it take the HBM path
+    // a block's 32 buckets together hold a fraction (knob agg.spill_pct,
+    // percent) of its rows; rows past a full bucket take the HBM path
     const uint64_t tiles_per_block = (ntiles + grid - 1) / grid;
-    const int64_t pct = std::min<int64_t>(100, std::max<int64_t>(1, tune_get("agg.spill_pct", 50)));
-    cap = std::max<uint64_t>(tile, tiles_per_block * tile * (uint64_t)pct / 100);
-    const uint64_t recs = grid * cap;
+    const int64_t pct = std::min<int64_t>(400, std::max<int64_t>(1, tune_get("agg.spill_pct", 60)));
+    cap = std::max<uint64_t>(64, (tiles_per_block * tile * (uint64_t)pct / 100 + kSpillBuckets - 1) / kSpillBuckets);
+    const uint64_t recs = grid * kSpillBuckets * cap;
     const uint64_t nh = (uint64_t)nparts * grid + 1;
-    const size_t bytes = recs * words * 8 * 2 + (grid + 2 * nh) * 4 + 1024;
+    const size_t bytes = recs * words * 8 + (grid * kSpillBuckets + 2 * nh) * 4 + 1024;
     if (stream_alloc(s, bytes, &spill_mem) == OXR_OK) {
       char *q = (char *)spill_mem;
       spill = (int64_t *)q;
       q += recs * words * 8;
-      sorted = (int64_t *)q;
-      q += recs * words * 8;
       spill_n = (uint32_t *)q;
-      q += grid * 4;
+      q += grid * kSpillBuckets * 4;
       hist = (uint32_t *)q;
       q += nh * 4;
       offs = (uint32_t *)q;
@@ -966,6 +970,7 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
       A.srange = kr->count;
       A.slog2 = slog2;
       A.sparts = nparts;
+      A.sper = sper;
     } else {
       clear_error();  // no room for the spill buffers: the HBM path takes every row
       nparts = 0;
@@ -984,10 +989,17 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     void *tmp = nullptr;
     if (e == hipSuccess) rc = stream_alloc(s, tmp_bytes + 16, &tmp);
     if (rc == OXR_OK && e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, hist, offs, (int)nh, s->s);
+    // the partitioned records, sized by the spill's total
+    uint32_t total = 0;
+    if (rc == OXR_OK && e == hipSuccess) e = hipMemcpyAsync(&total, offs + nh - 1, 4, hipMemcpyDeviceToHost, s->s);
+    if (rc == OXR_OK && e == hipSuccess) e = hipStreamSynchronize(s->s);
+    void *sorted_mem = nullptr;
+    if (rc == OXR_OK && e == hipSuccess) rc = stream_alloc(s, (size_t)std::max<uint32_t>(total, 1) * words * 8, &sorted_mem);
+    int64_t *sorted = (int64_t *)sorted_mem;
     if (rc == OXR_OK && e == hipSuccess) {
       ProfScope p2(s->s, "aggregate_scatter");
-      hipLaunchKernelGGL(spill_scatter_kernel, dim3((unsigned)grid), dim3(256), 0, s->s, spill, cap, spill_n, offs,
-                         nparts, slog2, words, sorted);
+      hipLaunchKernelGGL(spill_scatter_kernel, dim3((unsigned)(grid * kSpillBuckets)), dim3(256), 0, s->s, spill, cap,
+                         spill_n, offs, (uint32_t)grid, nparts, sper, slog2, words, sorted);
       e = hipGetLastError();
     }
     if (rc == OXR_OK && e == hipSuccess) {
@@ -1000,6 +1012,7 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     }
     if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "aggregate spill passes");
     stream_release(s, tmp);
+    stream_release(s, sorted_mem);
   }
   if (spill_mem) stream_release(s, spill_mem);
   stream_release(s, gos);

@@ -23,6 +23,7 @@ namespace oxr {
 
 constexpr int kMaxAggs = 8;
 constexpr int kMaxSpillParts = 1024;  // partitions of the spill range
+constexpr int kSpillBuckets = 32;     // coarse partitions a block appends to
 constexpr int kLdsSlots = 1024;       // block table slots (power of two)
 constexpr int kLdsAggs = 4;           // aggregates pre-aggregated in LDS
 constexpr int kLdsProbe = 8;          // LDS probe limit before going global
@@ -242,17 +243,20 @@ struct AggJitArgs {
   int64_t dense_min;
   uint32_t dense_keys, dense_null;
   // spill: rows the block table does not hold whose key is in [smin, smin +
-  // srange) go to the block's region of `spill` (records of 1 + kAggs words:
-  // group index | cm << 32, then the contributions) instead of HBM atomics;
-  // per block, its record count and a histogram over partitions of 2^slog2
-  // groups (hist[part * gridDim + block])
+  // srange) go to `spill` instead of HBM atomics, as records of 1 + kAggs
+  // words (group index | cm << 32, then the contributions). Block b appends
+  // to kSpillBuckets buckets of spill_cap records each (bucket c at record
+  // (b * kSpillBuckets + c) * spill_cap; partition p goes to bucket p /
+  // sper), so a few tails per block stay in L2 while they fill. Per block:
+  // the bucket counts (spill_n[b * kSpillBuckets + c]) and a histogram over
+  // partitions of 2^slog2 groups (hist[p * gridDim + b]).
   int64_t *spill;               // NULL: no spilling
-  uint64_t spill_cap;           // records per block region
+  uint64_t spill_cap;           // records per bucket
   uint32_t *spill_n;
   uint32_t *hist;
   int64_t smin;
   uint64_t srange;
-  uint32_t slog2, sparts;
+  uint32_t slog2, sparts, sper;
 };
 
 // group of a probe key through the join: false when nothing matches
@@ -312,7 +316,7 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
   constexpr int NLO = Q::kExactF64 ? lo_index<Q>(LA) : 0;  // compensated FP64 sums in LDS
   constexpr bool SP = LA == NA && !Q::kExactF64;            // rows may spill (see AggJitArgs)
   __shared__ uint32_t s_hist[SP ? kMaxSpillParts : 1];
-  __shared__ uint32_t s_spill;
+  __shared__ uint32_t s_bucket[SP ? kSpillBuckets : 1];
   __shared__ int64_t s_keys[kLdsSlots + 2];
   __shared__ int64_t s_acc[LAS][kLdsSlots + 2];
   __shared__ int64_t s_lo[NLO > 0 ? NLO : 1][NLO > 0 ? kLdsSlots + 2 : 1];
@@ -330,12 +334,11 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
 #pragma unroll
     for (int b = 0; b < NLO; b++) s_lo[b][i] = 0;
   }
-  if (tid == 0) {
-    s_used = 0;
-    s_spill = 0;
-  }
-  if (SP && p.spill)
+  if (tid == 0) s_used = 0;
+  if (SP && p.spill) {
     for (uint32_t i = tid; i < p.sparts; i += 256) s_hist[i] = 0;
+    if (tid < kSpillBuckets) s_bucket[tid] = 0;
+  }
   __syncthreads();
 
   bool err = false, ok = true;
@@ -446,29 +449,22 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
           }
         }
       }
-      // a row the block table does not hold goes to the spill buffer when its
-      // key is in the spill range (positions: one LDS atomic per wave), else
-      // to the HBM table
+      // a row the block table does not hold goes to its spill bucket when
+      // its key is in the spill range (and the bucket has room), else to the
+      // HBM table
       bool spilled = false;
-      if (SP) {
-        const bool want = p.spill && sel && lslot < 0 && kv && key >= p.smin &&
-                          (uint64_t)key - (uint64_t)p.smin < p.srange;
-        const uint64_t wm = __ballot(want);
-        if (wm) {
-          const int first = __builtin_ctzll(wm);
-          uint32_t at = 0;
-          if (lane == first) at = atomicAdd(&s_spill, (uint32_t)__popcll(wm));
-          at = (uint32_t)__shfl((int)at, first, 64);
-          const uint64_t pos = (uint64_t)at + __popcll(wm & ((1ull << lane) - 1));
-          if (want && pos < p.spill_cap) {
-            const uint32_t g = (uint32_t)((uint64_t)key - (uint64_t)p.smin);
-            int64_t *r = p.spill + ((uint64_t)blockIdx.x * p.spill_cap + pos) * (NA + 1);
-            r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
+      if (SP && p.spill && sel && lslot < 0 && kv && key >= p.smin &&
+          (uint64_t)key - (uint64_t)p.smin < p.srange) {
+        const uint32_t g = (uint32_t)((uint64_t)key - (uint64_t)p.smin);
+        const uint32_t part = g >> p.slog2, bucket = part / p.sper;
+        const uint32_t pos = atomicAdd(&s_bucket[SP ? bucket : 0], 1u);
+        if (pos < p.spill_cap) {
+          int64_t *r = p.spill + (((uint64_t)blockIdx.x * kSpillBuckets + bucket) * p.spill_cap + pos) * (NA + 1);
+          r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
 #pragma unroll
-            for (int a = 0; a < NA; a++) r[1 + a] = c[a];
-            atomicAdd(&s_hist[SP ? (g >> p.slog2) : 0], 1u);
-            spilled = true;
-          }
+          for (int a = 0; a < NA; a++) r[1 + a] = c[a];
+          atomicAdd(&s_hist[SP ? part : 0], 1u);
+          spilled = true;
         }
       }
       if (!sel || spilled) continue;
@@ -518,9 +514,10 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
     }
   }
   if (SP && p.spill) {
-    // s_hist and s_spill are final: every tile ended with a barrier
+    // s_hist and s_bucket are final: every tile ended with a barrier
     for (uint32_t i = tid; i < p.sparts; i += 256) p.hist[(uint64_t)i * gridDim.x + blockIdx.x] = s_hist[i];
-    if (tid == 0) p.spill_n[blockIdx.x] = s_spill < p.spill_cap ? s_spill : (uint32_t)p.spill_cap;
+    if (tid < kSpillBuckets)
+      p.spill_n[blockIdx.x * kSpillBuckets + tid] = s_bucket[tid] < p.spill_cap ? s_bucket[tid] : (uint32_t)p.spill_cap;
   }
   if (err) atomicOr(p.error, 1u);
   if (!ok) atomicOr(p.t.overflow, 1u);

@@ -264,19 +264,26 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
 
 // ------------------------------------------------------------------ finish
 // Compacts (key, slot) of the occupied ordinary slots; one counter atomic
-// per wave.
-__global__ void compact_kernel(Table t, int64_t *out_keys, uint64_t *out_slots, uint64_t *count) {
+// per block (same-address atomics serialise at ~10 ns each).
+__global__ __launch_bounds__(256) void compact_kernel(Table t, int64_t *out_keys, uint64_t *out_slots,
+                                                      uint64_t *count) {
+  __shared__ uint32_t s_wave[4];
+  __shared__ unsigned long long s_base;
   const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const int64_t k = i < t.cap ? t.keys[i] : kEmpty;
   const bool used = k != kEmpty;
   const uint64_t m = __ballot(used);
-  if (!m) return;  // the whole wave
-  const int lane = threadIdx.x & 63, first = __builtin_ctzll(m);
-  unsigned long long base = 0;
-  if (lane == first) base = atomicAdd((unsigned long long *)count, (unsigned long long)__popcll(m));
-  base = (unsigned long long)__shfl((long long)base, first, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s_wave[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    s_base = total ? atomicAdd((unsigned long long *)count, (unsigned long long)total) : 0;
+  }
+  __syncthreads();
   if (used) {
-    const uint64_t at = base + __popcll(m & ((1ull << lane) - 1));
+    uint64_t at = s_base + __popcll(m & ((1ull << lane) - 1));
+    for (int v = 0; v < w; v++) at += s_wave[v];
     out_keys[at] = k;
     out_slots[at] = i;
   }

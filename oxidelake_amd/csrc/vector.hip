@@ -284,6 +284,17 @@ struct Prec<P_FP8> {
   static constexpr int EB = 1, KG = 64, LB = 32;  // one 32x32x64
 };
 
+// Byte offset, within a row, of k group g's slice for lane half h. The k
+// axis is permuted (a dot product does not care) so that the R k groups of
+// one 128-byte line are read by the same two lanes, 64 contiguous bytes
+// each: a row's line is consumed by back-to-back loads instead of being
+// fetched again for every k group.
+template <int P>
+__device__ inline uint32_t koff(int g, int h) {
+  constexpr int LB = Prec<P>::LB, R = 128 / (Prec<P>::KG * Prec<P>::EB);
+  return (uint32_t)((g / R) * 128 + h * 64 + (g % R) * LB);
+}
+
 template <int P>
 struct Frag {
   uint4 v[Prec<P>::LB / 16];
@@ -310,7 +321,10 @@ __device__ inline void mfma_group(f32x16 &acc, const Frag<P> &a, const Frag<P> &
   }
 }
 
-template <int P, int NT, int RT>
+// BRES: the whole query tile (every k group) sits in dynamic LDS, loaded
+// once per block, so the k loop has no barriers and each wave streams its
+// rows at its own pace (small query tiles, where the scan is HBM-bound).
+template <int P, int NT, int RT, bool BRES>
 __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   constexpr int LB = Prec<P>::LB, KG = Prec<P>::KG, EB = Prec<P>::EB;
   constexpr int U = LB / 16;                 // 16-byte units per lane per group
@@ -318,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   constexpr int BUNITS = NT * 32 * QROW;     // units of one group's query slice
   constexpr int PER_T = (BUNITS + kBlock - 1) / kBlock;
   constexpr int BROWS = 4 * RT * 32;         // rows per block
-  __shared__ uint4 s_b[2][BUNITS];
+  __shared__ uint4 s_b[2][BRES ? 1 : BUNITS];
   // the block's rows' norms, scales and validity, read once per block into
   // LDS so the epilogue holds no row data in registers across the k loop
   __shared__ float s_info[BROWS];
@@ -330,10 +344,20 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   const uint8_t *xb = (const uint8_t *)a.x;
   const uint8_t *qb = (const uint8_t *)a.qm;
   const uint64_t tile_end = min(a.n, a.tile0 + a.tile_rows);
+  extern __shared__ uint4 s_bres[];  // BRES: [groups][BUNITS]
+  if (BRES) {
+    const int total = a.groups * BUNITS;
+    for (int unit = tid; unit < total; unit += kBlock) {
+      const int g = unit / BUNITS, rem = unit % BUNITS;
+      const int qrow = rem / QROW, part = rem % QROW;
+      s_bres[unit] = *(const uint4 *)(qb + (uint64_t)qrow * rstride + koff<P>(g, part / U) + (part % U) * 16);
+    }
+  }
 
   for (uint64_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
     const uint64_t blockrow = a.tile0 + blk * (uint64_t)BROWS;
     const uint64_t rowbase = blockrow + (uint64_t)wave * RT * 32;
+    if (BRES) __syncthreads();  // the previous block's epilogue has read s_info
     for (int t = tid; t < BROWS; t += kBlock) {
       const uint64_t row = blockrow + t;
       float info = -INFINITY, scl = 1.0f;  // -inf: the row cannot qualify
@@ -359,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         const bool in = row < tile_end;
         // rows past the tile read row tile0 (always present) and are zeroed
         const uint64_t arow = in ? row : a.tile0;
-        const uint4 *src = (const uint4 *)(xb + arow * rstride + (uint64_t)(g * KG + h * (KG / 2)) * EB);
+        const uint4 *src = (const uint4 *)(xb + arow * rstride + koff<P>(g, h));
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const uint4 v = src[u];
@@ -373,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         const int unit = p * kBlock + tid;
         if (unit < BUNITS) {
           const int qrow = unit / QROW, part = unit % QROW;
-          rb[p] = *(const uint4 *)(qb + (uint64_t)qrow * rstride + (uint64_t)g * KG * EB + part * 16);
+          rb[p] = *(const uint4 *)(qb + (uint64_t)qrow * rstride + koff<P>(g, part / U) + (part % U) * 16);
         }
       }
     };
@@ -386,12 +410,42 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
     };
 
     Frag<P> fa[RT], fn[RT];
+    if (BRES) {
+      __syncthreads();  // s_info (and, the first time, s_bres) written
+      // three fragment sets in rotation keep two k groups of loads in
+      // flight while the third is multiplied
+      auto mul = [&](int g, const Frag<P> (&f)[RT]) {
+        const uint4 *bg = s_bres + (uint64_t)g * BUNITS;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          Frag<P> fb;
+#pragma unroll
+          for (int u = 0; u < U; u++) fb.v[u] = bg[(t * 32 + i) * QROW + h * U + u];
+#pragma unroll
+          for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], f[r], fb);
+        }
+      };
+      Frag<P> fc[RT];
+      const int G = a.groups;
+      load_a(0, fa);
+      if (G > 1) load_a(1, fn);
+      for (int g = 0; g < G; g += 3) {
+        if (g + 2 < G) load_a(g + 2, fc);
+        mul(g, fa);
+        if (g + 3 < G) load_a(g + 3, fa);
+        if (g + 1 < G) mul(g + 1, fn);
+        if (g + 4 < G) load_a(g + 4, fn);
+        if (g + 2 < G) mul(g + 2, fc);
+      }
+    }
     uint4 rb[PER_T];
+    if (!BRES) {
     load_a(0, fa);
     load_b(0, rb);
     store_b(0, rb);
     __syncthreads();
-    for (int g = 0; g < a.groups; g++) {
+    }
+    for (int g = 0; !BRES && g < a.groups; g++) {
       const bool more = g + 1 < a.groups;
       if (more) {
         load_a(g + 1, fn);
@@ -426,19 +480,32 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         if (q >= a.nq) continue;
         const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
         float *out = a.scores + (uint64_t)q * a.tile_rows + (rowbase + r * 32 - a.tile0);
+        // elements 4j..4j+3 of the lane are rows 8j + 4h + 0..3: one 16-byte
+        // store each (tile_rows is a multiple of 128, so they are aligned)
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-          const int m = 8 * (e / 4) + 4 * h + (e % 4);
-          if (rowbase + r * 32 + m >= tile_end) continue;
-          const int br = wave * RT * 32 + r * 32 + m;
-          const float info = s_info[br];
-          float dot = acc[r][t][e];
-          if (P == P_FP8) dot = dot * s_scale[br] * qs;
-          float score;
-          if (info == -INFINITY) score = -INFINITY;  // the row cannot qualify
-          else if (a.metric == OXR_COSINE) score = dot * info;
-          else score = 2.0f * dot - info;
-          out[m] = score;
+        for (int jg = 0; jg < 4; jg++) {
+          const int m0 = 8 * jg + 4 * h;
+          float sc[4];
+#pragma unroll
+          for (int e4 = 0; e4 < 4; e4++) {
+            const int br = wave * RT * 32 + r * 32 + m0 + e4;
+            const float info = s_info[br];
+            float dot = acc[r][t][jg * 4 + e4];
+            if (P == P_FP8) dot = dot * s_scale[br] * qs;
+            float score;
+            if (info == -INFINITY) score = -INFINITY;  // the row cannot qualify
+            else if (a.metric == OXR_COSINE) score = dot * info;
+            else score = 2.0f * dot - info;
+            sc[e4] = score;
+          }
+          const uint64_t row0 = rowbase + r * 32 + m0;
+          if (row0 + 4 <= tile_end) {
+            *(float4 *)(out + m0) = make_float4(sc[0], sc[1], sc[2], sc[3]);
+          } else {
+#pragma unroll
+            for (int e4 = 0; e4 < 4; e4++)
+              if (row0 + e4 < tile_end) out[m0 + e4] = sc[e4];
+          }
         }
         // one (row tile, query tile) at a time: the scheduler may not batch
         // every tile's LDS reads ahead (that would cost occupancy)
@@ -653,32 +720,46 @@ int grid_for(oxr_device *d, uint64_t units, int per_cu) {
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(units, cap));
 }
 
+// Query tiles of at most this many bytes (every k group) sit in LDS for
+// the whole scan (scan_kernel's BRES).
+constexpr int kResidentQueryBytes = 64 * 1024;
+
 template <int P, int NT>
-void launch_scan_nt(dim3 grid, hipStream_t s, const ScanArgs &a) {
+void launch_scan_nt(oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
   constexpr int RT = NT == 1 ? 4 : NT == 2 ? 2 : 1;
-  hipLaunchKernelGGL((scan_kernel<P, NT, RT>), grid, dim3(kBlock), 0, s, a);
+  const size_t bres = (size_t)NT * 32 * a.dimp * Prec<P>::EB;
+  if (bres <= (size_t)kResidentQueryBytes && tune_get("vector.resident_queries", 1) > 0) {
+    // as many blocks per CU as LDS holds (a block keeps its query tile)
+    const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (bres + 8 * 1024))));
+    hipLaunchKernelGGL((scan_kernel<P, NT, RT, true>), dim3(grid_for(dev, units, per_cu)), dim3(kBlock), bres, s, a);
+  } else {
+    hipLaunchKernelGGL((scan_kernel<P, NT, RT, false>), dim3(grid_for(dev, units, NT >= 4 ? 2 : 8)), dim3(kBlock), 0,
+                       s, a);
+  }
 }
 
 int rows_per_block(int nt) { return 4 * 32 * (nt == 1 ? 4 : nt == 2 ? 2 : 1); }
 
 template <int P>
-void launch_scan_p(int nt, dim3 grid, hipStream_t s, const ScanArgs &a) {
+void launch_scan_p(int nt, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
   switch (nt) {
-    case 1: launch_scan_nt<P, 1>(grid, s, a); break;
-    case 2: launch_scan_nt<P, 2>(grid, s, a); break;
-    case 4: launch_scan_nt<P, 4>(grid, s, a); break;
-    default: launch_scan_nt<P, 8>(grid, s, a); break;
+    case 1: launch_scan_nt<P, 1>(dev, units, s, a); break;
+    case 2: launch_scan_nt<P, 2>(dev, units, s, a); break;
+    case 4: launch_scan_nt<P, 4>(dev, units, s, a); break;
+    default: launch_scan_nt<P, 8>(dev, units, s, a); break;
   }
 }
 
-void launch_scan(int prec, int nt, dim3 grid, hipStream_t s, const ScanArgs &a) {
-  if (prec == P_FP32) launch_scan_p<P_FP32>(nt, grid, s, a);
-  else if (prec == P_BF16) launch_scan_p<P_BF16>(nt, grid, s, a);
-  else launch_scan_p<P_FP8>(nt, grid, s, a);
+void launch_scan(int prec, int nt, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
+  if (prec == P_FP32) launch_scan_p<P_FP32>(nt, dev, units, s, a);
+  else if (prec == P_BF16) launch_scan_p<P_BF16>(nt, dev, units, s, a);
+  else launch_scan_p<P_FP8>(nt, dev, units, s, a);
 }
 
+// Rows are padded to whole 128-byte lines of the scan precision (the
+// scan's k permutation works a line at a time).
 int pad_dim(int dim, int prec) {
-  const int m = prec == P_FP8 ? 64 : prec == P_BF16 ? 16 : 8;
+  const int m = prec == P_FP8 ? 128 : prec == P_BF16 ? 64 : 32;
   return (dim + m - 1) / m * m;
 }
 
@@ -794,7 +875,7 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
       a.scores = scores;
       {
         ProfScope prof(st, "vector_scan");
-        launch_scan(ix->precision, nt, dim3(grid_for(ix->dev, a.nblocks, nt >= 4 ? 2 : 8)), st, a);
+        launch_scan(ix->precision, nt, ix->dev, a.nblocks, st, a);
         OXR_HIP(hipGetLastError());
       }
       SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand};

@@ -259,6 +259,19 @@ __global__ __launch_bounds__(kBlock) void prep_kernel(const PrepArgs a) {
 // (double-buffered). Verified lane layouts (32x32 instructions): A row and
 // B column = lane%32, both at the same k set for lane half l/32; result
 // element r of lane l is (row 8*(r/4) + 4*(l/32) + r%4, query l%32).
+// Keys that order candidates: larger is better. The score's bits made
+// order-preserving as an unsigned integer, then the row (smaller row wins a
+// tie). 0 = no candidate.
+__device__ inline uint32_t ordered(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline uint64_t score_key(float score, uint64_t row) {
+  return ((uint64_t)ordered(score) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)row);
+}
+__device__ inline uint64_t key_row(uint64_t key) { return 0xFFFFFFFFu - (uint32_t)key; }
+
 struct ScanArgs {
   const void *x;      // [n][dimp] in the scan precision
   const void *qm;     // [qpad][dimp]
@@ -267,6 +280,14 @@ struct ScanArgs {
   const float *rowinfo, *rscale, *qscale;
   const uint32_t *valid;
   float *scores;      // [nq][tile_rows]: scores ordered like the distance
+  // append mode (list != NULL): no scores; a score at or above its query's
+  // bound (the ordered score of tau_keys[q * C + C - 1]) is appended to
+  // list[q * cap + atomicAdd(&cnt[q], 1)] as a key (when below cap)
+  const uint64_t *tau_keys;
+  uint64_t *list;
+  uint32_t *cnt;
+  uint64_t cap;
+  int32_t C;
 };
 
 template <int P>
@@ -480,6 +501,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         if (q >= a.nq) continue;
         const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
         float *out = a.scores + (uint64_t)q * a.tile_rows + (rowbase + r * 32 - a.tile0);
+        const uint32_t tq = a.list ? (uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32) : 0;
         // elements 4j..4j+3 of the lane are rows 8j + 4h + 0..3: one 16-byte
         // store each (tile_rows is a multiple of 128, so they are aligned)
 #pragma unroll
@@ -499,7 +521,14 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
             sc[e4] = score;
           }
           const uint64_t row0 = rowbase + r * 32 + m0;
-          if (row0 + 4 <= tile_end) {
+          if (a.list) {
+#pragma unroll
+            for (int e4 = 0; e4 < 4; e4++) {
+              if (row0 + e4 >= tile_end || !(sc[e4] > -INFINITY) || ordered(sc[e4]) < tq) continue;
+              const uint32_t pos = atomicAdd(&a.cnt[q], 1u);
+              if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sc[e4], row0 + e4);
+            }
+          } else if (row0 + 4 <= tile_end) {
             *(float4 *)(out + m0) = make_float4(sc[0], sc[1], sc[2], sc[3]);
           } else {
 #pragma unroll
@@ -516,18 +545,6 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 }
 
 // ------------------------------------------------------------------ select
-// Keys that order candidates: larger is better. The score's bits made
-// order-preserving as an unsigned integer, then the row (smaller row wins a
-// tie). 0 = no candidate.
-__device__ inline uint32_t ordered(float f) {
-  uint32_t u;
-  __builtin_memcpy(&u, &f, 4);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ inline uint64_t score_key(float score, uint64_t row) {
-  return ((uint64_t)ordered(score) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)row);
-}
-__device__ inline uint64_t key_row(uint64_t key) { return 0xFFFFFFFFu - (uint32_t)key; }
 
 constexpr int kBuf = 4096;   // LDS candidate buffer (keys)
 constexpr int kChunk = 2048; // keys read per round (8 per thread)
@@ -644,6 +661,23 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(uint64_t *cand, int segs_
   const uint64_t *in = cand + (uint64_t)q * segs_total * C;
   top_c([&](uint64_t idx) -> uint64_t { return in[idx]; }, (uint64_t)segs_total * C, C, s_buf,
         &s_count, &s_tau);
+  for (int j = threadIdx.x; j < C; j += kBlock) best[(uint64_t)q * C + j] = s_buf[j];
+}
+
+// grid (queries): the best C of the pilot's best C and the query's
+// appended keys
+__global__ __launch_bounds__(kBlock) void merge_lists_kernel(const uint64_t *pilot_best, const uint64_t *list,
+                                                             const uint32_t *cnt, uint64_t cap, int C,
+                                                             uint64_t *best) {
+  __shared__ uint64_t s_buf[kBuf];
+  __shared__ int s_count;
+  __shared__ uint64_t s_tau;
+  const int q = blockIdx.x;
+  const uint64_t *pb = pilot_best + (uint64_t)q * C;
+  const uint64_t *ls = list + (uint64_t)q * cap;
+  const uint64_t len = (uint64_t)C + min((uint64_t)cnt[q], cap);
+  top_c([&](uint64_t idx) -> uint64_t { return idx < (uint64_t)C ? pb[idx] : ls[idx - C]; }, len, C, s_buf, &s_count,
+        &s_tau);
   for (int j = threadIdx.x; j < C; j += kBlock) best[(uint64_t)q * C + j] = s_buf[j];
 }
 
@@ -837,60 +871,115 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
     hipLaunchKernelGGL(prep_kernel, dim3(qpad), dim3(kBlock), 0, st, pa);
     OXR_HIP(hipGetLastError());
 
-    // score tiles of at most ~2 GB
     const uint64_t rpb = (uint64_t)rows_per_block(nt);
-    uint64_t tile = std::max<uint64_t>(rpb, ((2ull << 30) / (4ull * nqc)) / rpb * rpb);
-    if (tile > ix->n) tile = (ix->n + rpb - 1) / rpb * rpb;
-    const uint64_t ntiles = (ix->n + tile - 1) / tile;
-    // segments: about 2048 select blocks per tile, each at least 8K rows
-    const uint64_t want = std::max<uint64_t>(1, 2048 / (uint64_t)nqc);
-    uint64_t seg_rows = std::max<uint64_t>(8192, (tile + want - 1) / want);
-    const uint64_t segs_per_tile = (tile + seg_rows - 1) / seg_rows;
-    const uint64_t segs_total = segs_per_tile * ntiles;
-    float *scores;
-    uint64_t *cand, *best;
-    OXR_TRY(sc.get((size_t)nqc * tile * 4, &scores));
-    OXR_TRY(sc.get((size_t)nqc * segs_total * C * 8, &cand));
+    ScanArgs base{};
+    base.x = xscan;
+    base.qm = qm;
+    base.dimp = ix->dimp;
+    base.groups = ix->dimp / (ix->precision == P_FP8 ? 64 : ix->precision == P_BF16 ? 16 : 8);
+    base.nq = nqc;
+    base.metric = ix->metric;
+    base.rowinfo = ix->rowinfo;
+    base.rscale = ix->rscale;
+    base.qscale = qscale;
+    base.valid = ix->valid;
+    base.C = C;
+
+    // The best C keys per query of rows [begin, end), into out ([nqc][C]):
+    // score tiles of at most ~2 GB, each selected in segments, then merged.
+    auto scan_select = [&](uint64_t begin, uint64_t end, uint64_t *out) -> int {
+      const uint64_t rows = end - begin;
+      uint64_t tile = std::max<uint64_t>(rpb, ((2ull << 30) / (4ull * nqc)) / rpb * rpb);
+      if (tile > rows) tile = (rows + rpb - 1) / rpb * rpb;
+      const uint64_t ntiles = (rows + tile - 1) / tile;
+      // segments: about 2048 select blocks per tile, each at least 8K rows
+      const uint64_t want = std::max<uint64_t>(1, 2048 / (uint64_t)nqc);
+      const uint64_t seg_rows = std::max<uint64_t>(8192, (tile + want - 1) / want);
+      const uint64_t segs_per_tile = (tile + seg_rows - 1) / seg_rows;
+      const uint64_t segs_total = segs_per_tile * ntiles;
+      float *scores;
+      uint64_t *cand;
+      OXR_TRY(sc.get((size_t)nqc * tile * 4, &scores));
+      OXR_TRY(sc.get((size_t)nqc * segs_total * C * 8, &cand));
+      // segments past the end of the last tile stay empty
+      OXR_TRY(fill_zero(s, cand, (size_t)nqc * segs_total * C * 8));
+      for (uint64_t ti = 0; ti < ntiles; ti++) {
+        const uint64_t t0 = begin + ti * tile;
+        const uint64_t tlen = std::min(tile, end - t0);
+        ScanArgs a = base;
+        a.n = end;
+        a.tile0 = t0;
+        a.tile_rows = tile;
+        a.nblocks = (tlen + rpb - 1) / rpb;
+        a.scores = scores;
+        {
+          ProfScope prof(st, "vector_scan");
+          launch_scan(ix->precision, nt, ix->dev, a.nblocks, st, a);
+          OXR_HIP(hipGetLastError());
+        }
+        SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand};
+        const uint64_t segs_here = (tlen + seg_rows - 1) / seg_rows;
+        {
+          ProfScope prof(st, "vector_select");
+          hipLaunchKernelGGL(select_tile_kernel, dim3((unsigned)segs_here, nqc), dim3(kBlock), 0, st, sa);
+          OXR_HIP(hipGetLastError());
+        }
+      }
+      ProfScope prof(st, "vector_merge");
+      hipLaunchKernelGGL(merge_kernel, dim3(nqc), dim3(kBlock), 0, st, cand, (int)segs_total, C, out);
+      OXR_HIP(hipGetLastError());
+      return OXR_OK;
+    };
+
+    uint64_t *best;
     OXR_TRY(sc.get((size_t)nqc * C * 8, &best));
-    // segments past the end of the last tile stay empty
-    OXR_TRY(fill_zero(s, cand, (size_t)nqc * segs_total * C * 8));
-    for (uint64_t ti = 0; ti < ntiles; ti++) {
-      const uint64_t t0 = ti * tile;
-      const uint64_t tlen = std::min(tile, ix->n - t0);
-      ScanArgs a{};
-      a.x = xscan;
-      a.qm = qm;
+    // Pilot (knob vector.pilot): the best C of the first rows bound each
+    // query's final C-th best from below, so the other rows need no score
+    // array: the scan appends only the keys at or above the bound. A list
+    // that overflows sends the batch down the plain path.
+    const int64_t pilot_knob = tune_get("vector.pilot_rows", -1);
+    const uint64_t pilot_want = pilot_knob > 0 ? (uint64_t)pilot_knob : std::max<uint64_t>(1ull << 20, (uint64_t)C * 1024);
+    const uint64_t pilot = std::min<uint64_t>(ix->n, std::max<uint64_t>(rpb, pilot_want / rpb * rpb));
+    bool done = false;
+    if (tune_get("vector.pilot", 1) > 0 && ix->n >= 4 * pilot) {
+      uint64_t *pbest, *list;
+      uint32_t *cnt;
+      const uint64_t rest = ix->n - pilot;
+      const int64_t cap_knob = tune_get("vector.pilot_cap", -1);  // tests force the overflow path with it
+      const uint64_t cap = cap_knob > 0 ? (uint64_t)cap_knob
+                                        : std::max<uint64_t>(65536, 4 * (uint64_t)C * ((rest + pilot - 1) / pilot));
+      OXR_TRY(sc.get((size_t)nqc * C * 8, &pbest));
+      OXR_TRY(sc.get((size_t)nqc * cap * 8, &list));
+      OXR_TRY(sc.get((size_t)nqc * 4, &cnt));
+      OXR_TRY(fill_zero(s, cnt, (size_t)nqc * 4));
+      OXR_TRY(scan_select(0, pilot, pbest));
+      ScanArgs a = base;
       a.n = ix->n;
-      a.tile0 = t0;
-      a.tile_rows = tile;
-      a.nblocks = (tlen + rpb - 1) / rpb;
-      a.dimp = ix->dimp;
-      a.groups = ix->dimp / (ix->precision == P_FP8 ? 64 : ix->precision == P_BF16 ? 16 : 8);
-      a.nq = nqc;
-      a.metric = ix->metric;
-      a.rowinfo = ix->rowinfo;
-      a.rscale = ix->rscale;
-      a.qscale = qscale;
-      a.valid = ix->valid;
-      a.scores = scores;
+      a.tile0 = pilot;
+      a.tile_rows = (rest + rpb - 1) / rpb * rpb;
+      a.nblocks = a.tile_rows / rpb;
+      a.tau_keys = pbest;
+      a.list = list;
+      a.cnt = cnt;
+      a.cap = cap;
       {
         ProfScope prof(st, "vector_scan");
         launch_scan(ix->precision, nt, ix->dev, a.nblocks, st, a);
         OXR_HIP(hipGetLastError());
       }
-      SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand};
-      const uint64_t segs_here = (tlen + seg_rows - 1) / seg_rows;
-      {
-        ProfScope prof(st, "vector_select");
-        hipLaunchKernelGGL(select_tile_kernel, dim3((unsigned)segs_here, nqc), dim3(kBlock), 0, st, sa);
+      std::vector<uint32_t> hcnt(nqc);
+      OXR_HIP(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nqc * 4, hipMemcpyDeviceToHost, st));
+      OXR_HIP(hipStreamSynchronize(st));
+      bool overflow = false;
+      for (uint32_t c : hcnt) overflow |= c > cap;
+      if (!overflow) {
+        ProfScope prof(st, "vector_merge");
+        hipLaunchKernelGGL(merge_lists_kernel, dim3(nqc), dim3(kBlock), 0, st, pbest, list, cnt, cap, C, best);
         OXR_HIP(hipGetLastError());
+        done = true;
       }
     }
-    {
-      ProfScope prof(st, "vector_merge");
-      hipLaunchKernelGGL(merge_kernel, dim3(nqc), dim3(kBlock), 0, st, cand, (int)segs_total, C, best);
-      OXR_HIP(hipGetLastError());
-    }
+    if (!done) OXR_TRY(scan_select(0, ix->n, best));
     RerankArgs ra{ix->f32, ix->dim, ix->metric, C, Cpad, k, dq + (size_t)q0 * ix->dim, best,
                   drows + (size_t)q0 * k, ddist + (size_t)q0 * k};
     const size_t shm = (size_t)Cpad * 8 + (size_t)ix->dim * 4;

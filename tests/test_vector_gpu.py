@@ -214,3 +214,40 @@ def test_empty_and_all_null(backend, stream):
     assert rows.tolist() == [[-1, -1, -1]]
     out = backend.download(stream, backend.vector_distance(stream, dev, 1, [1, 1, 1, 1], M.L2, "d"))
     assert out.column(2).null_count == 10
+
+
+@pytest.mark.parametrize("precision", [VEC_FP32, VEC_BF16, VEC_FP8])
+@pytest.mark.parametrize("overflow", [False, True])
+def test_pilot_bound_matches_plain_scan(backend, stream, precision, overflow):
+    # the first rows' best C bound each query's C-th best from below; the
+    # other rows only append keys at or above it. Same keys, same results as
+    # the plain scan, bit for bit; an overflowing list falls back to it
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    n, dim = 60_000, 64
+    dev, vecs = mixture(backend, stream, n, dim, normalized=True, seed=9)
+    rng = np.random.default_rng(3)
+    ix = VectorIndex(stream, dev, 0, M.Cosine, precision)
+    for nq in (1, 40, 300):
+        qs = (vecs[rng.integers(0, n, nq)] + rng.normal(scale=0.2, size=(nq, dim))).astype(np.float32)
+        autotune.set_knob("vector.pilot", 0)
+        plain = ix.search(stream, qs, 10, candidates=100)
+        autotune.set_knob("vector.pilot", 1)
+        autotune.set_knob("vector.pilot_rows", 4096)
+        if overflow:
+            autotune.set_knob("vector.pilot_cap", 16)
+        lib.oxr_profile_enable(1)
+        lib.oxr_profile_reset()
+        try:
+            got = ix.search(stream, qs, 10, candidates=100)
+            scans = autotune.launches("vector_scan")
+        finally:
+            lib.oxr_profile_enable(0)
+            lib.oxr_profile_reset()
+            for k in ("vector.pilot", "vector.pilot_rows", "vector.pilot_cap"):
+                autotune.set_knob(k, -1)
+        assert np.array_equal(got[0], plain[0]) and np.array_equal(got[1], plain[1]), nq
+        chunks = -(-nq // 256)
+        # pilot + appending scan (+ the plain scan after an overflow) per chunk of 256 queries
+        assert scans == chunks * (3 if overflow else 2), scans

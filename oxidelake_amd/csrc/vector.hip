@@ -280,6 +280,7 @@ struct ScanArgs {
   const float *rowinfo, *rscale, *qscale;
   const uint32_t *valid;
   float *scores;      // [nq][tile_rows]: scores ordered like the distance
+  int32_t qoff;       // first query of this launch
   // append mode (list != NULL): no scores; a score at or above its query's
   // bound (the ordered score of tau_keys[q * C + C - 1]) is appended to
   // list[q * cap + atomicAdd(&cnt[q], 1)] as a key (when below cap)
@@ -345,14 +346,17 @@ __device__ inline void mfma_group(f32x16 &acc, const Frag<P> &a, const Frag<P> &
 // BRES: the whole query tile (every k group) sits in dynamic LDS, loaded
 // once per block, so the k loop has no barriers and each wave streams its
 // rows at its own pace (small query tiles, where the scan is HBM-bound).
-template <int P, int NT, int RT, bool BRES>
+// QW: waves per row group. The block's 4 waves form (4 / QW) row groups of
+// RT row tiles times QW query groups of NT query tiles: QW > 1 shares each
+// row's loads between QW waves (through L1) for large query batches.
+template <int P, int NT, int RT, bool BRES, int QW>
 __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   constexpr int LB = Prec<P>::LB, KG = Prec<P>::KG, EB = Prec<P>::EB;
   constexpr int U = LB / 16;                 // 16-byte units per lane per group
   constexpr int QROW = 2 * U;                // units per query per group
-  constexpr int BUNITS = NT * 32 * QROW;     // units of one group's query slice
+  constexpr int BUNITS = QW * NT * 32 * QROW;  // units of one group's query slice
   constexpr int PER_T = (BUNITS + kBlock - 1) / kBlock;
-  constexpr int BROWS = 4 * RT * 32;         // rows per block
+  constexpr int BROWS = (4 / QW) * RT * 32;  // rows per block
   __shared__ uint4 s_b[2][BRES ? 1 : BUNITS];
   // the block's rows' norms, scales and validity, read once per block into
   // LDS so the epilogue holds no row data in registers across the k loop
@@ -361,6 +365,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 31, h = lane >> 5;
+  const int rg = wave / QW, qg = wave % QW;  // the wave's row group and query group
   const uint64_t rstride = (uint64_t)a.dimp * EB;  // bytes per row
   const uint8_t *xb = (const uint8_t *)a.x;
   const uint8_t *qb = (const uint8_t *)a.qm;
@@ -371,13 +376,13 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
     for (int unit = tid; unit < total; unit += kBlock) {
       const int g = unit / BUNITS, rem = unit % BUNITS;
       const int qrow = rem / QROW, part = rem % QROW;
-      s_bres[unit] = *(const uint4 *)(qb + (uint64_t)qrow * rstride + koff<P>(g, part / U) + (part % U) * 16);
+      s_bres[unit] = *(const uint4 *)(qb + (uint64_t)(a.qoff + qrow) * rstride + koff<P>(g, part / U) + (part % U) * 16);
     }
   }
 
   for (uint64_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
     const uint64_t blockrow = a.tile0 + blk * (uint64_t)BROWS;
-    const uint64_t rowbase = blockrow + (uint64_t)wave * RT * 32;
+    const uint64_t rowbase = blockrow + (uint64_t)rg * RT * 32;
     if (BRES) __syncthreads();  // the previous block's epilogue has read s_info
     for (int t = tid; t < BROWS; t += kBlock) {
       const uint64_t row = blockrow + t;
@@ -418,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         const int unit = p * kBlock + tid;
         if (unit < BUNITS) {
           const int qrow = unit / QROW, part = unit % QROW;
-          rb[p] = *(const uint4 *)(qb + (uint64_t)qrow * rstride + koff<P>(g, part / U) + (part % U) * 16);
+          rb[p] = *(const uint4 *)(qb + (uint64_t)(a.qoff + qrow) * rstride + koff<P>(g, part / U) + (part % U) * 16);
         }
       }
     };
@@ -441,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         for (int t = 0; t < NT; t++) {
           Frag<P> fb;
 #pragma unroll
-          for (int u = 0; u < U; u++) fb.v[u] = bg[(t * 32 + i) * QROW + h * U + u];
+          for (int u = 0; u < U; u++) fb.v[u] = bg[((qg * NT + t) * 32 + i) * QROW + h * U + u];
 #pragma unroll
           for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], f[r], fb);
         }
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
       for (int t = 0; t < NT; t++) {
         Frag<P> fb;
 #pragma unroll
-        for (int u = 0; u < U; u++) fb.v[u] = s_b[buf][(t * 32 + i) * QROW + h * U + u];
+        for (int u = 0; u < U; u++) fb.v[u] = s_b[buf][((qg * NT + t) * 32 + i) * QROW + h * U + u];
 #pragma unroll
         for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], fa[r], fb);
       }
@@ -497,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
     for (int r = 0; r < RT; r++) {
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const int q = t * 32 + i;
+        const int q = a.qoff + (qg * NT + t) * 32 + i;
         if (q >= a.nq) continue;
         const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
         float *out = a.scores + (uint64_t)q * a.tile_rows + (rowbase + r * 32 - a.tile0);
@@ -510,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
           float sc[4];
 #pragma unroll
           for (int e4 = 0; e4 < 4; e4++) {
-            const int br = wave * RT * 32 + r * 32 + m0 + e4;
+            const int br = rg * RT * 32 + r * 32 + m0 + e4;
             const float info = s_info[br];
             float dot = acc[r][t][jg * 4 + e4];
             if (P == P_FP8) dot = dot * s_scale[br] * qs;
@@ -754,40 +759,79 @@ int grid_for(oxr_device *d, uint64_t units, int per_cu) {
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(units, cap));
 }
 
-// Query tiles of at most this many bytes (every k group) sit in LDS for
-// the whole scan (scan_kernel's BRES).
-constexpr int kResidentQueryBytes = 64 * 1024;
+// How a chunk of queries is scanned: per wave RT row tiles x NT query
+// tiles, QW query groups per block, the block's query tile resident in LDS
+// (bres) or streamed; qpl queries per launch.
+struct ScanCfg {
+  int nt, rt, qw;
+  bool bres;
+  int qpl;
+  uint64_t rpb() const { return (uint64_t)(4 / qw) * rt * 32; }
+};
 
-template <int P, int NT>
-void launch_scan_nt(oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
-  constexpr int RT = NT == 1 ? 4 : NT == 2 ? 2 : 1;
-  const size_t bres = (size_t)NT * 32 * a.dimp * Prec<P>::EB;
-  if (bres <= (size_t)kResidentQueryBytes && tune_get("vector.resident_queries", 1) > 0) {
+// A block's query tile of at most this many bytes (every k group) can sit
+// in LDS for the whole scan.
+constexpr size_t kResidentQueryBytes = 100 * 1024;
+
+ScanCfg pick_scan(int prec, int nqc, int dimp) {
+  const size_t eb = prec == P_FP8 ? 1 : prec == P_BF16 ? 2 : 4;
+  const bool res = tune_get("vector.resident_queries", 1) > 0;
+  auto fits = [&](int queries) { return res && (size_t)queries * dimp * eb <= (size_t)64 * 1024; };
+  if (nqc <= 32) return {1, 4, 1, fits(32), 32};
+  if (nqc <= 64) return {2, 2, 1, fits(64), 64};
+  // large FP8 batches: two query groups per block share each row's loads,
+  // the block's 128 queries resident (measured at 256 queries over 20M x
+  // 768: 23.0 ms against 30.8 streamed; BF16 at 64 queries per launch
+  // re-reads the rows too often: 49.2 against 46.5, so it streams)
+  if (prec == P_FP8 && res && tune_get("vector.wide", 1) > 0 && (size_t)128 * dimp * eb <= kResidentQueryBytes)
+    return {2, 4, 2, true, 128};
+  return nqc <= 128 ? ScanCfg{4, 1, 1, false, 128} : ScanCfg{8, 1, 1, false, 256};
+}
+
+template <int P, int NT, int RT, bool BRES, int QW>
+void launch_scan_cfg(oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
+  if (BRES) {
+    const size_t bres = (size_t)QW * NT * 32 * a.dimp * Prec<P>::EB;
     // as many blocks per CU as LDS holds (a block keeps its query tile)
     const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (bres + 8 * 1024))));
-    hipLaunchKernelGGL((scan_kernel<P, NT, RT, true>), dim3(grid_for(dev, units, per_cu)), dim3(kBlock), bres, s, a);
-  } else {
-    hipLaunchKernelGGL((scan_kernel<P, NT, RT, false>), dim3(grid_for(dev, units, NT >= 4 ? 2 : 8)), dim3(kBlock), 0,
+    hipLaunchKernelGGL((scan_kernel<P, NT, RT, BRES, QW>), dim3(grid_for(dev, units, per_cu)), dim3(kBlock), bres,
                        s, a);
+  } else {
+    hipLaunchKernelGGL((scan_kernel<P, NT, RT, BRES, QW>), dim3(grid_for(dev, units, NT >= 4 ? 2 : 8)),
+                       dim3(kBlock), 0, s, a);
   }
 }
-
-int rows_per_block(int nt) { return 4 * 32 * (nt == 1 ? 4 : nt == 2 ? 2 : 1); }
 
 template <int P>
-void launch_scan_p(int nt, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
-  switch (nt) {
-    case 1: launch_scan_nt<P, 1>(dev, units, s, a); break;
-    case 2: launch_scan_nt<P, 2>(dev, units, s, a); break;
-    case 4: launch_scan_nt<P, 4>(dev, units, s, a); break;
-    default: launch_scan_nt<P, 8>(dev, units, s, a); break;
+void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
+  if (P == P_FP8 && c.qw == 2) {
+    launch_scan_cfg<P, 2, 4, true, 2>(dev, units, s, a);
+    return;
+  }
+  switch (c.nt) {
+    case 1:
+      if (c.bres) launch_scan_cfg<P, 1, 4, true, 1>(dev, units, s, a);
+      else launch_scan_cfg<P, 1, 4, false, 1>(dev, units, s, a);
+      break;
+    case 2:
+      if (c.bres) launch_scan_cfg<P, 2, 2, true, 1>(dev, units, s, a);
+      else launch_scan_cfg<P, 2, 2, false, 1>(dev, units, s, a);
+      break;
+    case 4: launch_scan_cfg<P, 4, 1, false, 1>(dev, units, s, a); break;
+    default: launch_scan_cfg<P, 8, 1, false, 1>(dev, units, s, a); break;
   }
 }
 
-void launch_scan(int prec, int nt, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
-  if (prec == P_FP32) launch_scan_p<P_FP32>(nt, dev, units, s, a);
-  else if (prec == P_BF16) launch_scan_p<P_BF16>(nt, dev, units, s, a);
-  else launch_scan_p<P_FP8>(nt, dev, units, s, a);
+// Every query launch of a chunk over the rows `base` describes.
+void launch_scan(int prec, const ScanCfg &c, int nqc, oxr_device *dev, uint64_t units, hipStream_t s,
+                 const ScanArgs &base) {
+  for (int q0 = 0; q0 < nqc; q0 += c.qpl) {
+    ScanArgs a = base;
+    a.qoff = q0;
+    if (prec == P_FP32) launch_scan_p<P_FP32>(c, dev, units, s, a);
+    else if (prec == P_BF16) launch_scan_p<P_BF16>(c, dev, units, s, a);
+    else launch_scan_p<P_FP8>(c, dev, units, s, a);
+  }
 }
 
 // Rows are padded to whole 128-byte lines of the scan precision (the
@@ -861,8 +905,8 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
 
   for (int q0 = 0; q0 < nq; q0 += kMaxScanQueries) {
     const int nqc = std::min(kMaxScanQueries, nq - q0);
-    const int nt = nqc <= 32 ? 1 : nqc <= 64 ? 2 : nqc <= 128 ? 4 : 8;
-    const int qpad = nt * 32;
+    const ScanCfg cfg = pick_scan(ix->precision, nqc, ix->dimp);
+    const int qpad = (nqc + cfg.qpl - 1) / cfg.qpl * cfg.qpl;
     void *qm;
     float *qscale;
     OXR_TRY(sc.get((size_t)qpad * ix->dimp * eb, &qm));
@@ -871,7 +915,7 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
     hipLaunchKernelGGL(prep_kernel, dim3(qpad), dim3(kBlock), 0, st, pa);
     OXR_HIP(hipGetLastError());
 
-    const uint64_t rpb = (uint64_t)rows_per_block(nt);
+    const uint64_t rpb = cfg.rpb();
     ScanArgs base{};
     base.x = xscan;
     base.qm = qm;
@@ -914,7 +958,7 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
         a.scores = scores;
         {
           ProfScope prof(st, "vector_scan");
-          launch_scan(ix->precision, nt, ix->dev, a.nblocks, st, a);
+          launch_scan(ix->precision, cfg, nqc, ix->dev, a.nblocks, st, a);
           OXR_HIP(hipGetLastError());
         }
         SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand};
@@ -964,7 +1008,7 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
       a.cap = cap;
       {
         ProfScope prof(st, "vector_scan");
-        launch_scan(ix->precision, nt, ix->dev, a.nblocks, st, a);
+        launch_scan(ix->precision, cfg, nqc, ix->dev, a.nblocks, st, a);
         OXR_HIP(hipGetLastError());
       }
       std::vector<uint32_t> hcnt(nqc);

@@ -251,3 +251,33 @@ def test_pilot_bound_matches_plain_scan(backend, stream, precision, overflow):
         chunks = -(-nq // 256)
         # pilot + appending scan (+ the plain scan after an overflow) per chunk of 256 queries
         assert scans == chunks * (3 if overflow else 2), scans
+
+
+@pytest.mark.parametrize("precision", [VEC_FP8])
+def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision):
+    # large batches: two query groups per block with the block's queries in
+    # LDS, several launches per chunk; the same keys as the streamed kernel,
+    # so the same results bit for bit, and recall against the FP64 oracle
+    from oxidelake_amd import autotune
+
+    n, dim, nq, k = 40_000, 768, 200, 10
+    dev, vecs = mixture(backend, stream, n, dim, normalized=True, seed=21)
+    rng = np.random.default_rng(8)
+    qs = vecs[rng.integers(0, n, nq)] + rng.normal(scale=0.3, size=(nq, dim)).astype(np.float32)
+    qs = (qs / np.linalg.norm(qs, axis=1, keepdims=True)).astype(np.float32)
+    ix = VectorIndex(stream, dev, 0, M.Cosine, precision)
+    cand = 100 if precision == VEC_BF16 else 1000
+    autotune.set_knob("vector.pilot", 0)
+    try:
+        wide = ix.search(stream, qs, k, candidates=cand)
+        autotune.set_knob("vector.wide", 0)
+        streamed = ix.search(stream, qs, k, candidates=cand)
+    finally:
+        autotune.set_knob("vector.pilot", -1)
+        autotune.set_knob("vector.wide", -1)
+    assert np.array_equal(wide[0], streamed[0]) and np.array_equal(wide[1], streamed[1])
+    hit = 0
+    for i in range(0, nq, 10):
+        want, _ = oracle.topk64(vecs, None, qs[i], M.Cosine, k)
+        hit += len(set(want.tolist()) & set(wide[0][i].tolist()))
+    assert hit / (len(range(0, nq, 10)) * k) >= 0.99

@@ -754,24 +754,35 @@ __global__ void gos_kernel(JTable t, DCol grp, uint64_t nslots, uint64_t *gos32,
 // Workgroup w = b * kSpillBuckets + c moves block b's bucket c to its fine
 // partitions' ranges (offs[part * B + b] = start of block b's records of
 // that partition; bucket c holds partitions [c * sper, (c + 1) * sper)).
+// W = words per record; each thread moves 4 records per step, loads first.
+template <int W>
 __global__ __launch_bounds__(256) void spill_scatter_kernel(const int64_t *spill, uint64_t cap,
                                                             const uint32_t *spill_n, const uint32_t *offs,
                                                             uint32_t B, uint32_t nparts, uint32_t sper,
-                                                            uint32_t slog2, uint32_t words, int64_t *out) {
+                                                            uint32_t slog2, int64_t *out) {
   __shared__ uint32_t s_cur[kMaxSpillParts];
   const uint32_t w = blockIdx.x, b = w / kSpillBuckets, c = w % kSpillBuckets;
   const uint32_t p0 = c * sper;
   for (uint32_t f = threadIdx.x; f < sper && p0 + f < nparts; f += 256) s_cur[f] = offs[(uint64_t)(p0 + f) * B + b];
   __syncthreads();
   const uint64_t n = spill_n[w];
-  const int64_t *src = spill + (uint64_t)w * cap * words;
-  for (uint64_t r = threadIdx.x; r < n; r += 256) {
-    const int64_t *rec = src + r * words;
-    const int64_t w0 = rec[0];
-    const uint32_t pos = atomicAdd(&s_cur[((uint32_t)w0 >> slog2) - p0], 1u);
-    int64_t *dst = out + (uint64_t)pos * words;
-    dst[0] = w0;
-    for (uint32_t k = 1; k < words; k++) dst[k] = rec[k];
+  const int64_t *src = spill + (uint64_t)w * cap * W;
+  for (uint64_t r0 = threadIdx.x; r0 < n; r0 += 4 * 256) {
+    int64_t rec[4][W];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint64_t r = r0 + k * 256;
+#pragma unroll
+      for (int x = 0; x < W; x++) rec[k][x] = r < n ? src[r * W + x] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (r0 + k * 256 >= n) continue;
+      const uint32_t pos = atomicAdd(&s_cur[((uint32_t)rec[k][0] >> slog2) - p0], 1u);
+      int64_t *dst = out + (uint64_t)pos * W;
+#pragma unroll
+      for (int x = 0; x < W; x++) dst[x] = rec[k][x];
+    }
   }
 }
 
@@ -782,9 +793,10 @@ struct FoldKinds {
 // Workgroup p folds partition p's records into an LDS table of 2^slog2
 // groups (accumulators, non-NULL flags, used flags: plain byte stores), then
 // adds each used group's accumulators to the HBM table.
+template <int W>
 __global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, const uint32_t *offs, uint32_t B,
-                                                         uint32_t words, uint32_t slog2, int64_t smin, Table t,
-                                                         FoldKinds fk, int na) {
+                                                         uint32_t slog2, int64_t smin, Table t, FoldKinds fk) {
+  constexpr int na = W - 1;
   extern __shared__ int64_t s_dyn[];
   const uint32_t G = 1u << slog2;
   int64_t *s_acc = s_dyn;                                  // [na][G]
@@ -800,15 +812,26 @@ __global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, co
   __syncthreads();
   const uint32_t part = blockIdx.x;
   const uint64_t lo = offs[(uint64_t)part * B], hi = offs[(uint64_t)(part + 1) * B];
-  for (uint64_t r = lo + threadIdx.x; r < hi; r += 256) {
-    const int64_t *rec = recs + r * words;
-    const uint64_t w0 = (uint64_t)rec[0];
-    const uint32_t g = (uint32_t)w0 & (G - 1), cm = (uint32_t)(w0 >> 32);
-    s_used[g] = 1;
-    for (int a = 0; a < na; a++) {
-      if (!((cm >> a) & 1u)) continue;
-      acc_apply_lds(fk.kind[a], &s_acc[a * G + g], rec[1 + a]);
-      s_has[a * G + g] = 1;
+  for (uint64_t r0 = lo + threadIdx.x; r0 < hi; r0 += 4 * 256) {
+    int64_t rec[4][W];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint64_t r = r0 + k * 256;
+#pragma unroll
+      for (int x = 0; x < W; x++) rec[k][x] = r < hi ? recs[r * W + x] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (r0 + k * 256 >= hi) continue;
+      const uint64_t w0 = (uint64_t)rec[k][0];
+      const uint32_t g = (uint32_t)w0 & (G - 1), cm = (uint32_t)(w0 >> 32);
+      s_used[g] = 1;
+#pragma unroll
+      for (int a = 0; a < na; a++) {
+        if (!((cm >> a) & 1u)) continue;
+        acc_apply_lds(fk.kind[a], &s_acc[a * G + g], rec[k][1 + a]);
+        s_has[a * G + g] = 1;
+      }
     }
   }
   __syncthreads();
@@ -1005,16 +1028,33 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     int64_t *sorted = (int64_t *)sorted_mem;
     if (rc == OXR_OK && e == hipSuccess) {
       ProfScope p2(s->s, "aggregate_scatter");
-      hipLaunchKernelGGL(spill_scatter_kernel, dim3((unsigned)(grid * kSpillBuckets)), dim3(256), 0, s->s, spill, cap,
-                         spill_n, offs, (uint32_t)grid, nparts, sper, slog2, words, sorted);
+      const dim3 g((unsigned)(grid * kSpillBuckets)), b(256);
+      switch (words) {
+#define OXR_SCATTER(W)                                                                                     \
+  case W:                                                                                                  \
+    hipLaunchKernelGGL(spill_scatter_kernel<W>, g, b, 0, s->s, spill, cap, spill_n, offs, (uint32_t)grid, \
+                       nparts, sper, slog2, sorted);                                                       \
+    break;
+        OXR_SCATTER(2) OXR_SCATTER(3) OXR_SCATTER(4) OXR_SCATTER(5)
+#undef OXR_SCATTER
+      }
       e = hipGetLastError();
     }
     if (rc == OXR_OK && e == hipSuccess) {
       ProfScope p3(s->s, "aggregate_fold");
       FoldKinds fk{};
       for (int a = 0; a < na; a++) fk.kind[a] = st->aggs[a].kind;
-      hipLaunchKernelGGL(spill_fold_kernel, dim3(nparts), dim3(256), fold_lds(slog2, na), s->s, sorted, offs,
-                         (uint32_t)grid, words, slog2, kr->min, st->t, fk, na);
+      const dim3 g(nparts), b(256);
+      const size_t lds = fold_lds(slog2, na);
+      switch (words) {
+#define OXR_FOLD(W)                                                                                          \
+  case W:                                                                                                    \
+    hipLaunchKernelGGL(spill_fold_kernel<W>, g, b, lds, s->s, sorted, offs, (uint32_t)grid, slog2, kr->min, \
+                       st->t, fk);                                                                           \
+    break;
+        OXR_FOLD(2) OXR_FOLD(3) OXR_FOLD(4) OXR_FOLD(5)
+#undef OXR_FOLD
+      }
       e = hipGetLastError();
     }
     if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "aggregate spill passes");

@@ -177,6 +177,10 @@ int32_t oxr_batch_ncols(const oxr_batch *b);
 int oxr_batch_column(const oxr_batch *b, int32_t i, oxr_column *out);
 const char *oxr_batch_name(const oxr_batch *b, int32_t i);
 void oxr_batch_free(oxr_batch *b);
+/* Frees n batches (NULL entries skipped): one call for the many batches a
+   multi-batch filter returns. No reference counterpart (the reference frees
+   each batch on its own). */
+void oxr_batch_free_many(oxr_batch *const *batches, uint64_t n);
 
 /* ------------------------------------------------------------ expressions */
 /*

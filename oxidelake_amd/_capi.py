@@ -125,6 +125,7 @@ SIGNATURES: dict[str, tuple] = {
     "oxr_batch_column": (C.c_int, [P, I32, C.POINTER(Column)]),
     "oxr_batch_name": (C.c_char_p, [P, I32]),
     "oxr_batch_free": (None, [P]),
+    "oxr_batch_free_many": (None, [PP, U64]),
     "oxr_supports_predicate": (C.c_int, [C.POINTER(Expr)]),
     "oxr_filter_project": (
         C.c_int,

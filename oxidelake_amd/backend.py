@@ -115,11 +115,13 @@ _WIDTH = {plan.INT32: 4, plan.INT64: 8, plan.FLOAT32: 4, plan.FLOAT64: 8}
 
 
 class DeviceBatch:
-    """`DeviceBatch` (handles.rs:130): a batch resident in HBM."""
+    """`DeviceBatch` (handles.rs:130): a batch resident in HBM. `owned`
+    False: a view whose handle another object frees (DeviceBatches)."""
 
-    def __init__(self, handle: int, keepalive=None):
+    def __init__(self, handle: int, keepalive=None, owned: bool = True):
         self.handle = handle
         self._keep = keepalive
+        self._owned = owned
 
     @property
     def num_rows(self) -> int:
@@ -138,9 +140,36 @@ class DeviceBatch:
         return c
 
     def __del__(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and getattr(self, "_owned", True):
             lib.oxr_batch_free(self.handle)
-            self.handle = None
+        self.handle = None
+
+
+class DeviceBatches(Sequence):
+    """The batches of one multi-batch call, freed together
+    (oxr_batch_free_many) when the last reference goes; indexing gives
+    views that keep the set alive."""
+
+    def __init__(self, handles, n: int):
+        self._h = handles  # ctypes array of oxr_batch*
+        self._n = n
+
+    def __len__(self) -> int:
+        return self._n
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(self._n))]
+        if i < 0:
+            i += self._n
+        if not 0 <= i < self._n:
+            raise IndexError(i)
+        return DeviceBatch(self._h[i], keepalive=self, owned=False)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            lib.oxr_batch_free_many(self._h, self._n)
+            self._h = None
 
 
 class JoinTable:
@@ -361,9 +390,14 @@ class RocmBackend:
                                      narr, C.byref(out)))
         return DeviceBatch(out.value)
 
+    def prepare_filter(self, predicate: ExprT | None, projection: Sequence,
+                       names: Optional[Sequence[str]] = None) -> "PreparedFilter":
+        """A filter + projection encoded once, to run over many batches."""
+        return PreparedFilter(predicate, projection, names)
+
     def filter_project_batches(self, stream: Stream, batch: DeviceBatch, batch_rows: int,
                                predicate: ExprT | None, projection: Sequence,
-                               names: Optional[Sequence[str]] = None) -> list[DeviceBatch]:
+                               names: Optional[Sequence[str]] = None) -> "DeviceBatches":
         """`batch` read as consecutive batches of `batch_rows` rows, each
         filtered on its own, all in one launch (oxr_filter_project_batches)."""
         exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
@@ -378,7 +412,7 @@ class RocmBackend:
         nb = C.c_uint64()
         check(lib.oxr_filter_project_batches(stream.handle, batch.handle, batch_rows, pred.ref(), parr,
                                              len(exprs), narr, outs, cap, C.byref(nb)))
-        return [DeviceBatch(outs[i]) for i in range(nb.value)]
+        return DeviceBatches(outs, nb.value)
 
     def prepare_join_build(self, stream: Stream, build: DeviceBatch, key: int) -> JoinTable:
         """backend.rs:129-142 (the table is built on the device)."""
@@ -438,6 +472,39 @@ class RocmBackend:
                                   rows.ctypes.data_as(C.POINTER(C.c_int64)),
                                   dist.ctypes.data_as(C.POINTER(C.c_float))))
         return rows, dist
+
+
+class PreparedFilter:
+    """The plan of a filter + projection encoded once (RocmBackend.
+    prepare_filter): run and run_batches skip the per-call encoding."""
+
+    def __init__(self, predicate: ExprT | None, projection: Sequence, names: Optional[Sequence[str]]):
+        self.exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
+        self.names = names
+        self._pred = Encoded(predicate)
+        self._parr, self._penc = plan.encode_many(self.exprs)
+
+    def _cnames(self, batch: DeviceBatch):
+        names = self.names
+        if names is None:
+            names = [batch.name(p.index) if isinstance(p, plan.Col) else f"p{i}" for i, p in enumerate(self.exprs)]
+        return _cstrs(names, len(self.exprs))
+
+    def run(self, stream: Stream, batch: DeviceBatch) -> DeviceBatch:
+        narr, raw = self._cnames(batch)
+        out = C.c_void_p()
+        check(lib.oxr_filter_project(stream.handle, batch.handle, self._pred.ref(), self._parr, len(self.exprs),
+                                     narr, C.byref(out)))
+        return DeviceBatch(out.value)
+
+    def run_batches(self, stream: Stream, batch: DeviceBatch, batch_rows: int) -> "DeviceBatches":
+        narr, raw = self._cnames(batch)
+        cap = max(1, -(-batch.num_rows // batch_rows))
+        outs = (C.c_void_p * cap)()
+        nb = C.c_uint64()
+        check(lib.oxr_filter_project_batches(stream.handle, batch.handle, batch_rows, self._pred.ref(), self._parr,
+                                             len(self.exprs), narr, outs, cap, C.byref(nb)))
+        return DeviceBatches(outs, nb.value)
 
 
 class VectorIndex:

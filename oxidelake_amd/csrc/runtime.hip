@@ -939,4 +939,9 @@ void oxr_batch_free(oxr_batch *b) {
   delete b;
 }
 
+void oxr_batch_free_many(oxr_batch *const *batches, uint64_t n) {
+  if (!batches) return;
+  for (uint64_t i = 0; i < n; i++) oxr_batch_free(batches[i]);
+}
+
 }  // extern "C"

@@ -319,3 +319,25 @@ def test_many_batches_in_one_launch(backend, stream, oracle, path, batch_rows):
     last = (n - 1) // batch_rows
     assert_batches_equal(got, oracle.filter_project(host.slice(last * batch_rows, batch_rows), pred, proj,
                                                     ["a", "b2", "cd"]))
+
+
+def test_prepared_filter_and_batch_sets(backend, stream, path):
+    # a plan encoded once runs like the per-call form; a multi-batch result
+    # frees its batches together, after its last view is gone
+    from oxidelake_amd import datagen
+
+    dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=5), 300_000)
+    pred, proj = datagen.filter_query(120)
+    prep = backend.prepare_filter(pred, proj, ["a", "b2", "cd"])
+    want = backend.filter_project_batches(stream, dev, 65_536, pred, proj, ["a", "b2", "cd"])
+    got = prep.run_batches(stream, dev, 65_536)
+    assert len(got) == len(want) == 5
+    for g, w in zip(got, want):
+        assert backend.download(stream, g).equals(backend.download(stream, w))
+    one = prep.run(stream, dev)
+    assert one.num_rows == sum(b.num_rows for b in got)
+    last = got[-1]
+    del got, want
+    assert last.num_rows >= 0  # the view keeps its set alive
+    assert [b.num_rows for b in prep.run_batches(stream, dev, 65_536)[1:3]] == [
+        b.num_rows for b in backend.filter_project_batches(stream, dev, 65_536, pred, proj)[1:3]]

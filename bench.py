@@ -556,9 +556,21 @@ def bench_copy(args):
         torch.neg(x, out=y)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1000 / args.steps
+    # a read-only pass for reference: the sum of 2^29 float32 (2 GiB)
+    z = torch.randn(2 * n, device="cuda:0")
+    for _ in range(args.warmup):
+        z.sum()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        z.sum()
+    torch.cuda.synchronize()
+    rms = (time.perf_counter() - t0) * 1000 / args.steps
     print(json.dumps({"workload": "copy: y = -x over 2^28 float32", "read_bytes": 4 * n, "write_bytes": 4 * n,
                       "launches": args.steps + args.warmup, "ms_per_launch": ms,
-                      "gbs": 8 * n / (ms / 1000) / 1e9}), flush=True)
+                      "gbs": 8 * n / (ms / 1000) / 1e9,
+                      "read_only": {"workload": "sum of 2^29 float32", "ms": rms,
+                                    "gbs": 8 * n / (rms / 1000) / 1e9}}), flush=True)
 
 
 def recorded_traffic(kernel: str, **workload):

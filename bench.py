@@ -88,6 +88,18 @@ def cpu_baseline(host_sample, threshold: int):
     }
 
 
+def timed_cpu(fn, budget_s: float = 20.0, reps: int = 5) -> float:
+    """Best wall time of fn() over up to `reps` runs within about `budget_s`."""
+    fn()  # warm
+    best, n, t0 = float("inf"), 0, time.perf_counter()
+    while n < reps and time.perf_counter() - t0 < budget_s:
+        s0 = time.perf_counter()
+        fn()
+        best = min(best, time.perf_counter() - s0)
+        n += 1
+    return best
+
+
 def full_size_check(host, out_host, threshold: int) -> bool:
     """Size-independent parity at full size: the selected row count and the
     wrapping Int64 sums of both integer outputs equal an independent numpy
@@ -345,6 +357,16 @@ def bench_aggregate(args):
         kms, launches = _kernel_ms(lib, "aggregate_dense")
         kernel = "dense"
     lib.oxr_profile_reset()
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's aggregation, one core, on the first 2M rows of the
+        # same counter-based table
+        from oracle import oracle as O
+
+        sample = be.download(stream, datagen.table(be, stream, datagen.groupby_table_columns(seed=42), 2_000_000))
+        sec = timed_cpu(lambda: O.aggregate(sample, spec))
+        cpu = {"value": sample.num_rows / sec, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample.num_rows:,} rows of the same seeded table, same query, best of 5"}
     bytes_per_row = 4 + 8  # k, v
     achieved = n * bytes_per_row / (kms / 1000) / 1e9
     print(json.dumps({
@@ -355,7 +377,7 @@ def bench_aggregate(args):
         "data": "synthetic (seeded SplitMix64 generator, written in HBM)",
         "config": {"workload": "SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k", "rows": n,
                    "groups": int(len(keys))},
-        "kernel_ms": kms, "kernel": kernel, "parity_full_size": bool(ok),
+        "kernel_ms": kms, "kernel": kernel, "parity_full_size": bool(ok), "cpu_baseline": cpu,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }), flush=True)
@@ -433,6 +455,25 @@ def bench_join(args):
     fold_ms, _ = _kernel_ms(lib, "aggregate_fold")
     kms += spill_ms * spills / max(1, launches)
     lib.oxr_profile_reset()
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's hash join (build over the whole dimension) and
+        # aggregation, one core, on the first 2M fact rows of the same tables
+        from oracle import oracle as O
+
+        dim_host = be.download(stream, ddim)
+        fsample = be.download(stream, datagen.table(be, stream, fact_cols, 2_000_000))
+        gspec = datagen.star_query_spec()
+        jspec = type(gspec)(4, gspec.aggregates)  # grp is column 4 of fact ++ dim
+
+        def cpu_join():
+            O.aggregate(O.hash_join(fsample, dim_host, 0, 0), jspec)
+
+        sec = timed_cpu(cpu_join, reps=3)
+        cpu = {"value": fsample.num_rows / sec, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {fsample.num_rows:,} fact rows against the whole {nd:,}-row dimension "
+                         "(build + probe + aggregate), best of 3"}
+        del dim_host, fsample
     bytes_per_row = 8 + 4 + 8  # fk, qty, price
     achieved = nf * bytes_per_row / (kms / 1000) / 1e9
     print(json.dumps({
@@ -443,7 +484,7 @@ def bench_join(args):
         "data": "synthetic (seeded SplitMix64 generator: Zipf(1.1) keys, permuted dimension, in HBM)",
         "config": {"workload": "SUM(qty), SUM(qty*price), COUNT(*) FROM fact JOIN dim ON fk = pk GROUP BY grp",
                    "fact_rows": nf, "dim_rows": nd, "groups": int(out.num_rows)},
-        "kernel_ms": kms, "spill_ms": spill_ms if spills else 0.0,
+        "cpu_baseline": cpu, "kernel_ms": kms, "spill_ms": spill_ms if spills else 0.0,
         "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "build_s": build_s, "parity_full_size": ok,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},

@@ -341,3 +341,19 @@ def test_prepared_filter_and_batch_sets(backend, stream, path):
     assert last.num_rows >= 0  # the view keeps its set alive
     assert [b.num_rows for b in prep.run_batches(stream, dev, 65_536)[1:3]] == [
         b.num_rows for b in backend.filter_project_batches(stream, dev, 65_536, pred, proj)[1:3]]
+
+
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_streaming_host_batches(backend, stream, oracle, lanes):
+    # the plain drop-in path over host batches (upload, filter, download per
+    # batch, on `lanes` streams): every batch as the oracle filters it
+    from oxidelake_amd import datagen
+    from oxidelake_amd.streaming import StreamingFilter
+
+    rng = np.random.default_rng(31)
+    batches = [datagen.random_batch(rng, n) for n in (1, 5000, 0, 70_001, 333)]
+    pred = plan.or_(plan.cmp(col(1), Cmp.Lt, Lit.int32(0)), plan.cmp(col(2), Cmp.Gt, Lit.float64(0.25)))
+    proj = [0, plan.add(col(1), Lit.int64(3)), 3]
+    got = StreamingFilter(backend, pred, proj, lanes=lanes).run(batches)
+    for b, g in zip(batches, got):
+        assert_batches_equal(g, oracle.filter_project(b, pred, proj), names=False)

@@ -357,3 +357,30 @@ def test_streaming_host_batches(backend, stream, oracle, lanes):
     got = StreamingFilter(backend, pred, proj, lanes=lanes).run(batches)
     for b, g in zip(batches, got):
         assert_batches_equal(g, oracle.filter_project(b, pred, proj), names=False)
+
+
+def test_operator_sub_batch_and_host_columns(backend, stream, oracle):
+    # only the columns the predicate and computed projections read travel;
+    # a projected string column stays on the host and is gathered by row id;
+    # a predicate over a column the device does not take falls back
+    from oxidelake_amd.streaming import FilterOperator
+
+    rng = np.random.default_rng(12)
+    base = datagen.random_batch(rng, 20_000).select(["i", "j", "f", "g"])
+    tags = pa.array([f"r{k}" for k in range(base.num_rows)])
+    small = pa.array(rng.integers(0, 5, base.num_rows).astype(np.int8))
+    batch = pa.RecordBatch.from_arrays(base.columns + [tags, small], names=base.schema.names + ["tag", "small"])
+    pred = plan.cmp(col(0), Cmp.Gt, Lit.int64(10))
+    proj = [4, plan.mul(col(1), Lit.int64(2)), 3]
+    op = FilterOperator(backend, pred, proj, names=["tag", "j2", "g"])
+    got = op.process(batch)
+    numeric = batch.select(["i", "j", "f", "g"])
+    want = oracle.filter_project(numeric, pred, [plan.mul(col(1), Lit.int64(2)), 3])
+    kept = np.nonzero(np.nan_to_num(numeric.column(0).to_numpy(zero_copy_only=False), nan=-1) > 10)[0]
+    assert got.num_rows == want.num_rows == len(kept)
+    assert got.column(0).to_pylist() == tags.take(pa.array(kept)).to_pylist()
+    assert_batches_equal(got.select(["j2", "g"]), want, names=False)
+    assert (op.device_batches, op.fallback_batches) == (1, 0)
+    # a predicate on the Int8 column: the CPU path runs the batch
+    op2 = FilterOperator(backend, plan.cmp(col(5), Cmp.Eq, Lit.int64(1)), [0], cpu_fallback=lambda b: "cpu")
+    assert op2.process(batch) == "cpu" and (op2.device_batches, op2.fallback_batches) == (0, 1)

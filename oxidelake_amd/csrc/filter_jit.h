@@ -83,6 +83,8 @@ __device__ inline void copy_vector_rows(const float *src, float *dst, int dim, b
 
 template <class Q, int S>
 __device__ __forceinline__ void filter_tile(const JitArgs &p) {
+  // The tile scan below runs on one wave over the S * 4 (step, wave) counts.
+  static_assert(S >= 1 && S * 4 <= 64, "rows per lane: the tile scan covers at most 64 counts");
   constexpr int kTile = kJitBlock * S;
   constexpr int kWords = kTile / 32 + 2;  // output validity words a tile can touch
   constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;

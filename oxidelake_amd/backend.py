@@ -459,6 +459,12 @@ class RocmBackend:
                                       output_name.encode(), C.byref(out)))
         return DeviceBatch(out.value)
 
+    def aggregate_state(self, stream: Stream, like: DeviceBatch, spec: AggregateSpec) -> "AggState":
+        """Groups resident in HBM across batches (oxr_agg_create): each
+        `update` folds one batch of `like`'s schema, `finish` returns the
+        partial laid out as `aggregate`'s."""
+        return AggState(self, stream, like, spec)
+
     def vector_topk(self, stream: Stream, batch: DeviceBatch, column: int, queries: np.ndarray,
                     metric: DistanceMetric, k: int) -> tuple[np.ndarray, np.ndarray]:
         q = np.ascontiguousarray(np.asarray(queries, dtype=np.float32))

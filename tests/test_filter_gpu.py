@@ -382,5 +382,6 @@ def test_operator_sub_batch_and_host_columns(backend, stream, oracle):
     assert_batches_equal(got.select(["j2", "g"]), want, names=False)
     assert (op.device_batches, op.fallback_batches) == (1, 0)
     # a predicate on the Int8 column: the CPU path runs the batch
-    op2 = FilterOperator(backend, plan.cmp(col(5), Cmp.Eq, Lit.int64(1)), [0], cpu_fallback=lambda b: "cpu")
-    assert op2.process(batch) == "cpu" and (op2.device_batches, op2.fallback_batches) == (0, 1)
+    marker = batch.slice(0, 0)
+    op2 = FilterOperator(backend, plan.cmp(col(5), Cmp.Eq, Lit.int64(1)), [0], cpu_fallback=lambda b: marker)
+    assert op2.process(batch) is marker and (op2.device_batches, op2.fallback_batches) == (0, 1)

@@ -451,17 +451,25 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
           for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], f[r], fb);
         }
       };
-      Frag<P> fc[RT];
+      // D fragment sets in rotation (group x in set x % D): D - 1 k groups
+      // of loads stay in flight while one is multiplied. The BF16 wide
+      // tile (64 resident queries, one block per CU, so one wave per SIMD)
+      // goes six deep: 37.9 ms against 46.5 streamed for 256 queries over
+      // 20M x 768. FP8's wide tile measured no gain past three (its
+      // fragments are twice as large and spill into the AGPRs).
+      constexpr int D = NT * RT >= 8 && U == 1 ? 6 : 3;
+      Frag<P> fr[D][RT];
       const int G = a.groups;
-      load_a(0, fa);
-      if (G > 1) load_a(1, fn);
-      for (int g = 0; g < G; g += 3) {
-        if (g + 2 < G) load_a(g + 2, fc);
-        mul(g, fa);
-        if (g + 3 < G) load_a(g + 3, fa);
-        if (g + 1 < G) mul(g + 1, fn);
-        if (g + 4 < G) load_a(g + 4, fn);
-        if (g + 2 < G) mul(g + 2, fc);
+#pragma unroll
+      for (int d = 0; d < D - 1; d++)
+        if (d < G) load_a(d, fr[d]);
+      for (int g0 = 0; g0 < G; g0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          const int g = g0 + d;
+          if (g + D - 1 < G) load_a(g + D - 1, fr[(d + D - 1) % D]);
+          if (g < G) mul(g, fr[d]);
+        }
       }
     }
     uint4 rb[PER_T];
@@ -781,10 +789,13 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   if (nqc <= 64) return {2, 2, 1, fits(64), 64};
   // large FP8 batches: two query groups per block share each row's loads,
   // the block's 128 queries resident (measured at 256 queries over 20M x
-  // 768: 23.0 ms against 30.8 streamed; BF16 at 64 queries per launch
-  // re-reads the rows too often: 49.2 against 46.5, so it streams)
+  // 768: 23.0 ms against 30.8 streamed)
   if (prec == P_FP8 && res && tune_get("vector.wide", 1) > 0 && (size_t)128 * dimp * eb <= kResidentQueryBytes)
     return {2, 4, 2, true, 128};
+  // large BF16 batches: 64 queries resident per launch, 4 row tiles per
+  // wave, a deep load rotation (37.9 ms against 46.5 streamed at 256)
+  if (prec == P_BF16 && res && tune_get("vector.wide", 1) > 0 && (size_t)64 * dimp * eb <= kResidentQueryBytes)
+    return {2, 4, 1, true, 64};
   return nqc <= 128 ? ScanCfg{4, 1, 1, false, 128} : ScanCfg{8, 1, 1, false, 256};
 }
 
@@ -806,6 +817,10 @@ template <int P>
 void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
   if (P == P_FP8 && c.qw == 2) {
     launch_scan_cfg<P, 2, 4, true, 2>(dev, units, s, a);
+    return;
+  }
+  if (P == P_BF16 && c.nt == 2 && c.rt == 4) {
+    launch_scan_cfg<P, 2, 4, true, 1>(dev, units, s, a);
     return;
   }
   switch (c.nt) {

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void aggregate_kernel(const Params p) {
           lslot = kLdsSlots + 1;
           s_keys[kLdsSlots + 1] = 0;
         } else {
-          uint32_t h = (uint32_t)fmix64((uint64_t)k) & (kLdsSlots - 1);
+          uint32_t h = lds_hash(k);
           for (int pr = 0; pr < kLdsProbe; pr++) {
             const int64_t cur = s_keys[h];
             if (cur == k) {
@@ -974,6 +974,8 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     const uint64_t tiles_per_block = (ntiles + grid - 1) / grid;
     const int64_t pct = std::min<int64_t>(400, std::max<int64_t>(1, tune_get("agg.spill_pct", 60)));
     cap = std::max<uint64_t>(64, (tiles_per_block * tile * (uint64_t)pct / 100 + kSpillBuckets - 1) / kSpillBuckets);
+    // the kernel addresses a block's buckets with 32-bit offsets
+    cap = std::min<uint64_t>(cap, (uint64_t)(UINT32_MAX / 2) / (kSpillBuckets * words));
     const uint64_t recs = grid * kSpillBuckets * cap;
     const uint64_t nh = (uint64_t)nparts * grid + 1;
     const size_t bytes = recs * words * 8 + (grid * kSpillBuckets + 2 * nh) * 4 + 1024;

@@ -24,7 +24,8 @@ namespace oxr {
 constexpr int kMaxAggs = 8;
 constexpr int kMaxSpillParts = 1024;  // partitions of the spill range
 constexpr int kSpillBuckets = 32;     // coarse partitions a block appends to
-constexpr int kLdsSlots = 1024;       // block table slots (power of two)
+constexpr int kLdsLog2 = 10;
+constexpr int kLdsSlots = 1 << kLdsLog2;  // block table slots
 constexpr int kLdsAggs = 4;           // aggregates pre-aggregated in LDS
 constexpr int kLdsProbe = 8;          // LDS probe limit before going global
 constexpr int64_t kEmpty = kI64Min;   // empty-slot marker
@@ -56,6 +57,15 @@ __host__ __device__ inline int64_t acc_init(uint8_t kind) {
 }
 
 __host__ __device__ inline uint64_t fmix64(uint64_t h) { return join_hash(h); }
+
+// Block-table slot of a key: Fibonacci hashing of the key's two halves
+// folded together (3 instructions against fmix64's ~16). The block table
+// is a cache in front of the HBM table, so a weaker hash only costs probes
+// (a key it cannot place goes to HBM), never results.
+__device__ inline uint32_t lds_hash(int64_t key) {
+  const uint32_t f = (uint32_t)key ^ (uint32_t)((uint64_t)key >> 32);
+  return (f * 0x9E3779B1u) >> (32 - kLdsLog2);
+}
 
 // The HBM group table: open addressing, one 64-bit CAS claims a key.
 struct Table {
@@ -340,6 +350,8 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
     if (tid < kSpillBuckets) s_bucket[tid] = 0;
   }
   __syncthreads();
+  int64_t *const spill_blk =
+      SP && p.spill ? p.spill + (uint64_t)blockIdx.x * kSpillBuckets * p.spill_cap * (NA + 1) : nullptr;
 
   bool err = false, ok = true;
   const uint64_t ntiles = (p.n + kTile - 1) / kTile;
@@ -424,7 +436,7 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
           lslot = kLdsSlots + 1;
           s_keys[kLdsSlots + 1] = 0;
         } else {
-          uint32_t h = (uint32_t)fmix64((uint64_t)key) & (kLdsSlots - 1);
+          uint32_t h = lds_hash(key);
           for (int pr = 0; pr < kLdsProbe; pr++) {
             const int64_t cur = s_keys[h];
             if (cur == key) {
@@ -459,7 +471,9 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
         const uint32_t part = g >> p.slog2, bucket = part / p.sper;
         const uint32_t pos = atomicAdd(&s_bucket[SP ? bucket : 0], 1u);
         if (pos < p.spill_cap) {
-          int64_t *r = p.spill + (((uint64_t)blockIdx.x * kSpillBuckets + bucket) * p.spill_cap + pos) * (NA + 1);
+          // 32-bit offset inside the block's buckets (the host keeps
+          // kSpillBuckets * spill_cap * (NA + 1) below 2^31)
+          int64_t *r = spill_blk + (bucket * (uint32_t)p.spill_cap + pos) * (uint32_t)(NA + 1);
           r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
 #pragma unroll
           for (int a = 0; a < NA; a++) r[1 + a] = c[a];

@@ -90,10 +90,20 @@ StreamCore *retain(StreamCore *c);
 void release(StreamCore *c);
 }  // namespace oxr
 
+namespace oxr {
+// A pinned host buffer of at least `bytes`, owned by the stream and reused
+// by its calls (a call on a stream runs on one thread at a time), for small
+// device-to-host read-backs: a copy into pageable memory is staged through
+// the runtime's own pinned buffer and costs more than the copy.
+int stream_pinned(oxr_stream *s, size_t bytes, void **out);
+}  // namespace oxr
+
 struct oxr_stream {
   oxr_device *dev = nullptr;
   hipStream_t s = nullptr;  // == core->s
   oxr::StreamCore *core = nullptr;
+  void *pinned = nullptr;   // small result read-backs (see stream_pinned)
+  size_t pinned_len = 0;
 };
 
 struct oxr_buffer {

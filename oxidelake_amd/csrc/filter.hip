@@ -403,11 +403,19 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   uint64_t err = 0;
   counts->assign(nbatches, 0);
   if (rc == OXR_OK) {
-    hipError_t e = hipMemcpyAsync(&err, P.result, 8, hipMemcpyDeviceToHost, s->s);
-    if (e == hipSuccess && nbatches)
-      e = hipMemcpyAsync(counts->data(), P.counts, nbatches * 8, hipMemcpyDeviceToHost, s->s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->s);
-    if (e != hipSuccess) rc = fail_hip(e, "filter_project result");
+    // one copy into pinned memory: the error word, then the counts at +64
+    void *host = nullptr;
+    rc = stream_pinned(s, 64 + count_bytes, &host);
+    if (rc == OXR_OK) {
+      hipError_t e = hipMemcpyAsync(host, P.result, 64 + nbatches * 8, hipMemcpyDeviceToHost, s->s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+      if (e != hipSuccess) {
+        rc = fail_hip(e, "filter_project result");
+      } else {
+        std::memcpy(&err, host, 8);
+        if (nbatches) std::memcpy(counts->data(), (const char *)host + 64, nbatches * 8);
+      }
+    }
   }
   stream_release(s, scratch);
   if (rc == OXR_OK && err) rc = fail(OXR_EEXECUTION, "rocm.filter", "Divide by zero error");

@@ -432,6 +432,23 @@ void oxr_buffer_free(oxr_buffer *buf) {
   delete buf;
 }
 
+extern "C++" {
+namespace oxr {
+int stream_pinned(oxr_stream *s, size_t bytes, void **out) {
+  if (s->pinned_len < bytes) {
+    if (s->pinned) (void)hipHostFree(s->pinned);
+    s->pinned = nullptr;
+    s->pinned_len = 0;
+    const size_t want = std::max<size_t>(bytes, 4096);
+    OXR_HIP(hipHostMalloc(&s->pinned, want, hipHostMallocDefault));
+    s->pinned_len = want;
+  }
+  *out = s->pinned;
+  return OXR_OK;
+}
+}  // namespace oxr
+}
+
 int oxr_stream_create(oxr_device *dev, oxr_stream **out) {
   if (!dev || !out) return fail(OXR_EPLAN, "rocm.stream", "null argument");
   OXR_HIP(hipSetDevice(dev->ordinal));
@@ -461,6 +478,7 @@ int oxr_stream_wrap(oxr_device *dev, void *hip_stream, oxr_stream **out) {
 
 void oxr_stream_destroy(oxr_stream *s) {
   if (!s) return;
+  if (s->pinned) (void)hipHostFree(s->pinned);
   // the HIP stream itself goes when the last batch/table allocated on it does
   release(s->core);
   delete s;

@@ -608,8 +608,9 @@ class GpuOperator:
                 dev = self.backend.upload(s, sub)
                 if groups.state is None:
                     groups.state = self.backend.aggregate_state(s, dev, dev_spec)
+                # no wait: the batch's memory is released in stream order,
+                # and the next upload on this stream waits for the update
                 groups.state.update(s, dev)
-                s.synchronize()
                 moved = (transfer_bytes(sub), 0)
             except Unsupported as e:
                 why = e

@@ -288,3 +288,22 @@ def test_vector_distance_appends_one_column(oracle):
     assert out.schema.names == ["k", "vec", "d"] and out.schema.field(2).type == pa.float32()
     assert_batches_equal(out, oracle.vector_distance(w, 1, [1.0, 0.0], plan.DistanceMetric.L2, "d"))
     assert op.stats.bytes_h2d == transfer_bytes(w.select(["vec"]))
+
+
+def test_no_predicate_empty_batches_and_empty_probes(oracle):
+    op = GpuOperator(FakeBackend(oracle), OracleCpu(oracle))
+    b = batch()
+    # no predicate keeps every row; the string column comes back by row id
+    out = op.filter_project(b, None, [1, 0])
+    assert out.to_pydict() == {"s": ["a", "b", "c", "d"], "k": [1, None, 3, 4]}
+    # an empty batch, through every operator
+    e = b.slice(0, 0)
+    assert op.filter_project(e, plan.cmp(col(0), Cmp.Gt, Lit.int64(1)), [1, 0]).num_rows == 0
+    build = JoinBuild(keyed(20, "r"), 0)
+    pieces = op.join_all(keyed(10, "l").slice(0, 0), build, 0, None, 7)
+    assert len(pieces) == 1 and pieces[0].num_rows == 0 and pieces[0].num_columns == 6
+    groups = op.new_groups(b, SPEC)
+    op.aggregate_into(e, SPEC, groups)
+    op.aggregate_into(b, SPEC, groups)
+    assert_batches_equal(groups.finish(), oracle.aggregate(b, SPEC), names=False)
+    assert op.stats.fallback_batches == 0

@@ -271,6 +271,14 @@ __device__ inline uint64_t score_key(float score, uint64_t row) {
   return ((uint64_t)ordered(score) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)row);
 }
 __device__ inline uint64_t key_row(uint64_t key) { return 0xFFFFFFFFu - (uint32_t)key; }
+// The float whose `ordered` is k (k = 0, below every score, comes back NaN,
+// which no compare drops on).
+__device__ inline float from_ordered(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
 
 struct ScanArgs {
   const void *x;      // [n][dimp] in the scan precision
@@ -358,10 +366,12 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   constexpr int PER_T = (BUNITS + kBlock - 1) / kBlock;
   constexpr int BROWS = (4 / QW) * RT * 32;  // rows per block
   __shared__ uint4 s_b[2][BRES ? 1 : BUNITS];
-  // the block's rows' norms, scales and validity, read once per block into
-  // LDS so the epilogue holds no row data in registers across the k loop
-  __shared__ float s_info[BROWS];
-  __shared__ float s_scale[BROWS];
+  // per row of the block, read once into LDS (so the epilogue holds no row
+  // data in registers across the k loop): score = (dot * qscale) * mul -
+  // bias, with mul = scale / |x| and bias 0 (cosine) or mul = 2 * scale and
+  // bias |x|^2 (L2); bias = +inf marks a row that cannot qualify
+  __shared__ __align__(16) float s_mul[BROWS];
+  __shared__ __align__(16) float s_bias[BROWS];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 31, h = lane >> 5;
@@ -383,18 +393,23 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   for (uint64_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
     const uint64_t blockrow = a.tile0 + blk * (uint64_t)BROWS;
     const uint64_t rowbase = blockrow + (uint64_t)rg * RT * 32;
-    if (BRES) __syncthreads();  // the previous block's epilogue has read s_info
+    if (BRES) __syncthreads();  // the previous block's epilogue has read s_mul
     for (int t = tid; t < BROWS; t += kBlock) {
       const uint64_t row = blockrow + t;
-      float info = -INFINITY, scl = 1.0f;  // -inf: the row cannot qualify
+      float mul = 0.f, bias = INFINITY;  // NULL, zero norm, past the tile
       if (row < tile_end && valid_row(a.valid, row)) {
         const float ri = a.rowinfo[row];
-        if (a.metric == OXR_COSINE) info = ri > 0.f ? ri : -INFINITY;
-        else info = ri;
-        if (P == P_FP8) scl = a.rscale[row];
+        const float scl = P == P_FP8 ? a.rscale[row] : 1.0f;
+        if (a.metric != OXR_COSINE) {
+          mul = 2.0f * scl;
+          bias = ri;
+        } else if (ri > 0.f) {
+          mul = scl * ri;
+          bias = 0.f;
+        }
       }
-      s_info[t] = info;
-      s_scale[t] = scl;
+      s_mul[t] = mul;
+      s_bias[t] = bias;
     }
     f32x16 acc[RT][NT];
 #pragma unroll
@@ -504,52 +519,64 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 
     // epilogue: scores ordered like the distance (larger is nearer):
     // L2 2 x.q - |x|^2, cosine x.q / |x|; -inf where a row cannot qualify
-    // (NULL, zero norm). Row data comes from LDS (filled before the k loop;
-    // the loop's barriers order it).
+    // (NULL, zero norm, past the tile). Row factors come from LDS (filled
+    // before the k loop; the loop's barriers order it); per query tile the
+    // query's scale, and in append mode its bound as a float: a score below
+    // it is dropped (the float compare keeps a -0.0 score at a +0.0 bound,
+    // which ranks below every pilot key and so never reaches the top C).
+    int qv[NT];
+    float qsv[NT], tauv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int q = a.qoff + (qg * NT + t) * 32 + i;
+      qv[t] = q < a.nq ? q : -1;
+      qsv[t] = (P == P_FP8 && q < a.nq) ? a.qscale[q] : 1.0f;
+      tauv[t] = (a.list && q < a.nq) ? from_ordered((uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32))
+                                     : -INFINITY;
+    }
 #pragma unroll
     for (int r = 0; r < RT; r++) {
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        const int q = a.qoff + (qg * NT + t) * 32 + i;
-        if (q >= a.nq) continue;
-        const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
-        float *out = a.scores + (uint64_t)q * a.tile_rows + (rowbase + r * 32 - a.tile0);
-        const uint32_t tq = a.list ? (uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32) : 0;
-        // elements 4j..4j+3 of the lane are rows 8j + 4h + 0..3: one 16-byte
-        // store each (tile_rows is a multiple of 128, so they are aligned)
+      for (int jg = 0; jg < 4; jg++) {
+        // elements 4jg..4jg+3 of the lane are rows 8jg + 4h + 0..3: one
+        // 16-byte load of their factors, one 16-byte store of their scores
+        // (tile_rows is a multiple of 128, so they are aligned)
+        const int m0 = 8 * jg + 4 * h;
+        const int br = rg * RT * 32 + r * 32 + m0;
+        const float4 mul4 = *(const float4 *)&s_mul[br];
+        const float4 bias4 = *(const float4 *)&s_bias[br];
+        const float mulv[4] = {mul4.x, mul4.y, mul4.z, mul4.w};
+        const float biasv[4] = {bias4.x, bias4.y, bias4.z, bias4.w};
+        const uint64_t row0 = rowbase + r * 32 + m0;
 #pragma unroll
-        for (int jg = 0; jg < 4; jg++) {
-          const int m0 = 8 * jg + 4 * h;
+        for (int t = 0; t < NT; t++) {
+          const int q = qv[t];
+          if (q < 0) continue;
           float sc[4];
 #pragma unroll
           for (int e4 = 0; e4 < 4; e4++) {
-            const int br = rg * RT * 32 + r * 32 + m0 + e4;
-            const float info = s_info[br];
-            float dot = acc[r][t][jg * 4 + e4];
-            if (P == P_FP8) dot = dot * s_scale[br] * qs;
-            float score;
-            if (info == -INFINITY) score = -INFINITY;  // the row cannot qualify
-            else if (a.metric == OXR_COSINE) score = dot * info;
-            else score = 2.0f * dot - info;
-            sc[e4] = score;
+            const float s = (acc[r][t][jg * 4 + e4] * qsv[t]) * mulv[e4] - biasv[e4];
+            sc[e4] = biasv[e4] == INFINITY ? -INFINITY : s;
           }
-          const uint64_t row0 = rowbase + r * 32 + m0;
           if (a.list) {
 #pragma unroll
             for (int e4 = 0; e4 < 4; e4++) {
-              if (row0 + e4 >= tile_end || !(sc[e4] > -INFINITY) || ordered(sc[e4]) < tq) continue;
+              if (!(sc[e4] > -INFINITY) || sc[e4] < tauv[t]) continue;
               const uint32_t pos = atomicAdd(&a.cnt[q], 1u);
               if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sc[e4], row0 + e4);
             }
-          } else if (row0 + 4 <= tile_end) {
-            *(float4 *)(out + m0) = make_float4(sc[0], sc[1], sc[2], sc[3]);
           } else {
+            float *out = a.scores + (uint64_t)q * a.tile_rows + (row0 - a.tile0);
+            if (row0 + 4 <= tile_end) {
+              *(float4 *)out = make_float4(sc[0], sc[1], sc[2], sc[3]);
+            } else {
 #pragma unroll
-            for (int e4 = 0; e4 < 4; e4++)
-              if (row0 + e4 < tile_end) out[m0 + e4] = sc[e4];
+              for (int e4 = 0; e4 < 4; e4++)
+                if (row0 + e4 < tile_end) out[e4] = sc[e4];
+            }
           }
         }
-        // one (row tile, query tile) at a time: the scheduler may not batch
+        // one row group of four at a time: the scheduler may not batch
         // every tile's LDS reads ahead (that would cost occupancy)
         __builtin_amdgcn_sched_barrier(0);
       }

@@ -392,7 +392,20 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 
   for (uint64_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
     const uint64_t blockrow = a.tile0 + blk * (uint64_t)BROWS;
+    if (blockrow >= tile_end) break;  // never taken: the host sizes nblocks to the tile
     const uint64_t rowbase = blockrow + (uint64_t)rg * RT * 32;
+    // the block's rows from one uniform base, each lane's row as a 32-bit
+    // offset computed once per block; a row past the tile reads the tile's
+    // last row: its scores are dropped (its bias is +inf), so what it
+    // multiplies does not matter
+    const uint8_t *bbase = xb + blockrow * rstride;
+    uint32_t loff[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+      const uint64_t row = rowbase + r * 32 + i;
+      const uint64_t arow = row < tile_end ? row : tile_end - 1;
+      loff[r] = (uint32_t)((arow - blockrow) * rstride) + (uint32_t)(h * 64);
+    }
     if (BRES) __syncthreads();  // the previous block's epilogue has read s_mul
     for (int t = tid; t < BROWS; t += kBlock) {
       const uint64_t row = blockrow + t;
@@ -418,18 +431,12 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
       for (int t = 0; t < NT; t++) acc[r][t] = f32x16{};
 
     auto load_a = [&](int g, Frag<P> (&fa)[RT]) {
+      const uint8_t *gb = bbase + koff<P>(g, 0);  // uniform; the lane half is in loff
 #pragma unroll
       for (int r = 0; r < RT; r++) {
-        const uint64_t row = rowbase + r * 32 + i;
-        const bool in = row < tile_end;
-        // rows past the tile read row tile0 (always present) and are zeroed
-        const uint64_t arow = in ? row : a.tile0;
-        const uint4 *src = (const uint4 *)(xb + arow * rstride + koff<P>(g, h));
+        const uint4 *src = (const uint4 *)(gb + loff[r]);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const uint4 v = src[u];
-          fa[r].v[u] = in ? v : make_uint4(0, 0, 0, 0);
-        }
+        for (int u = 0; u < U; u++) fa[r].v[u] = src[u];
       }
     };
     auto load_b = [&](int g, uint4 (&rb)[PER_T]) {

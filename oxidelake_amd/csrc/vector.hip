@@ -821,14 +821,17 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   auto fits = [&](int queries) { return res && (size_t)queries * dimp * eb <= (size_t)64 * 1024; };
   if (nqc <= 32) return {1, 4, 1, fits(32), 32};
   if (nqc <= 64) return {2, 2, 1, fits(64), 64};
-  // large FP8 batches: two query groups per block share each row's loads,
-  // the block's 128 queries resident (measured at 256 queries over 20M x
-  // 768: 23.0 ms against 30.8 streamed)
-  if (prec == P_FP8 && res && tune_get("vector.wide", 1) > 0 && (size_t)128 * dimp * eb <= kResidentQueryBytes)
-    return {2, 4, 2, true, 128};
+  // large FP8 batches: the block's 128 queries resident (96 KB), each wave
+  // 64 rows x 128 queries, so every row is loaded by one wave (13.6 ms at
+  // 256 queries over 20M x 768). vector.wide = 2: two query groups of 64
+  // per block sharing each row's loads through L1 (16.2 ms: the duplicate
+  // loads fill the L1's 64 bytes a clock).
+  const int64_t wide = tune_get("vector.wide", 1);
+  if (prec == P_FP8 && res && wide > 0 && (size_t)128 * dimp * eb <= kResidentQueryBytes)
+    return wide == 2 ? ScanCfg{2, 4, 2, true, 128} : ScanCfg{4, 2, 1, true, 128};
   // large BF16 batches: 64 queries resident per launch, 4 row tiles per
   // wave, a deep load rotation (37.9 ms against 46.5 streamed at 256)
-  if (prec == P_BF16 && res && tune_get("vector.wide", 1) > 0 && (size_t)64 * dimp * eb <= kResidentQueryBytes)
+  if (prec == P_BF16 && res && wide > 0 && (size_t)64 * dimp * eb <= kResidentQueryBytes)
     return {2, 4, 1, true, 64};
   return nqc <= 128 ? ScanCfg{4, 1, 1, false, 128} : ScanCfg{8, 1, 1, false, 256};
 }
@@ -851,6 +854,10 @@ template <int P>
 void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
   if (P == P_FP8 && c.qw == 2) {
     launch_scan_cfg<P, 2, 4, true, 2>(dev, units, s, a);
+    return;
+  }
+  if (P == P_FP8 && c.nt == 4 && c.rt == 2) {
+    launch_scan_cfg<P, 4, 2, true, 1>(dev, units, s, a);
     return;
   }
   if (P == P_BF16 && c.nt == 2 && c.rt == 4) {

@@ -256,10 +256,10 @@ def test_pilot_bound_matches_plain_scan(backend, stream, precision, overflow):
 @pytest.mark.parametrize("precision", [VEC_FP8, VEC_BF16])
 def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision):
     # large batches with the block's queries in LDS, several launches per
-    # chunk (FP8: two query groups of 64 per block; BF16: 64 queries, four
-    # row tiles per wave, a six-deep load rotation); the same keys as the
-    # streamed kernel, so the same results bit for bit, and recall against
-    # the FP64 oracle
+    # chunk (FP8: 128 queries, 2 row tiles x 4 query tiles per wave; BF16:
+    # 64 queries, 4 x 2 per wave, a six-deep load rotation); the same keys
+    # as the streamed kernel, so the same results bit for bit, and recall
+    # against the FP64 oracle
     from oxidelake_amd import autotune
 
     n, dim, nq, k = 40_000, 768, 200, 10

@@ -121,6 +121,57 @@ __device__ inline void row_partials(const float *x, const float *q, int dim, int
   *p1 = s1;
 }
 
+// row_partials for R rows at once: the same per-row arithmetic in the same
+// order (so the same bits), with the R rows' loads in flight together.
+template <int R>
+__device__ inline void row_partials_multi(const float *x, const uint64_t (&rows)[R], const float *q, int dim,
+                                          int metric, int lane, float (&p0)[R], float (&p1)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; r++) p0[r] = p1[r] = 0.f;
+  if ((dim & 3) == 0) {
+    const f32x4 *q4 = (const f32x4 *)q;
+    for (int c = lane; c < dim / 4; c += 64) {
+      const f32x4 b = q4[c];
+      f32x4 av[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) av[r] = ((const f32x4 *)(x + rows[r] * (uint64_t)dim))[c];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (metric == OXR_COSINE) {
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            p0[r] = __fmaf_rn(av[r][e], b[e], p0[r]);
+            p1[r] = __fmaf_rn(av[r][e], av[r][e], p1[r]);
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const float d = av[r][e] - b[e];
+            p0[r] = __fmaf_rn(d, d, p0[r]);
+          }
+        }
+      }
+    }
+  } else {
+    for (int c = lane; c < dim; c += 64) {
+      const float b = q[c];
+      float av[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) av[r] = x[rows[r] * (uint64_t)dim + c];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (metric == OXR_COSINE) {
+          p0[r] = __fmaf_rn(av[r], b, p0[r]);
+          p1[r] = __fmaf_rn(av[r], av[r], p1[r]);
+        } else {
+          const float d = av[r] - b;
+          p0[r] = __fmaf_rn(d, d, p0[r]);
+        }
+      }
+    }
+  }
+}
+
 // L2 = sqrt(sum (a-b)^2); cosine = 1 - dot/(|a||b|), NaN when a norm is 0
 // (crates/oxidelake-device/src/cpu/kernels.rs:908-927).
 __device__ inline float finish_distance(int metric, float s0, float s1, float qnorm2) {
@@ -752,16 +803,27 @@ __global__ __launch_bounds__(kBlock) void rerank_kernel(const RerankArgs a) {
   float qn = 0.f;
   for (int c = lane; c < a.dim; c += 64) qn = __fmaf_rn(s_q[c], s_q[c], qn);
   const float qnorm2 = wave_sum(qn);
-  for (int j = wave; j < a.C; j += kBlock / 64) {
-    const uint64_t key = a.best[(uint64_t)q * a.C + j];
-    if (key == 0) continue;
-    const uint64_t row = key_row(key);
-    float s0, s1;
-    row_partials(a.x + row * (uint64_t)a.dim, s_q, a.dim, a.metric, lane, &s0, &s1);
-    s0 = wave_sum(s0);
-    s1 = wave_sum(s1);
-    const float d = finish_distance(a.metric, s0, s1, qnorm2);
-    if (lane == 0 && !__builtin_isnan(d)) s_key[j] = ((uint64_t)ordered(d) << 32) | row;
+  // each wave takes four candidates at a time, their row loads in flight
+  // together (one candidate's row at a time left the kernel waiting on
+  // memory); an empty slot (key 0) reads row 0 and is not kept
+  constexpr int R = 4;
+  for (int j0 = wave * R; j0 < a.C; j0 += (kBlock / 64) * R) {
+    uint64_t rows[R];
+    bool use[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int j = j0 + r;
+      const uint64_t key = j < a.C ? a.best[(uint64_t)q * a.C + j] : 0;
+      use[r] = key != 0;
+      rows[r] = use[r] ? key_row(key) : 0;
+    }
+    float s0[R], s1[R];
+    row_partials_multi<R>(a.x, rows, s_q, a.dim, a.metric, lane, s0, s1);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const float d = finish_distance(a.metric, wave_sum(s0[r]), wave_sum(s1[r]), qnorm2);
+      if (use[r] && lane == 0 && !__builtin_isnan(d)) s_key[j0 + r] = ((uint64_t)ordered(d) << 32) | rows[r];
+    }
   }
   __syncthreads();
   // ascending bitonic sort of Cpad keys (a power of two)

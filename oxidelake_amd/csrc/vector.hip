@@ -880,8 +880,14 @@ constexpr size_t kResidentQueryBytes = 100 * 1024;
 ScanCfg pick_scan(int prec, int nqc, int dimp) {
   const size_t eb = prec == P_FP8 ? 1 : prec == P_BF16 ? 2 : 4;
   const bool res = tune_get("vector.resident_queries", 1) > 0;
-  auto fits = [&](int queries) { return res && (size_t)queries * dimp * eb <= (size_t)64 * 1024; };
-  if (nqc <= 32) return {1, 4, 1, fits(32), 32};
+  // a query tile of up to kResidentQueryBytes stays in LDS (FP32, 32
+  // queries over 768 dims: 96 KB, one block per CU: 14.4 ms a scan against
+  // 18.5 streamed)
+  auto fits = [&](int queries) { return res && (size_t)queries * dimp * eb <= kResidentQueryBytes; };
+  // small batches: one query tile; FP8 with two row tiles per wave, which
+  // leaves registers for two waves per SIMD (one query: 3.15 ms against
+  // 3.64 with four)
+  if (nqc <= 32) return {1, prec == P_FP8 ? 2 : 4, 1, fits(32), 32};
   if (nqc <= 64) return {2, 2, 1, fits(64), 64};
   // large FP8 batches: the block's 128 queries resident (96 KB), each wave
   // 64 rows x 128 queries, so every row is loaded by one wave (13.6 ms at
@@ -920,6 +926,10 @@ void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_
   }
   if (P == P_FP8 && c.nt == 4 && c.rt == 2) {
     launch_scan_cfg<P, 4, 2, true, 1>(dev, units, s, a);
+    return;
+  }
+  if (P == P_FP8 && c.nt == 1 && c.rt == 2 && c.bres) {
+    launch_scan_cfg<P, 1, 2, true, 1>(dev, units, s, a);
     return;
   }
   if (P == P_BF16 && c.nt == 2 && c.rt == 4) {

@@ -884,10 +884,10 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   // queries over 768 dims: 96 KB, one block per CU: 14.4 ms a scan against
   // 18.5 streamed)
   auto fits = [&](int queries) { return res && (size_t)queries * dimp * eb <= kResidentQueryBytes; };
-  // small batches: one query tile; FP8 with two row tiles per wave, which
-  // leaves registers for two waves per SIMD (one query: 3.15 ms against
-  // 3.64 with four)
-  if (nqc <= 32) return {1, prec == P_FP8 ? 2 : 4, 1, fits(32), 32};
+  // small batches: one query tile; BF16 and FP8 with two row tiles per
+  // wave, which leaves registers for more waves per SIMD (one query, FP8:
+  // 3.15 ms against 3.64 with four; BF16: 7.28 against 7.84)
+  if (nqc <= 32) return {1, prec == P_FP32 ? 4 : 2, 1, fits(32), 32};
   if (nqc <= 64) return {2, 2, 1, fits(64), 64};
   // large FP8 batches: the block's 128 queries resident (96 KB), each wave
   // 64 rows x 128 queries, so every row is loaded by one wave (13.6 ms at
@@ -928,7 +928,7 @@ void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_
     launch_scan_cfg<P, 4, 2, true, 1>(dev, units, s, a);
     return;
   }
-  if (P == P_FP8 && c.nt == 1 && c.rt == 2 && c.bres) {
+  if (P != P_FP32 && c.nt == 1 && c.rt == 2 && c.bres) {
     launch_scan_cfg<P, 1, 2, true, 1>(dev, units, s, a);
     return;
   }

@@ -883,12 +883,13 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   // a query tile of up to kResidentQueryBytes stays in LDS (FP32, 32
   // queries over 768 dims: 96 KB, one block per CU: 14.4 ms a scan against
   // 18.5 streamed)
-  auto fits = [&](int queries) { return res && (size_t)queries * dimp * eb <= kResidentQueryBytes; };
+  auto fits = [&](int queries, size_t lim) { return res && (size_t)queries * dimp * eb <= lim; };
   // small batches: one query tile; BF16 and FP8 with two row tiles per
   // wave, which leaves registers for more waves per SIMD (one query, FP8:
   // 3.15 ms against 3.64 with four; BF16: 7.28 against 7.84)
-  if (nqc <= 32) return {1, prec == P_FP32 ? 4 : 2, 1, fits(32), 32};
-  if (nqc <= 64) return {2, 2, 1, fits(64), 64};
+  if (nqc <= 32) return {1, prec == P_FP32 ? 4 : 2, 1, fits(32, kResidentQueryBytes), 32};
+  // (a 64-query tile resident at 96 KB measured slower than streamed)
+  if (nqc <= 64) return {2, 2, 1, fits(64, 64 * 1024), 64};
   // large FP8 batches: the block's 128 queries resident (96 KB), each wave
   // 64 rows x 128 queries, so every row is loaded by one wave (13.6 ms at
   // 256 queries over 20M x 768). vector.wide = 2: two query groups of 64
@@ -897,11 +898,16 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   const int64_t wide = tune_get("vector.wide", 1);
   if (prec == P_FP8 && res && wide > 0 && (size_t)128 * dimp * eb <= kResidentQueryBytes)
     return wide == 2 ? ScanCfg{2, 4, 2, true, 128} : ScanCfg{4, 2, 1, true, 128};
-  // large BF16 batches: 64 queries resident per launch, 4 row tiles per
-  // wave, a deep load rotation (37.9 ms against 46.5 streamed at 256)
-  if (prec == P_BF16 && res && wide > 0 && (size_t)64 * dimp * eb <= kResidentQueryBytes)
+  // other large batches: the streamed 4-tile kernel in launches of 128
+  // queries (256 queries over 20M x 768: BF16 22.3 ms, FP32 76 ms). The
+  // 8-tile kernel in one launch of 256 spills registers (BF16 40.5, FP32
+  // 116 ms; vector.nt8 = 1); BF16 with 64 queries resident per launch, 4
+  // row tiles per wave and a six-deep load rotation reads the rows twice
+  // as often (33.2 ms; vector.wide = 3).
+  if (prec == P_BF16 && res && wide == 3 && (size_t)64 * dimp * eb <= kResidentQueryBytes)
     return {2, 4, 1, true, 64};
-  return nqc <= 128 ? ScanCfg{4, 1, 1, false, 128} : ScanCfg{8, 1, 1, false, 256};
+  if (nqc > 128 && tune_get("vector.nt8", 0) > 0) return {8, 1, 1, false, 256};
+  return {4, 1, 1, false, 128};
 }
 
 template <int P, int NT, int RT, bool BRES, int QW>

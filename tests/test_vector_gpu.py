@@ -253,13 +253,15 @@ def test_pilot_bound_matches_plain_scan(backend, stream, precision, overflow):
         assert scans == chunks * (3 if overflow else 2), scans
 
 
-@pytest.mark.parametrize("precision", [VEC_FP8, VEC_BF16])
-def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision):
-    # large batches with the block's queries in LDS, several launches per
-    # chunk (FP8: 128 queries, 2 row tiles x 4 query tiles per wave; BF16:
-    # 64 queries, 4 x 2 per wave, a six-deep load rotation); the same keys
-    # as the streamed kernel, so the same results bit for bit, and recall
-    # against the FP64 oracle
+@pytest.mark.parametrize("precision,knob,value", [(VEC_FP8, "vector.wide", 1), (VEC_BF16, "vector.wide", 3),
+                                                  (VEC_FP32, "vector.nt8", 1), (VEC_BF16, "vector.nt8", 1)])
+def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision, knob, value):
+    # other tile shapes for large batches against the streamed 4-tile kernel
+    # in launches of 128 (vector.wide = 0): FP8 with 128 queries resident,
+    # 2 row tiles x 4 query tiles per wave; BF16 with 64 resident, 4 x 2 per
+    # wave, a six-deep load rotation; the 8-tile streamed kernel in one
+    # launch of 256. The same keys, so the same results bit for bit, and
+    # recall against the FP64 oracle
     from oxidelake_amd import autotune
 
     n, dim, nq, k = 40_000, 768, 200, 10
@@ -268,15 +270,18 @@ def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision):
     qs = vecs[rng.integers(0, n, nq)] + rng.normal(scale=0.3, size=(nq, dim)).astype(np.float32)
     qs = (qs / np.linalg.norm(qs, axis=1, keepdims=True)).astype(np.float32)
     ix = VectorIndex(stream, dev, 0, M.Cosine, precision)
-    cand = 100 if precision == VEC_BF16 else 1000
+    cand = {VEC_BF16: 100, VEC_FP8: 1000, VEC_FP32: k}[precision]
     autotune.set_knob("vector.pilot", 0)
     try:
+        autotune.set_knob(knob, value)
         wide = ix.search(stream, qs, k, candidates=cand)
+        autotune.set_knob(knob, -1)
         autotune.set_knob("vector.wide", 0)
         streamed = ix.search(stream, qs, k, candidates=cand)
     finally:
         autotune.set_knob("vector.pilot", -1)
         autotune.set_knob("vector.wide", -1)
+        autotune.set_knob(knob, -1)
     assert np.array_equal(wide[0], streamed[0]) and np.array_equal(wide[1], streamed[1])
     hit = 0
     for i in range(0, nq, 10):

@@ -589,9 +589,14 @@ int key_range(oxr_stream *s, const DevCol &c, uint64_t n, KeyRange *out) {
       else hipLaunchKernelGGL((key_range_kernel<int64_t, false>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
     }
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(host, mm, 16, hipMemcpyDeviceToHost, s->s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->s);
-    if (e != hipSuccess) rc = fail_hip(e, "key range");
+    int64_t *pin = nullptr;
+    if (e == hipSuccess) rc = stream_pinned(s, 16, (void **)&pin);
+    if (rc == OXR_OK) {
+      if (e == hipSuccess) e = hipMemcpyAsync(pin, mm, 16, hipMemcpyDeviceToHost, s->s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+      if (e != hipSuccess) rc = fail_hip(e, "key range");
+      else std::memcpy(host, pin, 16);
+    }
   }
   stream_release(s, m);
   if (rc != OXR_OK) return rc;
@@ -1149,12 +1154,16 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
     if (e != hipSuccess) rc = fail_hip(e, "aggregate_kernel launch");
   }
   uint64_t host[4] = {0, 0, 0, 0};  // error, distinct, overflow
+  uint64_t *pin = nullptr;
+  if (rc == OXR_OK) rc = stream_pinned(s, 32, (void **)&pin);
   if (rc == OXR_OK) {
-    hipError_t e = hipMemcpyAsync(&host[0], err, 4, hipMemcpyDeviceToHost, s->s);
-    if (e == hipSuccess) e = hipMemcpyAsync(&host[1], st->t.distinct, 8, hipMemcpyDeviceToHost, s->s);
-    if (e == hipSuccess) e = hipMemcpyAsync(&host[2], st->t.overflow, 4, hipMemcpyDeviceToHost, s->s);
+    pin[0] = pin[1] = pin[2] = 0;
+    hipError_t e = hipMemcpyAsync(&pin[0], err, 4, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&pin[1], st->t.distinct, 8, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&pin[2], st->t.overflow, 4, hipMemcpyDeviceToHost, s->s);
     if (e == hipSuccess) e = hipStreamSynchronize(s->s);
     if (e != hipSuccess) rc = fail_hip(e, "aggregate result");
+    else std::memcpy(host, pin, 24);
   }
   stream_release(s, err);
   if (rc != OXR_OK) return rc;

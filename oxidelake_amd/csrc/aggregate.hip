@@ -1197,8 +1197,10 @@ int run_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {
     if (e != hipSuccess) rc = fail_hip(e, "compact_kernel launch");
   }
   uint32_t seen[2] = {0, 0};
+  uint32_t *pin_seen = nullptr;  // read after the wait below
+  if (rc == OXR_OK) rc = stream_pinned(s, 8, (void **)&pin_seen);
   if (rc == OXR_OK) {
-    hipError_t e = hipMemcpyAsync(seen, t.seen, 8, hipMemcpyDeviceToHost, s->s);
+    hipError_t e = hipMemcpyAsync(pin_seen, t.seen, 8, hipMemcpyDeviceToHost, s->s);
     if (e != hipSuccess) rc = fail_hip(e, "aggregate seen flags");
   }
   // sort ordinary keys ascending (stable radix sort of signed keys)
@@ -1223,6 +1225,7 @@ int run_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {
   if (rc == OXR_OK) {
     hipError_t e = hipStreamSynchronize(s->s);
     if (e != hipSuccess) rc = fail_hip(e, "aggregate finish");
+    else std::memcpy(seen, pin_seen, 8);
   }
   const bool global_only = st->key_col == -1;
   const int has_null = seen[0] ? 1 : 0;

@@ -1149,11 +1149,12 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
         launch_scan(ix->precision, cfg, nqc, ix->dev, a.nblocks, st, a);
         OXR_HIP(hipGetLastError());
       }
-      std::vector<uint32_t> hcnt(nqc);
-      OXR_HIP(hipMemcpyAsync(hcnt.data(), cnt, (size_t)nqc * 4, hipMemcpyDeviceToHost, st));
+      uint32_t *hcnt = nullptr;  // the list lengths, read back through pinned memory
+      OXR_TRY(stream_pinned(s, (size_t)nqc * 4, (void **)&hcnt));
+      OXR_HIP(hipMemcpyAsync(hcnt, cnt, (size_t)nqc * 4, hipMemcpyDeviceToHost, st));
       OXR_HIP(hipStreamSynchronize(st));
       bool overflow = false;
-      for (uint32_t c : hcnt) overflow |= c > cap;
+      for (int c = 0; c < nqc; c++) overflow |= hcnt[c] > cap;
       if (!overflow) {
         ProfScope prof(st, "vector_merge");
         hipLaunchKernelGGL(merge_lists_kernel, dim3(nqc), dim3(kBlock), 0, st, pbest, list, cnt, cap, C, best);

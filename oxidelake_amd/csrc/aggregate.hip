@@ -1028,8 +1028,11 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     if (rc == OXR_OK && e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, hist, offs, (int)nh, s->s);
     // the partitioned records, sized by the spill's total
     uint32_t total = 0;
-    if (rc == OXR_OK && e == hipSuccess) e = hipMemcpyAsync(&total, offs + nh - 1, 4, hipMemcpyDeviceToHost, s->s);
+    uint32_t *pin_total = nullptr;
+    if (rc == OXR_OK && e == hipSuccess) rc = stream_pinned(s, 4, (void **)&pin_total);
+    if (rc == OXR_OK && e == hipSuccess) e = hipMemcpyAsync(pin_total, offs + nh - 1, 4, hipMemcpyDeviceToHost, s->s);
     if (rc == OXR_OK && e == hipSuccess) e = hipStreamSynchronize(s->s);
+    if (rc == OXR_OK && e == hipSuccess) total = *pin_total;
     void *sorted_mem = nullptr;
     if (rc == OXR_OK && e == hipSuccess) rc = stream_alloc(s, (size_t)std::max<uint32_t>(total, 1) * words * 8, &sorted_mem);
     int64_t *sorted = (int64_t *)sorted_mem;

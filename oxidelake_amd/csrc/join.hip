@@ -427,9 +427,12 @@ int oxr_join_probe(oxr_stream *s, const oxr_join_table *table, const oxr_batch *
       e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, m->offsets, (int)(n + 1), s->s);
   }
   uint64_t tot = 0;
+  uint64_t *pin = nullptr;  // the match total, read back through pinned memory
+  if (rc == OXR_OK && e == hipSuccess) rc = stream_pinned(s, 8, (void **)&pin);
   if (rc == OXR_OK && e == hipSuccess)
-    e = hipMemcpyAsync(&tot, m->offsets + n, 8, hipMemcpyDeviceToHost, s->s);
+    e = hipMemcpyAsync(pin, m->offsets + n, 8, hipMemcpyDeviceToHost, s->s);
   if (rc == OXR_OK && e == hipSuccess) e = hipStreamSynchronize(s->s);
+  if (rc == OXR_OK && e == hipSuccess) tot = *pin;
   stream_release(s, tmp);
   if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "join probe");
   if (rc != OXR_OK) {

@@ -1118,9 +1118,13 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
     // Pilot (knob vector.pilot): the best C of the first rows bound each
     // query's final C-th best from below, so the other rows need no score
     // array: the scan appends only the keys at or above the bound. A list
-    // that overflows sends the batch down the plain path.
+    // that overflows sends the batch down the plain path. About 2048 C
+    // rows, at least 1M: a looser bound floods the scan with appends (at
+    // C = 1000 over 20M rows, 256 queries: 256K-row pilot 44 ms of scan,
+    // 1M 13.7, 2M 12.4), a larger pilot costs more selection (4M: 5.5 ms
+    // of select against 2.8).
     const int64_t pilot_knob = tune_get("vector.pilot_rows", -1);
-    const uint64_t pilot_want = pilot_knob > 0 ? (uint64_t)pilot_knob : std::max<uint64_t>(1ull << 20, (uint64_t)C * 1024);
+    const uint64_t pilot_want = pilot_knob > 0 ? (uint64_t)pilot_knob : std::max<uint64_t>(1ull << 20, (uint64_t)C * 2048);
     const uint64_t pilot = std::min<uint64_t>(ix->n, std::max<uint64_t>(rpb, pilot_want / rpb * rpb));
     bool done = false;
     if (tune_get("vector.pilot", 1) > 0 && ix->n >= 4 * pilot) {

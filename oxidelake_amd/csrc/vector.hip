@@ -884,10 +884,11 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   // queries over 768 dims: 96 KB, one block per CU: 14.4 ms a scan against
   // 18.5 streamed)
   auto fits = [&](int queries, size_t lim) { return res && (size_t)queries * dimp * eb <= lim; };
-  // small batches: one query tile; BF16 and FP8 with two row tiles per
-  // wave, which leaves registers for more waves per SIMD (one query, FP8:
-  // 3.15 ms against 3.64 with four; BF16: 7.28 against 7.84)
-  if (nqc <= 32) return {1, prec == P_FP32 ? 4 : 2, 1, fits(32, kResidentQueryBytes), 32};
+  // small batches: one query tile; fewer row tiles per wave leave
+  // registers for more waves per SIMD (one query: FP8 3.15 ms with two
+  // against 3.64 with four and 3.23 with one; BF16 6.89 with one against
+  // 7.28 with two and 7.84 with four)
+  if (nqc <= 32) return {1, prec == P_FP32 ? 4 : prec == P_BF16 ? 1 : 2, 1, fits(32, kResidentQueryBytes), 32};
   // (a 64-query tile resident at 96 KB measured slower than streamed)
   if (nqc <= 64) return {2, 2, 1, fits(64, 64 * 1024), 64};
   // large FP8 batches: the block's 128 queries resident (96 KB), each wave
@@ -936,6 +937,10 @@ void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_
   }
   if (P != P_FP32 && c.nt == 1 && c.rt == 2 && c.bres) {
     launch_scan_cfg<P, 1, 2, true, 1>(dev, units, s, a);
+    return;
+  }
+  if (P != P_FP32 && c.nt == 1 && c.rt == 1 && c.bres) {
+    launch_scan_cfg<P, 1, 1, true, 1>(dev, units, s, a);
     return;
   }
   if (P == P_BF16 && c.nt == 2 && c.rt == 4) {

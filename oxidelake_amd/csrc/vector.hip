@@ -413,7 +413,11 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   constexpr int LB = Prec<P>::LB, KG = Prec<P>::KG, EB = Prec<P>::EB;
   constexpr int U = LB / 16;                 // 16-byte units per lane per group
   constexpr int QROW = 2 * U;                // units per query per group
-  constexpr int BUNITS = QW * NT * 32 * QROW;  // units of one group's query slice
+  constexpr int QTOT = QW * NT * 32;         // queries of the block's tile
+  constexpr int BUNITS = QTOT * QROW;         // units of one group's query slice
+  // a group's slice sits in LDS part-major ([part][query]), so the lanes
+  // of a fragment read (one query each) read consecutive 16-byte units:
+  // query-major, they were a query row apart and shared banks
   constexpr int PER_T = (BUNITS + kBlock - 1) / kBlock;
   constexpr int BROWS = (4 / QW) * RT * 32;  // rows per block
   __shared__ uint4 s_b[2][BRES ? 1 : BUNITS];
@@ -437,7 +441,8 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
     for (int unit = tid; unit < total; unit += kBlock) {
       const int g = unit / BUNITS, rem = unit % BUNITS;
       const int qrow = rem / QROW, part = rem % QROW;
-      s_bres[unit] = *(const uint4 *)(qb + (uint64_t)(a.qoff + qrow) * rstride + koff<P>(g, part / U) + (part % U) * 16);
+      s_bres[g * BUNITS + part * QTOT + qrow] =
+          *(const uint4 *)(qb + (uint64_t)(a.qoff + qrow) * rstride + koff<P>(g, part / U) + (part % U) * 16);
     }
   }
 
@@ -504,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 #pragma unroll
       for (int p = 0; p < PER_T; p++) {
         const int unit = p * kBlock + tid;
-        if (unit < BUNITS) s_b[buf][unit] = rb[p];
+        if (unit < BUNITS) s_b[buf][(unit % QROW) * QTOT + unit / QROW] = rb[p];
       }
     };
 
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
         for (int t = 0; t < NT; t++) {
           Frag<P> fb;
 #pragma unroll
-          for (int u = 0; u < U; u++) fb.v[u] = bg[((qg * NT + t) * 32 + i) * QROW + h * U + u];
+          for (int u = 0; u < U; u++) fb.v[u] = bg[(h * U + u) * QTOT + (qg * NT + t) * 32 + i];
 #pragma unroll
           for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], f[r], fb);
         }
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
       for (int t = 0; t < NT; t++) {
         Frag<P> fb;
 #pragma unroll
-        for (int u = 0; u < U; u++) fb.v[u] = s_b[buf][((qg * NT + t) * 32 + i) * QROW + h * U + u];
+        for (int u = 0; u < U; u++) fb.v[u] = s_b[buf][(h * U + u) * QTOT + (qg * NT + t) * 32 + i];
 #pragma unroll
         for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], fa[r], fb);
       }

@@ -686,10 +686,16 @@ __device__ void compact(uint64_t *buf, int *count, uint64_t *tau, int C) {
   __syncthreads();
 }
 
-// Streams `len` keys from key_at(i) through the buffer; leaves the best C
-// (sorted, descending) in buf[0..C) with zeros after them.
-template <class F>
-__device__ void top_c(F key_at, uint64_t len, int C, uint64_t *buf, int *count, uint64_t *tau) {
+// Streams `len` keys through the buffer; leaves the best C (sorted,
+// descending) in buf[0..C) with zeros after them. Raw values are fetched a
+// chunk ahead: the next chunk's loads are issued before this chunk's keys
+// are inserted, so a round does not wait for memory from a standing start
+// (selection at 256 queries: FP8 2.76 -> 2.22 ms). `fetch(i)` reads value
+// i, `conv(v, i)` makes its key (0 = no key).
+template <class R, class Fetch, class Conv>
+__device__ void top_c_ahead(Fetch fetch, Conv conv, uint64_t len, int C, uint64_t *buf, int *count,
+                            uint64_t *tau) {
+  constexpr int KPT = kChunk / kBlock;
   if (threadIdx.x == 0) {
     *count = 0;
     *tau = 0;
@@ -697,17 +703,27 @@ __device__ void top_c(F key_at, uint64_t len, int C, uint64_t *buf, int *count, 
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1;
-  for (uint64_t base = 0; base < len; base += kChunk) {
-    const uint64_t th = *tau;
-    uint64_t keys[kChunk / kBlock];
+  R raw[KPT];
+  auto load = [&](uint64_t base) {
 #pragma unroll
-    for (int j = 0; j < kChunk / kBlock; j++) {
+    for (int j = 0; j < KPT; j++) {
       const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
-      keys[j] = idx < len ? key_at(idx) : 0;
+      if (idx < len) raw[j] = fetch(idx);
     }
+  };
+  if (len) load(0);
+  for (uint64_t base = 0; base < len; base += kChunk) {
+    uint64_t keys[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
+      keys[j] = idx < len ? conv(raw[j], idx) : 0;
+    }
+    if (base + kChunk < len) load(base + kChunk);
+    const uint64_t th = *tau;
     // wave-aggregated insertion: one LDS atomic per wave and step
 #pragma unroll
-    for (int j = 0; j < kChunk / kBlock; j++) {
+    for (int j = 0; j < KPT; j++) {
       const bool pass = keys[j] > th;
       const uint64_t m = __ballot(pass);
       if (m == 0) continue;
@@ -744,12 +760,9 @@ __global__ __launch_bounds__(kBlock) void select_tile_kernel(const SelectArgs a)
   const uint64_t len = min(a.seg_rows, a.tile_len - lo);
   const float *sp = a.scores + (uint64_t)q * a.tile_rows + lo;
   const uint64_t row0 = a.tile0 + lo;
-  top_c(
-      [&](uint64_t idx) -> uint64_t {
-        const float s = sp[idx];
-        return s > -INFINITY ? score_key(s, row0 + idx) : 0;
-      },
-      len, a.C, s_buf, &s_count, &s_tau);
+  top_c_ahead<float>([&](uint64_t idx) -> float { return sp[idx]; },
+                     [&](float s, uint64_t idx) -> uint64_t { return s > -INFINITY ? score_key(s, row0 + idx) : 0; },
+                     len, a.C, s_buf, &s_count, &s_tau);
   uint64_t *out = a.cand + ((uint64_t)q * a.segs_total + a.seg_first + blockIdx.x) * a.C;
   for (int j = threadIdx.x; j < a.C; j += kBlock) out[j] = s_buf[j];
 }
@@ -762,8 +775,9 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(uint64_t *cand, int segs_
   __shared__ uint64_t s_tau;
   const int q = blockIdx.x;
   const uint64_t *in = cand + (uint64_t)q * segs_total * C;
-  top_c([&](uint64_t idx) -> uint64_t { return in[idx]; }, (uint64_t)segs_total * C, C, s_buf,
-        &s_count, &s_tau);
+  top_c_ahead<uint64_t>([&](uint64_t idx) -> uint64_t { return in[idx]; },
+                        [](uint64_t v, uint64_t) -> uint64_t { return v; }, (uint64_t)segs_total * C, C, s_buf,
+                        &s_count, &s_tau);
   for (int j = threadIdx.x; j < C; j += kBlock) best[(uint64_t)q * C + j] = s_buf[j];
 }
 
@@ -779,8 +793,8 @@ __global__ __launch_bounds__(kBlock) void merge_lists_kernel(const uint64_t *pil
   const uint64_t *pb = pilot_best + (uint64_t)q * C;
   const uint64_t *ls = list + (uint64_t)q * cap;
   const uint64_t len = (uint64_t)C + min((uint64_t)cnt[q], cap);
-  top_c([&](uint64_t idx) -> uint64_t { return idx < (uint64_t)C ? pb[idx] : ls[idx - C]; }, len, C, s_buf, &s_count,
-        &s_tau);
+  top_c_ahead<uint64_t>([&](uint64_t idx) -> uint64_t { return idx < (uint64_t)C ? pb[idx] : ls[idx - C]; },
+                        [](uint64_t v, uint64_t) -> uint64_t { return v; }, len, C, s_buf, &s_count, &s_tau);
   for (int j = threadIdx.x; j < C; j += kBlock) best[(uint64_t)q * C + j] = s_buf[j];
 }
 

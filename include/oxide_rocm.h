@@ -311,9 +311,15 @@ typedef struct oxr_agg {
 /* `AggregateSpec` (params.rs:142-149) generalised: group key column (Int64
  * or Int32; -1 = no grouping, one global group) and aggregates over
  * expressions. `filter` (may be NULL) is applied first, fusing a
- * GpuFilterExec into the aggregate. `groups_hint` bounds the distinct keys
- * the state will see (0 = unknown: bounded by the rows of the first update).
- * A table that overflows its bound is OXR_EUNSUPPORTED ("rocm.aggregate"). */
+ * GpuFilterExec into the aggregate. `groups_hint` is the distinct keys the
+ * state is expected to see (0 = unknown): the group table is sized for it at
+ * create. It never limits the table: every update first grows the table so
+ * that each key the batch can add (at most one per row, per build row of a
+ * join, and per value of the key's measured range) keeps the load at most
+ * one half. An update that fails after some of its rows were applied (a
+ * division by zero, a device error) leaves the state unusable: every later
+ * update or finish on it is OXR_EEXECUTION, never OXR_EUNSUPPORTED, so a
+ * caller never re-runs on the CPU rows the device already counted. */
 typedef struct oxr_agg_spec {
   int32_t group_by;
   int32_t naggs;

@@ -24,7 +24,11 @@ AGG_GRIDS = [2, 4, 8, 16]
 
 
 def set_knob(name: str, value: int) -> None:
-    lib.oxr_tune_set(name.encode(), int(value))
+    """Sets a library knob (a negative value resets it). Raises PlanError for
+    an unknown knob or a value its kernels do not take."""
+    from .backend import check
+
+    check(lib.oxr_tune_set(name.encode(), int(value)))
 
 
 def get_knob(name: str) -> int:

@@ -439,6 +439,9 @@ struct oxr_agg_state {
   Table t{};
   void *mem = nullptr;  // one arena block holding the table
   uint64_t groups = 0;  // distinct ordinary keys seen (after the last update)
+  // an update failed after some of its rows were applied: the groups are no
+  // longer those of the batches folded so far, and every later call fails
+  bool poisoned = false;
   // the spec as expressions, for the generated kernels
   std::vector<oxr_expr_node> filter_nodes;
   std::vector<std::vector<oxr_expr_node>> input_nodes;
@@ -1076,7 +1079,12 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   return rc;
 }
 
+int poisoned_error() {
+  return fail(OXR_EEXECUTION, "rocm.aggregate", "an earlier update of this aggregation failed part way");
+}
+
 int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const JoinSide *js = nullptr) {
+  if (st->poisoned) return poisoned_error();
   if (in->cols.size() != st->col_types.size())
     return fail(OXR_EPLAN, "rocm.aggregate", "batch does not match the aggregation's schema");
   for (size_t i = 0; i < in->cols.size(); i++)
@@ -1089,7 +1097,8 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   // range (one pass over the key column measures it)
   uint64_t more = st->key_col == -1 ? 1 : n;
   if (js) more = std::min<uint64_t>(more, std::max<uint64_t>(js->table->nrows, 1));
-  if (st->hint) more = std::min<uint64_t>(more, st->hint > st->groups ? st->hint - st->groups : 16);
+  // (groups_hint only pre-sizes the table at create: capping `more` with it
+  // could overflow the table part way through an update)
   KeyRange kr;
   if (st->key_col != -1 && n > (uint64_t)std::max<int64_t>(0, tune_get("agg.range_rows", kRangeCheckRows))) {
     if (js) OXR_TRY(key_range(s, js->build->cols[js->build_group], js->build->rows, &kr));
@@ -1141,6 +1150,8 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.grid", 8));
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   bool generated = false;
+  // from here on a failure may leave some of the batch's rows applied
+  st->poisoned = true;
   if (rc == OXR_OK) {
     const int grc = run_update_generated(s, st, in, js, (uint32_t *)err, &kr);
     if (grc == OXR_OK) generated = true;
@@ -1171,19 +1182,23 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   stream_release(s, err);
   if (rc != OXR_OK) return rc;
   if ((uint32_t)host[0]) return fail(OXR_EEXECUTION, "rocm.aggregate", "Divide by zero error");
-  if ((uint32_t)host[2])
-    return fail(OXR_EUNSUPPORTED, "rocm.aggregate",
-                "the group table overflowed: more distinct keys than groups_hint allowed");
+  // cannot happen while the sizing above holds (load <= 1/2 for every key
+  // the batch can add); never a fallback signal: rows have been applied
+  if ((uint32_t)host[2]) return fail(OXR_EEXECUTION, "rocm.aggregate", "the group table overflowed");
   st->groups = host[1];
+  st->poisoned = false;
   return OXR_OK;
 }
 
 int run_finish(oxr_stream *s, oxr_agg_state *st, oxr_batch **out) {
+  if (st->poisoned) return poisoned_error();
   if (!st->mem) OXR_TRY(ensure_capacity(s, st, 1));
   const Table &t = st->t;
   // ordinary keys
   const uint64_t cap = t.cap;
   const uint64_t g = st->groups;
+  // the key sort takes a 32-bit signed item count
+  if (g >= (uint64_t)INT32_MAX) return fail(OXR_EUNSUPPORTED, "rocm.aggregate", "2^31 - 1 groups or more");
   void *buf = nullptr;
   const size_t gb = (g ? g : 1) * 8;
   OXR_TRY(stream_alloc(s, gb * 4 + 64, &buf));

@@ -41,13 +41,19 @@ struct FilterShape {
 };
 
 // The instantiated shapes; knobs "filter.v"/"filter.g" pick one (default
-// one row per lane, two row groups: at most ~90 VGPRs for four columns).
-FilterShape filter_shape() {
+// one row per lane, two row groups: at most ~90 VGPRs for four columns). A
+// pair that is not instantiated is refused, never replaced.
+int filter_shape(FilterShape *out) {
   const int v = (int)tune_get("filter.v", 1), g = (int)tune_get("filter.g", 2);
   const FilterShape known[] = {{1, 1}, {1, 2}, {1, 4}, {2, 1}, {2, 2}, {4, 1}};
   for (const FilterShape &k : known)
-    if (k.v == v && k.g == g) return k;
-  return {1, 2};
+    if (k.v == v && k.g == g) {
+      *out = k;
+      return OXR_OK;
+    }
+  return fail(OXR_EPLAN, "rocm.tune",
+              "filter.v x filter.g = " + std::to_string(v) + " x " + std::to_string(g) +
+                  " is not an instantiated shape (v * g <= 4)");
 }
 
 struct Out {
@@ -298,7 +304,8 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   }
 
   const uint64_t n = input->rows;
-  const FilterShape shape = filter_shape();
+  FilterShape shape{1, 2};
+  if (!use_jit) OXR_TRY(filter_shape(&shape));
   const uint64_t tile_rows =
       use_jit ? (uint64_t)kJitBlock * jf.rows_per_lane : (uint64_t)kBlock * shape.v * shape.g;
   const uint64_t nbatches = n ? (n + batch_rows - 1) / batch_rows : 0;

@@ -81,10 +81,17 @@ __device__ inline void copy_vector_rows(const float *src, float *dst, int dim, b
   }
 }
 
+// Rows per lane the skeleton takes: the kept flags of a lane are the bits of
+// one 32-bit register.
+constexpr int kMaxRowsPerLane = 32;
+
 template <class Q, int S>
 __device__ __forceinline__ void filter_tile(const JitArgs &p) {
-  // The tile scan below runs on one wave over the S * 4 (step, wave) counts.
-  static_assert(S >= 1 && S * 4 <= 64, "rows per lane: the tile scan covers at most 64 counts");
+  static_assert(S >= 1 && S <= kMaxRowsPerLane, "rows per lane: one kept bit per row in a 32-bit register");
+  // The tile scan runs on one wave over the S * 4 (step, wave) counts, PL
+  // consecutive counts per lane (PL = 1 up to 16 rows per lane).
+  constexpr int NC = S * 4;
+  constexpr int PL = (NC + 63) / 64;
   constexpr int kTile = kJitBlock * S;
   constexpr int kWords = kTile / 32 + 2;  // output validity words a tile can touch
   constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
@@ -141,9 +148,21 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   __syncthreads();
   // 3. tile scan over (step, wave) in row order, then the look-back
   if (wave == 0) {
-    const uint32_t v = lane < S * 4 ? s_cnt[lane] : 0u;
-    const uint32_t incl = wave_incl_scan(v, lane);
-    if (lane < S * 4) s_off[lane] = incl - v;
+    uint32_t v[PL], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PL; k++) {
+      const int c = lane * PL + k;
+      v[k] = c < NC ? s_cnt[c] : 0u;
+      sum += v[k];
+    }
+    const uint32_t incl = wave_incl_scan(sum, lane);
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (int k = 0; k < PL; k++) {
+      const int c = lane * PL + k;
+      if (c < NC) s_off[c] = run;
+      run += v[k];
+    }
     const uint32_t tot = __shfl(incl, 63, 64);
     const uint64_t pre = lookback(p.status, tile, tile - lt, tot, lane);
     if (lane == 0) {

@@ -278,9 +278,12 @@ void row_code(const std::vector<DevCol> &cols, const std::vector<char> &val_used
 // S (rows per lane) for a row of `vgprs` 32-bit registers: the largest power
 // of two up to 16 keeping the tile's registers near 128, unless the
 // "filter.rows" knob says otherwise.
+// Rows per lane of a generated kernel: the knob when set (oxr_tune_set only
+// accepts 1-32, what the skeletons take), else the largest power of two up to
+// 16 whose rows fit about 128 VGPRs.
 int pick_rows(int vgprs, const char *knob_name = "filter.rows") {
   const int64_t knob = tune_get(knob_name, 0);
-  if (knob == 1 || knob == 2 || knob == 4 || knob == 8 || knob == 16) return (int)knob;
+  if (knob >= 1 && knob <= kMaxRowsPerLane) return (int)knob;
   int s = 16;
   while (s > 2 && s * vgprs > 128) s /= 2;
   return s;

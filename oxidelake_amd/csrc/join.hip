@@ -236,6 +236,11 @@ int oxr_join_build(oxr_stream *s, const oxr_batch *build, int32_t key, oxr_join_
   OXR_TRY(key_col(build, key, &k, "build"));
   const uint64_t n = build->rows;
   if (n >= UINT32_MAX) return fail(OXR_EEXECUTION, "rocm.join", "build side exceeds u32::MAX rows");
+  // the device scan and sort of the build side take 32-bit signed counts
+  // (slots are at most 2 * n rounded up to a power of two, so n < 2^29 keeps
+  // them below 2^31)
+  if (n >= (1ull << 29))
+    return fail(OXR_EUNSUPPORTED, "rocm.join", "build side of 2^29 rows or more");
   std::unique_ptr<oxr_join_table> jt(new oxr_join_table());
   jt->dev = s->dev;
   jt->owner = retain(s->core);
@@ -400,6 +405,9 @@ int oxr_join_probe(oxr_stream *s, const oxr_join_table *table, const oxr_batch *
   OXR_TRY(key_col(probe, key, &k, "probe"));
   const uint64_t n = probe->rows;
   if (n >= UINT32_MAX) return fail(OXR_EEXECUTION, "rocm.join", "batch exceeds u32::MAX rows");
+  // the device scan of the n + 1 match counts takes a 32-bit signed count
+  if (n + 1 > (uint64_t)INT32_MAX)
+    return fail(OXR_EUNSUPPORTED, "rocm.join", "probe batch of 2^31 - 1 rows or more");
   std::unique_ptr<oxr_join_matches> m(new oxr_join_matches());
   m->dev = s->dev;
   m->owner = retain(s->core);

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <atomic>
 #include <memory>
 
@@ -265,7 +266,54 @@ int alloc_column(oxr_stream *s, int32_t type, int32_t dim, uint64_t rows, bool n
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, int64_t> g_tune;
+
+// Every knob the library reads, with the values its kernels take. A value
+// outside them is refused (OXR_EPLAN), so a benchmark never reports a shape
+// that did not run.
+struct KnobSpec {
+  const char *name;
+  int64_t lo, hi;      // inclusive range
+  uint32_t allowed;    // nonzero: bit v set for each allowed value v < 32
+};
+constexpr uint32_t kBool = 0x3u;
+const KnobSpec kKnobs[] = {
+    {"filter.jit", 0, 1, kBool},
+    {"filter.rows", 1, 32, 0},               // generated filter: rows per lane
+    {"filter.v", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
+    {"filter.g", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
+    {"jit.params", 0, 1, kBool},
+    {"agg.jit", 0, 1, kBool},
+    {"agg.rows", 1, 32, 0},                  // generated aggregate: rows per lane
+    {"agg.grid", 1, 64, 0},
+    {"agg.jit_grid", 1, 64, 0},
+    {"agg.exact_f64", 0, 1, kBool},
+    {"agg.dense", 0, 1, kBool},
+    {"agg.range_rows", 0, (int64_t)1 << 48, 0},
+    {"agg.spill", 0, 1, kBool},
+    {"agg.spill_rows", 0, (int64_t)1 << 48, 0},
+    {"agg.spill_pct", 1, 400, 0},
+    {"vector.resident_queries", 0, 1, kBool},
+    {"vector.wide", 0, 3, 0},
+    {"vector.nt8", 0, 1, kBool},
+    {"vector.pilot", 0, 1, kBool},
+    {"vector.pilot_rows", 1, (int64_t)1 << 40, 0},
+    {"vector.pilot_cap", 1, (int64_t)1 << 32, 0},
+};
 }  // namespace
+
+// OXR_OK when `knob` is known and takes `value` (a negative value resets a
+// known knob to its default).
+int tune_check(const char *knob, int64_t value) {
+  for (const KnobSpec &k : kKnobs) {
+    if (std::strcmp(k.name, knob) != 0) continue;
+    if (value < 0) return OXR_OK;
+    const bool in_range = value >= k.lo && value <= k.hi;
+    if (in_range && (!k.allowed || (value < 32 && ((k.allowed >> value) & 1u)))) return OXR_OK;
+    return fail(OXR_EPLAN, "rocm.tune",
+                std::string("knob ") + knob + " does not take the value " + std::to_string(value));
+  }
+  return fail(OXR_EPLAN, "rocm.tune", std::string("unknown knob ") + knob);
+}
 
 int64_t tune_get(const char *knob, int64_t dflt) {
   std::lock_guard<std::mutex> g(g_tune_mu);
@@ -902,6 +950,7 @@ const char *oxr_batch_name(const oxr_batch *b, int32_t i) {
 
 int oxr_tune_set(const char *knob, int64_t value) {
   if (!knob) return fail(OXR_EPLAN, "rocm.tune", "null knob");
+  OXR_TRY(tune_check(knob, value));
   std::lock_guard<std::mutex> g(g_tune_mu);
   if (value < 0) g_tune.erase(knob);
   else g_tune[knob] = value;

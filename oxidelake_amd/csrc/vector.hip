@@ -462,7 +462,10 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
       const uint64_t arow = row < tile_end ? row : tile_end - 1;
       loff[r] = (uint32_t)((arow - blockrow) * rstride) + (uint32_t)(h * 64);
     }
-    if (BRES) __syncthreads();  // the previous block's epilogue has read s_mul
+    // every wave has finished the previous block's epilogue (its reads of
+    // s_mul and s_bias) before they are rewritten; on the streamed path the
+    // k loop's barriers come after this fill, so they do not order it
+    __syncthreads();
     for (int t = tid; t < BROWS; t += kBlock) {
       const uint64_t row = blockrow + t;
       float mul = 0.f, bias = INFINITY;  // NULL, zero norm, past the tile

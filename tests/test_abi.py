@@ -40,3 +40,26 @@ def test_supports_predicate_is_host_only():
     assert lib.oxr_supports_predicate(p.ref()) == 1
     bad = plan.Encoded(plan.and_(plan.col(0), plan.col(1)))  # AND of non-booleans
     assert lib.oxr_supports_predicate(bad.ref()) == 0
+
+
+def test_tune_set_refuses_unknown_knobs_and_values():
+    # a shape the kernels do not take is refused, never replaced by another:
+    # a benchmark line reports only shapes that ran
+    import pytest
+
+    from oxidelake_amd import autotune
+    from oxidelake_amd.backend import PlanError
+
+    for knob, bad in [("filter.rows", 0), ("filter.rows", 33), ("agg.rows", 64), ("filter.v", 3),
+                      ("agg.spill_pct", 0), ("vector.wide", 4), ("agg.jit", 2), ("no.such_knob", 1)]:
+        with pytest.raises(PlanError):
+            autotune.set_knob(knob, bad)
+    with pytest.raises(PlanError):
+        autotune.set_knob("no.such_knob", -1)
+    try:
+        for rows in (1, 3, 16, 20, 32):
+            autotune.set_knob("filter.rows", rows)
+            assert autotune.get_knob("filter.rows") == rows
+    finally:
+        autotune.set_knob("filter.rows", -1)
+    assert autotune.get_knob("filter.rows") == -1

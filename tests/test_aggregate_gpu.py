@@ -228,7 +228,7 @@ def test_hot_keys_fold_across_the_wave(backend, stream, oracle):
     assert_agg_equal(device_aggregate(backend, stream, b, spec), oracle.aggregate(b, spec, sum_mode=1), spec)
 
 
-@pytest.mark.parametrize("rows", [1, 4, 16])
+@pytest.mark.parametrize("rows", [1, 4, 16, 32])
 def test_every_rows_per_lane(backend, stream, oracle, rows, path):
     from oxidelake_amd import autotune
 
@@ -373,3 +373,26 @@ def test_spill_many_groups(backend, stream, oracle, path, pct):
         lib.oxr_profile_reset()
     assert_agg_equal(got, oracle.aggregate(batch, spec, sum_mode=1), spec)
     assert spills == (0 if path in ("interpreter", "exact") else 1)
+
+
+def test_groups_hint_below_the_distinct_keys(backend, stream, oracle, path):
+    # a hint far below the keys the batches bring only pre-sizes the table:
+    # every update grows it first, so no row is lost or counted twice
+    from oxidelake_amd import autotune
+
+    autotune.set_knob("agg.range_rows", 0)  # measured key ranges on every batch size
+    rng = np.random.default_rng(41)
+    batches = []
+    for n in (5_000, 70_000, 200_000):
+        k = pa.array(rng.integers(0, 50_000, n).astype(np.int64), mask=rng.random(n) < 0.02)
+        v = pa.array(rng.integers(-1000, 1000, n).astype(np.int64))
+        batches.append(pa.record_batch({"k": k, "v": v}))
+    spec = AggregateSpec(0, [Aggregate(F.CountStar, None), Aggregate(F.Sum, col(1)), Aggregate(F.Max, col(1))],
+                         groups_hint=4)
+    dev = [backend.upload(stream, b) for b in batches]
+    st = backend.aggregate_state(stream, dev[0], spec)
+    for d in dev:
+        st.update(stream, d)
+    got = backend.download(stream, st.finish(stream))
+    want = oracle.aggregate(pa.Table.from_batches(batches).combine_chunks().to_batches()[0], spec, sum_mode=1)
+    assert_agg_equal(got, want, spec)

@@ -235,9 +235,10 @@ def test_every_launch_shape_is_bit_exact(backend, stream, oracle, shape):
         autotune.set_knob("filter.jit", -1)
 
 
-@pytest.mark.parametrize("rows", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("rows", [1, 2, 4, 8, 16, 20, 24, 32])
 def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows):
-    # generated kernels with S rows per lane (tiles of 256 * S rows)
+    # generated kernels with S rows per lane (tiles of 256 * S rows); above
+    # 16 the tile scan covers two (step, wave) counts per lane
     autotune.set_knob("filter.rows", rows)
     try:
         rng = np.random.default_rng(78)

@@ -1,35 +1,55 @@
 #!/usr/bin/env python3
-"""Benchmark: OxideLake's fused filter + projection (GpuFilterExec) on one MI355X.
+"""Benchmark: OxideLake's four GPU operators on one MI355X.
 
-Workload (the 200M-row filter + projection table, HBM-resident):
+Headline (the `value` of the line): the fused filter + projection
+(GpuFilterExec) over the 200M-row table, HBM-resident:
   SELECT a, b*2+1, c*d FROM t WHERE b < 50 AND c > 0.5 OR a % 7 = 3
   t: a Int64 U[0,2^40), b Int32 U[0,1000), c Float32 N(0,1) with 5% NULLs,
      d Float64 lognormal(0,1); 200,000,000 rows, synthetic, seeded.
 A step is one pass of the operator over the whole resident table (one
 launch; the table is 200 batches of 1M rows laid end to end in HBM).
-
-Prints ONE JSON line. `value` = input rows per second. `roofline` uses the
+`value` = input rows per second over whole steps; `roofline` uses the
 kernel's own duration from HIP events on its launch stream and the
 algorithmic bytes 24.125 B per input row + 24.125 B per selected row.
-`cpu_baseline` times the CPU oracle (a single-threaded C restatement of the
-reference CPU path) on a bounded sample of the same table.
+
+The same line carries, under `execs`, the other three operators on their
+workloads, each with its kernel time, roofline, full-size parity and CPU
+baseline: GpuAggregateExec (the k/v GROUP BY over 100M rows),
+GpuHashJoinExec + GpuAggregateExec (500M Zipf fact rows joined to a 10M-row
+dimension, SUM/SUM/COUNT by 1M groups) and GpuVectorDistanceExec (top-10 over
+20M x 768 rows, 256-query batches, FP32 / BF16 + rerank / FP8 + rerank, with
+recall@10 over all 256 queries against a certified FP64 top-10).
+
+CPU baselines run on this host: the oracle (a C restatement of the
+reference's CPU path) on one core and on every core this process may use,
+and Arrow's C++ engine (Acero, through pyarrow; not DataFusion) on every
+core, over bounded samples of the same seeded tables.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import glob
 import json
+import math
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3}
 IN_BYTES_PER_ROW = 8 + 4 + 4 + 1 / 8 + 8      # a, b, c (+validity bit), d
 OUT_BYTES_PER_ROW = 8 + 8 + 8 + 1 / 8         # a, b*2+1 (Int64), c*d (+validity)
 BATCH_ROWS = 1_000_000                        # 1M-row batches (a multiple of 32 rows)
+# (b threshold, c threshold, modulus) of the query's three selectivity points.
+# As written the query selects 14.3%-39.4% whatever the b threshold (a % 7 = 3
+# alone keeps 1/7); the points keep its shape and move the c threshold and
+# the modulus as well: 0.98%, 10.0% and 50.05% expected.
+SELECTIVITY_POINTS = {"1%": (30, 0.5, 1000), "10%": (180, 0.5, 20), "50%": (470, -1.0, 5)}
 
 
 def parse():
@@ -40,86 +60,95 @@ def parse():
     ap.add_argument("--rows", type=int, default=200_000_000)
     ap.add_argument("--b-threshold", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-execs", action="store_true",
+                    help="filter: leave out the other operators' measurements")
+    ap.add_argument("--exec-steps", type=int, default=5, help="timed steps per operator under `execs`")
+    ap.add_argument("--cpu-sample-rows", type=int, default=20_000_000,
+                    help="rows of the CPU baselines' samples (filter, aggregate)")
     ap.add_argument("--e2e", action="store_true",
                     help="filter: also time the plain drop-in path over host batches (PCIe included)")
-    ap.add_argument("--sweep", action="store_true", help="also time b thresholds 500 and 1000")
     ap.add_argument("--filter-rows", type=int, default=0,
                     help="rows per lane of the generated kernel (default: autotuned)")
     ap.add_argument("--workload", choices=["filter", "vector", "aggregate", "join", "serving", "copy"],
                     default="filter",
-                    help="filter: the 200M-row filter+projection (default); vector: 20M x 768 top-10; "
-                         "aggregate: the k/v GROUP BY query over 100M rows; join: 500M x 10M join + aggregate; "
-                         "serving: the mixed request loop; copy: a plain 1 GiB elementwise copy that "
-                         "calibrates the HBM byte counters")
+                    help="filter: the 200M-row filter+projection with every operator under `execs` "
+                         "(default); vector: 20M x 768 top-10; aggregate: the k/v GROUP BY query over "
+                         "100M rows; join: 500M x 10M join + aggregate; serving: the mixed request loop; "
+                         "copy: a plain 1 GiB elementwise copy that calibrates the HBM byte counters")
     ap.add_argument("--agg-rows", type=int, default=100_000_000)
     ap.add_argument("--fact-rows", type=int, default=500_000_000)
     ap.add_argument("--dim-rows", type=int, default=10_000_000)
     ap.add_argument("--groups", type=int, default=1_000_000)
     ap.add_argument("--vec-rows", type=int, default=20_000_000)
-    ap.add_argument("--recall-queries", type=int, default=16)
+    ap.add_argument("--recall-bruteforce", type=int, default=0,
+                    help="vector: also check the certified top-10 of this many queries against a full "
+                         "FP64 scan of every row on the host")
     ap.add_argument("--check", action="store_true", help="join: full-size check against numpy")
     ap.add_argument("--serve-rates", default="50,200,400",
                     help="serving: offered request rates (requests/s), comma separated")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="serving: seconds of Poisson arrivals per rate")
-    ap.add_argument("--serve-fact-rows", type=int, default=100_000_000)
+    ap.add_argument("--serve-fact-rows", type=int, default=500_000_000)
+    ap.add_argument("--serve-max-queries", type=int, default=64)
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="set a library tuning knob before running (repeatable)")
     return ap.parse_args()
 
 
-def cpu_baseline(host_sample, threshold: int):
-    """The oracle's filter + projection on a bounded sample (the first rows of
-    the same seeded table), single-threaded, timed on this host."""
-    from oracle import oracle as O
-    from oxidelake_amd import datagen
-
-    pred, proj = datagen.filter_query(threshold)
-    O.filter_project(host_sample, pred, proj)  # warm
-    best, reps, t0 = float("inf"), 0, time.perf_counter()
-    while reps < 5 and time.perf_counter() - t0 < 20:
-        s = time.perf_counter()
-        O.filter_project(host_sample, pred, proj)
-        best = min(best, time.perf_counter() - s)
-        reps += 1
-    rows = host_sample.num_rows
-    # the same restatement on host threads, one slice of the sample each (the
-    # oracle's C code releases the interpreter lock), as the reference's CPU
-    # path spreads batches over cores
-    from concurrent.futures import ThreadPoolExecutor
-
-    threads = max(1, min(16, os.cpu_count() or 1))
-    step = -(-rows // threads)
-    parts = [host_sample.slice(i, step) for i in range(0, rows, step)]
-    with ThreadPoolExecutor(threads) as pool:
-        def par():
-            list(pool.map(lambda b: O.filter_project(b, pred, proj), parts))
-
-        psec = timed_cpu(par)
-    return {
-        "value": rows / psec,
-        "unit": "rows/s",
-        "cores": threads,
-        "kind": "port",
-        "one_thread": rows / best,
-        "sample": f"first {rows:,} rows of the same seeded table, same query, best of {reps} "
-                  f"(value: {threads} host threads over slices of the sample; one_thread: one core)",
-    }
+def log(msg: str) -> None:
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def timed_cpu(fn, budget_s: float = 20.0, reps: int = 5) -> float:
-    """Best wall time of fn() over up to `reps` runs within about `budget_s`."""
+# ------------------------------------------------------------------ the host
+def host_threads() -> int:
+    """Cores this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def host_info() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "threads_usable": host_threads(), "cpu_model": model,
+            "numa_nodes": len(glob.glob("/sys/devices/system/node/node[0-9]*")) or None}
+
+
+def timed_cpu(fn, budget_s: float = 20.0, reps: int = 5) -> tuple[float, float, int]:
+    """(best, median) wall seconds of fn() over up to `reps` runs within about
+    `budget_s`, and the runs made."""
     fn()  # warm
-    best, n, t0 = float("inf"), 0, time.perf_counter()
-    while n < reps and time.perf_counter() - t0 < budget_s:
+    times, t0 = [], time.perf_counter()
+    while len(times) < reps and (not times or time.perf_counter() - t0 < budget_s):
         s0 = time.perf_counter()
         fn()
-        best = min(best, time.perf_counter() - s0)
-        n += 1
-    return best
+        times.append(time.perf_counter() - s0)
+    times.sort()
+    return times[0], times[len(times) // 2], len(times)
 
 
-def full_size_check(host, out_host, threshold: int) -> bool:
+def _kernel_ms(lib, name: str):
+    ms, cnt = C.c_double(), C.c_uint64()
+    lib.oxr_profile_get(name.encode(), C.byref(ms), C.byref(cnt))
+    return ms.value / max(1, cnt.value), int(cnt.value)
+
+
+def _roofline_hbm(alg_bytes: float, kernel_ms: float, traffic=None) -> dict:
+    achieved = alg_bytes / (kernel_ms / 1000) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
+
+
+# ------------------------------------------------------- filter + projection
+def filter_check(host, out_host, threshold: int, c_thr: float = 0.5, modulus: int = 7) -> bool:
     """Size-independent parity at full size: the selected row count and the
     wrapping Int64 sums of both integer outputs equal an independent numpy
     evaluation of the query over the whole table."""
@@ -130,7 +159,7 @@ def full_size_check(host, out_host, threshold: int) -> bool:
     c_arr = host.column(2)
     c = c_arr.fill_null(0.0).to_numpy()
     cv = c_arr.is_valid().to_numpy(zero_copy_only=False)
-    mask = ((b < threshold) & cv & (c > 0.5)) | (a % 7 == 3)
+    mask = ((b < threshold) & cv & (c > c_thr)) | (a % modulus == 3)
     want_rows = int(mask.sum())
     with np.errstate(over="ignore"):
         want_a = int(a[mask].sum(dtype=np.int64))
@@ -140,203 +169,222 @@ def full_size_check(host, out_host, threshold: int) -> bool:
     return out_host.num_rows == want_rows and got_a == want_a and got_b == want_b
 
 
-class _DevPtr:
-    """A device address with a shape, for RocmBackend.wrap."""
+def filter_cpu_baselines(sample, threshold: int) -> dict:
+    """The oracle's filter + projection on one core and on every core (one
+    slice of the sample per thread: the C code releases the interpreter
+    lock), and Acero's filter + project on every core, over the same
+    sample (the first rows of the same seeded table)."""
+    import pyarrow as pa
+    import pyarrow.compute as pc
 
-    def __init__(self, ptr: int, rows: int, dim: int):
-        self.ptr, self.shape = ptr, (rows, dim)
-
-    def data_ptr(self) -> int:
-        return self.ptr
-
-
-def bench_vector(args):
-    """Embedding search: top-10 over 20M x 768 L2-normalised mixture rows (cosine),
-    query batches of 1, 32 and 256; plain FP32, BF16 + FP32 rerank of 100,
-    FP8 e4m3 (per-row scale) + FP32 rerank of 100. QPS and latency p50/p99
-    per batch size; recall@10 and max distance error against an FP64 oracle
-    computed on the host over a sample of the queries."""
-    import numpy as np
-
-    from oxidelake_amd import datagen, plan
-    from oxidelake_amd._capi import lib
-    from oxidelake_amd.backend import VEC_BF16, VEC_FP8, VEC_FP32, RocmBackend, VectorIndex
-
-    be = RocmBackend(0)
-    stream = be.create_stream()
-    n, dim, qmax, k = args.vec_rows, 768, 256, 10
-    metric = plan.DistanceMetric.Cosine
-    # rows [n, n + 256) of the same counter-based mixture are the queries
-    table = datagen.table(be, stream, datagen.embedding_columns(seed=42, dim=dim), n + qmax)
-    base_ptr = table.column(0).values
-    rows_view = be.wrap([(_DevPtr(base_ptr, n, dim), None, plan.F32_VECTOR, dim)], ["emb"])
-    q_view = be.wrap([(_DevPtr(base_ptr + n * dim * 4, qmax, dim), None, plan.F32_VECTOR, dim)], ["emb"])
-    queries = np.asarray(be.download(stream, q_view).column(0).values.to_numpy(zero_copy_only=False),
-                         np.float32).reshape(qmax, dim)
-    t0 = time.perf_counter()
-    indexes = {}
-    for name, prec in [("fp32", VEC_FP32), ("bf16", VEC_BF16), ("fp8", VEC_FP8)]:
-        s0 = time.perf_counter()
-        indexes[name] = VectorIndex(stream, rows_view, 0, metric, prec)
-        stream.synchronize()
-        indexes[name + "_build_s"] = time.perf_counter() - s0
-    # FP8 e4m3 keeps ~4 significant bits: it needs a deeper rerank than BF16
-    # for recall@10 >= 0.99 (recall_vs_rerank_depth in the output)
-    paths = [("fp32_exact", "fp32", k), ("bf16_rerank100", "bf16", 100), ("fp8_rerank1000", "fp8", 1000)]
-    results = {}
-    for pname, iname, cand in paths:
-        ix = indexes[iname]
-        per_q = {}
-        for nq in (1, 32, 256):
-            reps = {1: 40, 32: 20, 256: 8}[nq]
-            for w in range(2):
-                ix.search(stream, queries[:nq], k, cand)
-            lat = []
-            lib.oxr_profile_reset()
-            lib.oxr_profile_enable(1)
-            for r in range(reps):
-                lo = (r * nq) % qmax
-                qb = np.roll(queries, -lo, axis=0)[:nq]
-                s0 = time.perf_counter()
-                ix.search(stream, qb, k, cand)
-                lat.append(time.perf_counter() - s0)
-            lib.oxr_profile_enable(0)
-            scan_ms = {}
-            for kern in ("vector_scan", "vector_select", "vector_merge", "vector_rerank"):
-                ms, cnt = C.c_double(), C.c_uint64()
-                lib.oxr_profile_get(kern.encode(), C.byref(ms), C.byref(cnt))
-                scan_ms[kern] = ms.value / reps
-            lib.oxr_profile_reset()
-            lat_ms = np.array(lat) * 1000
-            eb = {"fp32": 4, "bf16": 2, "fp8": 1}[iname]
-            scan = scan_ms["vector_scan"]
-            per_q[str(nq)] = {
-                "qps": nq / (lat_ms.mean() / 1000),
-                "latency_ms_p50": float(np.percentile(lat_ms, 50)),
-                "latency_ms_p99": float(np.percentile(lat_ms, 99)),
-                "device_ms": scan_ms,
-                "scan_tflops": 2.0 * n * dim * nq / (scan / 1000) / 1e12,
-                "scan_gbs": n * dim * eb / (scan / 1000) / 1e9,
-            }
-        results[pname] = per_q
-    # FP64 oracle on a sample of queries: exact cosine distances of every row,
-    # streamed through the host in 1M-row chunks
-    ns = min(args.recall_queries, qmax)
-    qs = queries[:ns].astype(np.float64)
-    qn = np.linalg.norm(qs, axis=1)
-    best = [np.empty(0, np.int64) for _ in range(ns)]
-    bestd = [np.empty(0) for _ in range(ns)]
-    got = {p: indexes[i].search(stream, queries[:ns], k, c) for p, i, c in paths}
-    # recall against the rerank depth of the reduced-precision scans
-    depth = {}
-    for iname in ("bf16", "fp8"):
-        for cand in (50, 100, 200, 400, 1000):
-            depth[(iname, cand)] = indexes[iname].search(stream, queries[:ns], k, cand)
-    d64_of = {}  # (query, row) -> FP64 distance for every row a path returned
-    need = {}
-    for rows_p, _ in list(got.values()) + list(depth.values()):
-        for qi in range(ns):
-            for r in rows_p[qi]:
-                need.setdefault(int(r), set()).add(qi)
-    chunk = 1_000_000
-    for c0 in range(0, n, chunk):
-        m = min(chunk, n - c0)
-        view = be.wrap([(_DevPtr(base_ptr + c0 * dim * 4, m, dim), None, plan.F32_VECTOR, dim)], ["emb"])
-        x = np.asarray(be.download(stream, view).column(0).values.to_numpy(zero_copy_only=False),
-                       np.float32).reshape(m, dim).astype(np.float64)
-        xn = np.linalg.norm(x, axis=1)
-        d = 1.0 - (x @ qs.T) / (xn[:, None] * qn[None, :])
-        for qi in range(ns):
-            col = d[:, qi]
-            idx = np.argpartition(col, k)[:k] if m > k else np.arange(m)
-            cand_rows = np.concatenate([best[qi], idx + c0])
-            cand_d = np.concatenate([bestd[qi], col[idx]])
-            order = np.lexsort((cand_rows, cand_d))[:k]
-            best[qi], bestd[qi] = cand_rows[order], cand_d[order]
-        for r in [r for r in need if c0 <= r < c0 + m]:
-            for qi in need[r]:
-                d64_of[(qi, r)] = d[r - c0, qi]
-        del x, d
-    for p, _, _ in paths:
-        rows_p, dist_p = got[p]
-        hit, err = 0, 0.0
-        for qi in range(ns):
-            hit += len(set(best[qi].tolist()) & set(rows_p[qi].tolist()))
-            for r, dv in zip(rows_p[qi], dist_p[qi]):
-                err = max(err, abs(float(dv) - d64_of[(qi, int(r))]))
-        results[p]["recall_at_10"] = hit / (ns * k)
-        results[p]["max_distance_error_vs_fp64"] = err
-    recall_vs_depth = {}
-    for (iname, cand), (rows_p, _) in depth.items():
-        hit = sum(len(set(best[qi].tolist()) & set(rows_p[qi].tolist())) for qi in range(ns))
-        recall_vs_depth.setdefault(iname, {})[str(cand)] = hit / (ns * k)
-    # CPU baseline: the oracle's FP64 brute force over a bounded sample of rows
     from oracle import oracle as O
+    from oxidelake_amd import datagen
 
-    sample = 200_000
-    view = be.wrap([(_DevPtr(base_ptr, sample, dim), None, plan.F32_VECTOR, dim)], ["emb"])
-    xs = np.asarray(be.download(stream, view).column(0).values.to_numpy(zero_copy_only=False),
-                    np.float32).reshape(sample, dim)
-    s0 = time.perf_counter()
-    O.topk64(xs, None, queries[0], metric, k)
-    cpu_s = (time.perf_counter() - s0) * n / sample
-    good = [(results[p]["256"]["qps"], p) for p, _, _ in paths if results[p]["recall_at_10"] >= 0.99]
-    value, best_path = max(good) if good else (0.0, None)
-    scan = results[best_path]["256"] if best_path else {}
+    pred, proj = datagen.filter_query(threshold)
+    rows = sample.num_rows
+    one, one_med, reps = timed_cpu(lambda: O.filter_project(sample, pred, proj), budget_s=30)
+    threads = host_threads()
+    step = -(-rows // threads)
+    parts = [sample.slice(i, step) for i in range(0, rows, step)]
+    with ThreadPoolExecutor(threads) as pool:
+        allc, all_med, _ = timed_cpu(lambda: list(pool.map(lambda b: O.filter_project(b, pred, proj), parts)))
+    out = {"value": rows / allc, "unit": "rows/s", "cores": threads, "kind": "port",
+           "one_thread": rows / one, "median_rows_per_s": rows / all_med,
+           "sample": f"first {rows:,} rows of the same seeded table, same query; best of up to 5 "
+                     f"(value: the oracle, a C restatement of the reference CPU path, on {threads} host "
+                     f"threads over slices; one_thread: one core, best of {reps})"}
+    try:
+        import pyarrow.acero as ac
+
+        table = pa.Table.from_batches([sample.slice(i, BATCH_ROWS) for i in range(0, rows, BATCH_ROWS)])
+        a, b, c, d = (pc.field(n) for n in ("a", "b", "c", "d"))
+        cond = ((b < threshold) & (c.cast(pa.float64()) > 0.5)) | \
+            ((a - (a / pc.scalar(7)) * pc.scalar(7)) == pc.scalar(3))
+        decl = ac.Declaration.from_sequence([
+            ac.Declaration("table_source", ac.TableSourceNodeOptions(table)),
+            ac.Declaration("filter", ac.FilterNodeOptions(cond)),
+            ac.Declaration("project", ac.ProjectNodeOptions(
+                [a, b.cast(pa.int64()) * pc.scalar(2) + pc.scalar(1), c.cast(pa.float64()) * d],
+                ["a", "b2", "cd"])),
+        ])
+        got = {}
+
+        def acero():
+            got["t"] = decl.to_table(use_threads=True)
+
+        best, med, _ = timed_cpu(acero)
+        want_rows = O.filter_project(sample, pred, proj).num_rows
+        out["arrow_acero"] = {"value": rows / best, "median_rows_per_s": rows / med, "unit": "rows/s",
+                              "cores": pa.cpu_count(), "rows_match_oracle": got["t"].num_rows == want_rows,
+                              "engine": "Arrow C++ engine (Acero via pyarrow " + pa.__version__ +
+                                        "; not DataFusion)"}
+    except Exception as e:  # noqa: BLE001 (a missing engine is reported, not fatal)
+        out["arrow_acero"] = {"error": f"{type(e).__name__}: {e}"}
+    return out
+
+
+def bench_filter(args, be, stream) -> dict:
+    """The headline: config 2's query over the resident 200M-row table."""
+    import pyarrow as pa
+
+    from oxidelake_amd import autotune, datagen
+    from oxidelake_amd._capi import lib
+
+    n = args.rows
+    dev = datagen.table(be, stream, datagen.filter_table_columns(seed=42), n)
+    prepared = {}
+
+    def run(threshold, c_thr=0.5, modulus=7):
+        # 1M-row batches, each filtered and compacted on its own, one launch;
+        # the plan is prepared once per query (as an engine prepares it)
+        key = (threshold, c_thr, modulus)
+        if key not in prepared:
+            pred, proj = datagen.filter_query(threshold, c_thr, modulus)
+            prepared[key] = be.prepare_filter(pred, proj, ["a", "b2", "cd"])
+        return prepared[key].run_batches(stream, dev, BATCH_ROWS)
+
+    def gather(outs):
+        return pa.Table.from_batches([be.download(stream, o) for o in outs]).combine_chunks().to_batches()[0]
+
+    def kernel_time(reps, *q):
+        out = run(*q)
+        out = None
+        stream.synchronize()
+        lib.oxr_profile_reset()
+        lib.oxr_profile_enable(1)
+        for _ in range(reps):
+            out = run(*q)
+            out = None
+        stream.synchronize()
+        lib.oxr_profile_enable(0)
+        km, launches = _kernel_ms(lib, "filter_project")
+        lib.oxr_profile_reset()
+        if launches != reps:
+            raise RuntimeError(f"expected {reps} launches of the generated filter kernel, saw {launches}")
+        return km
+
+    # full-size parity (count and checksums against numpy over the whole
+    # table), at the query's own literals and at the three selectivity points
+    host = be.download(stream, dev)
+    out_host = gather(run(args.b_threshold))
+    ok = filter_check(host, out_host, args.b_threshold)
+    want_rows = out_host.num_rows
+    out_host = None
+    points = {}
+    for name, (thr, c_thr, mod) in SELECTIVITY_POINTS.items():
+        o = gather(run(thr, c_thr, mod))
+        points[name] = {"b_threshold": thr, "c_threshold": c_thr, "modulus": mod,
+                        "selectivity": o.num_rows / n, "parity_full_size": filter_check(host, o, thr, c_thr, mod)}
+        o = None
+    e2e = None
+    if args.e2e:
+        from oxidelake_amd.streaming import StreamingFilter
+
+        hb = [host.slice(i, BATCH_ROWS) for i in range(0, host.num_rows, BATCH_ROWS)]
+        pred, proj = datagen.filter_query(args.b_threshold)
+        e2e = {"batches": len(hb), "transfers": "pageable host Arrow buffers, per batch"}
+        for lanes in (1, 4):
+            sf = StreamingFilter(be, pred, proj, ["a", "b2", "cd"], lanes=lanes)
+            sf.run(hb[:4])  # warm
+            t0 = time.perf_counter()
+            res = sf.run(hb)
+            sec = time.perf_counter() - t0
+            e2e[f"rows_per_s_{lanes}_stream{'s' if lanes > 1 else ''}"] = host.num_rows / sec
+            e2e[f"rows_ok_{lanes}"] = sum(r.num_rows for r in res) == want_rows
+            res = None
+        log(f"end to end: {e2e}")
+        hb = None
+    host = None
+    log(f"full-size parity checked: {ok}; selectivity points: {points}")
+
+    # launch shape picked by the project autotuner on this device and table
+    if args.filter_rows:
+        autotune.set_knob("filter.rows", args.filter_rows)
+        tuned = {"knobs": {"filter.rows": args.filter_rows}, "ms": None}
+    else:
+        tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
+    log(f"launch shape: {tuned['knobs']}")
+
+    for _ in range(args.warmup):
+        out = run(args.b_threshold)
+        out = None
+    stream.synchronize()
+    lib.oxr_profile_reset()
+    lib.oxr_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = run(args.b_threshold)
+        out = None
+    stream.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.oxr_profile_enable(0)
+    kernel_ms, launches = _kernel_ms(lib, "filter_project")
+    lib.oxr_profile_reset()
+    if launches != args.steps:
+        raise RuntimeError(f"expected {args.steps} launches of the generated filter kernel, saw {launches}")
+    for name, (thr, c_thr, mod) in SELECTIVITY_POINTS.items():
+        km = kernel_time(5, thr, c_thr, mod)
+        s = points[name]["selectivity"]
+        points[name]["kernel_ms"] = km
+        points[name]["hbm_gbs"] = (n * IN_BYTES_PER_ROW + s * n * OUT_BYTES_PER_ROW) / (km / 1000) / 1e9
+        points[name]["roofline_frac"] = points[name]["hbm_gbs"] / HBM_PEAK_GBS
+    ms_per_step = elapsed * 1000 / args.steps
+    alg_bytes = n * IN_BYTES_PER_ROW + want_rows * OUT_BYTES_PER_ROW
     result = {
-        "metric": "vector top-10 QPS at recall@10 >= 0.99 (GpuVectorDistanceExec, 20M x 768, batches of 256)",
-        "value": value,
-        "unit": "queries/s",
+        "metric": "filter+projection rows/s (GpuFilterExec, HBM-resident, 200M rows)",
+        "value": n / (ms_per_step / 1000),
+        "unit": "rows/s",
         "n_gpus": 1,
-        "steps": 8,
-        "warmup": 2,
-        "ms_per_step": scan.get("latency_ms_p50"),
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32_exact": "fp32", "bf16_rerank100": "bf16", "fp8_rerank1000": "fp8"}.get(best_path),
-        "data": "synthetic (seeded Gaussian mixture of 1024 centroids, sigma 0.3, L2-normalised, in HBM)",
-        "config": {"workload": "cosine top-10, 20M x 768 rows, query batches of 1/32/256", "rows": n,
-                   "dim": dim, "best_path": best_path},
-        "paths": results,
-        "index_build_s": {i: indexes[i + "_build_s"] for i in ("fp32", "bf16", "fp8")},
-        "recall_sample_queries": ns,
-        "recall_vs_rerank_depth": recall_vs_depth,
-        "roofline": {"bound": "mfma", "achieved": scan.get("scan_tflops"),
-                     "peak": {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3}.get(
-                         {"fp32_exact": "fp32", "bf16_rerank100": "bf16", "fp8_rerank1000": "fp8"}.get(best_path),
-                         2500.0),
-                     "unit": "TFLOP/s", "traffic": None},
-        "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
-                         "sample": f"FP64 brute-force top-10 of one query over the first {sample:,} rows, "
-                                   f"scaled to {n:,} rows"},
-        "total_s": time.perf_counter() - t0,
+        "dtype": "int64/f32/f64",
+        "data": "synthetic (seeded SplitMix64 generator, written in HBM)",
+        "config": {
+            "workload": "fused filter+projection: SELECT a, b*2+1, c*d FROM t WHERE b < 50 AND c > 0.5 OR a % 7 = 3",
+            "rows": n,
+            "batch_rows": BATCH_ROWS,
+            "selectivity": want_rows / n,
+            "parallelism": "single GPU",
+        },
+        "kernel_ms": kernel_ms,
+        "hbm_gbs": alg_bytes / (kernel_ms / 1000) / 1e9,
+        "parity_full_size": ok,
+        "selectivity_sweep": {
+            "note": "WHERE b < T AND c > C OR a % M = 3: the query's shape with T, C and M moved to reach "
+                    "about 1%, 10% and 50% (the b threshold alone spans 14.3%-39.4%)",
+            "points": points},
+        "end_to_end": e2e,
+        "autotune": tuned,
+        "roofline": _roofline_hbm(alg_bytes, kernel_ms,
+                                  recorded_traffic("filter_project", rows=n, b_threshold=args.b_threshold)),
     }
-    if result["roofline"]["achieved"]:
-        result["roofline"]["frac"] = result["roofline"]["achieved"] / result["roofline"]["peak"]
-    for i in ("fp32", "bf16", "fp8"):
-        indexes[i].free()
-    print(json.dumps(result), flush=True)
+    del dev
+    prepared.clear()
+    stream.synchronize()
+    if not args.no_cpu_baseline:
+        sample = be.download(stream, datagen.table(be, stream, datagen.filter_table_columns(seed=42),
+                                                   min(n, args.cpu_sample_rows)))
+        result["cpu_baseline"] = filter_cpu_baselines(sample, args.b_threshold)
+        sample = None
+    return result
 
 
-def _kernel_ms(lib, name: str):
-    ms, cnt = C.c_double(), C.c_uint64()
-    lib.oxr_profile_get(name.encode(), C.byref(ms), C.byref(cnt))
-    return ms.value / max(1, cnt.value), int(cnt.value)
-
-
-def bench_aggregate(args):
+# ------------------------------------------------------------- aggregation
+def exec_aggregate(args, be, stream, steps: int, warmup: int) -> dict:
     """SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k over 100M
-    HBM-resident rows (k Int32 U[0,16), v Float64 U[0,8)): one streaming
-    update of the resident group state per step, then the finish."""
+    HBM-resident rows (k Int32 U[0,16), v Float64 U[0,8)): a step is one
+    streaming update of a new group state, then its finish."""
     import numpy as np
+    import pyarrow as pa
 
     from oxidelake_amd import datagen
     from oxidelake_amd._capi import lib
-    from oxidelake_amd.backend import AggState, RocmBackend
+    from oxidelake_amd.backend import AggState
 
-    be = RocmBackend(0)
-    stream = be.create_stream()
     n = args.agg_rows
     dev = datagen.table(be, stream, datagen.groupby_table_columns(seed=42), n)
     spec = datagen.groupby_query_spec()
@@ -347,7 +395,6 @@ def bench_aggregate(args):
         return st.finish(stream)
 
     out = be.download(stream, step())
-    # full-size check against numpy over the whole table
     host = be.download(stream, dev)
     k = host.column(1).to_numpy()
     v = host.column(2).to_numpy()
@@ -359,13 +406,13 @@ def bench_aggregate(args):
     ok = list(keys) == [i for i in range(16) if cnt[i]] and all(
         abs(got[j] - sums[keys[j]]) <= 1e-9 * abs(sums[keys[j]]) for j in range(len(keys)))
     del host, k, v, m
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     stream.synchronize()
     lib.oxr_profile_reset()
     lib.oxr_profile_enable(1)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     stream.synchronize()
     elapsed = time.perf_counter() - t0
@@ -375,36 +422,416 @@ def bench_aggregate(args):
     if not launches:  # the key range is small: the dense kernel ran
         kms, launches = _kernel_ms(lib, "aggregate_dense")
         kernel = "dense"
+    range_ms, _ = _kernel_ms(lib, "key_range")
     lib.oxr_profile_reset()
     cpu = None
     if not args.no_cpu_baseline:
-        # the oracle's aggregation, one core, on the first 2M rows of the
-        # same counter-based table
         from oracle import oracle as O
 
-        sample = be.download(stream, datagen.table(be, stream, datagen.groupby_table_columns(seed=42), 2_000_000))
-        sec = timed_cpu(lambda: O.aggregate(sample, spec))
-        cpu = {"value": sample.num_rows / sec, "unit": "rows/s", "cores": 1, "kind": "port",
-               "sample": f"first {sample.num_rows:,} rows of the same seeded table, same query, best of 5"}
-    e2e = None
-    if args.e2e:
-        e2e = aggregate_end_to_end(be, stream, spec, args.no_cpu_baseline)
-        print(f"# end to end: {e2e}", file=sys.stderr, flush=True)
-    bytes_per_row = 4 + 8  # k, v
-    achieved = n * bytes_per_row / (kms / 1000) / 1e9
-    print(json.dumps({
+        rows = min(n, args.cpu_sample_rows)
+        sample = be.download(stream, datagen.table(be, stream, datagen.groupby_table_columns(seed=42), rows))
+        one, _, reps = timed_cpu(lambda: O.aggregate(sample, spec), budget_s=30)
+        threads = host_threads()
+        step_rows = -(-rows // threads)
+        parts = [sample.slice(i, step_rows) for i in range(0, rows, step_rows)]
+
+        def parallel():
+            # one partial per slice, merged (GroupedAggregator::merge)
+            with ThreadPoolExecutor(threads) as pool:
+                partials = list(pool.map(lambda b: O.aggregate(b, spec), parts))
+            g = O.GroupedAggregator(sample.slice(0, 0), spec)
+            for p_ in partials:
+                g.merge(p_)
+            return g.finish()
+
+        allc, all_med, _ = timed_cpu(parallel)
+        cpu = {"value": rows / allc, "unit": "rows/s", "cores": threads, "kind": "port", "one_thread": rows / one,
+               "median_rows_per_s": rows / all_med,
+               "sample": f"first {rows:,} rows of the same seeded table, same query (value: the oracle on "
+                         f"{threads} threads, one partial per slice then merged; one_thread: one core)"}
+        try:
+            import pyarrow.compute as pc
+
+            t = pa.Table.from_batches([sample])
+
+            def acero():
+                f = t.filter((pc.field("k") >= 2) & (pc.field("v") < 4.0))
+                return f.group_by("k", use_threads=True).aggregate([("v", "sum")])
+
+            best, med, _ = timed_cpu(acero)
+            cpu["arrow_acero"] = {"value": rows / best, "median_rows_per_s": rows / med, "unit": "rows/s",
+                                  "cores": pa.cpu_count(), "groups": acero().num_rows,
+                                  "engine": "Arrow C++ engine (Acero via pyarrow; not DataFusion)"}
+        except Exception as e:  # noqa: BLE001
+            cpu["arrow_acero"] = {"error": f"{type(e).__name__}: {e}"}
+        sample = None
+    alg = n * (4 + 8)  # k, v
+    del dev
+    return {
         "metric": "grouped aggregation rows/s (GpuAggregateExec, HBM-resident, 100M rows)",
-        "value": n / (elapsed / args.steps), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed * 1000 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32/f64",
-        "data": "synthetic (seeded SplitMix64 generator, written in HBM)",
+        "value": n / (elapsed / steps), "unit": "rows/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": elapsed * 1000 / steps, "dtype": "int32/f64",
         "config": {"workload": "SELECT k, SUM(v) FROM t WHERE k >= 2 AND v < 4.0 GROUP BY k", "rows": n,
                    "groups": int(len(keys))},
-        "kernel_ms": kms, "kernel": kernel, "parity_full_size": bool(ok), "cpu_baseline": cpu,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "end_to_end": e2e,
-    }), flush=True)
+        "kernel": kernel, "kernel_ms": kms, "key_range_ms": range_ms, "parity_full_size": bool(ok),
+        "roofline": _roofline_hbm(alg, kms, recorded_traffic("aggregate_" + kernel, rows=n)),
+        "cpu_baseline": cpu,
+    }
+
+
+# ------------------------------------------------------ join + aggregation
+def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
+    """Star join: fact (fk Int64 Zipf(1.1) over the dimension's keys, qty
+    Int32, price Float64) JOIN dim (pk Int64 a permutation, grp Int32 over 1M
+    groups) ON fk = pk, SUM(qty), SUM(qty*price), COUNT(*) GROUP BY grp, all
+    in HBM: the build once, then one fused probe + aggregate per step."""
+    import numpy as np
+    import pyarrow as pa
+
+    from oxidelake_amd import datagen
+    from oxidelake_amd._capi import lib
+    from oxidelake_amd.backend import AggState
+
+    nf, nd, groups = args.fact_rows, args.dim_rows, args.groups
+    fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=42)
+    dfact = datagen.table(be, stream, fact_cols, nf)
+    ddim = datagen.table(be, stream, dim_cols, nd)
+    spec = datagen.star_query_spec()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    table = be.prepare_join_build(stream, ddim, 0)
+    stream.synchronize()
+    build_s = time.perf_counter() - t0
+
+    def step():
+        st = AggState(be, stream, dfact, spec)
+        st.update_join(stream, table, ddim, 1, dfact, 0)
+        return st.finish(stream)
+
+    out = be.download(stream, step())
+    ok = None
+    if check:
+        # full size: integer sums and counts exact, the float sum within 1e-9
+        # of numpy's, per group
+        dim = be.download(stream, ddim)
+        pk, grp = dim.column(0).to_numpy(), dim.column(1).to_numpy()
+        inv = np.empty(nd, np.int64)
+        inv[pk] = np.arange(nd)
+        fk = be.download(stream, dfact).column(0).to_numpy()
+        g = grp[inv[fk]]
+        del fk
+        fact = be.download(stream, dfact)
+        qty = fact.column(1).to_numpy()
+        price = fact.column(2).to_numpy()
+        cnt = np.bincount(g, minlength=groups)
+        sq = np.bincount(g, weights=qty.astype(np.float64), minlength=groups)
+        sqp = np.bincount(g, weights=qty * price, minlength=groups)
+        keys = out.column(0).to_numpy()
+        ok = (np.array_equal(keys, np.nonzero(cnt)[0]) and np.array_equal(out.column(3).to_numpy(), cnt[keys])
+              and np.array_equal(out.column(1).to_numpy(), sq[keys].astype(np.int64))
+              and bool(np.all(np.abs(out.column(2).to_numpy() - sqp[keys]) <= 1e-9 * np.abs(sqp[keys]))))
+        del fact, qty, price, g
+    for _ in range(warmup):
+        step()
+    stream.synchronize()
+    lib.oxr_profile_reset()
+    lib.oxr_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    stream.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.oxr_profile_enable(0)
+    kms, launches = _kernel_ms(lib, "aggregate")
+    spill_ms, spills = _kernel_ms(lib, "aggregate_spill")  # the cold tail's partition + fold passes
+    scatter_ms, _ = _kernel_ms(lib, "aggregate_scatter")
+    fold_ms, _ = _kernel_ms(lib, "aggregate_fold")
+    total_ms = kms + spill_ms * spills / max(1, launches)
+    lib.oxr_profile_reset()
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        dim_host = be.download(stream, ddim)
+        rows = 5_000_000
+        fsample = be.download(stream, datagen.table(be, stream, fact_cols, rows))
+        jspec = type(spec)(4, spec.aggregates)  # grp is column 4 of fact ++ dim
+        one, _, reps = timed_cpu(lambda: O.aggregate(O.hash_join(fsample, dim_host, 0, 0), jspec), reps=3)
+        cpu = {"value": rows / one, "unit": "rows/s", "cores": 1, "kind": "port", "one_thread": rows / one,
+               "sample": f"first {rows:,} fact rows against the whole {nd:,}-row dimension (the oracle's "
+                         f"build + probe + gather + aggregate, one core, best of {reps})"}
+        try:
+            import pyarrow.compute as pc
+
+            ft, dt = pa.Table.from_batches([fsample]), pa.Table.from_batches([dim_host])
+
+            def acero():
+                j = ft.join(dt, keys="fk", right_keys="pk", join_type="inner", use_threads=True)
+                j = j.append_column("qp", pc.multiply(pc.cast(j["qty"], pa.float64()), j["price"]))
+                return j.group_by("grp", use_threads=True).aggregate([("qty", "sum"), ("qp", "sum"),
+                                                                      ([], "count_all")])
+
+            best, med, _ = timed_cpu(acero, reps=3)
+            cpu["value"] = max(cpu["value"], rows / best)
+            cpu["cores"] = pa.cpu_count()
+            cpu["kind"] = "port"
+            cpu["arrow_acero"] = {"value": rows / best, "median_rows_per_s": rows / med, "unit": "rows/s",
+                                  "cores": pa.cpu_count(),
+                                  "engine": "Arrow C++ engine (Acero hash join + group by via pyarrow; "
+                                            "not DataFusion)"}
+            cpu["sample"] += "; value: the faster of that and Acero on every core"
+        except Exception as e:  # noqa: BLE001
+            cpu["arrow_acero"] = {"error": f"{type(e).__name__}: {e}"}
+        del dim_host, fsample
+    alg = nf * (8 + 4 + 8)  # fk, qty, price: the probe side's compulsory bytes
+    del table, dfact, ddim
+    return {
+        "metric": "join + aggregate probe rows/s (GpuHashJoinExec + GpuAggregateExec, HBM-resident)",
+        "value": nf / (elapsed / steps), "unit": "rows/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": elapsed * 1000 / steps, "dtype": "int64/int32/f64",
+        "config": {"workload": "SUM(qty), SUM(qty*price), COUNT(*) FROM fact JOIN dim ON fk = pk GROUP BY grp",
+                   "fact_rows": nf, "dim_rows": nd, "groups": int(out.num_rows)},
+        "kernel_ms": total_ms, "probe_aggregate_ms": kms, "spill_ms": spill_ms if spills else 0.0,
+        "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "build_s": build_s,
+        "parity_full_size": ok,
+        "roofline": _roofline_hbm(alg, total_ms),
+        "cpu_baseline": cpu,
+    }
+
+
+# ------------------------------------------------------------------ vectors
+class _DevPtr:
+    """A device address with a shape, for RocmBackend.wrap."""
+
+    def __init__(self, ptr: int, rows: int, dim: int):
+        self.ptr, self.shape = ptr, (rows, dim)
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+
+def _fetch_rows(be, stream, base_ptr: int, dim: int, rows) -> dict:
+    """FP32 vectors of the given row ids, copied from HBM (runs of nearby rows
+    in one copy each)."""
+    import numpy as np
+
+    from oxidelake_amd import plan
+
+    rows = sorted(set(int(r) for r in rows))
+    out, i = {}, 0
+    while i < len(rows):
+        j = i
+        while j + 1 < len(rows) and rows[j + 1] - rows[i] < 64:
+            j += 1
+        lo, cnt = rows[i], rows[j] - rows[i] + 1
+        view = be.wrap([(_DevPtr(base_ptr + lo * dim * 4, cnt, dim), None, plan.F32_VECTOR, dim)], ["emb"])
+        x = np.asarray(be.download(stream, view).column(0).values.to_numpy(zero_copy_only=False),
+                       np.float32).reshape(cnt, dim)
+        for r in rows[i:j + 1]:
+            out[r] = x[r - lo]
+        i = j + 1
+    return out
+
+
+def certified_top10(be, stream, ix32, base_ptr, dim, queries, k=10, cand=100, eps=1e-4):
+    """The exact FP64 top-k of every query, certified without an FP64 pass
+    over every row. The FP32 path returns each query's `cand` nearest rows by
+    its FP32 scan (every other row scores no better in that scan) with their
+    FP32 distances. Each FP32 distance is within eps of the exact one (the
+    dot product of unit vectors in 768 terms: gamma_768 * |x| |q| < 4.6e-5,
+    Higham, Accuracy and Stability of Numerical Algorithms, 3.1; eps = 1e-4
+    covers it and the norm's and division's roundings). So a row outside the
+    candidates is at least D - 3 eps from the query, D the largest returned
+    FP32 distance; when the k-th smallest FP64 distance among the candidates
+    is below that, the candidates hold the exact top-k (ties broken by row).
+    Returns (rows [nq, k], FP64 distances of every fetched (query, row),
+    certified flags, fetched vectors)."""
+    import numpy as np
+
+    rows32, d32 = ix32.search(stream, queries, cand, cand)
+    vecs = _fetch_rows(be, stream, base_ptr, dim, rows32.reshape(-1))
+    q64 = queries.astype(np.float64)
+    qn = np.linalg.norm(q64, axis=1)
+    truth = np.empty((len(queries), k), np.int64)
+    d64 = {}
+    certified = np.zeros(len(queries), bool)
+    for qi in range(len(queries)):
+        r = rows32[qi]
+        x = np.stack([vecs[int(v)] for v in r]).astype(np.float64)
+        d = 1.0 - (x @ q64[qi]) / (np.linalg.norm(x, axis=1) * qn[qi])
+        for v, dv in zip(r, d):
+            d64[(qi, int(v))] = float(dv)
+        order = np.lexsort((r, d))
+        truth[qi] = r[order[:k]]
+        certified[qi] = d[order[k - 1]] < float(d32[qi].max()) - 3 * eps
+    return truth, d64, certified, vecs
+
+
+def exec_vector(args, be, stream, batch_sizes=(256,), reps_of=None) -> dict:
+    """Embedding search: top-10 over 20M x 768 L2-normalised mixture rows
+    (cosine), plain FP32, BF16 + FP32 rerank of 100, FP8 e4m3 (per-row scale)
+    + FP32 rerank of 1000. QPS and latency p50/p99 per batch size; recall@10
+    and max distance error over all 256 queries against the certified FP64
+    top-10."""
+    import numpy as np
+
+    from oxidelake_amd import datagen, plan
+    from oxidelake_amd._capi import lib
+    from oxidelake_amd.backend import VEC_BF16, VEC_FP8, VEC_FP32, VectorIndex
+
+    reps_of = reps_of or {1: 40, 32: 20, 256: 8}
+    n, dim, qmax, k = args.vec_rows, 768, 256, 10
+    metric = plan.DistanceMetric.Cosine
+    t_start = time.perf_counter()
+    # rows [n, n + 256) of the same counter-based mixture are the queries
+    table = datagen.table(be, stream, datagen.embedding_columns(seed=42, dim=dim), n + qmax)
+    base_ptr = table.column(0).values
+    rows_view = be.wrap([(_DevPtr(base_ptr, n, dim), None, plan.F32_VECTOR, dim)], ["emb"])
+    q_view = be.wrap([(_DevPtr(base_ptr + n * dim * 4, qmax, dim), None, plan.F32_VECTOR, dim)], ["emb"])
+    queries = np.asarray(be.download(stream, q_view).column(0).values.to_numpy(zero_copy_only=False),
+                         np.float32).reshape(qmax, dim)
+    indexes, build_s = {}, {}
+    for name, prec in [("fp32", VEC_FP32), ("bf16", VEC_BF16), ("fp8", VEC_FP8)]:
+        s0 = time.perf_counter()
+        indexes[name] = VectorIndex(stream, rows_view, 0, metric, prec)
+        stream.synchronize()
+        build_s[name] = time.perf_counter() - s0
+    # FP8 e4m3 keeps ~4 significant bits: it needs a deeper rerank than BF16
+    # for recall@10 >= 0.99 (recall_vs_rerank_depth below)
+    paths = [("fp32_exact", "fp32", k), ("bf16_rerank100", "bf16", 100), ("fp8_rerank1000", "fp8", 1000)]
+    results = {}
+    for pname, iname, cand in paths:
+        ix = indexes[iname]
+        per_q = {}
+        for nq in batch_sizes:
+            reps = reps_of[nq]
+            for _ in range(2):
+                ix.search(stream, queries[:nq], k, cand)
+            lat = []
+            lib.oxr_profile_reset()
+            lib.oxr_profile_enable(1)
+            for r in range(reps):
+                qb = np.roll(queries, -((r * nq) % qmax), axis=0)[:nq]
+                s0 = time.perf_counter()
+                ix.search(stream, qb, k, cand)
+                lat.append(time.perf_counter() - s0)
+            lib.oxr_profile_enable(0)
+            dev_ms = {kern: _kernel_ms(lib, kern)[0] * _kernel_ms(lib, kern)[1] / reps
+                      for kern in ("vector_scan", "vector_select", "vector_merge", "vector_rerank")}
+            scan_launch_ms, scan_launches = _kernel_ms(lib, "vector_scan")
+            lib.oxr_profile_reset()
+            lat_ms = np.array(lat) * 1000
+            eb = {"fp32": 4, "bf16": 2, "fp8": 1}[iname]
+            scan = dev_ms["vector_scan"]
+            per_q[str(nq)] = {
+                "qps": nq / (lat_ms.mean() / 1000),
+                "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+                "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+                "device_ms_per_search": dev_ms,
+                "scan_launches_per_search": scan_launches / reps,
+                "scan_tflops": 2.0 * n * dim * nq / (scan / 1000) / 1e12,
+                "scan_gbs": n * dim * eb / (scan / 1000) / 1e9,
+            }
+        results[pname] = per_q
+    # recall@10 and distance error over every query, against the certified
+    # FP64 top-10
+    truth, d64, certified, vecs = certified_top10(be, stream, indexes["fp32"], base_ptr, dim, queries, k)
+    got = {p: indexes[i].search(stream, queries, k, c) for p, i, c in paths}
+    depth = {(i, c): indexes[i].search(stream, queries, k, c)
+             for i in ("bf16", "fp8") for c in (50, 100, 200, 400, 1000)}
+    extra = set()
+    for rows_p, _ in got.values():
+        for qi in range(qmax):
+            extra.update(int(r) for r in rows_p[qi] if (qi, int(r)) not in d64)
+    vecs.update(_fetch_rows(be, stream, base_ptr, dim, extra))
+    q64 = queries.astype(np.float64)
+
+    def dist64(qi, r):
+        if (qi, r) not in d64:
+            x = vecs[r].astype(np.float64)
+            d64[(qi, r)] = float(1.0 - x @ q64[qi] / (np.linalg.norm(x) * np.linalg.norm(q64[qi])))
+        return d64[(qi, r)]
+
+    nc = int(certified.sum())
+    for p, _, _ in paths:
+        rows_p, dist_p = got[p]
+        hit, err = 0, 0.0
+        for qi in range(qmax):
+            hit += len(set(truth[qi].tolist()) & set(rows_p[qi].tolist()))
+            for r, dv in zip(rows_p[qi], dist_p[qi]):
+                err = max(err, abs(float(dv) - dist64(qi, int(r))))
+        results[p]["recall_at_10"] = hit / (qmax * k)
+        results[p]["max_distance_error_vs_fp64"] = err
+    recall_vs_depth = {}
+    for (iname, cand), (rows_p, _) in depth.items():
+        hit = sum(len(set(truth[qi].tolist()) & set(rows_p[qi].tolist())) for qi in range(qmax))
+        recall_vs_depth.setdefault(iname, {})[str(cand)] = hit / (qmax * k)
+    brute = None
+    if args.recall_bruteforce:
+        # the certificate itself, checked: a full FP64 scan of every row for
+        # the first queries, streamed through the host in 1M-row chunks
+        ns = min(args.recall_bruteforce, qmax)
+        qs = q64[:ns]
+        qn = np.linalg.norm(qs, axis=1)
+        best = [np.empty(0, np.int64) for _ in range(ns)]
+        bestd = [np.empty(0) for _ in range(ns)]
+        for c0 in range(0, n, 1_000_000):
+            m = min(1_000_000, n - c0)
+            view = be.wrap([(_DevPtr(base_ptr + c0 * dim * 4, m, dim), None, plan.F32_VECTOR, dim)], ["emb"])
+            x = np.asarray(be.download(stream, view).column(0).values.to_numpy(zero_copy_only=False),
+                           np.float32).reshape(m, dim).astype(np.float64)
+            d = 1.0 - (x @ qs.T) / (np.linalg.norm(x, axis=1)[:, None] * qn[None, :])
+            for qi in range(ns):
+                idx = np.argpartition(d[:, qi], k)[:k]
+                cr = np.concatenate([best[qi], idx + c0])
+                cd = np.concatenate([bestd[qi], d[idx, qi]])
+                o = np.lexsort((cr, cd))[:k]
+                best[qi], bestd[qi] = cr[o], cd[o]
+        brute = {"queries": ns, "equal": all(list(best[qi]) == list(truth[qi]) for qi in range(ns))}
+    # CPU baselines: the oracle's FP64 brute force (one query, one core) over
+    # a bounded sample, and numpy's FP32 GEMM (OpenBLAS, every core) over 256
+    # queries, each scaled to the whole table
+    from oracle import oracle as O
+
+    sample = 200_000
+    view = be.wrap([(_DevPtr(base_ptr, sample, dim), None, plan.F32_VECTOR, dim)], ["emb"])
+    xs = np.asarray(be.download(stream, view).column(0).values.to_numpy(zero_copy_only=False),
+                    np.float32).reshape(sample, dim)
+    one, _, _ = timed_cpu(lambda: O.topk64(xs, None, queries[0], metric, k), reps=3)
+    xn = np.linalg.norm(xs, axis=1)
+
+    def blas():
+        s = (xs @ queries.T) / xn[:, None]
+        np.argpartition(-s, k, axis=0)[:k]
+
+    gemm, _, _ = timed_cpu(blas, reps=3)
+    cpu = {"value": qmax / (gemm * n / sample), "unit": "queries/s", "cores": host_threads(), "kind": "port",
+           "one_thread": 1.0 / (one * n / sample),
+           "sample": f"first {sample:,} rows, scaled to {n:,}: value = numpy FP32 GEMM (BLAS on every core) "
+                     f"scoring all {qmax} queries plus a partial sort per query; one_thread = the oracle's FP64 "
+                     "brute-force top-10 of one query on one core"}
+    good = [(results[p]["256"]["qps"], p) for p, _, _ in paths if results[p]["recall_at_10"] >= 0.99]
+    value, best_path = max(good) if good else (0.0, None)
+    best = results[best_path]["256"] if best_path else {}
+    dtype = {"fp32_exact": "fp32", "bf16_rerank100": "bf16", "fp8_rerank1000": "fp8"}.get(best_path, "bf16")
+    scan_tf = best.get("scan_tflops")
+    for i in ("fp32", "bf16", "fp8"):
+        indexes[i].free()
+    del rows_view, q_view, table
+    return {
+        "metric": "vector top-10 QPS at recall@10 >= 0.99 (GpuVectorDistanceExec, 20M x 768, batches of 256)",
+        "value": value, "unit": "queries/s", "steps": reps_of[256], "warmup": 2,
+        "ms_per_step": best.get("latency_ms_p50"), "dtype": dtype,
+        "config": {"workload": "cosine top-10, 20M x 768 rows, query batches of " +
+                               "/".join(map(str, batch_sizes)), "rows": n, "dim": dim, "best_path": best_path},
+        "paths": results, "index_build_s": build_s,
+        "recall": {"queries": qmax, "certified_queries": nc,
+                   "truth": "FP64 top-10 certified from the FP32 path's 100 candidates per query"},
+        "recall_vs_rerank_depth": recall_vs_depth, "bruteforce_check": brute,
+        "roofline": {"bound": "mfma", "achieved": scan_tf, "peak": MFMA_PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
+                     "frac": (scan_tf / MFMA_PEAK_TFLOPS[dtype]) if scan_tf else None, "traffic": None},
+        "cpu_baseline": cpu, "total_s": time.perf_counter() - t_start,
+    }
 
 
 def aggregate_end_to_end(be, stream, spec, no_cpu: bool) -> dict:
@@ -444,7 +871,7 @@ def aggregate_end_to_end(be, stream, spec, no_cpu: bool) -> dict:
         keys, sums = out.column(0).to_numpy(), out.column(1).to_numpy()
         ok = list(keys) == [i for i in range(16) if m[k == i].any()] and all(
             abs(sums[j] - want[keys[j]]) <= 1e-9 * abs(want[keys[j]]) for j in range(len(keys)))
-        sec = timed_cpu(device_path, reps=3)
+        sec = timed_cpu(device_path, reps=3)[0]
         res = {"rows": n, "batches": st.batches, "rows_per_s": n / sec, "ms": sec * 1e3, "groups": int(out.num_rows),
                "parity": bool(ok), "bytes_h2d": st.bytes_h2d, "fallback_batches": st.fallback_batches,
                "path": "Parquet (8,192-row row groups) -> per-batch upload of k, v -> resident groups -> finish"}
@@ -459,118 +886,10 @@ def aggregate_end_to_end(be, stream, spec, no_cpu: bool) -> dict:
                     g.update(b)
                 return g.finish()
 
-            csec = timed_cpu(cpu_path, reps=3)
+            csec = timed_cpu(cpu_path, reps=3)[0]
             res["cpu_same_path"] = {"rows_per_s": n / csec, "ms": csec * 1e3, "cores": 1, "kind": "port",
                                     "path": "the same batches through the oracle's GroupedAggregator"}
     return res
-
-
-def bench_join(args):
-    """Star join: fact (fk Int64 Zipf(1.1) over the dimension's keys, qty Int32,
-    price Float64) JOIN dim (pk Int64 a permutation, grp Int32 over 1M groups)
-    ON fk = pk, SUM(qty), SUM(qty*price), COUNT(*) GROUP BY grp, all in HBM:
-    the build once, then one fused probe + aggregate pass per step."""
-    import numpy as np
-
-    from oxidelake_amd import datagen
-    from oxidelake_amd._capi import lib
-    from oxidelake_amd.backend import AggState, RocmBackend
-
-    be = RocmBackend(0)
-    stream = be.create_stream()
-    nf, nd, groups = args.fact_rows, args.dim_rows, args.groups
-    fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=42)
-    dfact = datagen.table(be, stream, fact_cols, nf)
-    ddim = datagen.table(be, stream, dim_cols, nd)
-    spec = datagen.star_query_spec()
-    lib.oxr_profile_reset()
-    lib.oxr_profile_enable(1)
-    t0 = time.perf_counter()
-    table = be.prepare_join_build(stream, ddim, 0)
-    stream.synchronize()
-    build_s = time.perf_counter() - t0
-    lib.oxr_profile_enable(0)
-    lib.oxr_profile_reset()
-
-    def step():
-        st = AggState(be, stream, dfact, spec)
-        st.update_join(stream, table, ddim, 1, dfact, 0)
-        return st.finish(stream)
-
-    out = be.download(stream, step())
-    ok = None
-    if args.check:
-        # full size: integer sums and counts exact, the float sum within
-        # 1e-9 of numpy's, per group
-        dim = be.download(stream, ddim)
-        pk, grp = dim.column(0).to_numpy(), dim.column(1).to_numpy()
-        inv = np.empty(nd, np.int64)
-        inv[pk] = np.arange(nd)
-        fk = be.download(stream, dfact).column(0).to_numpy()
-        g = grp[inv[fk]]
-        del fk
-        fact = be.download(stream, dfact)
-        qty = fact.column(1).to_numpy().astype(np.float64)
-        price = fact.column(2).to_numpy()
-        cnt = np.bincount(g, minlength=groups)
-        sq = np.bincount(g, weights=qty, minlength=groups)
-        sqp = np.bincount(g, weights=qty * price, minlength=groups)
-        keys = out.column(0).to_numpy()
-        ok = (np.array_equal(keys, np.nonzero(cnt)[0]) and np.array_equal(out.column(3).to_numpy(), cnt[keys])
-              and np.array_equal(out.column(1).to_numpy(), sq[keys].astype(np.int64))
-              and bool(np.all(np.abs(out.column(2).to_numpy() - sqp[keys]) <= 1e-9 * np.abs(sqp[keys]))))
-        del fact, qty, price, g
-    for _ in range(args.warmup):
-        step()
-    stream.synchronize()
-    lib.oxr_profile_reset()
-    lib.oxr_profile_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    stream.synchronize()
-    elapsed = time.perf_counter() - t0
-    lib.oxr_profile_enable(0)
-    kms, launches = _kernel_ms(lib, "aggregate")
-    spill_ms, spills = _kernel_ms(lib, "aggregate_spill")  # the cold tail's partition + fold passes
-    scatter_ms, _ = _kernel_ms(lib, "aggregate_scatter")
-    fold_ms, _ = _kernel_ms(lib, "aggregate_fold")
-    kms += spill_ms * spills / max(1, launches)
-    lib.oxr_profile_reset()
-    cpu = None
-    if not args.no_cpu_baseline:
-        # the oracle's hash join (build over the whole dimension) and
-        # aggregation, one core, on the first 2M fact rows of the same tables
-        from oracle import oracle as O
-
-        dim_host = be.download(stream, ddim)
-        fsample = be.download(stream, datagen.table(be, stream, fact_cols, 2_000_000))
-        gspec = datagen.star_query_spec()
-        jspec = type(gspec)(4, gspec.aggregates)  # grp is column 4 of fact ++ dim
-
-        def cpu_join():
-            O.aggregate(O.hash_join(fsample, dim_host, 0, 0), jspec)
-
-        sec = timed_cpu(cpu_join, reps=3)
-        cpu = {"value": fsample.num_rows / sec, "unit": "rows/s", "cores": 1, "kind": "port",
-               "sample": f"first {fsample.num_rows:,} fact rows against the whole {nd:,}-row dimension "
-                         "(build + probe + aggregate), best of 3"}
-        del dim_host, fsample
-    bytes_per_row = 8 + 4 + 8  # fk, qty, price
-    achieved = nf * bytes_per_row / (kms / 1000) / 1e9
-    print(json.dumps({
-        "metric": "join + aggregate probe rows/s (GpuHashJoinExec + GpuAggregateExec, HBM-resident)",
-        "value": nf / (elapsed / args.steps), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed * 1000 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int64/int32/f64",
-        "data": "synthetic (seeded SplitMix64 generator: Zipf(1.1) keys, permuted dimension, in HBM)",
-        "config": {"workload": "SUM(qty), SUM(qty*price), COUNT(*) FROM fact JOIN dim ON fk = pk GROUP BY grp",
-                   "fact_rows": nf, "dim_rows": nd, "groups": int(out.num_rows)},
-        "cpu_baseline": cpu, "kernel_ms": kms, "spill_ms": spill_ms if spills else 0.0,
-        "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "build_s": build_s, "parity_full_size": ok,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-    }), flush=True)
 
 
 def bench_serving(args):
@@ -711,6 +1030,19 @@ def recorded_traffic(kernel: str, **workload):
     return rec["bytes_per_launch"]
 
 
+def _line(d: dict, data: str) -> dict:
+    """An exec's result as a line of its own (--workload runs)."""
+    out = {"metric": d["metric"], "value": d["value"], "unit": d["unit"], "n_gpus": 1, "steps": d["steps"],
+           "warmup": d["warmup"], "ms_per_step": d["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": d["dtype"], "data": data}
+    out.update({k: v for k, v in d.items() if k not in out})
+    return out
+
+
+GEN_DATA = "synthetic (seeded SplitMix64 generator, written in HBM)"
+VEC_DATA = "synthetic (seeded Gaussian mixture of 1024 centroids, sigma 0.3, L2-normalised, in HBM)"
+
+
 def main():
     args = parse()
     if args.knob:
@@ -720,177 +1052,56 @@ def main():
             name, _, value = kv.partition("=")
             autotune.set_knob(name, int(value))
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding "
-              "is designed but not built in this tree", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): every box here has one GPU, "
+              "and multi-GPU sharding is designed but not built in this tree", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "vector":
-        bench_vector(args)
-        return
-    if args.workload == "aggregate":
-        bench_aggregate(args)
-        return
-    if args.workload == "join":
-        bench_join(args)
-        return
     if args.workload == "serving":
         bench_serving(args)
         return
     if args.workload == "copy":
         bench_copy(args)
         return
-    from oxidelake_amd import datagen
-    from oxidelake_amd._capi import lib
     from oxidelake_amd.backend import RocmBackend
 
     be = RocmBackend(0)
     stream = be.create_stream()
-    n = args.rows
-    dev = datagen.table(be, stream, datagen.filter_table_columns(seed=42), n)
-
-    prepared = {}
-
-    def run(threshold):
-        # 1M-row batches, each filtered and compacted on its own, one launch;
-        # the plan is prepared once per threshold (as an engine prepares it)
-        if threshold not in prepared:
-            pred, proj = datagen.filter_query(threshold)
-            prepared[threshold] = be.prepare_filter(pred, proj, ["a", "b2", "cd"])
-        return prepared[threshold].run_batches(stream, dev, BATCH_ROWS)
-
-    def rows_of(outs):
-        return sum(o.num_rows for o in outs)
-
-    # full-size parity (count and checksums against numpy over the whole table)
-    host = be.download(stream, dev)
-    outs = run(args.b_threshold)
-    sel = rows_of(outs)
-    import pyarrow as pa
-
-    out_host = pa.Table.from_batches([be.download(stream, o) for o in outs]).combine_chunks().to_batches()[0]
-    ok = full_size_check(host, out_host, args.b_threshold)
-    want_rows = out_host.num_rows
-    outs = out_host = None
-    # the plain drop-in path: host batches of 1M rows uploaded, filtered and
-    # downloaded one by one (the reference operator's shape), on one stream
-    # and on four streams from four host threads
-    e2e = None
-    if args.e2e:
-        from oxidelake_amd.streaming import StreamingFilter
-
-        hb = [host.slice(i, BATCH_ROWS) for i in range(0, host.num_rows, BATCH_ROWS)]
-        pred, proj = datagen.filter_query(args.b_threshold)
-        e2e = {"batches": len(hb), "transfers": "pageable host Arrow buffers, per batch"}
-        for lanes in (1, 4):
-            sf = StreamingFilter(be, pred, proj, ["a", "b2", "cd"], lanes=lanes)
-            sf.run(hb[:4])  # warm
-            t0 = time.perf_counter()
-            res = sf.run(hb)
-            sec = time.perf_counter() - t0
-            got = sum(r.num_rows for r in res)
-            e2e[f"rows_per_s_{lanes}_stream{'s' if lanes > 1 else ''}"] = host.num_rows / sec
-            e2e[f"rows_ok_{lanes}"] = got == want_rows
-            res = None
-        print(f"# end to end: {e2e}", file=sys.stderr, flush=True)
-        hb = None
-    del host
-    print(f"# full-size parity checked: {ok}", file=sys.stderr, flush=True)
-    # the generator is counter-based: these are the full table's first 2M rows
-    sample = be.download(stream, datagen.table(be, stream, datagen.filter_table_columns(seed=42), 2_000_000))
-
-    # launch shape picked by the project autotuner on this device and table
-    from oxidelake_amd import autotune
-
-    if args.filter_rows:
-        autotune.set_knob("filter.rows", args.filter_rows)
-        tuned = {"knobs": {"filter.rows": args.filter_rows}, "ms": None}
-    else:
-        tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
-    print(f"# launch shape: {tuned['knobs']}", file=sys.stderr, flush=True)
-
-    lib.oxr_profile_enable(0)
-    for _ in range(args.warmup):
-        out = run(args.b_threshold)
-        out = None
-    stream.synchronize()
-    lib.oxr_profile_reset()
-    lib.oxr_profile_enable(1)
+    if args.workload == "vector":
+        print(json.dumps(_line(exec_vector(args, be, stream, (1, 32, 256)), VEC_DATA)), flush=True)
+        return
+    if args.workload == "aggregate":
+        d = exec_aggregate(args, be, stream, args.steps, args.warmup)
+        if args.e2e:
+            d["end_to_end"] = aggregate_end_to_end(be, stream, datagen_groupby_spec(), args.no_cpu_baseline)
+        print(json.dumps(_line(d, GEN_DATA)), flush=True)
+        return
+    if args.workload == "join":
+        print(json.dumps(_line(exec_join(args, be, stream, args.steps, args.warmup, args.check), GEN_DATA)),
+              flush=True)
+        return
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = run(args.b_threshold)
-        out = None
-    stream.synchronize()
-    elapsed = time.perf_counter() - t0
-    lib.oxr_profile_enable(0)
-    kms, launches = C.c_double(), C.c_uint64()
-    lib.oxr_profile_get(b"filter_project", C.byref(kms), C.byref(launches))
-    if launches.value != args.steps:
-        raise RuntimeError(f"expected {args.steps} launches of the generated filter kernel, "
-                           f"saw {launches.value}")
-    kernel_ms = kms.value / launches.value
-    lib.oxr_profile_reset()
-
-    ms_per_step = elapsed * 1000 / args.steps
-    alg_bytes = n * IN_BYTES_PER_ROW + sel * OUT_BYTES_PER_ROW
-    achieved = alg_bytes / (kernel_ms / 1000) / 1e9
-    result = {
-        "metric": "filter+projection rows/s (GpuFilterExec, HBM-resident, 200M rows)",
-        "value": n / (ms_per_step / 1000),
-        "unit": "rows/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int64/f32/f64",
-        "data": "synthetic (seeded SplitMix64 generator, written in HBM)",
-        "config": {
-            "workload": "fused filter+projection: SELECT a, b*2+1, c*d FROM t WHERE b < 50 AND c > 0.5 OR a % 7 = 3",
-            "rows": n,
-            "batch_rows": 1_000_000,
-            "selectivity": sel / n,
-            "parallelism": "single GPU",
-        },
-        "kernel_ms": kernel_ms,
-        "hbm_gbs": achieved,
-        "parity_full_size": ok,
-        "end_to_end": e2e,
-        "autotune": tuned,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": recorded_traffic("filter_project", rows=n, b_threshold=args.b_threshold),
-        },
-    }
-    if args.sweep:
-        sweep = {}
-        for thr in (500, 1000):
-            o = run(thr)
-            s = rows_of(o)
-            o = None
+    result = bench_filter(args, be, stream)
+    log(f"filter done in {time.perf_counter() - t0:.1f} s")
+    result["host"] = host_info()
+    if not args.no_execs:
+        execs = {}
+        steps, warm = args.exec_steps, 2
+        for name, fn in [("aggregate", lambda: exec_aggregate(args, be, stream, steps, warm)),
+                         ("join", lambda: exec_join(args, be, stream, steps, warm, check=True)),
+                         ("vector", lambda: exec_vector(args, be, stream, (256,), {256: steps}))]:
+            t1 = time.perf_counter()
+            execs[name] = fn()
             stream.synchronize()
-            lib.oxr_profile_reset()
-            lib.oxr_profile_enable(1)
-            for _ in range(5):
-                o = run(thr)
-                o = None
-            stream.synchronize()
-            lib.oxr_profile_enable(0)
-            lib.oxr_profile_get(b"filter_project", C.byref(kms), C.byref(launches))
-            km = kms.value / max(1, launches.value)
-            lib.oxr_profile_reset()
-            sweep[str(thr)] = {"selectivity": s / n, "kernel_ms": km,
-                               "hbm_gbs": (n * IN_BYTES_PER_ROW + s * OUT_BYTES_PER_ROW) / (km / 1000) / 1e9}
-        result["selectivity_sweep"] = sweep
-    del dev
-    stream.synchronize()
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(sample, args.b_threshold)
+            log(f"{name} done in {time.perf_counter() - t1:.1f} s: {execs[name]['value']:.4g} "
+                f"{execs[name]['unit']}, roofline {execs[name]['roofline'].get('frac')}")
+        result["execs"] = execs
+    result["total_s"] = time.perf_counter() - t0
     print(json.dumps(result), flush=True)
+
+
+def datagen_groupby_spec():
+    from oxidelake_amd import datagen
+
+    return datagen.groupby_query_spec()
 
 
 if __name__ == "__main__":

@@ -124,36 +124,39 @@ int oxr_probe(int ordinal, char *why, size_t why_len);
  * `CudaBackend::new` (cuda/mod.rs:344-365). */
 int oxr_open(int ordinal, oxr_device **out);
 
-/* `GpuBackend::memory_info` (backend.rs:56): total/free HBM bytes. */
+/* `GpuBackend::memory_info` (backend.rs:52): total/free HBM bytes. */
 int oxr_mem_info(oxr_device *dev, uint64_t *total_bytes, uint64_t *free_bytes,
                  int *pinned_supported);
 
 /* Bytes the backend's HBM arena holds (in use + cached for reuse). */
 int oxr_arena_info(oxr_device *dev, uint64_t *in_use, uint64_t *cached);
 
-/* `GpuBackend::alloc_device` (backend.rs:59): zeroed, 256-byte aligned. */
+/* `GpuBackend::alloc_device` (backend.rs:54-55): zeroed, 256-byte aligned
+ * (the trait asks for 128). */
 int oxr_alloc_device(oxr_device *dev, size_t bytes, oxr_buffer **out);
-/* `GpuBackend::alloc_pinned_host` (backend.rs:63): page-locked host memory. */
+/* `GpuBackend::alloc_pinned_host` (backend.rs:57-59): page-locked host
+ * memory (`HostBuffer::class()` is always pinned here). */
 int oxr_alloc_pinned(oxr_device *dev, size_t bytes, oxr_buffer **out);
 void *oxr_buffer_ptr(const oxr_buffer *buf);
 size_t oxr_buffer_len(const oxr_buffer *buf);
 void oxr_buffer_free(oxr_buffer *buf);
 
-/* `GpuBackend::create_stream` (backend.rs:66). Default priority. */
+/* `GpuBackend::create_stream` (backend.rs:62). Default priority. */
 int oxr_stream_create(oxr_device *dev, oxr_stream **out);
 /* Borrows an existing hipStream_t (e.g. a framework's current stream);
  * destroying the wrapper leaves the stream alone. NULL = the null stream. */
 int oxr_stream_wrap(oxr_device *dev, void *hip_stream, oxr_stream **out);
 void oxr_stream_destroy(oxr_stream *s);
-/* `GpuBackend::synchronize` (backend.rs:83). */
+/* `GpuBackend::synchronize` (backend.rs:81). */
 int oxr_stream_sync(oxr_stream *s);
-/* `GpuBackend::copy_h2d` / `copy_d2h` (backend.rs:69-80), async on `s`. */
+/* `GpuBackend::copy_h2d` / `copy_d2h` (backend.rs:65-79), async on `s`: the
+ * host side must stay valid until the stream is synchronized. */
 int oxr_copy_h2d(oxr_stream *s, const void *src, oxr_buffer *dst, size_t bytes);
 int oxr_copy_d2h(oxr_stream *s, const oxr_buffer *src, void *dst, size_t bytes);
 
 /* ------------------------------------------------------------------ batches */
 
-/* `GpuBackend::upload` (backend.rs:86-90): a host RecordBatch exported as an
+/* `GpuBackend::upload` (backend.rs:84-88): a host RecordBatch exported as an
  * Arrow struct array. Columns of a type the device cannot hold, or a vector
  * column with a NULL element in a valid row (columns.rs:211-241), are
  * OXR_EUNSUPPORTED ("device.column_type" / "device.vector_null_element").
@@ -161,7 +164,7 @@ int oxr_copy_d2h(oxr_stream *s, const oxr_buffer *src, void *dst, size_t bytes);
 int oxr_upload(oxr_stream *s, const struct ArrowSchema *schema,
                const struct ArrowArray *array, oxr_batch **out);
 
-/* `GpuBackend::download` (backend.rs:93-97): exports the batch as an Arrow
+/* `GpuBackend::download` (backend.rs:91-95): exports the batch as an Arrow
  * struct array with host buffers; the caller releases both structs. Blocks
  * until the copies are done. */
 int oxr_download(oxr_stream *s, const oxr_batch *batch,
@@ -225,9 +228,15 @@ typedef struct oxr_expr {
   int32_t count;
 } oxr_expr;
 
-/* Capability hook asked before any upload: `GpuBackend::supports_predicate`
- * (backend.rs:182-186). Returns 1 when oxr_filter_project runs it. */
+/* Capability hooks asked before any upload: `GpuBackend::supports_predicate`
+ * (backend.rs:182-185), `supports_hash_join` (:189-191) and
+ * `supports_aggregate` (:195-197). Each returns 1 when the device runs the
+ * operation (oxr_filter_project for that predicate; oxr_join_* and
+ * oxr_aggregate / oxr_agg_*), 0 when every call would be OXR_EUNSUPPORTED.
+ * Host-only: they touch no device. */
 int oxr_supports_predicate(const oxr_expr *predicate);
+int oxr_supports_hash_join(void);
+int oxr_supports_aggregate(void);
 
 /* ------------------------------------------------------- filter + project */
 

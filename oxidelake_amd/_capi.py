@@ -127,6 +127,8 @@ SIGNATURES: dict[str, tuple] = {
     "oxr_batch_free": (None, [P]),
     "oxr_batch_free_many": (None, [PP, U64]),
     "oxr_supports_predicate": (C.c_int, [C.POINTER(Expr)]),
+    "oxr_supports_hash_join": (C.c_int, []),
+    "oxr_supports_aggregate": (C.c_int, []),
     "oxr_filter_project": (
         C.c_int,
         [P, P, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(C.c_char_p), PP],

@@ -217,6 +217,44 @@ class MemoryInfo:
         return max(0, self.total_bytes - self.free_bytes)
 
 
+class DeviceBuffer:
+    """`DeviceBuffer` (crates/oxidelake-device/src/handles.rs): zeroed HBM
+    from the backend's arena, freed when dropped."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    def __len__(self) -> int:
+        return int(lib.oxr_buffer_len(self.handle))
+
+    @property
+    def ptr(self) -> int:
+        return int(lib.oxr_buffer_ptr(self.handle) or 0)
+
+    def free(self) -> None:
+        if self.handle:
+            lib.oxr_buffer_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostBuffer(DeviceBuffer):
+    """`HostBuffer` (crates/oxidelake-memory/src/pinned.rs:24-35): page-locked
+    host memory; `class_` is "pinned" (the class the trait's `class()` reports)."""
+
+    class_ = "pinned"
+
+    def array(self) -> np.ndarray:
+        """The buffer's bytes as a writable numpy view."""
+        n = len(self)
+        return np.ctypeslib.as_array((C.c_uint8 * n).from_address(self.ptr)) if n else np.empty(0, np.uint8)
+
+
 def filter_kernel_source(columns: Sequence[tuple], predicate: ExprT | None, projection: Sequence,
                          compile: bool = False) -> str:
     """The kernel source the filter generates for this query shape over
@@ -295,7 +333,30 @@ class RocmBackend:
         check(lib.oxr_mem_info(self.dev, C.byref(t), C.byref(f), C.byref(p)))
         return MemoryInfo(int(t.value), int(f.value), bool(p.value))
 
-    # -- streams (backend.rs:66, 83)
+    # -- memory (backend.rs:54-79)
+    def alloc_device(self, nbytes: int) -> DeviceBuffer:
+        """backend.rs:54-55: zeroed device memory."""
+        out = C.c_void_p()
+        check(lib.oxr_alloc_device(self.dev, int(nbytes), C.byref(out)))
+        return DeviceBuffer(out.value)
+
+    def alloc_pinned_host(self, nbytes: int) -> HostBuffer:
+        """backend.rs:57-59: page-locked, zeroed host memory."""
+        out = C.c_void_p()
+        check(lib.oxr_alloc_pinned(self.dev, int(nbytes), C.byref(out)))
+        return HostBuffer(out.value)
+
+    def copy_h2d(self, stream: Stream, src: HostBuffer, dst: DeviceBuffer) -> None:
+        """backend.rs:65-71: all of `src` into the start of `dst`, async on
+        `stream` (a copy larger than `dst` is an ExecutionError)."""
+        check(lib.oxr_copy_h2d(stream.handle, src.ptr, dst.handle, len(src)))
+
+    def copy_d2h(self, stream: Stream, src: DeviceBuffer, dst: HostBuffer) -> None:
+        """backend.rs:73-79: the start of `src` into all of `dst`, async on
+        `stream`."""
+        check(lib.oxr_copy_d2h(stream.handle, src.handle, dst.ptr, len(dst)))
+
+    # -- streams (backend.rs:62, 81)
     def create_stream(self) -> Stream:
         s = C.c_void_p()
         check(lib.oxr_stream_create(self.dev, C.byref(s)))
@@ -364,15 +425,17 @@ class RocmBackend:
 
     # -- operators
     def supports_predicate(self, predicate: ExprT | None) -> bool:
-        """backend.rs:182-186."""
+        """backend.rs:182-185."""
         enc = Encoded(predicate)
         return bool(lib.oxr_supports_predicate(enc.ref()))
 
     def supports_hash_join(self) -> bool:
-        return True
+        """backend.rs:189-191."""
+        return bool(lib.oxr_supports_hash_join())
 
     def supports_aggregate(self) -> bool:
-        return True
+        """backend.rs:195-197."""
+        return bool(lib.oxr_supports_aggregate())
 
     def filter_project(self, stream: Stream, batch: DeviceBatch, predicate: ExprT | None,
                        projection: Sequence, names: Optional[Sequence[str]] = None) -> DeviceBatch:

@@ -584,12 +584,15 @@ int key_range(oxr_stream *s, const DevCol &c, uint64_t n, KeyRange *out) {
     const uint64_t grid = std::max<uint64_t>(
         1, std::min<uint64_t>((n / 8 + 255) / 256, (uint64_t)std::max(1, s->dev->cu_count) * 2));
     const dim3 g((unsigned)grid), b(256);
-    if (c.type == OXR_INT32) {
-      if (valid) hipLaunchKernelGGL((key_range_kernel<int32_t, true>), g, b, 0, s->s, (const int32_t *)c.values, valid, n, mm);
-      else hipLaunchKernelGGL((key_range_kernel<int32_t, false>), g, b, 0, s->s, (const int32_t *)c.values, valid, n, mm);
-    } else {
-      if (valid) hipLaunchKernelGGL((key_range_kernel<int64_t, true>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
-      else hipLaunchKernelGGL((key_range_kernel<int64_t, false>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
+    {
+      ProfScope prof(s->s, "key_range");
+      if (c.type == OXR_INT32) {
+        if (valid) hipLaunchKernelGGL((key_range_kernel<int32_t, true>), g, b, 0, s->s, (const int32_t *)c.values, valid, n, mm);
+        else hipLaunchKernelGGL((key_range_kernel<int32_t, false>), g, b, 0, s->s, (const int32_t *)c.values, valid, n, mm);
+      } else {
+        if (valid) hipLaunchKernelGGL((key_range_kernel<int64_t, true>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
+        else hipLaunchKernelGGL((key_range_kernel<int64_t, false>), g, b, 0, s->s, (const int64_t *)c.values, valid, n, mm);
+      }
     }
     hipError_t e = hipGetLastError();
     int64_t *pin = nullptr;

@@ -423,3 +423,9 @@ extern "C" int oxr_supports_predicate(const oxr_expr *predicate) {
   std::string why;
   return oxr::predicate_supported(predicate, &why) ? 1 : 0;
 }
+
+// The device runs every hash join and aggregation the C ABI accepts; what it
+// declines per call (a key type, a size) comes back as OXR_EUNSUPPORTED from
+// that call.
+extern "C" int oxr_supports_hash_join(void) { return 1; }
+extern "C" int oxr_supports_aggregate(void) { return 1; }

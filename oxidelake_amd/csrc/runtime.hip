@@ -475,6 +475,10 @@ size_t oxr_buffer_len(const oxr_buffer *buf) { return buf ? buf->len : 0; }
 
 void oxr_buffer_free(oxr_buffer *buf) {
   if (!buf) return;
+  // a buffer carries no stream: copies into or out of it may still be in
+  // flight on any stream, so its memory goes back only after the device is
+  // idle (buffers are the trait's raw memory, not the operators' path)
+  (void)hipDeviceSynchronize();
   if (buf->pinned) hipHostFree(buf->ptr);
   else buf->dev->arena->release(buf->ptr, nullptr);
   delete buf;

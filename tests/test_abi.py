@@ -63,3 +63,11 @@ def test_tune_set_refuses_unknown_knobs_and_values():
     finally:
         autotune.set_knob("filter.rows", -1)
     assert autotune.get_knob("filter.rows") == -1
+
+
+def test_capability_hooks_are_host_only():
+    # backend.rs:189-197: asked before any upload, so they touch no device
+    from oxidelake_amd._capi import lib
+
+    assert lib.oxr_supports_hash_join() == 1
+    assert lib.oxr_supports_aggregate() == 1

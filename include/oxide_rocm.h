@@ -263,6 +263,34 @@ int oxr_filter_project_batches(oxr_stream *s, const oxr_batch *input, uint64_t b
                                int32_t nproj, const char *const *names, oxr_batch **outs,
                                uint64_t max_batches, uint64_t *nbatches);
 
+/* Bytes of device scratch every oxr_filter_project_noalloc call on a device
+ * uses (the array is per device, held until the process ends). */
+#define OXR_NOALLOC_SCRATCH_BYTES (2u << 20)
+
+/* No counterpart in the reference: oxr_filter_project for callers that
+ * cannot allocate (serving loops and harnesses that forbid device
+ * allocation, code that must not call hipMalloc). The same rows, values and
+ * NULLs as oxr_filter_project, written into the caller's `outputs` (nproj
+ * columns: the type and dim each projection gives, as oxr_filter_project's
+ * result reports them, `len` = capacity >= input rows, values 16-byte
+ * aligned, a validity bitmap (4-byte aligned) when the projection can be
+ * NULL; the first ceil(rows/32) validity words are cleared by the call).
+ * *out_rows = the rows kept. The launch's scratch (look-back words, ticket,
+ * error and count words) is a static device array of
+ * OXR_NOALLOC_SCRATCH_BYTES: one 8-byte word per tile (512 rows on the
+ * interpreter kernel's default shape, 4096 on a generated kernel of 16 rows
+ * per lane) plus 128 bytes; an input needing more is refused (OXR_EPLAN)
+ * before anything runs (about 134M rows at 512-row tiles). No device
+ * memory is allocated, and no code is compiled or loaded: the query's
+ * generated kernel runs when an earlier oxr_filter_project of the same query
+ * shape loaded it, the interpreter kernel built into the library otherwise
+ * (the same results). Calls on one device share its scratch array and take
+ * turns (a lock held until the call's work has finished), so they never
+ * overlap; the call returns when its results are in `outputs`. */
+int oxr_filter_project_noalloc(oxr_stream *s, const oxr_batch *input, const oxr_expr *predicate,
+                               const oxr_expr *projection, int32_t nproj, const oxr_column *outputs,
+                               uint64_t *out_rows);
+
 /* The kernel source oxr_filter_project generates for this query shape (an
  * EXPLAIN of the device plan; no counterpart in the reference, whose CUDA
  * kernels are fixed per operator, cuda/ops.rs:143-373). Only the type, dim

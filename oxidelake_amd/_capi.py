@@ -143,6 +143,10 @@ SIGNATURES: dict[str, tuple] = {
         [P, P, U64, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(C.c_char_p), PP, U64,
          C.POINTER(U64)],
     ),
+    "oxr_filter_project_noalloc": (
+        C.c_int,
+        [P, P, C.POINTER(Expr), C.POINTER(Expr), I32, C.POINTER(Column), C.POINTER(U64)],
+    ),
     "oxr_join_build": (C.c_int, [P, P, I32, PP]),
     "oxr_join_table_info": (C.c_int, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(C.c_int)]),
     "oxr_join_table_free": (None, [P]),

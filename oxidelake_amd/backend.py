@@ -477,6 +477,26 @@ class RocmBackend:
                                              len(exprs), narr, outs, cap, C.byref(nb)))
         return DeviceBatches(outs, nb.value)
 
+    def filter_project_noalloc(self, stream: Stream, batch: DeviceBatch, predicate: ExprT | None,
+                               projection: Sequence, outputs: Sequence[tuple]) -> int:
+        """oxr_filter_project_noalloc: the filter's rows written into caller
+        device memory, `outputs` one (values_ptr, validity_ptr or 0, oxr_type,
+        dim, capacity_rows) per projection; no device allocation inside.
+        Returns the rows kept."""
+        exprs = [plan.Col(p) if isinstance(p, int) else p for p in projection]
+        if len(outputs) != len(exprs):
+            raise ValueError(f"{len(exprs)} projections, {len(outputs)} output columns")
+        cols = (Column * max(1, len(outputs)))()
+        for i, (values, validity, typ, dim, cap) in enumerate(outputs):
+            cols[i].type, cols[i].dim, cols[i].len = typ, dim, cap
+            cols[i].values, cols[i].validity = values or None, validity or None
+        pred = Encoded(predicate)
+        parr, penc = plan.encode_many(exprs)
+        rows = C.c_uint64()
+        check(lib.oxr_filter_project_noalloc(stream.handle, batch.handle, pred.ref(), parr, len(exprs), cols,
+                                             C.byref(rows)))
+        return int(rows.value)
+
     def prepare_join_build(self, stream: Stream, build: DeviceBatch, key: int) -> JoinTable:
         """backend.rs:129-142 (the table is built on the device)."""
         out = C.c_void_p()

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
 
 #include "common.h"
 #include "compact.h"
@@ -276,13 +277,22 @@ using namespace oxr;
 namespace oxr {
 namespace {
 
+// Where a filter's outputs and scratch come from: by default the arena;
+// for the no-allocation entry point the caller's output columns and the
+// static scratch array, and only kernels already loaded.
+struct FilterTarget {
+  const oxr_column *outs = nullptr;  // nproj caller-owned output columns
+  void *scratch = nullptr;           // device scratch of scratch_bytes
+  size_t scratch_bytes = 0;
+};
+
 // Runs the filter over `input` cut into batches of `batch_rows` rows, all in
 // one launch. Output column buffers have the input's capacity; batch i's
 // output rows start at row i * batch_rows of them. counts[i] = its rows.
 int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const oxr_expr *predicate,
                const oxr_expr *projection, int32_t nproj, const char *const *names,
                std::vector<DevCol> *out_cols, std::vector<std::string> *out_names,
-               std::vector<uint64_t> *counts) {
+               std::vector<uint64_t> *counts, const FilterTarget *tgt = nullptr) {
   if (nproj < 0 || nproj > kMaxOuts)
     return fail(OXR_EUNSUPPORTED, "rocm.filter",
                 "at most " + std::to_string(kMaxOuts) + " projections per launch");
@@ -298,7 +308,7 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   JitFilter jf;
   bool use_jit = false;
   if (tune_get("filter.jit", 1) != 0 && input->rows > 0) {
-    const int jrc = jit_filter(s->dev, input->cols, predicate, projection, nproj, vps, &jf);
+    const int jrc = jit_filter(s->dev, input->cols, predicate, projection, nproj, vps, &jf, tgt != nullptr);
     if (jrc == OXR_OK) use_jit = true;
     else if (jrc != OXR_EUNSUPPORTED) return jrc;
   }
@@ -338,18 +348,54 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   P.nb = (int32_t)prog.bprog.size();
   P.nouts = nproj;
 
+  // scratch: tile status words | error flags | ticket | per-batch counts
+  const size_t status_bytes = (size_t)std::max<uint64_t>(P.ntiles, 1) * 8;
+  const size_t count_bytes = (size_t)std::max<uint64_t>(nbatches, 1) * 8;
+  if (tgt && status_bytes + 64 + count_bytes > tgt->scratch_bytes)
+    return fail(OXR_EPLAN, "rocm.filter",
+                std::to_string(P.ntiles) + " tiles need " + std::to_string(status_bytes + 64 + count_bytes) +
+                    " bytes of scratch; the no-allocation scratch holds " + std::to_string(tgt->scratch_bytes));
   auto release_outputs = [&]() {
-    for (auto &c : *out_cols) {
-      stream_release(s, c.values);
-      stream_release(s, c.validity);
-    }
+    if (!tgt)
+      for (auto &c : *out_cols) {
+        stream_release(s, c.values);
+        stream_release(s, c.validity);
+      }
     out_cols->clear();
   };
   for (int i = 0; i < nproj; i++) {
     const ValueProg &v = vps[i];
     DevCol c;
     const int32_t dim = v.col >= 0 ? input->cols[v.col].dim : 0;
-    int rc = alloc_column(s, v.type, dim, n, v.nullable, &c);
+    int rc = OXR_OK;
+    if (tgt) {
+      // the caller's column: the projection's type, room for every input
+      // row, and a validity bitmap when the projection can be NULL (its
+      // words cleared here: the kernels OR the boundary words in)
+      const oxr_column &o = tgt->outs[i];
+      if (o.type != v.type || (v.type == OXR_F32_VECTOR && o.dim != dim))
+        rc = fail(OXR_EPLAN, "rocm.filter",
+                  "output " + std::to_string(i) + " has type " + std::to_string(o.type) + "/" +
+                      std::to_string(o.dim) + ", the projection gives " + std::to_string(v.type) + "/" +
+                      std::to_string(dim));
+      else if (o.len < n)
+        rc = fail(OXR_EPLAN, "rocm.filter",
+                  "output " + std::to_string(i) + " holds " + std::to_string(o.len) + " rows, the input has " +
+                      std::to_string(n));
+      else if ((n && !o.values) || (v.nullable && n && !o.validity))
+        rc = fail(OXR_EPLAN, "rocm.filter", "output " + std::to_string(i) + " lacks a buffer");
+      else if (((uintptr_t)o.values % 16) != 0 || ((uintptr_t)o.validity % 4) != 0)
+        rc = fail(OXR_EPLAN, "rocm.filter", "output values must be 16-byte and validity 4-byte aligned");
+      c.type = v.type;
+      c.dim = dim;
+      c.len = n;
+      c.values = o.values;
+      c.validity = v.nullable ? o.validity : nullptr;
+      c.owned = false;
+      if (rc == OXR_OK && c.validity) rc = fill_zero(s, c.validity, (n + 31) / 32 * 4);
+    } else {
+      rc = alloc_column(s, v.type, dim, n, v.nullable, &c);
+    }
     if (rc != OXR_OK) {
       release_outputs();
       return rc;
@@ -363,11 +409,13 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
     o.dim = dim;
   }
 
-  // scratch: tile status words | error flags | ticket | per-batch counts
-  const size_t status_bytes = (size_t)std::max<uint64_t>(P.ntiles, 1) * 8;
-  const size_t count_bytes = (size_t)std::max<uint64_t>(nbatches, 1) * 8;
   void *scratch = nullptr;
-  int rc = stream_alloc(s, status_bytes + 64 + count_bytes, &scratch);
+  int rc = OXR_OK;
+  if (tgt) {
+    scratch = tgt->scratch;  // its size was checked before any output was touched
+  } else {
+    rc = stream_alloc(s, status_bytes + 64 + count_bytes, &scratch);
+  }
   if (rc != OXR_OK) {
     release_outputs();
     return rc;
@@ -409,10 +457,17 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   }
   uint64_t err = 0;
   counts->assign(nbatches, 0);
+  std::vector<uint64_t> unpinned;
   if (rc == OXR_OK) {
     // one copy into pinned memory: the error word, then the counts at +64
+    // (no-allocation calls: into host memory the call already holds)
     void *host = nullptr;
-    rc = stream_pinned(s, 64 + count_bytes, &host);
+    if (tgt) {
+      unpinned.assign(8 + std::max<uint64_t>(nbatches, 1), 0);
+      host = unpinned.data();
+    } else {
+      rc = stream_pinned(s, 64 + count_bytes, &host);
+    }
     if (rc == OXR_OK) {
       hipError_t e = hipMemcpyAsync(host, P.result, 64 + nbatches * 8, hipMemcpyDeviceToHost, s->s);
       if (e == hipSuccess) e = hipStreamSynchronize(s->s);
@@ -424,7 +479,7 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
       }
     }
   }
-  stream_release(s, scratch);
+  if (!tgt) stream_release(s, scratch);
   if (rc == OXR_OK && err) rc = fail(OXR_EEXECUTION, "rocm.filter", "Divide by zero error");
   if (rc != OXR_OK) release_outputs();
   return rc;
@@ -492,5 +547,55 @@ extern "C" int oxr_filter_project_batches(oxr_stream *s, const oxr_batch *input,
     }
     outs[i] = b;
   }
+  return OXR_OK;
+}
+
+// ------------------------------------------------- no-allocation entry point
+// The scratch of oxr_filter_project_noalloc: tile look-back words, the tile
+// ticket, the error word and the kept-row count, for every call on a device.
+// Each device holds its own copy until the process ends. In host code the
+// name is only a stand-in for the symbol: kernels get the device address
+// hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) uint8_t scratch[OXR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu[kMaxDevices];   // one call at a time per device
+void *g_noalloc_ptr[kMaxDevices] = {};  // scratch's address on each device
+
+}  // namespace
+
+extern "C" int oxr_filter_project_noalloc(oxr_stream *s, const oxr_batch *input, const oxr_expr *predicate,
+                                          const oxr_expr *projection, int32_t nproj, const oxr_column *outputs,
+                                          uint64_t *out_rows) {
+  clear_error();
+  if (!s || !input || !out_rows || (nproj > 0 && (!projection || !outputs)))
+    return fail(OXR_EPLAN, "rocm.filter", "null argument");
+  const int dev = s->dev->ordinal;
+  if (dev < 0 || dev >= kMaxDevices) return fail(OXR_EDEVICE, "rocm.filter", "device ordinal out of range");
+  // held until the call's work has finished: the calls on one device take
+  // turns on its scratch
+  std::lock_guard<std::mutex> lock(g_noalloc_mu[dev]);
+  if (!g_noalloc_ptr[dev]) {
+    int prev = 0;
+    OXR_HIP(hipGetDevice(&prev));
+    if (prev != dev) OXR_HIP(hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != dev) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(no-allocation scratch)");
+    g_noalloc_ptr[dev] = ptr;
+  }
+  FilterTarget tgt;
+  tgt.outs = outputs;
+  tgt.scratch = g_noalloc_ptr[dev];
+  tgt.scratch_bytes = OXR_NOALLOC_SCRATCH_BYTES;
+  std::vector<DevCol> cols;
+  std::vector<std::string> names;
+  std::vector<uint64_t> counts;
+  OXR_TRY(run_filter(s, input, std::max<uint64_t>(input->rows, 1), predicate, projection, nproj, nullptr, &cols,
+                     &names, &counts, &tgt));
+  *out_rows = counts.empty() ? 0 : counts[0];
   return OXR_OK;
 }

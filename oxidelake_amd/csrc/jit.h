@@ -31,10 +31,12 @@ struct JitFilter {
 // Finds or builds the filter kernel of this query shape for `dev`. `vps`
 // are the interpreter's compiled projections (their types and nullability
 // fix the output columns). OXR_EUNSUPPORTED (no error recorded) means the
-// query is outside what generated kernels cover and the interpreter runs it.
+// query is outside what generated kernels cover and the interpreter runs it;
+// with `cached_only` also that its kernel was not compiled before (nothing
+// is compiled or loaded then).
 int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr *pred,
                const oxr_expr *proj, int32_t nproj, const std::vector<ValueProg> &vps,
-               JitFilter *out);
+               JitFilter *out, bool cached_only = false);
 
 // Launches a generated kernel: `grid` blocks of 256 threads.
 int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args);

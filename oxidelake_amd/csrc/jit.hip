@@ -412,7 +412,10 @@ int rtc_compile(const std::string &arch, const std::string &src, std::vector<cha
 }
 
 // The code object for `src` on `dev`, compiled once per process and device.
-int compile(oxr_device *dev, const std::string &src, const char *kernel, hipFunction_t *fn) {
+// With `cached_only`, a source not compiled before is OXR_EUNSUPPORTED (no
+// error recorded): nothing is compiled or loaded.
+int compile(oxr_device *dev, const std::string &src, const char *kernel, hipFunction_t *fn,
+            bool cached_only = false) {
   const std::string arch = dev->arch.substr(0, dev->arch.find(':'));
   const std::string key = arch + "/" + std::to_string(dev->ordinal) + "\n" + src;
   std::lock_guard<std::mutex> lock(g_jit_mu);
@@ -430,6 +433,7 @@ int compile(oxr_device *dev, const std::string &src, const char *kernel, hipFunc
     *fn = h;
     return OXR_OK;
   }
+  if (cached_only) return OXR_EUNSUPPORTED;
   std::vector<char> code;
   OXR_TRY(rtc_compile(arch, src, &code));
   int prev = 0;
@@ -590,11 +594,11 @@ namespace {
 
 int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr *pred,
                const oxr_expr *proj, int32_t nproj, const std::vector<ValueProg> &vps,
-               JitFilter *out) {
+               JitFilter *out, bool cached_only) {
   JitFilter jf;
   const int rc = generate(cols, pred, proj, nproj, vps, &jf);
   if (rc != OXR_OK) return rc;
-  OXR_TRY(compile(dev, jf.source, "oxr_filter_jit", &jf.fn));
+  OXR_TRY(compile(dev, jf.source, "oxr_filter_jit", &jf.fn, cached_only));
   *out = std::move(jf);
   return OXR_OK;
 }

@@ -19,7 +19,7 @@ from ._capi import lib
 CACHE = Path(os.environ.get("OXR_TUNE_CACHE", Path.home() / ".cache" / "oxidelake_amd" / "tune.json"))
 
 FILTER_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (4, 1)]  # interpreter kernel
-FILTER_ROWS = [4, 8, 16]  # generated kernels: rows per lane
+FILTER_ROWS = [8, 16, 24]  # generated kernels: rows per lane
 AGG_GRIDS = [2, 4, 8, 16]
 
 

@@ -446,9 +446,23 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
       A.dim[i] = (*out_cols)[i].dim;
     }
     A.nouts = nproj;
+    A.ntiles = P.ntiles;
     std::copy(jf.lits.begin(), jf.lits.end(), A.lit);
-    ProfScope prof(s->s, "filter_project");
-    rc = jit_launch(jf.fn, (unsigned)P.ntiles, s->s, A);
+    uint64_t grid = P.ntiles;
+    if (jf.persistent) {
+      // as many workgroups as are resident at once (knob filter.grid: per CU)
+      int per_cu = (int)tune_get("filter.grid", 0);
+      if (per_cu <= 0) {
+        const hipError_t e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jf.fn, kJitBlock, 0);
+        if (e != hipSuccess) rc = fail_hip(e, "occupancy of the generated filter kernel");
+        per_cu = std::max(per_cu, 1);
+      }
+      grid = std::min<uint64_t>(P.ntiles, (uint64_t)std::max(1, s->dev->cu_count) * (uint64_t)per_cu);
+    }
+    if (rc == OXR_OK) {
+      ProfScope prof(s->s, "filter_project");
+      rc = jit_launch(jf.fn, (unsigned)grid, s->s, A);
+    }
   } else if (rc == OXR_OK && P.ntiles) {
     ProfScope prof(s->s, "filter_project_vm");
     launch_filter(slot_bucket(P.nslots), shape, dim3((unsigned)P.ntiles), s->s, P);

@@ -47,6 +47,7 @@ struct JitArgs {
   uint32_t *counter;           // tile ticket, zeroed
   uint64_t *result;            // [0]: error flags (1 = division by zero)
   uint64_t *counts;            // kept rows per batch
+  uint64_t ntiles;             // tiles of the launch (the persistent kernel's ticket bound)
   const void *col[kJitCols];   // input column values
   const uint32_t *valid[kJitCols];
   void *out[kJitOuts];         // output column values
@@ -209,6 +210,167 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
       }
     }
   }
+}
+
+// Persistent form of filter_tile: a grid of about as many workgroups as fit
+// on the device at once, each taking tiles by ticket until none is left.
+// Two register sets of S rows: the next tile's loads are issued before the
+// current tile's predicate, look-back and stores, so a workgroup keeps
+// loads in flight while it compacts (one tile per workgroup launch leaves
+// the loads and the compaction of the resident workgroups in step). A
+// workgroup takes the ticket of its next tile when it starts the current
+// one, so the tiles compacted at any time are one round of tickets and a
+// tile's look-back mostly waits on tiles compacted beside it (tickets taken
+// two at a time up front chained every tile to a tile its owner had not
+// started: 3x slower). A look-back waits only on tiles with smaller
+// tickets, each held by a running workgroup that reaches it without waiting
+// on a later tile, so every wait ends.
+template <class Q, int S>
+__device__ __forceinline__ void filter_persistent(const JitArgs &p) {
+  static_assert(S >= 1 && S <= kMaxRowsPerLane, "rows per lane: one kept bit per row in a 32-bit register");
+  constexpr int NC = S * 4;
+  constexpr int PL = (NC + 63) / 64;
+  constexpr int kTile = kJitBlock * S;
+  constexpr int kWords = kTile / 32 + 2;
+  constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
+  __shared__ uint32_t s_tile, s_next;
+  __shared__ uint32_t s_total;
+  __shared__ uint32_t s_cnt[NC];
+  __shared__ uint32_t s_off[NC];
+  __shared__ uint64_t s_mask[NC];
+  __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_bits[kNV * kWords];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(p.counter, 1u);
+  if (Q::kValidOuts > 0)
+    for (int i = tid; i < kNV * kWords; i += kJitBlock) s_bits[i] = 0;
+  __syncthreads();
+  uint64_t cur = s_tile;
+  if (cur >= p.ntiles) return;  // uniform: no tile for this workgroup
+
+  // the rows [base, base + lim) of tile t
+  auto tile_rows = [&](uint64_t t, uint64_t &base, uint32_t &lim) {
+    const uint64_t batch = t / p.tiles_per_batch;
+    const uint64_t lt = t - batch * p.tiles_per_batch;
+    const uint64_t b0 = batch * p.batch_rows;
+    const uint64_t bend = b0 + p.batch_rows;
+    const uint64_t end = p.n < bend ? p.n : bend;
+    base = b0 + lt * (uint64_t)kTile;
+    lim = end > base ? (uint32_t)(end - base < (uint64_t)kTile ? end - base : kTile) : 0u;
+  };
+  auto load_tile = [&](uint64_t t, typename Q::Row (&x)[S]) {
+    uint64_t base;
+    uint32_t lim;
+    tile_rows(t, base, lim);
+    const typename Q::In in_cols = Q::ins(p, base);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+  };
+  // takes the next tile's ticket, issues its loads into xn, compacts tile t
+  // from rows x; returns the next tile. Past the last tile the loads read
+  // tile t again (in L2 by now): unconditional loads let the compiler wait
+  // on x's loads alone while xn's stay in flight.
+  bool err = false;
+  auto step = [&](uint64_t t, typename Q::Row (&x)[S], typename Q::Row (&xn)[S]) {
+    if (tid == 0) s_next = atomicAdd(p.counter, 1u);
+    __syncthreads();  // the ticket visible; the previous tile's reads of LDS done
+    const uint64_t next = s_next;
+    load_tile(next < p.ntiles ? next : t, xn);
+    uint64_t base;
+    uint32_t lim;
+    tile_rows(t, base, lim);
+    const uint64_t batch = t / p.tiles_per_batch;
+    const uint64_t lt = t - batch * p.tiles_per_batch;
+    const uint64_t b0 = batch * p.batch_rows;
+    uint32_t selbits = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+      const bool in = j < lim;
+      const bool k = Q::pred(p.lit, x[s], j, in, err) && in;
+      selbits |= (uint32_t)k << s;
+      const uint64_t mk = __ballot(k);
+      if (lane == 0) {
+        s_cnt[s * 4 + wave] = (uint32_t)__popcll(mk);
+        s_mask[s * 4 + wave] = mk;
+      }
+    }
+    __syncthreads();  // ballots in LDS
+    if (wave == 0) {
+      uint32_t v[PL], sum = 0;
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        const int c = lane * PL + k;
+        v[k] = c < NC ? s_cnt[c] : 0u;
+        sum += v[k];
+      }
+      const uint32_t incl = wave_incl_scan(sum, lane);
+      uint32_t run = incl - sum;
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        const int c = lane * PL + k;
+        if (c < NC) s_off[c] = run;
+        run += v[k];
+      }
+      const uint32_t tot = __shfl(incl, 63, 64);
+      const uint64_t pre = lookback(p.status, t, t - lt, tot, lane);
+      if (lane == 0) {
+        s_total = tot;
+        s_prefix = b0 + pre;
+        if (lt == p.tiles_per_batch - 1) p.counts[batch] = pre + tot;
+      }
+    }
+    __syncthreads();  // offsets and prefix visible
+    const uint64_t prefix = s_prefix;
+    const uint64_t w0 = prefix >> 5;
+    const uint32_t bit0 = (uint32_t)(prefix & 31);
+    const uint64_t below = (1ull << lane) - 1;
+    const typename Q::Out o = Q::outs(p, base, prefix);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+      const bool k = (selbits >> s) & 1u;
+      const uint32_t q = s_off[s * 4 + wave] + (uint32_t)__popcll(s_mask[s * 4 + wave] & below);
+      if (k) Q::project(p.lit, o, x[s], j, q, bit0 + q, s_bits, kWords, err);
+      if (Q::kVecOuts > 0) Q::vectors(o, k, j, q, lane);
+    }
+    const uint32_t tot = s_total;
+    __syncthreads();  // every output bit set; every read of s_off, s_mask, s_total done
+    if (Q::kValidOuts > 0 && tot > 0) {
+      const uint64_t last = (prefix + tot - 1) >> 5;
+      const int nw = (int)(last - w0 + 1);
+      for (int k = 0; k < p.nouts; k++) {
+        const int v = p.vslot[k];
+        if (v < 0) continue;
+        uint32_t *g = p.outv[k];
+        for (int w = tid; w < nw; w += kJitBlock) {
+          const uint32_t bits = s_bits[v * kWords + w];
+          s_bits[v * kWords + w] = 0;  // clear for the next tile (read only by this thread)
+          if (w == 0 || w == nw - 1) {
+            if (bits) atomicOr(&g[w0 + w], bits);
+          } else {
+            g[w0 + w] = bits;
+          }
+        }
+      }
+    }
+    return next;
+  };
+  typename Q::Row xa[S], xb[S];
+  load_tile(cur, xa);
+  while (true) {
+    cur = step(cur, xa, xb);
+    if (cur >= p.ntiles) break;
+    cur = step(cur, xb, xa);
+    if (cur >= p.ntiles) break;
+  }
+  if (err) atomicOr((unsigned long long *)&p.result[0], 1ull);
 }
 
 }  // namespace oxr

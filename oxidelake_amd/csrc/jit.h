@@ -23,6 +23,7 @@ namespace oxr {
 struct JitFilter {
   hipFunction_t fn = nullptr;
   int rows_per_lane = 0;       // S: a tile is 256 * S rows
+  bool persistent = false;     // filter_persistent: a grid that fits at once, tiles by ticket
   std::vector<int32_t> vslot;  // JitArgs::vslot per projection
   std::vector<uint64_t> lits;  // JitArgs::lit (prepared mode, knob jit.params)
   std::string source;

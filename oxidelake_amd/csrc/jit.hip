@@ -345,7 +345,11 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   std::string in_fields, in_init, row, load;
   int vgprs = 0;
   row_code(cols, val_used, valid_used, &in_fields, &in_init, &row, &load, &vgprs);
-  jf->rows_per_lane = pick_rows(std::max(vgprs, 1));
+  // the persistent kernel (knob filter.persistent, default off: 3x slower
+  // on the 200M-row query, its look-backs walk back through whole rounds of
+  // tiles compacted in step) holds two tiles' rows: half the rows per lane
+  jf->persistent = tune_get("filter.persistent", 0) > 0;
+  jf->rows_per_lane = pick_rows(std::max(vgprs, 1) * (jf->persistent ? 2 : 1));
   std::string src;
   src += "// generated filter + projection kernel\n";
   src += "#include \"filter_jit.h\"\nusing namespace oxr;\nnamespace {\nstruct Q {\n";
@@ -370,7 +374,8 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
          vec_code + "  }\n";
   src += "};\n}  // namespace\n";
   src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_filter_jit(const JitArgs p) {\n";
-  src += "  filter_tile<Q, " + std::to_string(jf->rows_per_lane) + ">(p);\n}\n";
+  src += std::string("  ") + (jf->persistent ? "filter_persistent" : "filter_tile") + "<Q, " +
+         std::to_string(jf->rows_per_lane) + ">(p);\n}\n";
   jf->source = src;
   return OXR_OK;
 }

@@ -279,6 +279,8 @@ constexpr uint32_t kBool = 0x3u;
 const KnobSpec kKnobs[] = {
     {"filter.jit", 0, 1, kBool},
     {"filter.rows", 1, 32, 0},               // generated filter: rows per lane
+    {"filter.persistent", 0, 1, kBool},      // generated filter: persistent grid, tiles by ticket
+    {"filter.grid", 1, 16, 0},               // persistent filter: workgroups per CU (default: what fits)
     {"filter.v", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
     {"filter.g", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
     {"jit.params", 0, 1, kBool},

@@ -20,14 +20,19 @@ from tests.util import assert_batches_equal
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["generated", "prepared", "interpreter"])
+@pytest.fixture(params=["generated", "prepared", "persistent", "interpreter"])
 def path(request):
-    # prepared: the generated kernel with literals passed as arguments
+    # generated: the generated kernel, one tile per workgroup launch;
+    # prepared: the same with literals passed as arguments; persistent: the
+    # persistent generated kernel (tiles by ticket, the next tile's loads in
+    # flight)
     autotune.set_knob("filter.jit", 0 if request.param == "interpreter" else 1)
     autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
+    autotune.set_knob("filter.persistent", 1 if request.param == "persistent" else 0)
     yield request.param
     autotune.set_knob("filter.jit", -1)
     autotune.set_knob("jit.params", -1)
+    autotune.set_knob("filter.persistent", -1)
 
 
 @pytest.fixture
@@ -235,11 +240,13 @@ def test_every_launch_shape_is_bit_exact(backend, stream, oracle, shape):
         autotune.set_knob("filter.jit", -1)
 
 
-@pytest.mark.parametrize("rows", [1, 2, 4, 8, 16, 20, 24, 32])
-def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows):
+@pytest.mark.parametrize("persistent", [1, 0])
+@pytest.mark.parametrize("rows", [1, 2, 4, 6, 8, 16, 20, 24, 32])
+def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows, persistent):
     # generated kernels with S rows per lane (tiles of 256 * S rows); above
     # 16 the tile scan covers two (step, wave) counts per lane
     autotune.set_knob("filter.rows", rows)
+    autotune.set_knob("filter.persistent", persistent)
     try:
         rng = np.random.default_rng(78)
         for n in (1, 1000, 9001):
@@ -256,6 +263,29 @@ def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows):
             assert_batches_equal(backend.download(stream, o), want, names=False)
     finally:
         autotune.set_knob("filter.rows", -1)
+        autotune.set_knob("filter.persistent", -1)
+
+
+@pytest.mark.parametrize("rows,batch_rows", [(4, 65_536), (8, 1_000_000), (2, 4_096)])
+def test_persistent_grid_takes_many_tiles_per_workgroup(backend, stream, oracle, rows, batch_rows):
+    # one workgroup per CU: every workgroup compacts several tiles in turn,
+    # across batch boundaries, with the next tile's loads in flight
+    autotune.set_knob("filter.rows", rows)
+    autotune.set_knob("filter.persistent", 1)
+    autotune.set_knob("filter.grid", 1)
+    try:
+        n = 2_000_003
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=11), n)
+        host = backend.download(stream, dev)
+        pred, proj = datagen.filter_query(300)
+        outs = backend.filter_project_batches(stream, dev, batch_rows, pred, proj)
+        assert len(outs) == -(-n // batch_rows)
+        for i, o in enumerate(outs):
+            want = oracle.filter_project(host.slice(i * batch_rows, batch_rows), pred, proj)
+            assert_batches_equal(backend.download(stream, o), want, names=False)
+    finally:
+        for k in ("filter.rows", "filter.persistent", "filter.grid"):
+            autotune.set_knob(k, -1)
 
 
 def test_generated_kernel_is_the_one_that_runs(backend, stream):

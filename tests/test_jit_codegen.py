@@ -38,8 +38,14 @@ def test_filter_query_kernel_shape():
     autotune.set_knob("filter.rows", -1)
     pred, proj = datagen.filter_query()
     src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
-    # four 1-2 register columns plus one validity word: 16 rows per lane
+    # four 1-2 register columns plus one validity word: 16 rows per lane (8
+    # on the persistent kernel, which holds two tiles' rows)
     assert "filter_tile<Q, 16>" in src
+    autotune.set_knob("filter.persistent", 1)
+    try:
+        assert "filter_persistent<Q, 8>" in backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
+    finally:
+        autotune.set_knob("filter.persistent", -1)
     # a % 7 by a constant divisor (the compiler's multiply-high), no error path
     assert "(x.c0 % ((int64_t)0x0000000000000007ull))" in src
     # literals are constants: b < 50, then c > 0.5 by its bit pattern
@@ -74,12 +80,17 @@ def test_vector_projection_is_a_wave_copy():
     assert "copy_vector_rows" in src and "kVecOuts = 1;" in src
 
 
-@pytest.mark.parametrize("rows", [1, 2, 8, 16])
-def test_every_rows_per_lane_compiles(rows):
+@pytest.mark.parametrize("persistent", [1, 0])
+@pytest.mark.parametrize("rows", [1, 2, 8, 16, 24])
+def test_every_rows_per_lane_compiles(rows, persistent):
     autotune.set_knob("filter.rows", rows)
-    pred, proj = datagen.filter_query()
-    src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
-    assert f"filter_tile<Q, {rows}>" in src
+    autotune.set_knob("filter.persistent", persistent)
+    try:
+        pred, proj = datagen.filter_query()
+        src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
+        assert f"{'filter_persistent' if persistent else 'filter_tile'}<Q, {rows}>" in src
+    finally:
+        autotune.set_knob("filter.persistent", -1)
 
 
 def test_plan_errors_match_the_interpreter():

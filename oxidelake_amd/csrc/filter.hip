@@ -317,7 +317,7 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   FilterShape shape{1, 2};
   if (!use_jit) OXR_TRY(filter_shape(&shape));
   const uint64_t tile_rows =
-      use_jit ? (uint64_t)kJitBlock * jf.rows_per_lane : (uint64_t)kBlock * shape.v * shape.g;
+      use_jit ? (uint64_t)64 * jf.waves * jf.rows_per_lane : (uint64_t)kBlock * shape.v * shape.g;
   const uint64_t nbatches = n ? (n + batch_rows - 1) / batch_rows : 0;
   std::unique_ptr<Params> guard(new Params());
   Params &P = *guard;
@@ -453,7 +453,7 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
       // as many workgroups as are resident at once (knob filter.grid: per CU)
       int per_cu = (int)tune_get("filter.grid", 0);
       if (per_cu <= 0) {
-        const hipError_t e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jf.fn, kJitBlock, 0);
+        const hipError_t e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jf.fn, 64 * jf.waves, 0);
         if (e != hipSuccess) rc = fail_hip(e, "occupancy of the generated filter kernel");
         per_cu = std::max(per_cu, 1);
       }
@@ -461,7 +461,7 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
     }
     if (rc == OXR_OK) {
       ProfScope prof(s->s, "filter_project");
-      rc = jit_launch(jf.fn, (unsigned)grid, s->s, A);
+      rc = jit_launch(jf.fn, (unsigned)grid, s->s, A, (unsigned)(64 * jf.waves));
     }
   } else if (rc == OXR_OK && P.ntiles) {
     ProfScope prof(s->s, "filter_project_vm");

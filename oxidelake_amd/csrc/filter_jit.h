@@ -22,8 +22,8 @@
 //   kValidOuts, kVecOuts           outputs with a validity bitmap / vectors
 // Semantics are those of the interpreter kernel in filter.hip
 // (crates/oxidelake-device/src/cpu/kernels.rs:99-152); this skeleton does
-// the compaction: a tile is 256 threads x S rows, row s*256 + tid of the tile
-// on thread tid (every load instruction of a wave reads one contiguous run),
+// the compaction: a tile is B = 64 W threads x S rows, row s*B + tid of the
+// tile on thread tid (every load instruction of a wave reads one contiguous run),
 // wave ballots rank the kept rows, and a decoupled look-back (compact.h)
 // places the tile inside its batch. Compiles under hipcc and hipRTC.
 #pragma once
@@ -58,6 +58,22 @@ struct JitArgs {
   uint64_t lit[kJitLits];      // prepared mode: literal bits, in each literal's type
 };
 
+// A column value read once by a generated kernel: nontemporal, so the
+// stream does not displace what the caches hold (ld_plain: an ordinary load).
+template <typename T>
+__device__ inline T ld_stream(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ inline T ld_plain(const T *p) {
+  return *p;
+}
+// A projection value written once (knob jit.stream_stores): nontemporal.
+template <typename T>
+__device__ inline void st_stream(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // Validity bit of tile row j (tiles start on 32-row boundaries).
 __device__ inline bool row_bit(uint32_t w, uint32_t j) { return (w >> (j & 31)) & 1u; }
 
@@ -86,21 +102,23 @@ __device__ inline void copy_vector_rows(const float *src, float *dst, int dim, b
 // one 32-bit register.
 constexpr int kMaxRowsPerLane = 32;
 
-template <class Q, int S>
+template <class Q, int S, int W = 4>
 __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   static_assert(S >= 1 && S <= kMaxRowsPerLane, "rows per lane: one kept bit per row in a 32-bit register");
-  // The tile scan runs on one wave over the S * 4 (step, wave) counts, PL
-  // consecutive counts per lane (PL = 1 up to 16 rows per lane).
-  constexpr int NC = S * 4;
+  static_assert(W >= 1 && W <= 16, "waves per workgroup");
+  constexpr int B = 64 * W;  // threads per workgroup
+  // The tile scan runs on one wave over the S * W (step, wave) counts, PL
+  // consecutive counts per lane (PL = 1 while S * W <= 64).
+  constexpr int NC = S * W;
   constexpr int PL = (NC + 63) / 64;
-  constexpr int kTile = kJitBlock * S;
+  constexpr int kTile = B * S;
   constexpr int kWords = kTile / 32 + 2;  // output validity words a tile can touch
   constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_total;
-  __shared__ uint32_t s_cnt[S * 4];
-  __shared__ uint32_t s_off[S * 4];
-  __shared__ uint64_t s_mask[S * 4];  // each (step, wave)'s ballot of kept rows
+  __shared__ uint32_t s_cnt[S * W];
+  __shared__ uint32_t s_off[S * W];
+  __shared__ uint64_t s_mask[S * W];  // each (step, wave)'s ballot of kept rows
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_bits[kNV * kWords];
 
@@ -109,7 +127,7 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   const int wave = tid >> 6;
   if (tid == 0) s_tile = atomicAdd(p.counter, 1u);
   if (Q::kValidOuts > 0)
-    for (int i = tid; i < kNV * kWords; i += kJitBlock) s_bits[i] = 0;
+    for (int i = tid; i < kNV * kWords; i += B) s_bits[i] = 0;
   __syncthreads();
   const uint64_t tile = s_tile;  // ticket order: a batch's tiles in row order
   const uint64_t batch = tile / p.tiles_per_batch;
@@ -126,7 +144,7 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   typename Q::Row x[S];
 #pragma unroll
   for (int s = 0; s < S; s++) {
-    const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+    const uint32_t j = (uint32_t)(s * B + tid);
     Q::load(in_cols, j, j < lim, x[s]);
   }
   // 2. predicate and ranks. The kept flags live as bits of one register and
@@ -136,14 +154,14 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   uint32_t selbits = 0;
 #pragma unroll
   for (int s = 0; s < S; s++) {
-    const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+    const uint32_t j = (uint32_t)(s * B + tid);
     const bool in = j < lim;
     const bool k = Q::pred(p.lit, x[s], j, in, err) && in;
     selbits |= (uint32_t)k << s;
     const uint64_t mk = __ballot(k);
     if (lane == 0) {
-      s_cnt[s * 4 + wave] = (uint32_t)__popcll(mk);
-      s_mask[s * 4 + wave] = mk;
+      s_cnt[s * W + wave] = (uint32_t)__popcll(mk);
+      s_mask[s * W + wave] = mk;
     }
   }
   __syncthreads();
@@ -182,9 +200,9 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   const typename Q::Out o = Q::outs(p, base, prefix);
 #pragma unroll
   for (int s = 0; s < S; s++) {
-    const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+    const uint32_t j = (uint32_t)(s * B + tid);
     const bool k = (selbits >> s) & 1u;
-    const uint32_t q = s_off[s * 4 + wave] + (uint32_t)__popcll(s_mask[s * 4 + wave] & below);
+    const uint32_t q = s_off[s * W + wave] + (uint32_t)__popcll(s_mask[s * W + wave] & below);
     if (k) Q::project(p.lit, o, x[s], j, q, bit0 + q, s_bits, kWords, err);
     if (Q::kVecOuts > 0) Q::vectors(o, k, j, q, lane);
   }
@@ -201,7 +219,7 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
     const int v = p.vslot[k];
     if (v < 0) continue;
     uint32_t *g = p.outv[k];
-    for (int w = tid; w < nw; w += kJitBlock) {
+    for (int w = tid; w < nw; w += B) {
       const uint32_t bits = s_bits[v * kWords + w];
       if (w == 0 || w == nw - 1) {
         if (bits) atomicOr(&g[w0 + w], bits);
@@ -225,12 +243,14 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
 // started: 3x slower). A look-back waits only on tiles with smaller
 // tickets, each held by a running workgroup that reaches it without waiting
 // on a later tile, so every wait ends.
-template <class Q, int S>
+template <class Q, int S, int W = 4>
 __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
   static_assert(S >= 1 && S <= kMaxRowsPerLane, "rows per lane: one kept bit per row in a 32-bit register");
-  constexpr int NC = S * 4;
+  static_assert(W >= 1 && W <= 16, "waves per workgroup");
+  constexpr int B = 64 * W;  // threads per workgroup
+  constexpr int NC = S * W;
   constexpr int PL = (NC + 63) / 64;
-  constexpr int kTile = kJitBlock * S;
+  constexpr int kTile = B * S;
   constexpr int kWords = kTile / 32 + 2;
   constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
   __shared__ uint32_t s_tile, s_next;
@@ -246,7 +266,7 @@ __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
   const int wave = tid >> 6;
   if (tid == 0) s_tile = atomicAdd(p.counter, 1u);
   if (Q::kValidOuts > 0)
-    for (int i = tid; i < kNV * kWords; i += kJitBlock) s_bits[i] = 0;
+    for (int i = tid; i < kNV * kWords; i += B) s_bits[i] = 0;
   __syncthreads();
   uint64_t cur = s_tile;
   if (cur >= p.ntiles) return;  // uniform: no tile for this workgroup
@@ -268,7 +288,7 @@ __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
     const typename Q::In in_cols = Q::ins(p, base);
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+      const uint32_t j = (uint32_t)(s * B + tid);
       Q::load(in_cols, j, j < lim, x[s]);
     }
   };
@@ -291,14 +311,14 @@ __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
     uint32_t selbits = 0;
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+      const uint32_t j = (uint32_t)(s * B + tid);
       const bool in = j < lim;
       const bool k = Q::pred(p.lit, x[s], j, in, err) && in;
       selbits |= (uint32_t)k << s;
       const uint64_t mk = __ballot(k);
       if (lane == 0) {
-        s_cnt[s * 4 + wave] = (uint32_t)__popcll(mk);
-        s_mask[s * 4 + wave] = mk;
+        s_cnt[s * W + wave] = (uint32_t)__popcll(mk);
+        s_mask[s * W + wave] = mk;
       }
     }
     __syncthreads();  // ballots in LDS
@@ -334,9 +354,9 @@ __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
     const typename Q::Out o = Q::outs(p, base, prefix);
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      const uint32_t j = (uint32_t)(s * kJitBlock + tid);
+      const uint32_t j = (uint32_t)(s * B + tid);
       const bool k = (selbits >> s) & 1u;
-      const uint32_t q = s_off[s * 4 + wave] + (uint32_t)__popcll(s_mask[s * 4 + wave] & below);
+      const uint32_t q = s_off[s * W + wave] + (uint32_t)__popcll(s_mask[s * W + wave] & below);
       if (k) Q::project(p.lit, o, x[s], j, q, bit0 + q, s_bits, kWords, err);
       if (Q::kVecOuts > 0) Q::vectors(o, k, j, q, lane);
     }
@@ -349,7 +369,7 @@ __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
         const int v = p.vslot[k];
         if (v < 0) continue;
         uint32_t *g = p.outv[k];
-        for (int w = tid; w < nw; w += kJitBlock) {
+        for (int w = tid; w < nw; w += B) {
           const uint32_t bits = s_bits[v * kWords + w];
           s_bits[v * kWords + w] = 0;  // clear for the next tile (read only by this thread)
           if (w == 0 || w == nw - 1) {

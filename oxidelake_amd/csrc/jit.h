@@ -24,6 +24,7 @@ struct JitFilter {
   hipFunction_t fn = nullptr;
   int rows_per_lane = 0;       // S: a tile is 256 * S rows
   bool persistent = false;     // filter_persistent: a grid that fits at once, tiles by ticket
+  int waves = 4;               // W: a workgroup is 64 * W threads
   std::vector<int32_t> vslot;  // JitArgs::vslot per projection
   std::vector<uint64_t> lits;  // JitArgs::lit (prepared mode, knob jit.params)
   std::string source;
@@ -39,9 +40,10 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
                const oxr_expr *proj, int32_t nproj, const std::vector<ValueProg> &vps,
                JitFilter *out, bool cached_only = false);
 
-// Launches a generated kernel: `grid` blocks of 256 threads.
-int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args);
-int jit_launch_raw(hipFunction_t fn, unsigned grid, hipStream_t s, void *args, size_t size);
+// Launches a generated kernel: `grid` blocks of `block` threads.
+int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args, unsigned block = kJitBlock);
+int jit_launch_raw(hipFunction_t fn, unsigned grid, hipStream_t s, void *args, size_t size,
+                   unsigned block = kJitBlock);
 
 // An aggregation shape: the fused filter, the key (a column of the input,
 // -1 for one global group, or with `join` the probe key column whose match

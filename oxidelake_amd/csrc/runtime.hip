@@ -281,9 +281,12 @@ const KnobSpec kKnobs[] = {
     {"filter.rows", 1, 32, 0},               // generated filter: rows per lane
     {"filter.persistent", 0, 1, kBool},      // generated filter: persistent grid, tiles by ticket
     {"filter.grid", 1, 16, 0},               // persistent filter: workgroups per CU (default: what fits)
+    {"filter.waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 12) | (1u << 16)},  // generated filter: waves per workgroup
     {"filter.v", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
     {"filter.g", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
     {"jit.params", 0, 1, kBool},
+    {"jit.stream_loads", 0, 1, kBool},       // generated kernels: nontemporal column loads
+    {"jit.stream_stores", 0, 1, kBool},      // generated filter: nontemporal projection stores
     {"agg.jit", 0, 1, kBool},
     {"agg.rows", 1, 32, 0},                  // generated aggregate: rows per lane
     {"agg.grid", 1, 64, 0},

@@ -266,6 +266,31 @@ def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows, persist
         autotune.set_knob("filter.persistent", -1)
 
 
+@pytest.mark.parametrize("waves,rows", [(8, 16), (12, 16), (16, 8), (8, 24), (12, 3), (16, 1)])
+def test_every_waves_per_workgroup_is_bit_exact(backend, stream, oracle, waves, rows):
+    # tiles of 64 * W * S rows; S * W above 64 (step, wave) counts scans two
+    # or more counts per lane
+    autotune.set_knob("filter.rows", rows)
+    autotune.set_knob("filter.waves", waves)
+    try:
+        rng = np.random.default_rng(80 + waves)
+        for n in (1, 1000, 30_001):
+            batch = datagen.random_batch(rng, n)
+            for pred, proj in [(PREDICATES[10], PROJECTIONS[2]), (PREDICATES[12], PROJECTIONS[1])]:
+                got, want = run_both(backend, stream, oracle, batch, pred, proj)
+                assert_batches_equal(got, want, names=False)
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=6), 1_000_003)
+        host = backend.download(stream, dev)
+        pred, proj = datagen.filter_query(50)
+        outs = backend.filter_project_batches(stream, dev, 65_536, pred, proj)
+        for i, o in enumerate(outs):
+            want = oracle.filter_project(host.slice(i * 65_536, 65_536), pred, proj)
+            assert_batches_equal(backend.download(stream, o), want, names=False)
+    finally:
+        autotune.set_knob("filter.rows", -1)
+        autotune.set_knob("filter.waves", -1)
+
+
 @pytest.mark.parametrize("rows,batch_rows", [(4, 65_536), (8, 1_000_000), (2, 4_096)])
 def test_persistent_grid_takes_many_tiles_per_workgroup(backend, stream, oracle, rows, batch_rows):
     # one workgroup per CU: every workgroup compacts several tiles in turn,

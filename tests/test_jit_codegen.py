@@ -40,10 +40,10 @@ def test_filter_query_kernel_shape():
     src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
     # four 1-2 register columns plus one validity word: 16 rows per lane (8
     # on the persistent kernel, which holds two tiles' rows)
-    assert "filter_tile<Q, 16>" in src
+    assert "filter_tile<Q, 16, 4>" in src
     autotune.set_knob("filter.persistent", 1)
     try:
-        assert "filter_persistent<Q, 8>" in backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
+        assert "filter_persistent<Q, 8, 4>" in backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
     finally:
         autotune.set_knob("filter.persistent", -1)
     # a % 7 by a constant divisor (the compiler's multiply-high), no error path
@@ -88,7 +88,7 @@ def test_every_rows_per_lane_compiles(rows, persistent):
     try:
         pred, proj = datagen.filter_query()
         src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
-        assert f"{'filter_persistent' if persistent else 'filter_tile'}<Q, {rows}>" in src
+        assert f"{'filter_persistent' if persistent else 'filter_tile'}<Q, {rows}, 4>" in src
     finally:
         autotune.set_knob("filter.persistent", -1)
 

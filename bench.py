@@ -542,11 +542,19 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
     stream.synchronize()
     elapsed = time.perf_counter() - t0
     lib.oxr_profile_enable(0)
-    kms, launches = _kernel_ms(lib, "aggregate")
-    spill_ms, spills = _kernel_ms(lib, "aggregate_spill")  # the cold tail's partition + fold passes
-    scatter_ms, _ = _kernel_ms(lib, "aggregate_scatter")
-    fold_ms, _ = _kernel_ms(lib, "aggregate_fold")
-    total_ms = kms + spill_ms * spills / max(1, launches)
+    # device time per step of every pass of the update: the probe + aggregate
+    # kernel (hashed block tables, or hot groups with their sampled map and
+    # the reduce of the blocks' hot partials), then the cold tail's scan,
+    # scatter and fold
+    parts = {}
+    for name in ("aggregate", "aggregate_hot", "aggregate_hot_map", "aggregate_hot_reduce", "aggregate_spill",
+                 "aggregate_scatter", "aggregate_fold", "key_range"):
+        ms, cnt = _kernel_ms(lib, name)
+        parts[name] = ms * cnt / steps
+    total_ms = sum(parts[k] for k in ("aggregate", "aggregate_hot", "aggregate_hot_map", "aggregate_hot_reduce",
+                                      "aggregate_spill", "key_range"))
+    kms = parts["aggregate"] + parts["aggregate_hot"]
+    spill_ms, scatter_ms, fold_ms = parts["aggregate_spill"], parts["aggregate_scatter"], parts["aggregate_fold"]
     lib.oxr_profile_reset()
     cpu = None
     if not args.no_cpu_baseline:
@@ -591,8 +599,9 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
         "ms_per_step": elapsed * 1000 / steps, "dtype": "int64/int32/f64",
         "config": {"workload": "SUM(qty), SUM(qty*price), COUNT(*) FROM fact JOIN dim ON fk = pk GROUP BY grp",
                    "fact_rows": nf, "dim_rows": nd, "groups": int(out.num_rows)},
-        "kernel_ms": total_ms, "probe_aggregate_ms": kms, "spill_ms": spill_ms if spills else 0.0,
-        "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "build_s": build_s,
+        "kernel_ms": total_ms, "probe_aggregate_ms": kms, "spill_ms": spill_ms,
+        "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "device_ms_per_step": parts,
+        "build_s": build_s,
         "parity_full_size": ok,
         "roofline": _roofline_hbm(alg, total_ms),
         "cpu_baseline": cpu,

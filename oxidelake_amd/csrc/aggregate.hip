@@ -867,6 +867,134 @@ size_t fold_lds(uint32_t slog2, int na) {
   return G * 8 * na + G + G * na;
 }
 
+// ------------------------------------------------------------- hot groups
+// Where a sampled row's group comes from: the key column itself, or the
+// probe key through a unique-key join (the group of each join table slot in
+// gos32 / gos64, as the generated kernel reads it).
+struct HotKeySrc {
+  DCol key;
+  int32_t join;
+  JTable jt;
+  const uint64_t *gos32;
+  const uint64_t *gos64;
+};
+
+// The group index (key - smin) of every `stride`-th row into grp[i], or
+// 0xFFFFFFFF when the row has no group in [smin, smin + srange): the sample
+// the hot map is picked from. (Counting with atomics instead serialises on
+// the hottest group: a Zipf key's top group is a tenth of the sample.)
+__global__ void hot_sample_kernel(HotKeySrc src, uint64_t samples, uint64_t stride, int64_t smin,
+                                  uint64_t srange, uint32_t *grp) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= samples) return;
+  grp[i] = 0xFFFFFFFFu;
+  const uint64_t r = i * stride;  // host: (samples - 1) * stride < rows
+  const DCol &c = src.key;
+  if (c.valid && !((c.valid[r >> 5] >> (r & 31)) & 1u)) return;
+  int64_t k = c.type == OXR_INT32 ? (int64_t)((const int32_t *)c.values)[r] : ((const int64_t *)c.values)[r];
+  if (src.join) {
+    const uint64_t slot = join_find(src.jt, k);
+    if (slot == kNoSlot) return;
+    uint32_t state;
+    if (src.gos32) {
+      const uint64_t e = src.gos32[slot];
+      state = (uint32_t)(e >> 32);
+      k = (int64_t)(int32_t)(uint32_t)e;
+    } else {
+      k = (int64_t)src.gos64[2 * slot];
+      state = (uint32_t)src.gos64[2 * slot + 1];
+    }
+    if ((state & 0xFFu) != 1) return;  // no build row, or its group is NULL
+  }
+  if (k < smin || (uint64_t)k - (uint64_t)smin >= srange) return;
+  grp[i] = (uint32_t)((uint64_t)k - (uint64_t)smin);
+}
+
+// Stamps each join table slot's group entry with its group's hot id (the
+// state word's bits 8 and up: hot id + 1, 0 for a cold group), so the hot
+// kernel's one lookup gives group and hot id together.
+__global__ void gos_hot_kernel(uint64_t *gos32, uint64_t *gos64, uint64_t nslots, const int32_t *hotmap, int64_t smin,
+                               uint64_t srange) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  int64_t g;
+  uint32_t state;
+  if (gos32) {
+    g = (int64_t)(int32_t)(uint32_t)gos32[i];
+    state = (uint32_t)(gos32[i] >> 32);
+  } else {
+    g = (int64_t)gos64[2 * i];
+    state = (uint32_t)gos64[2 * i + 1];
+  }
+  if (state != 1 || g < smin || (uint64_t)g - (uint64_t)smin >= srange) return;
+  const uint32_t h1 = (uint32_t)(hotmap[(uint64_t)g - (uint64_t)smin] + 1);
+  if (!h1) return;
+  if (gos32) gos32[i] |= (uint64_t)h1 << 40;
+  else gos64[2 * i + 1] = state | (h1 << 8);
+}
+
+__global__ void fill_i32_kernel(int32_t *v, uint64_t n, int32_t x) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    v[i] = x;
+}
+
+// Hot id h -> the h-th most sampled group (sorted descending), for groups
+// sampled at least `min_count` times; hotkeys[h] = its key (smin + index).
+// Groups past srange (no group) and runs past the run count (zero counts)
+// get no hot id.
+__global__ void hot_assign_kernel(const uint32_t *cnt_sorted, const uint32_t *grp_sorted, uint32_t nhot,
+                                  uint32_t min_count, int64_t smin, uint64_t srange, int32_t *hotmap,
+                                  int64_t *hotkeys) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nhot) return;
+  const uint32_t g = grp_sorted[h];
+  hotkeys[h] = (int64_t)((uint64_t)smin + g);
+  if (cnt_sorted[h] >= min_count && g < srange) hotmap[g] = (int32_t)h;
+}
+
+// Folds the blocks' hot partials into the HBM table: block (x, y) takes 64
+// hot ids and the source blocks y, y + gridDim.y, ...; 4 threads per hot id
+// split those, combine in LDS, and one update per (hot id, aggregate) that
+// any of its source blocks used goes to the table.
+__global__ __launch_bounds__(256) void hot_reduce_kernel(const int64_t *part, const uint8_t *flags, uint32_t nblocks,
+                                                         uint32_t nhot, const int64_t *hotkeys, Table t, FoldKinds fk,
+                                                         int na) {
+  __shared__ int64_t s_v[4][kMaxAggs][64];
+  __shared__ uint32_t s_f[4][64];
+  const int hl = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const uint32_t h = blockIdx.x * 64 + hl;
+  int64_t v[kMaxAggs];
+  for (int a = 0; a < na; a++) v[a] = acc_init(fk.kind[a]);
+  uint32_t f = 0;
+  if (h < nhot) {
+    for (uint32_t b = blockIdx.y + gridDim.y * q; b < nblocks; b += 4 * gridDim.y) {
+      const uint32_t fb = flags[(uint64_t)b * nhot + h];
+      if (!(fb & 1u)) continue;
+      f |= fb;
+      for (int a = 0; a < na; a++) v[a] = combine_kind(fk.kind[a], v[a], part[((uint64_t)b * na + a) * nhot + h]);
+    }
+  }
+  for (int a = 0; a < na; a++) s_v[q][a][hl] = v[a];
+  s_f[q][hl] = f;
+  __syncthreads();
+  if (q != 0 || h >= nhot) return;
+  for (int k = 1; k < 4; k++) {
+    f |= s_f[k][hl];
+    for (int a = 0; a < na; a++) v[a] = combine_kind(fk.kind[a], v[a], s_v[k][a][hl]);
+  }
+  if (!(f & 1u)) return;
+  bool ok = true;
+  const uint64_t gs = global_slot(t, hotkeys[h], &ok);
+  if (!ok) {
+    atomicOr(t.overflow, 1u);
+    return;
+  }
+  for (int a = 0; a < na; a++) {
+    if (v[a] != acc_init(fk.kind[a]) || fk.kind[a] == A_SUM_F64) acc_apply_global(fk.kind[a], &t.acc[a][gs], v[a]);
+    if (t.has[a] && ((f >> (1 + a)) & 1u)) t.has[a][gs] = 1;
+  }
+}
+
 // The join an update folds through (oxr_agg_update_join), or none.
 struct JoinSide {
   const oxr_join_table *table = nullptr;
@@ -895,6 +1023,97 @@ struct ShapeOf {
     }
   }
 };
+
+// Samples the update's input (every stride-th row, about 1M rows), counts
+// each sampled group of the spill range, and gives the most sampled ones
+// (up to `cap`, each sampled at least twice) dense hot ids: A.hotmap,
+// A.nhot, and the per-block partials A.hot_part / A.hot_flags the hot
+// kernel writes. *mem holds it all (released by the caller after the
+// reduce); *nhot = 0 when there is nothing to do.
+int build_hot_map(oxr_stream *s, const oxr_agg_state *st, const oxr_batch *in, const JoinSide *js,
+                  const KeyRange *kr, AggJitArgs &A, uint32_t grid, uint32_t cap, void **mem, int64_t **hotkeys,
+                  uint32_t *nhot) {
+  const uint64_t srange = kr->count;
+  const int na = (int)st->aggs.size();
+  const uint64_t n = in->rows;
+  const uint64_t samples = std::min<uint64_t>(n, 1ull << 20);
+  const uint32_t nh = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(cap, srange), samples);
+  const uint64_t stride = n / samples;  // (samples - 1) * stride < n
+  // the sampled groups, sorted, run-length encoded into (group, count)
+  // runs, the runs sorted by count (descending): no atomics
+  size_t b_sort1 = 0, b_rle = 0, b_sort2 = 0;
+  const int ns = (int)samples;
+  hipError_t e = hipcub::DeviceRadixSort::SortKeys(nullptr, b_sort1, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                   ns, 0, 32, s->s);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRunLengthEncode::Encode(nullptr, b_rle, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr, ns, s->s);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b_sort2, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                     (const uint32_t *)nullptr, (uint32_t *)nullptr, ns, 0, 32,
+                                                     s->s);
+  if (e != hipSuccess) return fail_hip(e, "hot map sizes");
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t b_s = up(samples * 4), b_part = up((size_t)grid * na * nh * 8), b_flags = up((size_t)grid * nh);
+  const size_t b_tmp = up(std::max(b_sort1, std::max(b_rle, b_sort2)));
+  const size_t total = 6 * b_s + 256 + up(srange * 4) + up((size_t)nh * 8) + b_part + b_flags + b_tmp;
+  OXR_TRY(stream_alloc(s, total, mem));
+  char *q = (char *)*mem;
+  uint32_t *grp = (uint32_t *)q, *grp_sorted = (uint32_t *)(q + b_s), *runs = (uint32_t *)(q + 2 * b_s),
+           *counts = (uint32_t *)(q + 3 * b_s), *cnt_sorted = (uint32_t *)(q + 4 * b_s),
+           *run_sorted = (uint32_t *)(q + 5 * b_s), *nruns = (uint32_t *)(q + 6 * b_s);
+  q += 6 * b_s + 256;
+  int32_t *hotmap = (int32_t *)q;
+  q += up(srange * 4);
+  *hotkeys = (int64_t *)q;
+  q += up((size_t)nh * 8);
+  A.hot_part = (int64_t *)q;
+  q += b_part;
+  A.hot_flags = (uint8_t *)q;
+  q += b_flags;
+  void *tmp = q;
+  // counts past the run count stay 0 (sorted last, never hot)
+  int rc = fill_zero(s, counts, samples * 4);
+  if (rc != OXR_OK) return rc;
+  const unsigned tb = 256;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<uint64_t>((srange + tb - 1) / tb, 4096)), dim3(tb), 0,
+                     s->s, hotmap, srange, -1);
+  HotKeySrc src{};
+  if (js) {
+    const DevCol &k = in->cols[js->probe_key];
+    src.key.values = k.values;
+    src.key.valid = (const uint32_t *)k.validity;
+    src.key.type = k.type;
+    src.join = 1;
+    src.jt = A.jt;
+    src.gos32 = A.gos;
+    src.gos64 = A.gos64;
+  } else {
+    const DevCol &k = in->cols[st->key_col];
+    src.key.values = k.values;
+    src.key.valid = (const uint32_t *)k.validity;
+    src.key.type = k.type;
+  }
+  hipLaunchKernelGGL(hot_sample_kernel, dim3((unsigned)((samples + tb - 1) / tb)), dim3(tb), 0, s->s, src, samples,
+                     stride, kr->min, srange, grp);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortKeys(tmp, b_sort1, grp, grp_sorted, ns, 0, 32, s->s);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRunLengthEncode::Encode(tmp, b_rle, grp_sorted, runs, counts, nruns, ns, s->s);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(tmp, b_sort2, counts, cnt_sorted, runs, run_sorted, ns, 0, 32,
+                                                     s->s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(hot_assign_kernel, dim3((nh + tb - 1) / tb), dim3(tb), 0, s->s, cnt_sorted, run_sorted, nh,
+                       2u, kr->min, srange, hotmap, *hotkeys);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail_hip(e, "hot map");
+  A.hotmap = hotmap;
+  A.nhot = nh;
+  *nhot = nh;
+  return OXR_OK;
+}
 
 // The update as the kernel generated for this aggregation's shape, when the
 // generator covers it (a plain key or one global group, or a join on unique
@@ -955,11 +1174,6 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
       A.dense_null = in->cols[st->key_col].validity ? 1 : 0;
     }
   }
-  const uint64_t tile = 256ull * ja.rows_per_lane;
-  const uint64_t ntiles = (in->rows + tile - 1) / tile;
-  const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.jit_grid", 4));
-  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
-
   // spill the cold tail of a high-cardinality key (see spill_fold_kernel):
   // a measured key range that the fold's partitions cover, every aggregate
   // in the block table, plain FP64 sums, and enough rows to pay for it
@@ -973,6 +1187,21 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     const uint64_t parts = (kr->count + (1ull << slog2) - 1) >> slog2;
     if (parts <= (uint64_t)kMaxSpillParts && kr->count < (1ull << 32)) nparts = (uint32_t)parts;
   }
+  // hot groups: with the spill on (a wide key range) over enough rows, a
+  // sample of the input picks the groups that get dense LDS accumulators
+  // (aggregate_hot) instead of the hashed block table
+  bool any_nullable = false;
+  for (int a = 0; a < na; a++) any_nullable |= st->aggs[a].nullable != 0;
+  const uint32_t hot_cap = (uint32_t)hot_capacity(na, any_nullable);
+  const bool hot_try = nparts && ja.hot && fn == ja.fn && tune_get("agg.hot", 1) > 0 && kr->count > 4ull * hot_cap &&
+                       in->rows >= (uint64_t)std::max<int64_t>(1, tune_get("agg.hot_rows", 1 << 22));
+  // workgroups per CU: the hot kernel hides its LDS atomics' latency with
+  // more resident blocks (knob agg.hot_grid), the others take agg.jit_grid
+  const uint64_t tile = 256ull * ja.rows_per_lane;
+  const uint64_t ntiles = (in->rows + tile - 1) / tile;
+  const uint64_t per_cu =
+      (uint64_t)std::max<int64_t>(1, hot_try ? tune_get("agg.hot_grid", 8) : tune_get("agg.jit_grid", 4));
+  const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   const uint32_t words = (uint32_t)na + 1;
   void *spill_mem = nullptr;
   int64_t *spill = nullptr;
@@ -1019,11 +1248,40 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
       nparts = 0;
     }
   }
-  int rc;
-  {
-    ProfScope prof(s->s, fn == ja.fn ? "aggregate" : "aggregate_dense");
+  void *hot_mem = nullptr;
+  int64_t *hotkeys = nullptr;
+  uint32_t nhot = 0;
+  int rc = OXR_OK;
+  if (nparts && hot_try) {
+    ProfScope prof(s->s, "aggregate_hot_map");
+    rc = build_hot_map(s, st, in, js, kr, A, (uint32_t)grid, hot_cap, &hot_mem, &hotkeys, &nhot);
+    if (rc == OXR_OK && nhot) fn = ja.hot_fn;
+    if (rc == OXR_OK && nhot && js) {
+      // the join's group entries carry the hot ids: one lookup per row
+      const uint64_t nslots = js->table->t.slots + 1;
+      hipLaunchKernelGGL(gos_hot_kernel, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s->s,
+                         A.gos ? (uint64_t *)gos : nullptr, A.gos64 ? (uint64_t *)gos : nullptr, nslots, A.hotmap,
+                         kr->min, kr->count);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail_hip(e, "gos_hot_kernel launch");
+      A.gos_hot = 1;
+    }
+  }
+  if (rc == OXR_OK) {
+    ProfScope prof(s->s, fn == ja.fn ? "aggregate" : fn == ja.hot_fn ? "aggregate_hot" : "aggregate_dense");
     rc = jit_launch_raw(fn, (unsigned)grid, s->s, &A, sizeof A);
   }
+  if (rc == OXR_OK && nhot) {
+    ProfScope prof(s->s, "aggregate_hot_reduce");
+    FoldKinds fk{};
+    for (int a = 0; a < na; a++) fk.kind[a] = st->aggs[a].kind;
+    const dim3 g((unsigned)((nhot + 63) / 64), 8), b(256);
+    hipLaunchKernelGGL(hot_reduce_kernel, g, b, 0, s->s, A.hot_part, A.hot_flags, (uint32_t)grid, nhot, hotkeys,
+                       st->t, fk, na);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail_hip(e, "hot_reduce_kernel launch");
+  }
+  if (hot_mem) stream_release(s, hot_mem);
   if (rc == OXR_OK && nparts) {
     ProfScope prof(s->s, "aggregate_spill");
     const uint64_t nh = (uint64_t)nparts * grid + 1;

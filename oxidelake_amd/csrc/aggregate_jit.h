@@ -231,8 +231,11 @@ __device__ inline uint64_t slot_of(const Table &t, bool key_valid, int64_t key, 
 //   struct In, ins(p, base); struct Row, load(i, j, in, x)   (as for filters)
 //   pred(L, x, j, in, err)              the fused filter (true: keep; L: the
 //                                       literals, in prepared mode)
-//   key(p, x, j, in, key, kv)           the group key (false: no group, i.e.
-//                                       no join match); kv false = NULL key
+//   key(p, x, j, in, key, kv, hid)      the group key (false: no group, i.e.
+//                                       no join match); kv false = NULL key;
+//                                       hid: the group's hot id when the join
+//                                       table's group entries carry it (-1:
+//                                       cold), else -2
 //   contrib(L, x, j, sel, c, cm, err)   each aggregate's contribution; bit a
 //                                       of cm set where input a is not NULL
 constexpr int kAggCols = 16;
@@ -247,6 +250,7 @@ struct AggJitArgs {
   JTable jt;
   const uint64_t *gos;          // Int32 groups: (state << 32) | group
   const uint64_t *gos64;        // Int64 groups: {group, state} pairs
+  uint32_t gos_hot;             // the states carry hot ids (aggregate_hot): state | (hid + 1) << 8
   uint64_t lit[32];             // prepared mode: literal bits
   // dense kernels: keys [dense_min, dense_min + dense_keys), plus the NULL
   // key's slot at index dense_keys when dense_null
@@ -267,10 +271,27 @@ struct AggJitArgs {
   int64_t smin;
   uint64_t srange;
   uint32_t slog2, sparts, sper;
+  // hot groups (aggregate_hot): hot id of each group of the spill range
+  // (hotmap[key - smin], -1 cold); each block's hot accumulators go to
+  // hot_part[(block * kAggs + a) * nhot + h] and its flags to
+  // hot_flags[block * nhot + h] (bit 0: a row of the group was selected;
+  // bit 1 + a: aggregate a saw a non-NULL input) for the reduce that follows
+  const int32_t *hotmap;
+  uint32_t nhot;
+  int64_t *hot_part;
+  uint8_t *hot_flags;
 };
 
-// group of a probe key through the join: false when nothing matches
-__device__ inline bool join_group(const AggJitArgs &p, int64_t pk, int64_t &g, bool &gv) {
+// Hot groups a block accumulates in LDS (aggregate_hot): 40 KB of
+// accumulators and flags for NA aggregates, a multiple of 256.
+__host__ __device__ constexpr int hot_capacity(int na, bool nullable) {
+  return (40960 / (8 * na + 1 + (nullable ? na : 0))) / 256 * 256;
+}
+
+// group of a probe key through the join: false when nothing matches. With
+// p.gos_hot the group entries also carry the group's hot id (the state
+// word's bits 8 and up hold hot id + 1, 0 for a cold group).
+__device__ inline bool join_group(const AggJitArgs &p, int64_t pk, int64_t &g, bool &gv, int32_t &hid) {
   uint64_t slot;
   if (p.jt.dense) {
     // every slot of a dense table has an entry (state 0: no build row)
@@ -289,6 +310,8 @@ __device__ inline bool join_group(const AggJitArgs &p, int64_t pk, int64_t &g, b
     g = (int64_t)p.gos64[2 * slot];
     state = (uint32_t)p.gos64[2 * slot + 1];
   }
+  if (p.gos_hot) hid = (int32_t)(state >> 8) - 1;
+  state &= 0xFFu;
   gv = state == 1;  // 2: the build row's group is NULL
   return state != 0;
 }
@@ -314,6 +337,29 @@ __host__ __device__ constexpr int lo_index(int a) {
   int n = 0;
   for (int i = 0; i < a; i++) n += Q::kKind[i] == A_SUM_F64;
   return n;
+}
+
+// A selected row (key, kv, contributions c under mask cm) that the block's
+// LDS does not hold: its spill bucket when its key is in the spill range and
+// the bucket has room (true), else the caller applies it to the HBM table.
+template <class Q>
+__device__ __forceinline__ bool spill_row(const AggJitArgs &p, int64_t *spill_blk, uint32_t *s_bucket,
+                                          uint32_t *s_hist, int64_t key, bool kv, const int64_t *c,
+                                          uint32_t cm) {
+  constexpr int NA = Q::kAggs;
+  if (!(p.spill && kv && key >= p.smin && (uint64_t)key - (uint64_t)p.smin < p.srange)) return false;
+  const uint32_t g = (uint32_t)((uint64_t)key - (uint64_t)p.smin);
+  const uint32_t part = g >> p.slog2, bucket = part / p.sper;
+  const uint32_t pos = atomicAdd(&s_bucket[bucket], 1u);
+  if (pos >= p.spill_cap) return false;
+  // 32-bit offset inside the block's buckets (the host keeps
+  // kSpillBuckets * spill_cap * (NA + 1) below 2^31)
+  int64_t *r = spill_blk + (bucket * (uint32_t)p.spill_cap + pos) * (uint32_t)(NA + 1);
+  r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
+#pragma unroll
+  for (int a = 0; a < NA; a++) r[1 + a] = c[a];
+  atomicAdd(&s_hist[part], 1u);
+  return true;
 }
 
 template <class Q, int S>
@@ -375,7 +421,8 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       bool sel = in && Q::pred(p.lit, x[s], j, in, err);
       int64_t key = 0;
       bool kv = false;
-      if (sel) sel = Q::key(p, x[s], j, in, key, kv);
+      int32_t hid_unused;
+      if (sel) sel = Q::key(p, x[s], j, in, key, kv, hid_unused);
       if (!kv) key = 0;  // every NULL key is the same group
       int64_t c[NA];
       double cl[NA];  // FP64 sums: the rounding error carried with c
@@ -464,23 +511,7 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       // a row the block table does not hold goes to its spill bucket when
       // its key is in the spill range (and the bucket has room), else to the
       // HBM table
-      bool spilled = false;
-      if (SP && p.spill && sel && lslot < 0 && kv && key >= p.smin &&
-          (uint64_t)key - (uint64_t)p.smin < p.srange) {
-        const uint32_t g = (uint32_t)((uint64_t)key - (uint64_t)p.smin);
-        const uint32_t part = g >> p.slog2, bucket = part / p.sper;
-        const uint32_t pos = atomicAdd(&s_bucket[SP ? bucket : 0], 1u);
-        if (pos < p.spill_cap) {
-          // 32-bit offset inside the block's buckets (the host keeps
-          // kSpillBuckets * spill_cap * (NA + 1) below 2^31)
-          int64_t *r = spill_blk + (bucket * (uint32_t)p.spill_cap + pos) * (uint32_t)(NA + 1);
-          r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
-#pragma unroll
-          for (int a = 0; a < NA; a++) r[1 + a] = c[a];
-          atomicAdd(&s_hist[SP ? part : 0], 1u);
-          spilled = true;
-        }
-      }
+      const bool spilled = SP && sel && lslot < 0 && spill_row<Q>(p, spill_blk, s_bucket, s_hist, key, kv, c, cm);
       if (!sel || spilled) continue;
       uint64_t gslot = 0;
       if (lslot < 0 || LA < NA) gslot = slot_of(p.t, kv, key, &ok);
@@ -529,6 +560,156 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
   }
   if (SP && p.spill) {
     // s_hist and s_bucket are final: every tile ended with a barrier
+    for (uint32_t i = tid; i < p.sparts; i += 256) p.hist[(uint64_t)i * gridDim.x + blockIdx.x] = s_hist[i];
+    if (tid < kSpillBuckets)
+      p.spill_n[blockIdx.x * kSpillBuckets + tid] = s_bucket[tid] < p.spill_cap ? s_bucket[tid] : (uint32_t)p.spill_cap;
+  }
+  if (err) atomicOr(p.error, 1u);
+  if (!ok) atomicOr(p.t.overflow, 1u);
+}
+
+// ------------------------------------------------------ hot-group kernel
+// For a wide key range with a skewed key distribution (a Zipf fact table
+// joined to its dimension): the host samples the input, gives the most
+// frequent groups dense hot ids (p.hotmap), and each row of a hot group is
+// one LDS atomic per aggregate at its hot id: no hashing, probing or
+// matching round. Other rows spill (spill_row) or take the HBM table. Each
+// row's join lookup and hot id are loaded for all S row steps before any is
+// applied, so those random reads are in flight together. A block's hot
+// accumulators end as plain stores into p.hot_part, which a reduce folds
+// into the HBM table with one update per (hot group, aggregate). The hot
+// map only routes rows: every group's result is the same sum whichever way
+// its rows went.
+template <class Q, int S>
+__device__ __forceinline__ void aggregate_hot(const AggJitArgs &p) {
+  constexpr int NA = Q::kAggs;
+  constexpr int H = hot_capacity(NA, Q::kAnyNullable);
+  constexpr int kTile = 256 * S;
+  static_assert(NA <= kLdsAggs, "hot groups keep every aggregate in LDS");
+  __shared__ int64_t s_acc[NA][H];
+  __shared__ uint8_t s_used[H];
+  __shared__ uint8_t s_has[Q::kAnyNullable ? NA : 1][Q::kAnyNullable ? H : 1];
+  __shared__ uint32_t s_hist[kMaxSpillParts];
+  __shared__ uint32_t s_bucket[kSpillBuckets];
+
+  const int tid = threadIdx.x;
+  const uint32_t nhot = p.nhot < (uint32_t)H ? p.nhot : (uint32_t)H;  // the host keeps nhot <= H
+  for (uint32_t i = tid; i < (uint32_t)H; i += 256) {
+    s_used[i] = 0;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      s_acc[a][i] = acc_init(Q::kKind[a]);
+      if (Q::kAnyNullable) s_has[a][i] = 0;
+    }
+  }
+  if (p.spill) {
+    for (uint32_t i = tid; i < p.sparts; i += 256) s_hist[i] = 0;
+    if (tid < kSpillBuckets) s_bucket[tid] = 0;
+  }
+  __syncthreads();
+  int64_t *const spill_blk = p.spill ? p.spill + (uint64_t)blockIdx.x * kSpillBuckets * p.spill_cap * (NA + 1) : nullptr;
+
+  bool err = false, ok = true;
+  const uint64_t ntiles = (p.n + kTile - 1) / kTile;
+  auto load_tile = [&](uint64_t tile, typename Q::Row (&x)[S]) {
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    const typename Q::In in_cols = Q::ins(p, base);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+  };
+  // folds tile `tile` from rows x, with the next tile's loads into xn
+  // issued first (past the last tile they read `tile` again: unconditional
+  // loads keep the compiler's waits on x alone)
+  auto step = [&](uint64_t tile, typename Q::Row (&x)[S], typename Q::Row (&xn)[S]) {
+    const uint64_t next = tile + gridDim.x;
+    load_tile(next < ntiles ? next : tile, xn);
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    // every row step's key (the join lookups, with their hot ids) before any
+    // is applied, loads in flight together
+    int64_t key[S];
+    int32_t hid[S];
+    uint32_t selb = 0, kvb = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      const bool in = j < lim;
+      bool sel = in && Q::pred(p.lit, x[s], j, in, err);
+      int64_t k = 0;
+      bool kv = false;
+      int32_t h = -1;
+      if (sel) sel = Q::key(p, x[s], j, in, k, kv, h);
+      if (!kv) k = 0;  // every NULL key is the same group
+      key[s] = k;
+      hid[s] = h;
+      selb |= (uint32_t)sel << s;
+      kvb |= (uint32_t)kv << s;
+    }
+    // keys whose hot id the lookup did not carry: the hot map
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      if (hid[s] != -2) continue;
+      const bool inr = ((kvb >> s) & 1u) && key[s] >= p.smin && (uint64_t)key[s] - (uint64_t)p.smin < p.srange;
+      hid[s] = inr ? p.hotmap[(uint64_t)key[s] - (uint64_t)p.smin] : -1;
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      const bool sel = (selb >> s) & 1u, kv = (kvb >> s) & 1u;
+      int64_t c[NA];
+      uint32_t cm = 0;
+      Q::contrib(p.lit, x[s], j, sel, c, cm, err);
+      if (!sel) continue;
+      const int32_t h = hid[s];
+      if (h >= 0 && (uint32_t)h < nhot) {
+        s_used[h] = 1;
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+          if (!((cm >> a) & 1u)) continue;
+          acc_apply_lds(Q::kKind[a], &s_acc[a][h], c[a]);
+          if (Q::kAnyNullable && Q::kNullable[a]) s_has[a][h] = 1;
+        }
+        continue;
+      }
+      if (spill_row<Q>(p, spill_blk, s_bucket, s_hist, key[s], kv, c, cm)) continue;
+      const uint64_t gslot = slot_of(p.t, kv, key[s], &ok);
+      if (!ok) continue;
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (!((cm >> a) & 1u)) continue;
+        acc_apply_global(Q::kKind[a], &p.t.acc[a][gslot], c[a]);
+        if (p.t.has[a]) p.t.has[a][gslot] = 1;
+      }
+    }
+    return next;
+  };
+  typename Q::Row xa[S], xb[S];
+  uint64_t tile = blockIdx.x;
+  if (tile < ntiles) {
+    load_tile(tile, xa);
+    while (true) {
+      tile = step(tile, xa, xb);
+      if (tile >= ntiles) break;
+      tile = step(tile, xb, xa);
+      if (tile >= ntiles) break;
+    }
+  }
+  __syncthreads();  // every row applied: the LDS state is final
+  // the block's hot partials, for the reduce
+  for (uint32_t h = tid; h < nhot; h += 256) {
+    uint32_t flags = s_used[h];
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      p.hot_part[((uint64_t)blockIdx.x * NA + a) * nhot + h] = s_acc[a][h];
+      if (Q::kAnyNullable && Q::kNullable[a]) flags |= (uint32_t)s_has[a][h] << (1 + a);
+    }
+    p.hot_flags[(uint64_t)blockIdx.x * nhot + h] = (uint8_t)flags;
+  }
+  if (p.spill) {
     for (uint32_t i = tid; i < p.sparts; i += 256) p.hist[(uint64_t)i * gridDim.x + blockIdx.x] = s_hist[i];
     if (tid < kSpillBuckets)
       p.spill_n[blockIdx.x * kSpillBuckets + tid] = s_bucket[tid] < p.spill_cap ? s_bucket[tid] : (uint32_t)p.spill_cap;
@@ -604,7 +785,8 @@ __device__ __forceinline__ void aggregate_dense(const AggJitArgs &p) {
       bool sel = in && Q::pred(p.lit, x[s], j, in, err);
       int64_t key = 0;
       bool kv = false;
-      if (sel) sel = Q::key(p, x[s], j, in, key, kv);
+      int32_t hid_unused;
+      if (sel) sel = Q::key(p, x[s], j, in, key, kv, hid_unused);
       if (!kv) key = 0;
       int64_t c[NA];
       uint32_t cm = 0;

@@ -62,6 +62,8 @@ struct JitAgg {
   hipFunction_t fn = nullptr;
   bool dense = false;  // a plain key: the small-range kernels below exist
   hipFunction_t dense16 = nullptr, dense32 = nullptr;
+  bool hot = false;  // the hot-group kernel below exists
+  hipFunction_t hot_fn = nullptr;
   int rows_per_lane = 0;
   std::vector<uint64_t> lits;  // AggJitArgs::lit (prepared mode)
   std::string source;

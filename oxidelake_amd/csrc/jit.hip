@@ -496,7 +496,7 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
     const int c = sh.key_col;  // the probe key
     const std::string ok = column_ok(c);
     key_code = "    if (!" + ok + ") return false;\n    return join_group(p, (int64_t)x.c" + std::to_string(c) +
-               ", key, kv);\n";
+               ", key, kv, hid);\n";
   } else if (sh.key_col >= 0) {
     const int c = sh.key_col;
     const std::string ok = column_ok(c);
@@ -559,12 +559,23 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   src += "  __device__ static inline bool pred(const uint64_t *L, const Row &x, uint32_t j, bool in, "
          "bool &err) {\n" + pred_code + "  }\n";
   src += "  __device__ static inline bool key(const AggJitArgs &p, const Row &x, uint32_t j, bool in, "
-         "int64_t &key, bool &kv) {\n" + key_code + "  }\n";
+         "int64_t &key, bool &kv, int32_t &hid) {\n    hid = -2;\n" + key_code + "  }\n";
   src += "  __device__ static inline void contrib(const uint64_t *L, const Row &x, uint32_t j, bool sel, "
          "int64_t (&c)[kAggs], uint32_t &cm, bool &err) {\n" + contrib + "  }\n";
   src += "};\n}  // namespace\n";
   src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_jit(const AggJitArgs p) {\n";
   src += "  aggregate_tiles<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";
+  // high-cardinality skewed keys: the hot-group kernel (aggregate_hot), for
+  // shapes whose aggregates all fit the block's LDS and sum FP64 plainly
+  ja->hot = la == na && tune_get("agg.exact_f64", 0) == 0 && sh.key_col != -1;
+  if (ja->hot) {
+    // the shape's rows per lane (two tiles of them in registers, the next
+    // tile's loads in flight), or the knob agg.hot_lane_rows
+    const int64_t hs = tune_get("agg.hot_lane_rows", 0);
+    const int hot_rows = hs > 0 ? (int)hs : ja->rows_per_lane;
+    src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_hot(const AggJitArgs p) {\n"
+           "  aggregate_hot<Q, " + std::to_string(hot_rows) + ">(p);\n}\n";
+  }
   ja->dense = sh.key_col >= 0 && !sh.join;
   if (ja->dense) {
     // small key ranges: per-lane LDS accumulators (aggregate_dense)
@@ -588,6 +599,7 @@ int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJit
     OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense16", &ja.dense16));
     OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense32", &ja.dense32));
   }
+  if (ja.hot) OXR_TRY(compile(dev, ja.source, "oxr_aggregate_hot", &ja.hot_fn));
   *out = std::move(ja);
   return OXR_OK;
 }

@@ -297,6 +297,10 @@ const KnobSpec kKnobs[] = {
     {"agg.spill", 0, 1, kBool},
     {"agg.spill_rows", 0, (int64_t)1 << 48, 0},
     {"agg.spill_pct", 1, 400, 0},
+    {"agg.hot", 0, 1, kBool},                 // hot-group kernel for wide skewed key ranges
+    {"agg.hot_rows", 1, (int64_t)1 << 48, 0},  // rows from which an update samples for hot groups
+    {"agg.hot_lane_rows", 1, 32, 0},           // hot-group kernel: rows per lane (default: the shape's)
+    {"agg.hot_grid", 1, 64, 0},                // hot-group kernel: workgroups per CU (default 8)
     {"vector.resident_queries", 0, 1, kBool},
     {"vector.wide", 0, 3, 0},
     {"vector.nt8", 0, 1, kBool},

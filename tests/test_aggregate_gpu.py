@@ -26,9 +26,11 @@ SUM_REL = 1e-12
 
 
 SPILL_KNOBS = {"agg.range_rows": 0, "agg.spill_rows": 0, "agg.dense": 0, "agg.spill_pct": 100}
+# hot: the spill on, and a sampled hot-group map from any input size
+HOT_KNOBS = {**SPILL_KNOBS, "agg.hot_rows": 1}
 
 
-@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "dense", "spill", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "prepared", "exact", "dense", "spill", "hot", "interpreter"])
 def path(request):
     # every test runs on the kernel generated per aggregation shape (with
     # literals as constants; prepared: literals as kernel arguments; exact:
@@ -42,11 +44,11 @@ def path(request):
     autotune.set_knob("jit.params", 1 if request.param == "prepared" else 0)
     autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
     autotune.set_knob("agg.range_rows", 0 if request.param == "dense" else -1)
-    if request.param == "spill":
-        for k, v in SPILL_KNOBS.items():
+    if request.param in ("spill", "hot"):
+        for k, v in (SPILL_KNOBS if request.param == "spill" else HOT_KNOBS).items():
             autotune.set_knob(k, v)
     yield request.param
-    for k in ("agg.jit", "jit.params", "agg.exact_f64", "agg.range_rows", *SPILL_KNOBS):
+    for k in ("agg.jit", "jit.params", "agg.exact_f64", "agg.range_rows", *HOT_KNOBS):
         autotune.set_knob(k, -1)
 
 
@@ -339,7 +341,7 @@ def test_dense_small_key_ranges(backend, stream, oracle, path, keys):
         # 4 aggregates x (range + NULL slot) slots fit 32 unless the range is wide
         # (twice that in exact mode: too many for every case here; the spill
         # path turns the dense kernel off)
-        assert dense == (0 if keys == "wide" or path in ("exact", "spill") else 1), dense
+        assert dense == (0 if keys == "wide" or path in ("exact", "spill", "hot") else 1), dense
 
 
 @pytest.mark.parametrize("pct", [1, 100])
@@ -360,7 +362,7 @@ def test_spill_many_groups(backend, stream, oracle, path, pct):
     batch = pa.record_batch({"k": k, "v": v, "w": w})
     spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.CountStar, None), Aggregate(F.Max, col(2)),
                              Aggregate(F.Sum, col(2))], filter=plan.cmp(col(2), Cmp.Lt, Lit.int32(900)))
-    for kk, vv in SPILL_KNOBS.items():
+    for kk, vv in (HOT_KNOBS if path == "hot" else SPILL_KNOBS).items():
         autotune.set_knob(kk, vv)
     autotune.set_knob("agg.spill_pct", pct)
     lib.oxr_profile_enable(1)
@@ -368,11 +370,13 @@ def test_spill_many_groups(backend, stream, oracle, path, pct):
     try:
         got = device_aggregate(backend, stream, batch, spec)
         spills = autotune.launches("aggregate_spill")
+        hot = autotune.launches("aggregate_hot")
     finally:
         lib.oxr_profile_enable(0)
         lib.oxr_profile_reset()
     assert_agg_equal(got, oracle.aggregate(batch, spec, sum_mode=1), spec)
     assert spills == (0 if path in ("interpreter", "exact") else 1)
+    assert hot == (1 if path == "hot" else 0)
 
 
 def test_groups_hint_below_the_distinct_keys(backend, stream, oracle, path):
@@ -396,3 +400,33 @@ def test_groups_hint_below_the_distinct_keys(backend, stream, oracle, path):
     got = backend.download(stream, st.finish(stream))
     want = oracle.aggregate(pa.Table.from_batches(batches).combine_chunks().to_batches()[0], spec, sum_mode=1)
     assert_agg_equal(got, want, spec)
+
+
+@pytest.mark.parametrize("null_keys", [False, True])
+def test_hot_groups_of_a_skewed_key(backend, stream, oracle, path, null_keys):
+    # a Zipf-like key over 200K values: the sampled hot groups take most
+    # rows in LDS, the rest spill or go to the HBM table; every group's
+    # result is the same whichever way its rows went
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    rng = np.random.default_rng(57)
+    n = 3_000_000
+    ranks = np.minimum((rng.pareto(0.4, n) * 3).astype(np.int64), 199_999)
+    perm = rng.permutation(200_000).astype(np.int64) - 50_000
+    k = pa.array(perm[ranks], mask=(rng.random(n) < 0.02) if null_keys else None)
+    x = pa.array(rng.integers(-500, 500, n).astype(np.int64), mask=rng.random(n) < 0.03)
+    v = pa.array(rng.uniform(0.5, 5, n))
+    batch = pa.record_batch({"k": k, "x": x, "v": v})
+    spec = AggregateSpec(0, [Aggregate(F.CountStar, None), Aggregate(F.Sum, col(1)), Aggregate(F.Min, col(1)),
+                             Aggregate(F.Sum, col(2))], filter=plan.cmp(col(1), Cmp.NotEq, Lit.int64(7)))
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        got = device_aggregate(backend, stream, batch, spec)
+        hot = autotune.launches("aggregate_hot")
+    finally:
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
+    assert_agg_equal(got, oracle.aggregate(batch, spec, sum_mode=1), spec)
+    assert hot == (1 if path == "hot" else 0)

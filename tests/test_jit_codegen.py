@@ -123,7 +123,7 @@ def test_groupby_and_star_kernels():
     assert "kAnyNullable = false" in src and "aggregate_tiles<Q, 16>" in src
     c3 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False)]
     src = backend.aggregate_kernel_source(c3, datagen.star_query_spec(), probe_key=0, compile=True)
-    assert "join_group(p, (int64_t)x.c0, key, kv)" in src
+    assert "join_group(p, (int64_t)x.c0, key, kv, hid)" in src
 
 
 def test_aggregation_without_aggregates_uses_the_interpreter():

@@ -18,21 +18,22 @@ from tests.test_aggregate_gpu import assert_agg_equal
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["generated", "exact", "spill", "interpreter"])
+@pytest.fixture(autouse=True, params=["generated", "exact", "spill", "hot", "interpreter"])
 def path(request):
     # unique-key joins run on the generated kernel, others on the
     # interpreter; exact: the generated kernel with compensated FP64 sums;
-    # spill: cold groups spilled, partitioned and folded in LDS
+    # spill: cold groups spilled, partitioned and folded in LDS; hot: the
+    # spill plus sampled hot groups in dense LDS accumulators
     from oxidelake_amd import autotune
-    from tests.test_aggregate_gpu import SPILL_KNOBS
+    from tests.test_aggregate_gpu import HOT_KNOBS, SPILL_KNOBS
 
     autotune.set_knob("agg.jit", 0 if request.param == "interpreter" else 1)
     autotune.set_knob("agg.exact_f64", 1 if request.param == "exact" else 0)
-    if request.param == "spill":
-        for k, v in SPILL_KNOBS.items():
+    if request.param in ("spill", "hot"):
+        for k, v in (SPILL_KNOBS if request.param == "spill" else HOT_KNOBS).items():
             autotune.set_knob(k, v)
     yield request.param
-    for k in ("agg.jit", "agg.exact_f64", *SPILL_KNOBS):
+    for k in ("agg.jit", "agg.exact_f64", *HOT_KNOBS):
         autotune.set_knob(k, -1)
 
 
@@ -93,7 +94,7 @@ def test_filter_and_streaming_pieces(backend, stream, oracle):
     assert_agg_equal(got, want, spec)
 
 
-def test_star_query_with_zipf_skew(backend, stream, oracle):
+def test_star_query_with_zipf_skew(backend, stream, oracle, path):
     # SUM(qty), SUM(qty*price), COUNT(*) GROUP BY grp over fact JOIN dim ON fk = pk
     nf, nd, groups = 2_000_000, 100_000, 10_000
     fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=5)
@@ -102,12 +103,23 @@ def test_star_query_with_zipf_skew(backend, stream, oracle):
     fact, dim = backend.download(stream, dfact), backend.download(stream, ddim)
     aggs = datagen.star_query_spec().aggregates
     table = backend.prepare_join_build(stream, ddim, 0)
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
     st = AggState(backend, stream, dfact, AggregateSpec(-1, aggs))
-    st.update_join(stream, table, ddim, 1, dfact, 0)
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        st.update_join(stream, table, ddim, 1, dfact, 0)
+        hot = autotune.launches("aggregate_hot")
+    finally:
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
     got = backend.download(stream, st.finish(stream))
     want, spec = oracle_join_agg(oracle, fact, dim, aggs)
     assert got.num_rows == want.num_rows <= groups
     assert_agg_equal(got, want, spec)
+    assert hot == (1 if path == "hot" else 0)
     # Zipf: the hottest key alone is a large share of the rows
     counts = got.column(3).to_numpy()
     assert counts.sum() == nf and counts.max() > nf // 20

@@ -751,8 +751,9 @@ __global__ void gos_kernel(JTable t, DCol grp, uint64_t nslots, uint64_t *gos32,
 
 // ------------------------------------------------------------------- spill
 // Rows the generated kernel's block tables do not hold (the cold tail of a
-// high-cardinality key) are spilled as records (group index | cm << 32, then
-// one contribution per aggregate), each block appending to 32 coarse
+// high-cardinality key) are spilled as records (the group index, then the
+// row's contributions in as few 32-bit words as their types allow), each
+// block appending to 32 coarse
 // buckets (their tails stay in L2 while they fill, so the writes reach HBM
 // as whole lines), with a histogram over partitions of 2^slog2 groups.
 // Random HBM atomics run at ~24 G/s on this device whatever their scope or
@@ -760,112 +761,13 @@ __global__ void gos_kernel(JTable t, DCol grp, uint64_t nslots, uint64_t *gos32,
 // partitioned (scatter: one coarse bucket of one block per workgroup, a few
 // fine partitions each) and each partition is folded in LDS by one
 // workgroup (fold), which adds one update per (group, aggregate) to the HBM
-// table.
-
-// Workgroup w = b * kSpillBuckets + c moves block b's bucket c to its fine
-// partitions' ranges (offs[part * B + b] = start of block b's records of
-// that partition; bucket c holds partitions [c * sper, (c + 1) * sper)).
-// W = words per record; each thread moves 4 records per step, loads first.
-template <int W>
-__global__ __launch_bounds__(256) void spill_scatter_kernel(const int64_t *spill, uint64_t cap,
-                                                            const uint32_t *spill_n, const uint32_t *offs,
-                                                            uint32_t B, uint32_t nparts, uint32_t sper,
-                                                            uint32_t slog2, int64_t *out) {
-  __shared__ uint32_t s_cur[kMaxSpillParts];
-  const uint32_t w = blockIdx.x, b = w / kSpillBuckets, c = w % kSpillBuckets;
-  const uint32_t p0 = c * sper;
-  for (uint32_t f = threadIdx.x; f < sper && p0 + f < nparts; f += 256) s_cur[f] = offs[(uint64_t)(p0 + f) * B + b];
-  __syncthreads();
-  const uint64_t n = spill_n[w];
-  const int64_t *src = spill + (uint64_t)w * cap * W;
-  for (uint64_t r0 = threadIdx.x; r0 < n; r0 += 4 * 256) {
-    int64_t rec[4][W];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const uint64_t r = r0 + k * 256;
-#pragma unroll
-      for (int x = 0; x < W; x++) rec[k][x] = r < n ? src[r * W + x] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (r0 + k * 256 >= n) continue;
-      const uint32_t pos = atomicAdd(&s_cur[((uint32_t)rec[k][0] >> slog2) - p0], 1u);
-      int64_t *dst = out + (uint64_t)pos * W;
-#pragma unroll
-      for (int x = 0; x < W; x++) dst[x] = rec[k][x];
-    }
-  }
-}
+// table. The records' layout depends on the query's aggregates (SpillRec),
+// so both passes are generated with the aggregation kernel (spill_scatter,
+// spill_fold).
 
 struct FoldKinds {
   uint8_t kind[kMaxAggs];
 };
-
-// Workgroup p folds partition p's records into an LDS table of 2^slog2
-// groups (accumulators, non-NULL flags, used flags: plain byte stores), then
-// adds each used group's accumulators to the HBM table.
-template <int W>
-__global__ __launch_bounds__(256) void spill_fold_kernel(const int64_t *recs, const uint32_t *offs, uint32_t B,
-                                                         uint32_t slog2, int64_t smin, Table t, FoldKinds fk) {
-  constexpr int na = W - 1;
-  extern __shared__ int64_t s_dyn[];
-  const uint32_t G = 1u << slog2;
-  int64_t *s_acc = s_dyn;                                  // [na][G]
-  uint8_t *s_used = (uint8_t *)(s_acc + (uint64_t)na * G);  // [G]
-  uint8_t *s_has = s_used + G;                             // [na][G]
-  for (uint32_t i = threadIdx.x; i < G; i += 256) {
-    for (int a = 0; a < na; a++) {
-      s_acc[a * G + i] = acc_init(fk.kind[a]);
-      s_has[a * G + i] = 0;
-    }
-    s_used[i] = 0;
-  }
-  __syncthreads();
-  const uint32_t part = blockIdx.x;
-  const uint64_t lo = offs[(uint64_t)part * B], hi = offs[(uint64_t)(part + 1) * B];
-  for (uint64_t r0 = lo + threadIdx.x; r0 < hi; r0 += 4 * 256) {
-    int64_t rec[4][W];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const uint64_t r = r0 + k * 256;
-#pragma unroll
-      for (int x = 0; x < W; x++) rec[k][x] = r < hi ? recs[r * W + x] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (r0 + k * 256 >= hi) continue;
-      const uint64_t w0 = (uint64_t)rec[k][0];
-      const uint32_t g = (uint32_t)w0 & (G - 1), cm = (uint32_t)(w0 >> 32);
-      s_used[g] = 1;
-#pragma unroll
-      for (int a = 0; a < na; a++) {
-        if (!((cm >> a) & 1u)) continue;
-        acc_apply_lds(fk.kind[a], &s_acc[a * G + g], rec[k][1 + a]);
-        s_has[a * G + g] = 1;
-      }
-    }
-  }
-  __syncthreads();
-  bool ok = true;
-  for (uint32_t i = threadIdx.x; i < G && ok; i += 256) {
-    if (!s_used[i]) continue;
-    const int64_t key = (int64_t)((uint64_t)smin + (uint64_t)part * G + i);
-    const uint64_t gs = slot_of(t, true, key, &ok);
-    if (!ok) break;
-    for (int a = 0; a < na; a++) {
-      const int64_t v = s_acc[a * G + i];
-      if (v != acc_init(fk.kind[a]) || fk.kind[a] == A_SUM_F64) acc_apply_global(fk.kind[a], &t.acc[a][gs], v);
-      if (t.has[a] && s_has[a * G + i]) t.has[a][gs] = 1;
-    }
-  }
-  if (!ok) atomicOr(t.overflow, 1u);
-}
-
-// LDS bytes of the fold for 2^slog2 groups and na aggregates.
-size_t fold_lds(uint32_t slog2, int na) {
-  const size_t G = (size_t)1 << slog2;
-  return G * 8 * na + G + G * na;
-}
 
 // ------------------------------------------------------------- hot groups
 // Where a sampled row's group comes from: the key column itself, or the
@@ -1174,16 +1076,17 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
       A.dense_null = in->cols[st->key_col].validity ? 1 : 0;
     }
   }
-  // spill the cold tail of a high-cardinality key (see spill_fold_kernel):
+  // spill the cold tail of a high-cardinality key (see spill_fold):
   // a measured key range that the fold's partitions cover, every aggregate
   // in the block table, plain FP64 sums, and enough rows to pay for it
   const int na = (int)st->aggs.size();
   const bool exact = tune_get("agg.exact_f64", 0) > 0;
   uint32_t slog2 = 0, nparts = 0;
-  if (fn == ja.fn && kr && kr->known && kr->count > 0 && !exact && na <= kLdsAggs &&
+  int nnull = 0;
+  for (int a = 0; a < na; a++) nnull += st->aggs[a].nullable != 0;
+  if (fn == ja.fn && ja.spill && kr && kr->known && kr->count > 0 && !exact && na <= kLdsAggs &&
       tune_get("agg.spill", 1) > 0 && in->rows >= (uint64_t)std::max<int64_t>(0, tune_get("agg.spill_rows", 1 << 20))) {
-    slog2 = 14;
-    while (slog2 > 5 && fold_lds(slog2, na) > 64 * 1024) slog2--;
+    slog2 = fold_log2(na, nnull);
     const uint64_t parts = (kr->count + (1ull << slog2) - 1) >> slog2;
     if (parts <= (uint64_t)kMaxSpillParts && kr->count < (1ull << 32)) nparts = (uint32_t)parts;
   }
@@ -1202,9 +1105,9 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   const uint64_t per_cu =
       (uint64_t)std::max<int64_t>(1, hot_try ? tune_get("agg.hot_grid", 8) : tune_get("agg.jit_grid", 4));
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
-  const uint32_t words = (uint32_t)na + 1;
+  const uint32_t words = (uint32_t)ja.spill_words;  // 32-bit words per record
   void *spill_mem = nullptr;
-  int64_t *spill = nullptr;
+  uint32_t *spill = nullptr;
   uint32_t *spill_n = nullptr, *hist = nullptr, *offs = nullptr;
   uint64_t cap = 0;
   const uint32_t sper = nparts ? (nparts + kSpillBuckets - 1) / kSpillBuckets : 1;
@@ -1218,11 +1121,11 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     cap = std::min<uint64_t>(cap, (uint64_t)(UINT32_MAX / 2) / (kSpillBuckets * words));
     const uint64_t recs = grid * kSpillBuckets * cap;
     const uint64_t nh = (uint64_t)nparts * grid + 1;
-    const size_t bytes = recs * words * 8 + (grid * kSpillBuckets + 2 * nh) * 4 + 1024;
+    const size_t bytes = recs * words * 4 + (grid * kSpillBuckets + 2 * nh) * 4 + 1024;
     if (stream_alloc(s, bytes, &spill_mem) == OXR_OK) {
       char *q = (char *)spill_mem;
-      spill = (int64_t *)q;
-      q += recs * words * 8;
+      spill = (uint32_t *)q;
+      q += (recs * words * 4 + 255) / 256 * 256;
       spill_n = (uint32_t *)q;
       q += grid * kSpillBuckets * 4;
       hist = (uint32_t *)q;
@@ -1298,38 +1201,28 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     if (rc == OXR_OK && e == hipSuccess) e = hipStreamSynchronize(s->s);
     if (rc == OXR_OK && e == hipSuccess) total = *pin_total;
     void *sorted_mem = nullptr;
-    if (rc == OXR_OK && e == hipSuccess) rc = stream_alloc(s, (size_t)std::max<uint32_t>(total, 1) * words * 8, &sorted_mem);
-    int64_t *sorted = (int64_t *)sorted_mem;
+    if (rc == OXR_OK && e == hipSuccess) rc = stream_alloc(s, (size_t)std::max<uint32_t>(total, 1) * words * 4, &sorted_mem);
+    SpillArgs sa;
+    std::memset(&sa, 0, sizeof sa);
+    sa.spill = spill;
+    sa.spill_n = spill_n;
+    sa.offs = offs;
+    sa.recs = (uint32_t *)sorted_mem;
+    sa.cap = cap;
+    sa.B = (uint32_t)grid;
+    sa.nparts = nparts;
+    sa.sper = sper;
+    sa.slog2 = slog2;
+    sa.smin = kr->min;
+    sa.t = st->t;
+    sa.error = err;
     if (rc == OXR_OK && e == hipSuccess) {
       ProfScope p2(s->s, "aggregate_scatter");
-      const dim3 g((unsigned)(grid * kSpillBuckets)), b(256);
-      switch (words) {
-#define OXR_SCATTER(W)                                                                                     \
-  case W:                                                                                                  \
-    hipLaunchKernelGGL(spill_scatter_kernel<W>, g, b, 0, s->s, spill, cap, spill_n, offs, (uint32_t)grid, \
-                       nparts, sper, slog2, sorted);                                                       \
-    break;
-        OXR_SCATTER(2) OXR_SCATTER(3) OXR_SCATTER(4) OXR_SCATTER(5)
-#undef OXR_SCATTER
-      }
-      e = hipGetLastError();
+      rc = jit_launch_raw(ja.scatter_fn, (unsigned)(grid * kSpillBuckets), s->s, &sa, sizeof sa, 256);
     }
     if (rc == OXR_OK && e == hipSuccess) {
       ProfScope p3(s->s, "aggregate_fold");
-      FoldKinds fk{};
-      for (int a = 0; a < na; a++) fk.kind[a] = st->aggs[a].kind;
-      const dim3 g(nparts), b(256);
-      const size_t lds = fold_lds(slog2, na);
-      switch (words) {
-#define OXR_FOLD(W)                                                                                          \
-  case W:                                                                                                    \
-    hipLaunchKernelGGL(spill_fold_kernel<W>, g, b, lds, s->s, sorted, offs, (uint32_t)grid, slog2, kr->min, \
-                       st->t, fk);                                                                           \
-    break;
-        OXR_FOLD(2) OXR_FOLD(3) OXR_FOLD(4) OXR_FOLD(5)
-#undef OXR_FOLD
-      }
-      e = hipGetLastError();
+      rc = jit_launch_raw(ja.fold_fn, nparts, s->s, &sa, sizeof sa, 256);
     }
     if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "aggregate spill passes");
     stream_release(s, tmp);
@@ -1428,12 +1321,12 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail_hip(e, "aggregate_kernel launch");
   }
-  uint64_t host[4] = {0, 0, 0, 0};  // error, distinct, overflow
+  uint64_t host[4] = {0, 0, 0, 0};  // errors (divide by zero | spill fold << 32), distinct, overflow
   uint64_t *pin = nullptr;
   if (rc == OXR_OK) rc = stream_pinned(s, 32, (void **)&pin);
   if (rc == OXR_OK) {
     pin[0] = pin[1] = pin[2] = 0;
-    hipError_t e = hipMemcpyAsync(&pin[0], err, 4, hipMemcpyDeviceToHost, s->s);
+    hipError_t e = hipMemcpyAsync(&pin[0], err, 8, hipMemcpyDeviceToHost, s->s);
     if (e == hipSuccess) e = hipMemcpyAsync(&pin[1], st->t.distinct, 8, hipMemcpyDeviceToHost, s->s);
     if (e == hipSuccess) e = hipMemcpyAsync(&pin[2], st->t.overflow, 4, hipMemcpyDeviceToHost, s->s);
     if (e == hipSuccess) e = hipStreamSynchronize(s->s);
@@ -1443,6 +1336,8 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   stream_release(s, err);
   if (rc != OXR_OK) return rc;
   if ((uint32_t)host[0]) return fail(OXR_EEXECUTION, "rocm.aggregate", "Divide by zero error");
+  if ((uint32_t)(host[0] >> 32))
+    return fail(OXR_EEXECUTION, "rocm.aggregate", "internal: the spill fold's partitions exceed its LDS table");
   // cannot happen while the sizing above holds (load <= 1/2 for every key
   // the batch can add); never a fallback signal: rows have been applied
   if ((uint32_t)host[2]) return fail(OXR_EEXECUTION, "rocm.aggregate", "the group table overflowed");

@@ -257,14 +257,14 @@ struct AggJitArgs {
   int64_t dense_min;
   uint32_t dense_keys, dense_null;
   // spill: rows the block table does not hold whose key is in [smin, smin +
-  // srange) go to `spill` instead of HBM atomics, as records of 1 + kAggs
-  // words (group index | cm << 32, then the contributions). Block b appends
+  // srange) go to `spill` instead of HBM atomics, as records of
+  // SpillRec<Q>::kWords 32-bit words (see SpillRec). Block b appends
   // to kSpillBuckets buckets of spill_cap records each (bucket c at record
   // (b * kSpillBuckets + c) * spill_cap; partition p goes to bucket p /
   // sper), so a few tails per block stay in L2 while they fill. Per block:
   // the bucket counts (spill_n[b * kSpillBuckets + c]) and a histogram over
   // partitions of 2^slog2 groups (hist[p * gridDim + b]).
-  int64_t *spill;               // NULL: no spilling
+  uint32_t *spill;              // NULL: no spilling
   uint64_t spill_cap;           // records per bucket
   uint32_t *spill_n;
   uint32_t *hist;
@@ -339,27 +339,237 @@ __host__ __device__ constexpr int lo_index(int a) {
   return n;
 }
 
+// A spilled row is a record of 32-bit words: the group index (key - smin);
+// when any input is nullable, the mask of aggregates whose input was not
+// NULL; then aggregate a's contribution in Q::kSpillWords[a] words: 0 for
+// COUNT and COUNT(*) (a contribution of 1, under the mask), 1 for a value
+// that fits 32 bits (Int32 inputs of SUM / MIN / MAX, Float32 MIN / MAX
+// keys; sign-extended when read back), 2 for 64 bits, low word first.
+template <class Q>
+struct SpillRec {
+  static constexpr int kHdr = Q::kAnyNullable ? 2 : 1;
+  static constexpr int off(int a) {
+    int o = kHdr;
+    for (int i = 0; i < a; i++) o += Q::kSpillWords[i];
+    return o;
+  }
+  static constexpr int kWords = off(Q::kAggs);
+  __device__ static inline void pack(uint32_t (&w)[kWords], uint32_t g, uint32_t cm, const int64_t *c) {
+    w[0] = g;
+    if (kHdr == 2) w[1] = cm;
+#pragma unroll
+    for (int a = 0; a < Q::kAggs; a++) {
+      if (Q::kSpillWords[a] >= 1) w[off(a)] = (uint32_t)(uint64_t)c[a];
+      if (Q::kSpillWords[a] == 2) w[off(a) + 1] = (uint32_t)((uint64_t)c[a] >> 32);
+    }
+  }
+  __device__ static inline int64_t value(const uint32_t (&w)[kWords], int a) {
+    if (Q::kSpillWords[a] == 0) return 1;
+    if (Q::kSpillWords[a] == 1) return (int64_t)(int32_t)w[off(a)];
+    return (int64_t)((uint64_t)w[off(a)] | ((uint64_t)w[off(a) + 1] << 32));
+  }
+  __device__ static inline uint32_t mask(const uint32_t (&w)[kWords]) {
+    return kHdr == 2 ? w[1] : (1u << Q::kAggs) - 1u;
+  }
+};
+
+// R consecutive 32-bit words at dst / src (aligned to 4 * R bytes when R is
+// even), in 16- or 8-byte accesses where R allows.
+template <int R>
+__device__ __forceinline__ void store_words(uint32_t *dst, const uint32_t (&w)[R]) {
+  if constexpr (R % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < R / 4; i++) ((uint4 *)dst)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+  } else if constexpr (R % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) ((uint2 *)dst)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; i++) dst[i] = w[i];
+  }
+}
+template <int R>
+__device__ __forceinline__ void load_words(const uint32_t *src, uint32_t (&w)[R]) {
+  if constexpr (R % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < R / 4; i++) {
+      const uint4 v = ((const uint4 *)src)[i];
+      w[4 * i] = v.x;
+      w[4 * i + 1] = v.y;
+      w[4 * i + 2] = v.z;
+      w[4 * i + 3] = v.w;
+    }
+  } else if constexpr (R % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      const uint2 v = ((const uint2 *)src)[i];
+      w[2 * i] = v.x;
+      w[2 * i + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; i++) w[i] = src[i];
+  }
+}
+
 // A selected row (key, kv, contributions c under mask cm) that the block's
 // LDS does not hold: its spill bucket when its key is in the spill range and
 // the bucket has room (true), else the caller applies it to the HBM table.
 template <class Q>
-__device__ __forceinline__ bool spill_row(const AggJitArgs &p, int64_t *spill_blk, uint32_t *s_bucket,
+__device__ __forceinline__ bool spill_row(const AggJitArgs &p, uint32_t *spill_blk, uint32_t *s_bucket,
                                           uint32_t *s_hist, int64_t key, bool kv, const int64_t *c,
                                           uint32_t cm) {
-  constexpr int NA = Q::kAggs;
+  using SR = SpillRec<Q>;
   if (!(p.spill && kv && key >= p.smin && (uint64_t)key - (uint64_t)p.smin < p.srange)) return false;
   const uint32_t g = (uint32_t)((uint64_t)key - (uint64_t)p.smin);
   const uint32_t part = g >> p.slog2, bucket = part / p.sper;
   const uint32_t pos = atomicAdd(&s_bucket[bucket], 1u);
   if (pos >= p.spill_cap) return false;
   // 32-bit offset inside the block's buckets (the host keeps
-  // kSpillBuckets * spill_cap * (NA + 1) below 2^31)
-  int64_t *r = spill_blk + (bucket * (uint32_t)p.spill_cap + pos) * (uint32_t)(NA + 1);
-  r[0] = (int64_t)((uint64_t)g | ((uint64_t)cm << 32));
-#pragma unroll
-  for (int a = 0; a < NA; a++) r[1 + a] = c[a];
+  // kSpillBuckets * spill_cap * kWords below 2^31)
+  uint32_t w[SR::kWords];
+  SR::pack(w, g, cm, c);
+  store_words<SR::kWords>(spill_blk + (bucket * (uint32_t)p.spill_cap + pos) * (uint32_t)SR::kWords, w);
   atomicAdd(&s_hist[part], 1u);
   return true;
+}
+
+// The spill's block region: block b's kSpillBuckets buckets of spill_cap
+// records.
+template <class Q>
+__device__ __forceinline__ uint32_t *spill_block(const AggJitArgs &p) {
+  return p.spill ? p.spill + (uint64_t)blockIdx.x * kSpillBuckets * p.spill_cap * SpillRec<Q>::kWords : nullptr;
+}
+
+// log2 of the groups one fold workgroup holds in LDS (8-byte accumulators,
+// a used byte, and a non-NULL byte per nullable aggregate), within 64 KB.
+__host__ __device__ constexpr uint32_t fold_log2(int na, int nnull) {
+  uint32_t l = 14;
+  while (l > 5 && ((uint64_t)1 << l) * (uint64_t)(8 * na + 1 + nnull) > 65536) l--;
+  return l;
+}
+
+// Arguments of the spill's scatter and fold kernels (see aggregate.hip's
+// spill passes): the block buckets (spill, spill_n, spill_cap), the
+// exclusive scan of the histogram (offs[part * B + block] = where block's
+// records of partition part start in recs), and the partitioned records.
+struct SpillArgs {
+  const uint32_t *spill;
+  const uint32_t *spill_n;
+  const uint32_t *offs;
+  uint32_t *recs;
+  uint64_t cap;
+  uint32_t B, nparts, sper, slog2;
+  int64_t smin;
+  Table t;
+  uint32_t *error;  // [1]: the fold's LDS does not hold 2^slog2 groups
+};
+
+// Workgroup w = b * kSpillBuckets + c moves block b's bucket c to its fine
+// partitions' ranges (bucket c holds partitions [c * sper, (c + 1) * sper));
+// each thread moves 4 records per step, loads first.
+template <class Q>
+__device__ __forceinline__ void spill_scatter(const SpillArgs &a) {
+  constexpr int R = SpillRec<Q>::kWords;
+  __shared__ uint32_t s_cur[kMaxSpillParts];
+  const uint32_t w = blockIdx.x, b = w / kSpillBuckets, c = w % kSpillBuckets;
+  const uint32_t p0 = c * a.sper;
+  for (uint32_t f = threadIdx.x; f < a.sper && p0 + f < a.nparts; f += 256) s_cur[f] = a.offs[(uint64_t)(p0 + f) * a.B + b];
+  __syncthreads();
+  const uint64_t n = a.spill_n[w] < a.cap ? a.spill_n[w] : a.cap;
+  const uint32_t *src = a.spill + (uint64_t)w * a.cap * R;
+  for (uint64_t r0 = threadIdx.x; r0 < n; r0 += 4 * 256) {
+    uint32_t rec[4][R];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint64_t r = r0 + k * 256;
+      if (r < n) load_words<R>(src + r * R, rec[k]);
+      else
+#pragma unroll
+        for (int x = 0; x < R; x++) rec[k][x] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (r0 + k * 256 >= n) continue;
+      const uint32_t pos = atomicAdd(&s_cur[(rec[k][0] >> a.slog2) - p0], 1u);
+      store_words<R>(a.recs + (uint64_t)pos * R, rec[k]);
+    }
+  }
+}
+
+// Index of aggregate a among the nullable ones before it.
+template <class Q>
+__host__ __device__ constexpr int null_index(int a) {
+  int n = 0;
+  for (int i = 0; i < a; i++) n += Q::kNullable[i] != 0;
+  return n;
+}
+
+// Workgroup p folds partition p's records into an LDS table of 2^slog2
+// groups (accumulators, used and non-NULL flags: plain byte stores), then
+// adds each used group's accumulators to the HBM table.
+template <class Q>
+__device__ __forceinline__ void spill_fold(const SpillArgs &sa) {
+  using SR = SpillRec<Q>;
+  constexpr int NA = Q::kAggs, R = SR::kWords, NN = null_index<Q>(NA);
+  constexpr uint32_t kG = 1u << fold_log2(NA, NN);
+  __shared__ int64_t s_acc[NA][kG];
+  __shared__ uint8_t s_used[kG];
+  __shared__ uint8_t s_has[NN > 0 ? NN : 1][NN > 0 ? kG : 1];
+  const uint32_t G = 1u << sa.slog2;
+  if (G > kG) {  // the host sizes partitions with fold_log2 too
+    if (threadIdx.x == 0) atomicOr(&sa.error[1], 1u);
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < G; i += 256) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      s_acc[a][i] = acc_init(Q::kKind[a]);
+      if (Q::kNullable[a]) s_has[null_index<Q>(a)][i] = 0;
+    }
+    s_used[i] = 0;
+  }
+  __syncthreads();
+  const uint32_t part = blockIdx.x;
+  const uint64_t lo = sa.offs[(uint64_t)part * sa.B], hi = sa.offs[(uint64_t)(part + 1) * sa.B];
+  for (uint64_t r0 = lo + threadIdx.x; r0 < hi; r0 += 4 * 256) {
+    uint32_t rec[4][R];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint64_t r = r0 + k * 256;
+      if (r < hi) load_words<R>(sa.recs + r * R, rec[k]);
+      else
+#pragma unroll
+        for (int x = 0; x < R; x++) rec[k][x] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (r0 + k * 256 >= hi) continue;
+      const uint32_t g = rec[k][0] & (G - 1), cm = SR::mask(rec[k]);
+      s_used[g] = 1;
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (!((cm >> a) & 1u)) continue;
+        acc_apply_lds(Q::kKind[a], &s_acc[a][g], SR::value(rec[k], a));
+        if (Q::kNullable[a]) s_has[null_index<Q>(a)][g] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  bool ok = true;
+  for (uint32_t i = threadIdx.x; i < G && ok; i += 256) {
+    if (!s_used[i]) continue;
+    const int64_t key = (int64_t)((uint64_t)sa.smin + (uint64_t)part * G + i);
+    const uint64_t gs = slot_of(sa.t, true, key, &ok);
+    if (!ok) break;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      const int64_t v = s_acc[a][i];
+      if (v != acc_init(Q::kKind[a]) || Q::kKind[a] == A_SUM_F64) acc_apply_global(Q::kKind[a], &sa.t.acc[a][gs], v);
+      if (sa.t.has[a] && (!Q::kNullable[a] || s_has[null_index<Q>(a)][i])) sa.t.has[a][gs] = 1;
+    }
+  }
+  if (!ok) atomicOr(sa.t.overflow, 1u);
 }
 
 template <class Q, int S>
@@ -396,8 +606,7 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
     if (tid < kSpillBuckets) s_bucket[tid] = 0;
   }
   __syncthreads();
-  int64_t *const spill_blk =
-      SP && p.spill ? p.spill + (uint64_t)blockIdx.x * kSpillBuckets * p.spill_cap * (NA + 1) : nullptr;
+  uint32_t *const spill_blk = SP ? spill_block<Q>(p) : nullptr;
 
   bool err = false, ok = true;
   const uint64_t ntiles = (p.n + kTile - 1) / kTile;
@@ -433,6 +642,7 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       // one matching round: the lanes sharing the first selected lane's key
       // fold into it, so a hot key costs one update per wave
       const uint64_t act = __ballot(sel);
+      bool merged = false;  // this lane holds other lanes' rows too
       if (act) {
         const int lead = __builtin_ctzll(act);
         const int64_t lk = (int64_t)__shfl((long long)key, lead, 64);
@@ -468,8 +678,12 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
             cmc |= (uint32_t)(__ballot(use) != 0) << a;
           }
           if (same) {
-            if (lane == lead) cm = cmc;
-            else sel = false;
+            if (lane == lead) {
+              cm = cmc;
+              merged = true;
+            } else {
+              sel = false;
+            }
           }
         }
       }
@@ -510,8 +724,10 @@ __device__ __forceinline__ void aggregate_tiles(const AggJitArgs &p) {
       }
       // a row the block table does not hold goes to its spill bucket when
       // its key is in the spill range (and the bucket has room), else to the
-      // HBM table
-      const bool spilled = SP && sel && lslot < 0 && spill_row<Q>(p, spill_blk, s_bucket, s_hist, key, kv, c, cm);
+      // HBM table; a merged row always to HBM (a spill record is one row's:
+      // its COUNT is implied and its Int32 values take 32 bits)
+      const bool spilled =
+          SP && sel && lslot < 0 && !merged && spill_row<Q>(p, spill_blk, s_bucket, s_hist, key, kv, c, cm);
       if (!sel || spilled) continue;
       uint64_t gslot = 0;
       if (lslot < 0 || LA < NA) gslot = slot_of(p.t, kv, key, &ok);
@@ -607,7 +823,7 @@ __device__ __forceinline__ void aggregate_hot(const AggJitArgs &p) {
     if (tid < kSpillBuckets) s_bucket[tid] = 0;
   }
   __syncthreads();
-  int64_t *const spill_blk = p.spill ? p.spill + (uint64_t)blockIdx.x * kSpillBuckets * p.spill_cap * (NA + 1) : nullptr;
+  uint32_t *const spill_blk = spill_block<Q>(p);
 
   bool err = false, ok = true;
   const uint64_t ntiles = (p.n + kTile - 1) / kTile;

@@ -506,19 +506,29 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   }
   // the contributions
   std::string contrib = "    const bool in = sel;\n    (void)in;\n";
-  std::string kinds, nulls;
+  std::string kinds, nulls, swords;
+  bool any_null = false;
+  int spill_words = 1;  // the group index, then the mask and the contributions
   for (int a = 0; a < na; a++) {
+    any_null |= sh.nullable[a] != 0;
     const std::string as = std::to_string(a);
     const uint8_t k = sh.kinds[a];
     kinds += (a ? ", " : "") + std::to_string(k);
     nulls += (a ? ", " : "") + std::string(sh.nullable[a] ? "1" : "0");
     if (k == A_COUNT_STAR) {
       contrib += "    c[" + as + "] = 1;\n    cm |= (uint32_t)sel << " + as + ";\n";
+      swords += (a ? ", 0" : "0");
       continue;
     }
     g.e = sh.inputs[a];
     g.code.clear();
     const Val v = g.value(0, sh.inputs[a]->count);
+    // the 32-bit words of a spilled contribution (SpillRec)
+    const bool narrow = ((k == A_SUM_I64 || k == A_MIN_I64 || k == A_MAX_I64) && v.t == OXR_INT32) ||
+                        k == A_MIN_F32 || k == A_MAX_F32;
+    const int w = k == A_COUNT ? 0 : narrow ? 1 : 2;
+    swords += (a ? ", " : "") + std::to_string(w);
+    spill_words += w;
     std::string cv;
     switch (k) {
       case A_COUNT: cv = "1"; break;
@@ -546,8 +556,7 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   src += "  static constexpr int kLdsAggs = " + std::to_string(la) + ";\n";
   src += "  static constexpr uint8_t kKind[kAggs] = {" + kinds + "};\n";
   src += "  static constexpr uint8_t kNullable[kAggs] = {" + nulls + "};\n";
-  bool any_null = false;
-  for (uint8_t v : sh.nullable) any_null |= v != 0;
+  src += "  static constexpr uint8_t kSpillWords[kAggs] = {" + swords + "};\n";
   src += std::string("  static constexpr bool kAnyNullable = ") + (any_null ? "true" : "false") + ";\n";
   src += std::string("  static constexpr bool kExactF64 = ") + (tune_get("agg.exact_f64", 0) > 0 ? "true" : "false") +
          ";\n";
@@ -567,7 +576,17 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   src += "  aggregate_tiles<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";
   // high-cardinality skewed keys: the hot-group kernel (aggregate_hot), for
   // shapes whose aggregates all fit the block's LDS and sum FP64 plainly
-  ja->hot = la == na && tune_get("agg.exact_f64", 0) == 0 && sh.key_col != -1;
+  // the spill's scatter and fold for this shape's records: shapes whose
+  // rows may spill (a key, every aggregate in the block table, plain FP64
+  // sums)
+  ja->spill = la == na && tune_get("agg.exact_f64", 0) == 0 && sh.key_col != -1;
+  if (ja->spill)
+    src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_spill_scatter(const SpillArgs a) {\n"
+           "  spill_scatter<Q>(a);\n}\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void oxr_spill_fold(const SpillArgs a) {\n"
+           "  spill_fold<Q>(a);\n}\n";
+  ja->spill_words = spill_words + (any_null ? 1 : 0);
+  ja->hot = ja->spill;
   if (ja->hot) {
     // the shape's rows per lane (two tiles of them in registers, the next
     // tile's loads in flight), or the knob agg.hot_lane_rows
@@ -600,6 +619,10 @@ int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJit
     OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense32", &ja.dense32));
   }
   if (ja.hot) OXR_TRY(compile(dev, ja.source, "oxr_aggregate_hot", &ja.hot_fn));
+  if (ja.spill) {
+    OXR_TRY(compile(dev, ja.source, "oxr_spill_scatter", &ja.scatter_fn));
+    OXR_TRY(compile(dev, ja.source, "oxr_spill_fold", &ja.fold_fn));
+  }
   *out = std::move(ja);
   return OXR_OK;
 }

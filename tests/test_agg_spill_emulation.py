@@ -1,6 +1,7 @@
 """CPU emulation of the generated aggregation kernel's row routing with the
-spill enabled (oxidelake_amd/csrc/aggregate_jit.h `aggregate_tiles`, and the
-spill scan, scatter and fold in oxidelake_amd/csrc/aggregate.hip), on the
+spill enabled (oxidelake_amd/csrc/aggregate_jit.h `aggregate_tiles`,
+`spill_scatter` and `spill_fold`, and the spill scan in
+oxidelake_amd/csrc/aggregate.hip), on the
 input of tests/test_aggregate_gpu.py::test_spill_many_groups.
 
 Each wave step is run with the kernel's SIMT semantics: the matching round
@@ -62,9 +63,10 @@ def emulate(k, kvalid, w, S, grid_cap, pct, seed):
     # the key range of the valid keys (key_range_kernel) and the spill shape
     kmin, kmax = int(k[kvalid].min()), int(k[kvalid].max())
     count = kmax - kmin + 1
+    # fold_log2: 4 aggregates (8-byte accumulators), a used byte, one
+    # nullable input (SUM(v))
     slog2 = 14
-    fold_lds = lambda s2, na: (1 << s2) * 8 * na + (1 << s2) + (1 << s2) * na  # noqa: E731
-    while slog2 > 5 and fold_lds(slog2, 4) > 64 * 1024:
+    while slog2 > 5 and (1 << slog2) * (8 * 4 + 1 + 1) > 64 * 1024:
         slog2 -= 1
     nparts = (count + (1 << slog2) - 1) >> slog2
     assert nparts <= MAX_PARTS
@@ -112,6 +114,7 @@ def emulate(k, kvalid, w, S, grid_cap, pct, seed):
                 c = [[1, int(w[r]), int(w[r])] for r in rows]
                 # matching round
                 act = np.nonzero(sel)[0]
+                merged = -1
                 if act.size:
                     lead = act[0]
                     same = sel & (key == key[lead]) & (kv == kv[lead])
@@ -119,6 +122,7 @@ def emulate(k, kvalid, w, S, grid_cap, pct, seed):
                         m = np.nonzero(same)[0]
                         c[lead] = [int(same.sum()), sum(c[i][1] for i in m), max(c[i][2] for i in m)]
                         sel = sel & ~(same & (lanes != lead))
+                        merged = lead
                 # block-table probe in lock step
                 lslot = np.full(64, -1)
                 todo = []
@@ -154,16 +158,18 @@ def emulate(k, kvalid, w, S, grid_cap, pct, seed):
                         h[i] = (h[i] + 1) & (LDS_SLOTS - 1)
                         nxt.append(i)
                     todo = nxt
-                # spill, else HBM, else the block table
+                # spill (single rows only: a record's COUNT(*) is implied 1 and
+                # its Int32 values are 32 bits), else HBM, else the block table
                 for i in np.nonzero(sel)[0]:
                     kk = int(key[i])
-                    if lslot[i] < 0 and kv[i] and kmin <= kk < kmin + count:
+                    if lslot[i] < 0 and i != merged and kv[i] and kmin <= kk < kmin + count:
                         g = kk - kmin
                         part = g >> slog2
                         bucket = part // sper
                         pos = s_bucket[bucket]
                         s_bucket[bucket] += 1
                         if pos < cap:
+                            assert c[i][0] == 1, "a spill record holds one row"
                             blk[bucket].append((g, *c[i]))
                             hist[part, b] += 1
                             continue

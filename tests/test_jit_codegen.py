@@ -124,6 +124,9 @@ def test_groupby_and_star_kernels():
     c3 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False)]
     src = backend.aggregate_kernel_source(c3, datagen.star_query_spec(), probe_key=0, compile=True)
     assert "join_group(p, (int64_t)x.c0, key, kv, hid)" in src
+    # spill records of 4 words: group, SUM(qty) (Int32), SUM(qty*price) (FP64), COUNT(*) (none)
+    assert "kSpillWords[kAggs] = {1, 2, 0}" in src
+    assert "spill_scatter<Q>(a)" in src and "spill_fold<Q>(a)" in src
 
 
 def test_aggregation_without_aggregates_uses_the_interpreter():

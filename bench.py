@@ -603,7 +603,8 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
         "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "device_ms_per_step": parts,
         "build_s": build_s,
         "parity_full_size": ok,
-        "roofline": _roofline_hbm(alg, total_ms),
+        "roofline": _roofline_hbm(alg, total_ms, recorded_traffic("join_aggregate", fact_rows=nf, dim_rows=nd,
+                                                                  groups=args.groups)),
         "cpu_baseline": cpu,
     }
 

@@ -195,6 +195,21 @@ def test_filter_query_parity(backend, stream, oracle, path, threshold):
     assert abs(got.num_rows / n - datagen.filter_selectivity(threshold)) < 0.01
 
 
+@pytest.mark.parametrize("threshold,c_thr,modulus,want", [(30, 0.5, 1000, 0.0098), (180, 0.5, 20, 0.100),
+                                                          (470, -1.0, 5, 0.5006)])
+def test_filter_query_selectivity_points(backend, stream, oracle, path, threshold, c_thr, modulus, want):
+    # the query's shape at about 1%, 10% and 50% selected (the benchmark's
+    # three points: T, C and M moved together)
+    n = 300_001
+    dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=11), n)
+    host = backend.download(stream, dev)
+    pred, proj = datagen.filter_query(threshold, c_thr, modulus)
+    got = backend.download(stream, backend.filter_project(stream, dev, pred, proj, ["a", "b2", "cd"]))
+    assert_batches_equal(got, oracle.filter_project(host, pred, proj, ["a", "b2", "cd"]))
+    assert abs(got.num_rows / n - want) < 0.01
+    assert abs(datagen.filter_selectivity(threshold, c_thr, modulus=modulus) - want) < 0.002
+
+
 def test_batches_outlive_their_stream(backend, oracle):
     # a batch allocated on a stream may be used and freed after the caller
     # destroyed that stream's handle (the stream lives until its last batch)

@@ -268,6 +268,14 @@ def test_float32_min_max_follow_total_order_without_zero_normalisation(oracle):
             continue
         assert int(mins[gi]) == min(members, key=key32), key
         assert int(maxs[gi]) == max(members, key=key32), key
+    # groups of zeros only: totalOrder puts -0.0 below +0.0 whichever comes
+    # first (a normalised order would keep the first one seen)
+    zk = np.repeat(np.arange(20), 2)
+    zv = np.array([[0.0, -0.0], [-0.0, 0.0]] * 10, np.float32).reshape(-1)
+    zb = pa.record_batch({"k": pa.array(zk, pa.int64()), "v": pa.array(zv)})
+    zo = oracle.aggregate(zb, AggregateSpec(0, [Aggregate(F.Min, col(1)), Aggregate(F.Max, col(1))]))
+    assert set(np.frombuffer(zo.column(1).buffers()[1], np.uint32)[:20].tolist()) == {0x80000000}
+    assert set(np.frombuffer(zo.column(2).buffers()[1], np.uint32)[:20].tolist()) == {0}
 
 
 def test_sum_of_int32_is_an_int64_wrapping_sum(oracle):

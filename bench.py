@@ -907,9 +907,9 @@ def bench_serving(args):
     same Poisson arrival stream per offered rate: the serving loop
     (continuous batching of vector queries, one stream per request kind,
     admission by footprint) and the plain one-at-a-time path. Tables: the
-    200M-row filter table, a 100M-row fact table joined to a 10M-row
+    200M-row filter table, the 500M-row fact table joined to the 10M-row
     dimension (1M groups), and a 20M x 768 BF16 vector index with an FP32
-    rerank of 100. Reports throughput and latency p50/p99 per kind, and
+    rerank of 100; vector requests of 1-64 queries. Reports throughput and latency p50/p99 per kind, and
     whether every request's result is the same in the mix as alone."""
     import copy
 
@@ -939,7 +939,7 @@ def bench_serving(args):
     stream.synchronize()
     tables = serving.Tables(ftab, fact, dim, join, index, n, candidates=100, k=10, keep=[emb, q_view])
     # compile each kind's kernels once (prepared: every literal is an argument)
-    warm = serving.poisson_mix(1000.0, 30, queries, nd, seed=1)
+    warm = serving.poisson_mix(1000.0, 30, queries, nd, seed=1, max_queries=args.serve_max_queries)
     serving.run_one_at_a_time(be, tables, warm)
 
     per_rate, same = {}, True
@@ -948,7 +948,7 @@ def bench_serving(args):
     lib.oxr_profile_enable(1)
     for rate in [float(r) for r in args.serve_rates.split(",")]:
         reqs = serving.poisson_mix(rate, max(8, int(rate * args.serve_seconds)), queries, nd,
-                                   seed=int(rate))
+                                   seed=int(rate), max_queries=args.serve_max_queries)
         alone = copy.deepcopy(reqs)
         server = serving.Server(be, tables)
         s_stats = server.run(reqs)
@@ -982,7 +982,8 @@ def bench_serving(args):
         "ms_per_step": 1000.0 / top["serving_loop"]["throughput_rps"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f32/int64/f64",
         "data": "synthetic (seeded SplitMix64 generator, written in HBM); Poisson arrivals",
-        "config": {"workload": "60% vector top-10 of 1-16 queries (20M x 768, BF16 scan + FP32 rerank 100), "
+        "config": {"workload": f"60% vector top-10 of 1-{args.serve_max_queries} queries (20M x 768, BF16 scan "
+                               "+ FP32 rerank 100), "
                                "30% COUNT/SUM WHERE b < T over 200M rows, 10% SUM ... GROUP BY grp over "
                                f"{nf} fact rows JOIN {nd} dimension rows WHERE fk in a 1-10% range",
                    "rates_rps": args.serve_rates, "seconds_per_rate": args.serve_seconds},

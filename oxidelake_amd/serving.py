@@ -2,7 +2,7 @@
 batching, admission by device-memory footprint, overlap on the device.
 
 Three request kinds, over tables that stay resident for the server's life:
-  - "vector": top-k of 1..16 queries over a vector index (BF16 scan + FP32
+  - "vector": top-k of 1..64 queries over a vector index (BF16 scan + FP32
     rerank by default);
   - "filter_agg": COUNT(*), SUM(a), SUM(c*d) FROM t WHERE b < T over the
     200M-row filter table (prepared: one compiled kernel for every T);
@@ -70,6 +70,7 @@ class Tables:
     index: VectorIndex
     vec_rows: int             # rows the index holds
     candidates: int = 100     # rerank depth of the vector index
+    vec_dim: int = 768        # the index's vector size
     k: int = 10
     keep: list = field(default_factory=list)  # anything the tables view into
 
@@ -119,13 +120,43 @@ class MemoryGate:
             self.cv.notify_all()
 
 
+def vector_scratch(rows: int, dim: int, nq: int, candidates: int) -> int:
+    """Device scratch of one top-k search (oxidelake_amd/csrc/vector.hip
+    `search`): queries and results, then per pass of up to 256 queries the
+    prepared queries and either the pilot path (an index of at least 4 pilots
+    of max(1M, 2048 C) rows: the pilot's score tile and segment candidates,
+    the bests, the append list of max(64K, 4 C x pilots in the rest) keys per
+    query) or the plain path (score tiles of at most 2 GiB, segment
+    candidates of at least 32 C rows). A pilot whose list overflows reruns
+    the plain path on top: rare, and outside this estimate."""
+    C = max(candidates, 10)
+    total = nq * dim * 4 + nq * 12 * 16
+    pilot = max(1 << 20, 2048 * C)
+    for q0 in range(0, nq, 256):
+        q = min(256, nq - q0)
+        total += q * dim * 4 + q * 4 + 2 * q * C * 8  # prepared queries, scales, bests
+
+        def scan_select(n):
+            tile = min(n, max(128, (2 << 30) // (4 * q)))
+            seg = max(8192, 32 * C, -(-tile // max(1, 2048 // q)))
+            segs = -(-n // tile) * -(-tile // seg)
+            return q * tile * 4 + q * segs * C * 8
+
+        if rows >= 4 * pilot:
+            rest = rows - pilot
+            cap = max(65536, 4 * C * -(-rest // pilot))
+            total += scan_select(pilot) + q * C * 8 + q * cap * 8 + q * 4
+        else:
+            total += scan_select(rows)
+    return total
+
+
 def footprint(tables: Tables, kind: str, reqs: list[Request]) -> int:
     """Estimated device bytes a batch holds while it runs (scratch and
     results; the resident tables are not counted)."""
     if kind == "vector":
-        # scores of one scan tile (at most 2 GiB) and the per-block candidates
         nq = sum(len(r.payload) for r in reqs)
-        return min(2 << 30, nq * tables.vec_rows * 4) + nq * tables.candidates * 8 * 4096
+        return vector_scratch(tables.vec_rows, tables.vec_dim, nq, tables.candidates)
     if kind == "join_agg":
         # a slot-to-group entry per dimension row and one group state per row
         return tables.dim.num_rows * (8 + 64)
@@ -268,17 +299,19 @@ def stats(requests: list[Request], elapsed: float) -> dict:
     return out
 
 
-def poisson_mix(rate: float, n: int, query_pool: np.ndarray, dim_rows: int, seed: int = 7) -> list[Request]:
+def poisson_mix(rate: float, n: int, query_pool: np.ndarray, dim_rows: int, seed: int = 7,
+                max_queries: int = 64) -> list[Request]:
     """n requests arriving as a Poisson process at `rate` per second: 60%
-    vector (1-16 queries from the pool), 30% filter_agg (T in [1, 500]: 0.1%
-    to 50% of the rows), 10% join_agg (an fk range of 1-10% of the keys)."""
+    vector (1 to max_queries queries from the pool), 30% filter_agg (T in
+    [1, 500]: 0.1% to 50% of the rows), 10% join_agg (an fk range of 1-10%
+    of the keys)."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n))
     out = []
     for i in range(n):
         u = rng.random()
         if u < 0.6:
-            q = int(rng.integers(1, 17))
+            q = int(rng.integers(1, max_queries + 1))
             at = rng.integers(0, len(query_pool), q)
             out.append(Request(i, "vector", float(t[i]), query_pool[at]))
         elif u < 0.9:

@@ -28,6 +28,7 @@ class _Backend:
 
 class _Tables:
     vec_rows = 1000
+    vec_dim = 8
     candidates = 10
 
     class dim:
@@ -61,7 +62,8 @@ def test_every_request_answered_once_and_vector_batches_capped(monkeypatch):
     for kind, ids in log:
         assert all(by_id[i].kind == kind for i in ids)
         if kind == "vector":
-            assert sum(len(by_id[i].payload) for i in ids) <= 40
+            # capped, except a lone request larger than the cap
+            assert len(ids) == 1 or sum(len(by_id[i].payload) for i in ids) <= 40
         else:
             assert len(ids) == 1  # aggregates run one request per call
     # within a kind, batches keep arrival order
@@ -133,7 +135,7 @@ def test_poisson_mix_shape():
     assert np.all(np.diff(arr) >= 0) and 15 < arr[-1] < 25  # ~2000 / 100 s
     for r in reqs:
         if r.kind == "vector":
-            assert 1 <= len(r.payload) <= 16
+            assert 1 <= len(r.payload) <= 64
         elif r.kind == "filter_agg":
             assert 1 <= r.payload <= 500
         else:
@@ -193,3 +195,15 @@ def test_specs_compile_in_prepared_mode(kind):
         assert len(srcs) == 1 and "L[0]" in srcs.pop()
     finally:
         autotune.set_knob("jit.params", -1)
+
+
+def test_vector_footprint_follows_the_pilot_path():
+    # an index of at least 4 pilots scans its pilot into a score tile and
+    # appends the rest to a list: nowhere near a score tile of every row
+    big = serving.vector_scratch(20_000_000, 768, 16, 100)
+    assert big < 200 << 20
+    assert big > 16 * (1 << 20) * 4  # the pilot's scores (1M rows) are there
+    # below 4 pilots the plain path scores every row (tiles of at most 2 GiB)
+    small = serving.vector_scratch(3_000_000, 768, 16, 100)
+    assert 16 * 3_000_000 * 4 <= small < 16 * 3_000_000 * 4 + (64 << 20)
+    assert serving.vector_scratch(20_000_000, 768, 512, 1000) > serving.vector_scratch(20_000_000, 768, 256, 1000)

@@ -282,16 +282,28 @@ def bench_filter(args, be, stream) -> dict:
 
         hb = [host.slice(i, BATCH_ROWS) for i in range(0, host.num_rows, BATCH_ROWS)]
         pred, proj = datagen.filter_query(args.b_threshold)
-        e2e = {"batches": len(hb), "transfers": "pageable host Arrow buffers, per batch"}
-        for lanes in (1, 4):
-            sf = StreamingFilter(be, pred, proj, ["a", "b2", "cd"], lanes=lanes)
-            sf.run(hb[:4])  # warm
+        e2e = {"batches": len(hb), "transfers": "host Arrow buffers (pageable), per batch: uploaded as they "
+                                                 "are (pageable) or through each stream's pinned ring (staged)"}
+        in_bytes = sum(b.nbytes for b in hb[:8])  # the slices' own byte ranges
+        for staging in (0, 1):
+            autotune.set_knob("transfer.staging", staging)
+            mode = "staged" if staging else "pageable"
+            # upload alone: host to HBM bytes per second of 8 batches
+            be.upload(stream, hb[0])
             t0 = time.perf_counter()
-            res = sf.run(hb)
-            sec = time.perf_counter() - t0
-            e2e[f"rows_per_s_{lanes}_stream{'s' if lanes > 1 else ''}"] = host.num_rows / sec
-            e2e[f"rows_ok_{lanes}"] = sum(r.num_rows for r in res) == want_rows
-            res = None
+            for b in hb[:8]:
+                be.upload(stream, b)
+            e2e[f"upload_gbs_{mode}"] = in_bytes / (time.perf_counter() - t0) / 1e9
+            for lanes in (1, 4):
+                sf = StreamingFilter(be, pred, proj, ["a", "b2", "cd"], lanes=lanes)
+                sf.run(hb[:4])  # warm
+                t0 = time.perf_counter()
+                res = sf.run(hb)
+                sec = time.perf_counter() - t0
+                e2e[f"rows_per_s_{lanes}_stream{'s' if lanes > 1 else ''}_{mode}"] = host.num_rows / sec
+                e2e[f"rows_ok_{lanes}_{mode}"] = sum(r.num_rows for r in res) == want_rows
+                res = None
+        autotune.set_knob("transfer.staging", -1)
         log(f"end to end: {e2e}")
         hb = None
     host = None

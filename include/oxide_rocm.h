@@ -160,13 +160,17 @@ int oxr_copy_d2h(oxr_stream *s, const oxr_buffer *src, void *dst, size_t bytes);
  * Arrow struct array. Columns of a type the device cannot hold, or a vector
  * column with a NULL element in a valid row (columns.rs:211-241), are
  * OXR_EUNSUPPORTED ("device.column_type" / "device.vector_null_element").
- * The caller keeps ownership of schema/array. */
+ * The caller keeps ownership of schema/array. The buffers travel through the
+ * stream's double-buffered pinned staging ring (knobs transfer.staging,
+ * transfer.chunk_kib): the reference copies from pageable memory
+ * (cuda/mod.rs:445-498); the ring overlaps each chunk's host copy with the
+ * other chunk's DMA. Blocks until the copies are done. */
 int oxr_upload(oxr_stream *s, const struct ArrowSchema *schema,
                const struct ArrowArray *array, oxr_batch **out);
 
 /* `GpuBackend::download` (backend.rs:91-95): exports the batch as an Arrow
- * struct array with host buffers; the caller releases both structs. Blocks
- * until the copies are done. */
+ * struct array with host buffers; the caller releases both structs. Through
+ * the same pinned ring as oxr_upload. Blocks until the copies are done. */
 int oxr_download(oxr_stream *s, const oxr_batch *batch,
                  struct ArrowSchema *out_schema, struct ArrowArray *out_array);
 

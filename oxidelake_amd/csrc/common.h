@@ -104,6 +104,11 @@ struct oxr_stream {
   oxr::StreamCore *core = nullptr;
   void *pinned = nullptr;   // small result read-backs (see stream_pinned)
   size_t pinned_len = 0;
+  // the pinned staging ring of upload / download: two halves of stage_len
+  // bytes, each with the event of the last copy that used it
+  void *stage[2] = {nullptr, nullptr};
+  size_t stage_len = 0;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct oxr_buffer {

@@ -287,6 +287,8 @@ const KnobSpec kKnobs[] = {
     {"jit.params", 0, 1, kBool},
     {"jit.stream_loads", 0, 1, kBool},       // generated kernels: nontemporal column loads
     {"jit.stream_stores", 0, 1, kBool},      // generated filter: nontemporal projection stores
+    {"transfer.staging", 0, 1, kBool},       // upload / download through the stream's pinned ring
+    {"transfer.chunk_kib", 64, 65536, 0},    // its chunk size in KiB (default 1024)
     {"agg.jit", 0, 1, kBool},
     {"agg.rows", 1, 32, 0},                  // generated aggregate: rows per lane
     {"agg.grid", 1, 64, 0},
@@ -540,6 +542,12 @@ int oxr_stream_wrap(oxr_device *dev, void *hip_stream, oxr_stream **out) {
 void oxr_stream_destroy(oxr_stream *s) {
   if (!s) return;
   if (s->pinned) (void)hipHostFree(s->pinned);
+  // no copy through the ring is in flight: upload and download return
+  // after their stream synchronised
+  for (int i = 0; i < 2; i++) {
+    if (s->stage[i]) (void)hipHostFree(s->stage[i]);
+    if (s->stage_ev[i]) (void)hipEventDestroy(s->stage_ev[i]);
+  }
   // the HIP stream itself goes when the last batch/table allocated on it does
   release(s->core);
   delete s;
@@ -603,6 +611,96 @@ static std::vector<uint8_t> repack_bits(const uint8_t *bits, uint64_t offset, ui
   return out;
 }
 
+// ------------------------------------------------------ pinned staging ring
+// Arrow buffers arrive in pageable host memory. With the knob
+// transfer.staging on (the default), upload and download go through the
+// stream's two pinned halves of transfer.chunk_kib KiB: the host copy of
+// one chunk runs while the other half's DMA is in flight, and a half is
+// reused only once its last DMA has finished (its event). Off, every buffer
+// is one hipMemcpyAsync from pageable memory.
+namespace {
+struct HostCopy {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
+int stage_ring(oxr_stream *s, size_t *chunk) {
+  const size_t want = (size_t)tune_get("transfer.chunk_kib", 1024) * 1024;
+  if (s->stage_len != want) {
+    for (int i = 0; i < 2; i++) {
+      if (s->stage[i]) (void)hipHostFree(s->stage[i]);
+      s->stage[i] = nullptr;
+    }
+    s->stage_len = 0;
+    for (int i = 0; i < 2; i++) {
+      OXR_HIP(hipHostMalloc(&s->stage[i], want, hipHostMallocDefault));
+      if (!s->stage_ev[i]) OXR_HIP(hipEventCreateWithFlags(&s->stage_ev[i], hipEventDisableTiming));
+    }
+    s->stage_len = want;
+  }
+  *chunk = want;
+  return OXR_OK;
+}
+
+// Host to device through the ring; returns with the last DMAs queued.
+int stage_h2d(oxr_stream *s, const std::vector<HostCopy> &copies) {
+  size_t chunk = 0;
+  OXR_TRY(stage_ring(s, &chunk));
+  bool busy[2] = {false, false};
+  int slot = 0;
+  for (const HostCopy &c : copies) {
+    for (size_t off = 0; off < c.bytes; off += chunk) {
+      const size_t m = std::min(chunk, c.bytes - off);
+      if (busy[slot]) OXR_HIP(hipEventSynchronize(s->stage_ev[slot]));
+      std::memcpy(s->stage[slot], (const char *)c.src + off, m);
+      OXR_HIP(hipMemcpyAsync((char *)c.dst + off, s->stage[slot], m, hipMemcpyHostToDevice, s->s));
+      OXR_HIP(hipEventRecord(s->stage_ev[slot], s->s));
+      busy[slot] = true;
+      slot ^= 1;
+    }
+  }
+  return OXR_OK;
+}
+
+// Device to host through the ring: two DMAs in flight, each chunk copied
+// out of its half as soon as its DMA is done; returns with every byte home.
+int stage_d2h(oxr_stream *s, const std::vector<HostCopy> &copies) {
+  size_t chunk = 0;
+  OXR_TRY(stage_ring(s, &chunk));
+  struct Pending {
+    int slot;
+    void *dst;
+    size_t bytes;
+  };
+  Pending pend[2];
+  int npend = 0, head = 0, slot = 0;
+  auto drain_one = [&]() -> int {
+    const Pending p = pend[head];
+    head ^= 1;
+    npend--;
+    OXR_HIP(hipEventSynchronize(s->stage_ev[p.slot]));
+    std::memcpy(p.dst, s->stage[p.slot], p.bytes);
+    return OXR_OK;
+  };
+  for (const HostCopy &c : copies) {
+    for (size_t off = 0; off < c.bytes; off += chunk) {
+      const size_t m = std::min(chunk, c.bytes - off);
+      if (npend == 2) OXR_TRY(drain_one());  // the oldest chunk used `slot`
+      OXR_HIP(hipMemcpyAsync(s->stage[slot], (const char *)c.src + off, m, hipMemcpyDeviceToHost, s->s));
+      OXR_HIP(hipEventRecord(s->stage_ev[slot], s->s));
+      pend[(head + npend) & 1] = {slot, (char *)c.dst + off, m};
+      npend++;
+      slot ^= 1;
+    }
+  }
+  while (npend) OXR_TRY(drain_one());
+  return OXR_OK;
+}
+
+bool staging_on() { return tune_get("transfer.staging", 1) > 0; }
+}  // namespace
+
 int oxr_upload(oxr_stream *s, const struct ArrowSchema *schema, const struct ArrowArray *array,
                oxr_batch **out) {
   clear_error();
@@ -627,6 +725,8 @@ int oxr_upload(oxr_stream *s, const struct ArrowSchema *schema, const struct Arr
     }
   };
   std::vector<std::vector<uint8_t>> staged;  // kept alive until the copies finish
+  staged.reserve(schema->n_children);        // (their addresses go into `copies`)
+  std::vector<HostCopy> copies;
   for (int64_t i = 0; i < schema->n_children; i++) {
     const struct ArrowArray *a = array->children[i];
     const struct ArrowSchema *f = schema->children[i];
@@ -677,19 +777,27 @@ int oxr_upload(oxr_stream *s, const struct ArrowSchema *schema, const struct Arr
       const uint8_t *src = (const uint8_t *)values;
       if (types[i] == OXR_F32_VECTOR) src += (child_off + off * (uint64_t)dims[i]) * 4;
       else src += off * w;
-      hipError_t e = hipMemcpyAsync(c.values, src, w * rows, hipMemcpyHostToDevice, s->s);
-      if (e != hipSuccess) {
-        cleanup();
-        return fail_hip(e, "hipMemcpyAsync(upload values)");
-      }
+      copies.push_back({c.values, src, w * rows});
       if (nullable) {
         staged.push_back(repack_bits(valid, off, rows));
-        e = hipMemcpyAsync(c.validity, staged.back().data(), staged.back().size(),
-                           hipMemcpyHostToDevice, s->s);
-        if (e != hipSuccess) {
-          cleanup();
-          return fail_hip(e, "hipMemcpyAsync(upload validity)");
-        }
+        copies.push_back({c.validity, staged.back().data(), staged.back().size()});
+      }
+    }
+  }
+  if (staging_on()) {
+    const int rc = stage_h2d(s, copies);
+    if (rc != OXR_OK) {
+      (void)hipStreamSynchronize(s->s);  // nothing still reads the host buffers
+      cleanup();
+      return rc;
+    }
+  } else {
+    for (const HostCopy &c : copies) {
+      const hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, s->s);
+      if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s->s);
+        cleanup();
+        return fail_hip(e, "hipMemcpyAsync(upload)");
       }
     }
   }
@@ -800,6 +908,7 @@ int oxr_download(oxr_stream *s, const oxr_batch *batch, struct ArrowSchema *out_
     return rc;
   };
   std::vector<struct ArrowArray *> nulls_fix;
+  std::vector<HostCopy> copies;
   for (size_t i = 0; i < ncols; i++) {
     const DevCol &c = batch->cols[i];
     const size_t w = type_width(c.type, c.dim);
@@ -827,19 +936,11 @@ int oxr_download(oxr_stream *s, const oxr_batch *batch, struct ArrowSchema *out_
       vbits = (uint8_t *)host_alloc(validity_bytes(rows));
       if (!vbits) return abort_all(fail(OXR_EALLOCATION, "rocm.download", "host allocation"));
       p->mem.push_back(vbits);
-      hipError_t e = hipMemcpyAsync(vbits, c.validity, validity_bytes(rows),
-                                    hipMemcpyDeviceToHost, s->s);
-      if (e != hipSuccess) return abort_all(fail_hip(e, "hipMemcpyAsync(download validity)"));
+      copies.push_back({vbits, c.validity, validity_bytes(rows)});
     }
     void *vals = host_alloc(w * rows);
     if (!vals) return abort_all(fail(OXR_EALLOCATION, "rocm.download", "host allocation"));
-    if (rows) {
-      hipError_t e = hipMemcpyAsync(vals, c.values, w * rows, hipMemcpyDeviceToHost, s->s);
-      if (e != hipSuccess) {
-        std::free(vals);
-        return abort_all(fail_hip(e, "hipMemcpyAsync(download values)"));
-      }
-    }
+    if (rows) copies.push_back({vals, c.values, w * rows});
     if (c.type == OXR_F32_VECTOR) {
       // parent: validity only; child: Float32 values
       p->buffers = {vbits};
@@ -874,6 +975,21 @@ int oxr_download(oxr_stream *s, const oxr_batch *batch, struct ArrowSchema *out_
       nulls_fix.push_back(a);
     } else {
       a->null_count = 0;
+    }
+  }
+  if (staging_on()) {
+    const int rc = stage_d2h(s, copies);
+    if (rc != OXR_OK) {
+      (void)hipStreamSynchronize(s->s);  // nothing still writes the host buffers
+      return abort_all(rc);
+    }
+  } else {
+    for (const HostCopy &c : copies) {
+      const hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s->s);
+      if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s->s);
+        return abort_all(fail_hip(e, "hipMemcpyAsync(download)"));
+      }
     }
   }
   hipError_t e = hipStreamSynchronize(s->s);

@@ -90,6 +90,9 @@ def parse():
                     help="serving: seconds of Poisson arrivals per rate")
     ap.add_argument("--serve-fact-rows", type=int, default=500_000_000)
     ap.add_argument("--serve-max-queries", type=int, default=64)
+    ap.add_argument("--serve-no-batch-agg", action="store_true",
+                    help="serving: run filter_agg requests one create/update/finish chain each "
+                         "instead of batched shared scans replayed from hipGraphs")
     ap.add_argument("--knob", action="append", default=[], metavar="NAME=VALUE",
                     help="set a library tuning knob before running (repeatable)")
     return ap.parse_args()
@@ -953,6 +956,9 @@ def bench_serving(args):
     # compile each kind's kernels once (prepared: every literal is an argument)
     warm = serving.poisson_mix(1000.0, 30, queries, nd, seed=1, max_queries=args.serve_max_queries)
     serving.run_one_at_a_time(be, tables, warm)
+    if not args.serve_no_batch_agg:
+        serving.prepare_filter_batch(be, stream, tables)  # filter_agg requests batched, hipGraph replays
+        serving.Server(be, tables).run(copy.deepcopy(warm))
 
     per_rate, same = {}, True
     mismatches = 0
@@ -1000,6 +1006,7 @@ def bench_serving(args):
                                f"{nf} fact rows JOIN {nd} dimension rows WHERE fk in a 1-10% range",
                    "rates_rps": args.serve_rates, "seconds_per_rate": args.serve_seconds},
         "rates": per_rate, "same_alone_and_in_mix": same, "mismatches": mismatches,
+        "filter_agg_batched": tables.filter_batch is not None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "vector_scan (BF16): every search's rows over the time of every scan launch "

@@ -404,6 +404,48 @@ int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols, const oxr
                                 int32_t probe_key, int32_t compile, char *buf, size_t cap,
                                 size_t *len);
 
+/* ------------------------------------------------ batched aggregations */
+/* No counterpart in the reference, whose operator runs one aggregation per
+ * plan over its batches (crates/oxidelake-compute/src/operator.rs:994-1038).
+ * For a server answering many aggregation requests over one HBM-resident
+ * table: requests of one prepared shape -- the same filter and aggregates
+ * over one global group (group_by -1), differing only in literal values --
+ * run as one pass over the table (each tile's rows loaded once, every
+ * request's filter and aggregates evaluated on them), and the chain of a
+ * run (the literals copied from pinned host memory, the scan, a final fold,
+ * the results copied back to pinned memory) is captured once per batch size
+ * into a hipGraph and replayed (knob agg.batch_graph; 0 launches it
+ * directly). No device memory is allocated per run.
+ * Every result is that of oxr_aggregate with the request's spec (one row:
+ * COUNT 0 and the rest NULL over no selected row), FP64 sums compensated;
+ * a request's additions run in a fixed order, so its result is the same
+ * bits alone, in any batch, and from run to run. */
+typedef struct oxr_agg_batch oxr_agg_batch;
+
+/* Prepares `spec` over `table` (kept alive by the batch object; its values
+ * must not change while it lives). *max_requests = the requests one run
+ * takes (16 for up to 3 aggregates, fewer for more). OXR_EUNSUPPORTED: a
+ * group key, or a shape outside the generated kernels. */
+int oxr_agg_batch_prepare(oxr_stream *s, const oxr_batch *table, const oxr_agg_spec *spec,
+                          int32_t *max_requests, oxr_agg_batch **out);
+/* The output columns: their count, types (as oxr_aggregate's result
+ * columns) and names (valid while the batch object lives). */
+int oxr_agg_batch_columns(const oxr_agg_batch *b, int32_t *naggs, int32_t *types, const char **names);
+/* Runs n (0..max_requests) requests: requests[r] is the prepared spec with
+ * request r's literal values (OXR_EPLAN otherwise). Writes n * naggs values
+ * (row-major; Int64 and Float64 as their 64 bits, Int32 and Float32 in the
+ * low 32 bits), n * naggs valid flags (0 = NULL) and n statuses (OXR_OK, or
+ * OXR_EEXECUTION for a request whose filter or inputs divided by zero in a
+ * row: that request fails alone). Runs on one batch object take turns;
+ * returns when the results are written. */
+int oxr_agg_batch_run(oxr_stream *s, oxr_agg_batch *b, const oxr_agg_spec *requests, int32_t n,
+                      int64_t *values, uint8_t *valid, int32_t *status);
+void oxr_agg_batch_free(oxr_agg_batch *b);
+/* The batched kernels' generated source for `spec` (as
+ * oxr_aggregate_kernel_source; no device needed). */
+int oxr_agg_batch_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec,
+                                int32_t compile, char *buf, size_t cap, size_t *len);
+
 /* --------------------------------------------------------- vector distance */
 enum { OXR_L2 = 1, OXR_COSINE = 2 }; /* DistanceMetric, params.rs:151-157 */
 

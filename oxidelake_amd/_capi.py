@@ -164,6 +164,17 @@ SIGNATURES: dict[str, tuple] = {
         [C.POINTER(Column), I32, C.POINTER(AggSpec), I32, I32, C.c_char_p, C.c_size_t,
          C.POINTER(C.c_size_t)],
     ),
+    "oxr_agg_batch_prepare": (C.c_int, [P, P, C.POINTER(AggSpec), C.POINTER(I32), PP]),
+    "oxr_agg_batch_columns": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(C.c_char_p)]),
+    "oxr_agg_batch_run": (
+        C.c_int,
+        [P, P, C.POINTER(AggSpec), I32, C.POINTER(C.c_int64), C.POINTER(C.c_uint8), C.POINTER(I32)],
+    ),
+    "oxr_agg_batch_free": (None, [P]),
+    "oxr_agg_batch_kernel_source": (
+        C.c_int,
+        [C.POINTER(Column), I32, C.POINTER(AggSpec), I32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)],
+    ),
     "oxr_vector_distance": (
         C.c_int,
         [P, P, I32, C.POINTER(C.c_float), I32, I32, C.c_char_p, PP],

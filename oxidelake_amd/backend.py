@@ -542,6 +542,11 @@ class RocmBackend:
                                       output_name.encode(), C.byref(out)))
         return DeviceBatch(out.value)
 
+    def prepare_agg_batch(self, stream: Stream, table: DeviceBatch, spec: AggregateSpec) -> "AggBatch":
+        """Batched scalar aggregations of `spec`'s shape over `table`
+        (oxr_agg_batch_prepare); see AggBatch."""
+        return AggBatch(stream, table, spec)
+
     def aggregate_state(self, stream: Stream, like: DeviceBatch, spec: AggregateSpec) -> "AggState":
         """Groups resident in HBM across batches (oxr_agg_create): each
         `update` folds one batch of `like`'s schema, `finish` returns the
@@ -665,6 +670,94 @@ class AggState:
         if getattr(self, "handle", None):
             lib.oxr_agg_free(self.handle)
             self.handle = None
+
+
+class AggBatch:
+    """Batched scalar aggregations over one resident table (oxr_agg_batch_*):
+    requests of one prepared shape (the same filter and aggregates, no group
+    key, only literal values differing) answered by one pass over the table,
+    the run's chain replayed from a captured hipGraph. No counterpart in the
+    reference (its operator runs one aggregation per plan,
+    crates/oxidelake-compute/src/operator.rs:994-1038)."""
+
+    _ARROW = {plan.INT32: pa.int32(), plan.INT64: pa.int64(), plan.FLOAT32: pa.float32(),
+              plan.FLOAT64: pa.float64()}
+
+    def __init__(self, stream: Stream, table: DeviceBatch, spec: AggregateSpec):
+        cspec, keep = _agg_spec(spec)
+        out, kmax = C.c_void_p(), C.c_int32()
+        check(lib.oxr_agg_batch_prepare(stream.handle, table.handle, C.byref(cspec), C.byref(kmax),
+                                        C.byref(out)))
+        self.handle = out.value
+        self.table = table  # the object keeps its buffers alive; this keeps the Python side too
+        self.max_requests = int(kmax.value)
+        na = C.c_int32()
+        check(lib.oxr_agg_batch_columns(self.handle, C.byref(na), None, None))
+        types = (C.c_int32 * max(1, na.value))()
+        names = (C.c_char_p * max(1, na.value))()
+        check(lib.oxr_agg_batch_columns(self.handle, C.byref(na), types, names))
+        self.schema = pa.schema([pa.field(names[i].decode(), self._ARROW[types[i]]) for i in range(na.value)])
+
+    def run(self, stream: Stream, specs: Sequence[AggregateSpec]) -> list:
+        """One result per request: a one-row RecordBatch (as oxr_aggregate's
+        global aggregate), or the ExecutionError of a request that divided by
+        zero (it fails alone)."""
+        n = len(specs)
+        if n == 0:
+            return []
+        arr = (AggSpec * n)()
+        keep = []
+        for i, sp in enumerate(specs):
+            cs, k = _agg_spec(sp)
+            arr[i] = cs
+            keep.append(k)
+        na = len(self.schema)
+        values = np.zeros(n * na, np.int64)
+        valid = np.zeros(n * na, np.uint8)
+        status = np.zeros(n, np.int32)
+        check(lib.oxr_agg_batch_run(stream.handle, self.handle, arr, n,
+                                    values.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    valid.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                    status.ctypes.data_as(C.POINTER(C.c_int32))))
+        out = []
+        for r in range(n):
+            if status[r] != 0:
+                out.append(ExecutionError(f"request {r}: division by zero"))
+                continue
+            cols = []
+            for a, f in enumerate(self.schema):
+                # the value's bits: 32-bit types in the low word (little endian: element 0)
+                word = values[r * na + a:r * na + a + 1]
+                v = word.view(np.dtype(f.type.to_pandas_dtype()))[:1]
+                cols.append(pa.array(v, f.type, mask=np.array([valid[r * na + a] == 0])))
+            out.append(pa.RecordBatch.from_arrays(cols, schema=self.schema))
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "handle", None):
+            lib.oxr_agg_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.free()
+
+
+def agg_batch_kernel_source(columns: Sequence[tuple], spec: AggregateSpec, compile: bool = False) -> str:
+    """The batched-request kernels generated for `spec` over columns given as
+    (type, nullable). No device is needed."""
+    cols = (Column * max(1, len(columns)))()
+    for i, c in enumerate(columns):
+        typ, dim, nullable = (c[0], 0, c[1]) if len(c) == 2 else c
+        cols[i].type = typ
+        cols[i].dim = dim
+        cols[i].validity = 1 if nullable else None
+    cspec, keep = _agg_spec(spec)
+    n = C.c_size_t()
+    check(lib.oxr_agg_batch_kernel_source(cols, len(columns), C.byref(cspec), 0, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib.oxr_agg_batch_kernel_source(cols, len(columns), C.byref(cspec), 1 if compile else 0, buf,
+                                          n.value + 1, C.byref(n)))
+    return buf.value.decode()
 
 
 def _agg_spec(spec: AggregateSpec):

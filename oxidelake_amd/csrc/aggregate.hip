@@ -1581,9 +1581,9 @@ int oxr_aggregate(oxr_stream *s, const oxr_batch *input, const oxr_agg_spec *spe
 
 }  // extern "C"
 
-extern "C" int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec,
-                                           int32_t probe_key, int32_t compile, char *buf, size_t cap,
-                                           size_t *len) {
+namespace {
+int kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec, int32_t probe_key, bool batched,
+                  int32_t compile, char *buf, size_t cap, size_t *len) {
   clear_error();
   if (!spec || ncols < 0 || (ncols > 0 && !cols) || (spec->naggs > 0 && !spec->aggs))
     return fail(OXR_EPLAN, "rocm.jit", "null argument");
@@ -1599,7 +1599,10 @@ extern "C" int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols
   oxr_agg_state st;
   OXR_TRY(compile_spec(&like, spec, &st));
   if (probe_key >= ncols) return fail(OXR_EPLAN, "rocm.jit", "probe key column out of range");
-  ShapeOf so(&st, probe_key >= 0 ? probe_key : -2);
+  if (batched && (st.key_col != -1 || spec->naggs < 1))
+    return fail(OXR_EUNSUPPORTED, "rocm.aggregate_batch", "batched requests have no group key and some aggregate");
+  ShapeOf so(&st, probe_key >= 0 && !batched ? probe_key : -2);
+  if (batched) so.shape.multi = multi_max_requests((int)st.aggs.size());
   std::string src;
   const int rc = jit_aggregate_source(like.cols, so.shape, compile != 0, &src);
   if (rc == OXR_EUNSUPPORTED)
@@ -1613,3 +1616,254 @@ extern "C" int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols
   }
   return OXR_OK;
 }
+}  // namespace
+
+extern "C" int oxr_aggregate_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec,
+                                           int32_t probe_key, int32_t compile, char *buf, size_t cap,
+                                           size_t *len) {
+  return kernel_source(cols, ncols, spec, probe_key, false, compile, buf, cap, len);
+}
+
+extern "C" int oxr_agg_batch_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_agg_spec *spec,
+                                           int32_t compile, char *buf, size_t cap, size_t *len) {
+  return kernel_source(cols, ncols, spec, -1, true, compile, buf, cap, len);
+}
+
+// ---------------------------------------------------------- batched requests
+// Many scalar aggregations of one prepared shape over one resident table,
+// answered by one pass over the table (aggregate_multi, aggregate_jit.h).
+// The chain of one run -- the requests' literals copied from pinned host
+// memory, the shared scan, the final fold, the result records copied back
+// to pinned host memory -- is captured once per batch size into a hipGraph
+// and replayed for every run; nothing in it is a memset, and no device
+// memory is allocated per run.
+struct oxr_agg_batch {
+  oxr_device *dev = nullptr;
+  StreamCore *owner = nullptr;
+  std::mutex mu;  // runs take turns on the buffers below
+  std::unique_ptr<oxr_agg_state> proto;  // the prepared spec
+  std::vector<DevCol> cols;              // the table's columns (views)
+  std::shared_ptr<void> hold;            // keeps the table's buffers alive
+  uint64_t rows = 0;
+  std::string source;                    // the shape's generated source
+  JitAgg ja;
+  int kmax = 1;                          // requests per run
+  uint32_t nlit = 0;                     // literals per request
+  unsigned grid = 1;
+  AggJitArgs args{};
+  void *dmem = nullptr;                  // lits | partials | results (arena)
+  uint64_t *dlits = nullptr;
+  int64_t *dpart = nullptr, *dres = nullptr;
+  uint64_t *hlits = nullptr;             // pinned
+  int64_t *hres = nullptr;               // pinned
+  hipStream_t cap = nullptr;             // capture-only stream
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace oxr {
+namespace {
+
+// The generated shape of `spec` over the batch's table and its literals.
+int batch_shape(const oxr_agg_batch *b, const oxr_agg_spec *spec, std::string *src, std::vector<uint64_t> *lits) {
+  oxr_batch like;
+  like.rows = 0;
+  like.cols = b->cols;
+  like.names.assign(b->cols.size(), std::string());
+  oxr_agg_state st;
+  OXR_TRY(compile_spec(&like, spec, &st));
+  if (st.key_col != -1) return fail(OXR_EPLAN, "rocm.aggregate_batch", "a request with a group key");
+  ShapeOf so(&st, -2);
+  so.shape.multi = b->kmax;
+  JitAgg ja;
+  const int rc = jit_aggregate_generate(b->cols, so.shape, &ja);
+  if (rc == OXR_EUNSUPPORTED)
+    return fail(OXR_EUNSUPPORTED, "rocm.aggregate_batch", "the shape is outside the generated kernels");
+  OXR_TRY(rc);
+  *src = std::move(ja.source);
+  *lits = std::move(ja.lits);
+  return OXR_OK;
+}
+
+int batch_bucket(int n) {
+  for (int i = 0; i < 4; i++)
+    if (kMultiK[i] >= n) return i;
+  return 3;
+}
+
+// The chain of one run of bucket `ki` on `st`.
+int batch_enqueue(oxr_agg_batch *b, hipStream_t st, int ki) {
+  const int K = kMultiK[ki];
+  const int na = (int)b->proto->aggs.size();
+  OXR_HIP(hipMemcpyAsync(b->dlits, b->hlits, (1 + (size_t)K * b->nlit) * 8, hipMemcpyHostToDevice, st));
+  void *params[] = {&b->args};
+  OXR_HIP(hipModuleLaunchKernel(b->ja.multi_fn[ki], b->grid, 1, 1, 256, 1, 1, 0, st, params, nullptr));
+  OXR_HIP(hipModuleLaunchKernel(b->ja.multi_final[ki], (unsigned)K, 1, 1, 256, 1, 1, 0, st, params, nullptr));
+  OXR_HIP(hipMemcpyAsync(b->hres, b->dres, (size_t)K * (na + 1) * 8, hipMemcpyDeviceToHost, st));
+  return OXR_OK;
+}
+
+int batch_graph(oxr_agg_batch *b, int ki) {
+  if (b->exec[ki]) return OXR_OK;
+  if (!b->cap) OXR_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+  OXR_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_enqueue(b, b->cap, ki);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->cap, &g);
+  if (rc != OXR_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail_hip(e, "capturing the batched aggregation");
+  const hipError_t ei = hipGraphInstantiate(&b->exec[ki], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    b->exec[ki] = nullptr;
+    return fail_hip(ei, "instantiating the batched aggregation graph");
+  }
+  return OXR_OK;
+}
+
+void batch_destroy(oxr_agg_batch *b) {
+  for (auto &x : b->exec)
+    if (x) hipGraphExecDestroy(x);
+  if (b->cap) hipStreamDestroy(b->cap);
+  if (b->hlits) hipHostFree(b->hlits);
+  if (b->hres) hipHostFree(b->hres);
+  if (b->dmem) b->dev->arena->release(b->dmem, b->owner->s);
+  if (b->owner) release(b->owner);
+  delete b;
+}
+
+}  // namespace
+}  // namespace oxr
+
+extern "C" {
+
+int oxr_agg_batch_prepare(oxr_stream *s, const oxr_batch *table, const oxr_agg_spec *spec,
+                          int32_t *max_requests, oxr_agg_batch **out) {
+  clear_error();
+  if (!s || !table || !spec || !out || (spec->naggs > 0 && !spec->aggs))
+    return fail(OXR_EPLAN, "rocm.aggregate_batch", "null argument");
+  if (spec->group_by != -1) return fail(OXR_EUNSUPPORTED, "rocm.aggregate_batch", "batched requests have no group key");
+  if (spec->naggs < 1) return fail(OXR_EPLAN, "rocm.aggregate_batch", "no aggregates");
+  std::unique_ptr<oxr_agg_batch, void (*)(oxr_agg_batch *)> b(new oxr_agg_batch(), batch_destroy);
+  b->dev = s->dev;
+  b->owner = retain(s->core);
+  b->proto.reset(new oxr_agg_state());
+  OXR_TRY(compile_spec(table, spec, b->proto.get()));
+  b->cols = table->cols;
+  for (auto &c : b->cols) c.owned = false;
+  b->hold = share_columns(table);
+  b->rows = table->rows;
+  const int na = (int)b->proto->aggs.size();
+  b->kmax = multi_max_requests(na);
+  {
+    ShapeOf so(b->proto.get(), -2);
+    so.shape.multi = b->kmax;
+    const int rc = jit_aggregate(s->dev, b->cols, so.shape, &b->ja);
+    if (rc == OXR_EUNSUPPORTED)
+      return fail(OXR_EUNSUPPORTED, "rocm.aggregate_batch", "the shape is outside the generated kernels");
+    OXR_TRY(rc);
+  }
+  b->source = b->ja.source;
+  b->nlit = (uint32_t)b->ja.lits.size();
+  // the grid: a fixed function of the table and the device, so a request's
+  // order of additions never changes
+  const uint64_t tile = 256ull * (uint64_t)b->ja.multi_rows;
+  const uint64_t ntiles = std::max<uint64_t>(1, (b->rows + tile - 1) / tile);
+  const uint64_t per_cu = (uint64_t)std::max<int64_t>(1, tune_get("agg.multi_grid", 2));
+  b->grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
+  const int K = b->kmax;
+  const size_t lit_b = ((1 + (size_t)K * b->nlit) * 8 + 255) / 256 * 256;
+  const size_t part_b = ((size_t)b->grid * K * (2 * na + 1) * 8 + 255) / 256 * 256;
+  const size_t res_b = (size_t)K * (na + 1) * 8;
+  OXR_TRY(stream_alloc(s, lit_b + part_b + res_b, &b->dmem));
+  b->dlits = (uint64_t *)b->dmem;
+  b->dpart = (int64_t *)((char *)b->dmem + lit_b);
+  b->dres = (int64_t *)((char *)b->dmem + lit_b + part_b);
+  OXR_HIP(hipHostMalloc((void **)&b->hlits, lit_b, hipHostMallocDefault));
+  OXR_HIP(hipHostMalloc((void **)&b->hres, res_b, hipHostMallocDefault));
+  std::memset(&b->args, 0, sizeof b->args);
+  b->args.n = b->rows;
+  for (size_t i = 0; i < b->cols.size() && i < (size_t)kAggCols; i++) {
+    b->args.col[i] = b->cols[i].values;
+    b->args.valid[i] = (const uint32_t *)b->cols[i].validity;
+  }
+  b->args.mlits = b->dlits;
+  b->args.mnlit = b->nlit;
+  b->args.mblocks = b->grid;
+  b->args.mpart = b->dpart;
+  b->args.mres = b->dres;
+  // every run waits for the table's producer: the table was made on `s`
+  OXR_HIP(hipStreamSynchronize(s->s));
+  if (max_requests) *max_requests = K;
+  *out = b.release();
+  return OXR_OK;
+}
+
+int oxr_agg_batch_columns(const oxr_agg_batch *b, int32_t *naggs, int32_t *types, const char **names) {
+  clear_error();
+  if (!b || !naggs) return fail(OXR_EPLAN, "rocm.aggregate_batch", "null argument");
+  const int na = (int)b->proto->aggs.size();
+  *naggs = na;
+  for (int a = 0; a < na; a++) {
+    if (types) types[a] = b->proto->out_types[a];
+    if (names) names[a] = b->proto->names[a].c_str();
+  }
+  return OXR_OK;
+}
+
+int oxr_agg_batch_run(oxr_stream *s, oxr_agg_batch *b, const oxr_agg_spec *requests, int32_t n,
+                      int64_t *values, uint8_t *valid, int32_t *status) {
+  clear_error();
+  if (!s || !b || (n > 0 && (!requests || !values || !valid || !status)))
+    return fail(OXR_EPLAN, "rocm.aggregate_batch", "null argument");
+  if (s->dev != b->dev) return fail(OXR_EPLAN, "rocm.aggregate_batch", "the stream is on another device");
+  if (n < 0 || n > b->kmax)
+    return fail(OXR_EPLAN, "rocm.aggregate_batch",
+                "a run takes 0 to " + std::to_string(b->kmax) + " requests, not " + std::to_string(n));
+  if (n == 0) return OXR_OK;
+  const int na = (int)b->proto->aggs.size();
+  // every request must be the prepared shape; its literals fill its row
+  std::vector<uint64_t> lits((size_t)n * b->nlit);
+  for (int r = 0; r < n; r++) {
+    std::string src;
+    std::vector<uint64_t> l;
+    OXR_TRY(batch_shape(b, &requests[r], &src, &l));
+    if (src != b->source || l.size() != b->nlit)
+      return fail(OXR_EPLAN, "rocm.aggregate_batch",
+                  "request " + std::to_string(r) + " is not the prepared shape (only literal values may differ)");
+    std::copy(l.begin(), l.end(), lits.begin() + (size_t)r * b->nlit);
+  }
+  const int ki = batch_bucket(n);
+  std::lock_guard<std::mutex> lock(b->mu);
+  b->hlits[0] = (uint64_t)n;
+  std::copy(lits.begin(), lits.end(), b->hlits + 1);
+  if (tune_get("agg.batch_graph", 1) > 0) {
+    OXR_TRY(batch_graph(b, ki));
+    ProfScope prof(s->s, "aggregate_batch");
+    OXR_HIP(hipGraphLaunch(b->exec[ki], s->s));
+  } else {
+    ProfScope prof(s->s, "aggregate_batch");
+    OXR_TRY(batch_enqueue(b, s->s, ki));
+  }
+  OXR_HIP(hipStreamSynchronize(s->s));
+  for (int r = 0; r < n; r++) {
+    const int64_t *rec = b->hres + (size_t)r * (na + 1);
+    const uint32_t fl = (uint32_t)rec[na];
+    status[r] = (fl & kMultiErr) ? OXR_EEXECUTION : OXR_OK;
+    for (int a = 0; a < na; a++) {
+      const uint8_t k = b->proto->aggs[a].kind;
+      const bool cnt = k == A_COUNT || k == A_COUNT_STAR;
+      values[(size_t)r * na + a] = rec[a];
+      valid[(size_t)r * na + a] = (uint8_t)(cnt || ((fl >> a) & 1u));
+    }
+  }
+  return OXR_OK;
+}
+
+void oxr_agg_batch_free(oxr_agg_batch *b) {
+  if (b) batch_destroy(b);
+}
+
+}  // extern "C"

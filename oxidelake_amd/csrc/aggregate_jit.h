@@ -280,6 +280,14 @@ struct AggJitArgs {
   uint32_t nhot;
   int64_t *hot_part;
   uint8_t *hot_flags;
+  // batched requests (aggregate_multi): mlits[0] = the requests of this
+  // run, request r's literals at mlits[1 + r * mnlit ...]; each of the
+  // multi kernel's mblocks blocks writes its partials to mpart, and the
+  // final fold writes one result record per request to mres
+  const uint64_t *mlits;
+  uint32_t mnlit, mblocks;
+  int64_t *mpart;
+  int64_t *mres;
 };
 
 // Hot groups a block accumulates in LDS (aggregate_hot): 40 KB of
@@ -1094,6 +1102,206 @@ __device__ __forceinline__ void aggregate_dense(const AggJitArgs &p) {
   }
   if (err) atomicOr(p.error, 1u);
   if (!ok) atomicOr(p.t.overflow, 1u);
+}
+
+
+// ------------------------------------------- batched scalar aggregations
+// Up to K requests of one prepared shape (the same filter and aggregates
+// over one global group, each request with its own literal values) in one
+// pass over a table: every tile's rows are loaded once, and each request's
+// filter and contributions are evaluated on them into per-lane
+// accumulators held in registers. Tiles go to blocks in a fixed order (tile
+// += gridDim.x) and every reduction below runs in a fixed order, so a
+// request's result depends neither on the requests beside it, nor on the
+// batch's size, nor on anything else running on the device; FP64 sums are
+// compensated (TwoSum on every row). The run's literals: p.mlits (see
+// AggJitArgs). Block b writes request r's partial to
+// p.mpart[(b * K + r) * (2 * NA + 1) ...]: NA accumulators, NA lo parts
+// (FP64 sums), a flag word (bit a: aggregate a took a value; bit 30: a
+// division by zero).
+
+constexpr uint32_t kMultiErr = 1u << 30;
+
+// (acc, lo) of kind `kind` += (v, vl), the compensated add for FP64 sums.
+__device__ inline void multi_combine(uint8_t kind, int64_t &acc, double &lo, int64_t v, double vl) {
+  if (kind == A_SUM_F64) {
+    double hi = as_f64((uint64_t)acc);
+    add_f64_pair(hi, lo, as_f64((uint64_t)v), vl);
+    acc = (int64_t)f64_bits(hi);
+  } else {
+    acc = combine_kind(kind, acc, v);
+  }
+}
+
+// Combines (acc, lo, fl) over the wave by a fixed xor tree: every lane ends
+// with the same values whatever the order its partners' were made in
+// (addition is commutative and TwoSum's error exact).
+template <class Q>
+__device__ inline void multi_wave(int64_t (&acc)[Q::kAggs], double (&lo)[Q::kAggs], uint32_t &fl) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+#pragma unroll
+    for (int a = 0; a < Q::kAggs; a++) {
+      const int64_t ov = (int64_t)__shfl_xor((long long)acc[a], d, 64);
+      const double ol = Q::kKind[a] == A_SUM_F64 ? __shfl_xor(lo[a], d, 64) : 0.0;
+      multi_combine(Q::kKind[a], acc[a], lo[a], ov, ol);
+    }
+    fl |= (uint32_t)__shfl_xor((int)fl, d, 64);
+  }
+}
+
+template <class Q, int S, int K>
+__device__ __forceinline__ void aggregate_multi(const AggJitArgs &p) {
+  constexpr int NA = Q::kAggs;
+  constexpr int W = 2 * NA + 1;
+  constexpr int kTile = 256 * S;
+  __shared__ int64_t s_acc[4][K][NA];
+  __shared__ double s_lo[4][K][NA];
+  __shared__ uint32_t s_fl[4][K];
+  const int tid = threadIdx.x;
+  const uint32_t nreq = (uint32_t)p.mlits[0];
+  int64_t acc[K][NA];
+  double lo[K][NA];
+  uint32_t fl[K];
+#pragma unroll
+  for (int r = 0; r < K; r++) {
+    fl[r] = 0;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      acc[r][a] = acc_init(Q::kKind[a]);
+      lo[r][a] = 0.0;
+    }
+  }
+  const uint64_t ntiles = (p.n + kTile - 1) / kTile;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    const typename Q::In in_cols = Q::ins(p, base);
+    typename Q::Row x[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+      if ((uint32_t)r < nreq) {  // uniform
+        const uint64_t *L = p.mlits + 1 + (uint64_t)r * p.mnlit;
+        bool err = false;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          const uint32_t j = (uint32_t)(s * 256 + tid);
+          const bool in = j < lim;
+          const bool sel = in && Q::pred(L, x[s], j, in, err);
+          int64_t c[NA];
+          uint32_t cm = 0;
+          Q::contrib(L, x[s], j, sel, c, cm, err);
+#pragma unroll
+          for (int a = 0; a < NA; a++) {
+            if (!((cm >> a) & 1u)) continue;
+            multi_combine(Q::kKind[a], acc[r][a], lo[r][a], c[a], 0.0);
+            fl[r] |= 1u << a;
+          }
+        }
+        if (err) fl[r] |= kMultiErr;
+      }
+    }
+  }
+  // the block's partial of each request: waves by a fixed tree, then the
+  // four waves in order
+  const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int r = 0; r < K; r++) {
+    if ((uint32_t)r < nreq) {
+      multi_wave<Q>(acc[r], lo[r], fl[r]);
+      if (lane == 0) {
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+          s_acc[wave][r][a] = acc[r][a];
+          s_lo[wave][r][a] = lo[r][a];
+        }
+        s_fl[wave][r] = fl[r];
+      }
+    }
+  }
+  __syncthreads();
+  if ((uint32_t)tid < nreq * NA) {
+    const int r = tid / NA, a = tid % NA;
+    const uint8_t kind = Q::kKind[a];
+    int64_t v = s_acc[0][r][a];
+    double vl = s_lo[0][r][a];
+    for (int w = 1; w < 4; w++) multi_combine(kind, v, vl, s_acc[w][r][a], s_lo[w][r][a]);
+    int64_t *o = p.mpart + ((uint64_t)blockIdx.x * K + r) * W;
+    o[a] = v;
+    o[NA + a] = (int64_t)f64_bits(vl);
+  }
+  if ((uint32_t)tid < nreq) {
+    const uint32_t f = s_fl[0][tid] | s_fl[1][tid] | s_fl[2][tid] | s_fl[3][tid];
+    p.mpart[((uint64_t)blockIdx.x * K + tid) * W + 2 * NA] = f;
+  }
+}
+
+// The final fold: block r (of K) folds request r's partials over the
+// p.mblocks blocks in a fixed order and writes its result record to
+// p.mres[r * (NA + 1) ...]: each aggregate's value as its output type's
+// bits (an FP64 sum as hi + lo, a float MIN/MAX decoded from its order
+// key), then the flag word.
+template <class Q, int K>
+__device__ __forceinline__ void aggregate_multi_final(const AggJitArgs &p) {
+  constexpr int NA = Q::kAggs;
+  constexpr int W = 2 * NA + 1;
+  __shared__ int64_t s_acc[4][NA];
+  __shared__ double s_lo[4][NA];
+  __shared__ uint32_t s_fl[4];
+  const uint32_t r = blockIdx.x;
+  if (r >= (uint32_t)p.mlits[0]) return;  // uniform over the block
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  int64_t acc[NA];
+  double lo[NA];
+  uint32_t fl = 0;
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    acc[a] = acc_init(Q::kKind[a]);
+    lo[a] = 0.0;
+  }
+  for (uint32_t b = (uint32_t)tid; b < p.mblocks; b += 256) {
+    const int64_t *src = p.mpart + ((uint64_t)b * K + r) * W;
+#pragma unroll
+    for (int a = 0; a < NA; a++) multi_combine(Q::kKind[a], acc[a], lo[a], src[a], as_f64((uint64_t)src[NA + a]));
+    fl |= (uint32_t)src[2 * NA];
+  }
+  multi_wave<Q>(acc, lo, fl);
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      s_acc[wave][a] = acc[a];
+      s_lo[wave][a] = lo[a];
+    }
+    s_fl[wave] = fl;
+  }
+  __syncthreads();
+  if (tid < NA) {
+    const int a = tid;
+    const uint8_t kind = Q::kKind[a];
+    int64_t v = s_acc[0][a];
+    double vl = s_lo[0][a];
+    for (int w = 1; w < 4; w++) multi_combine(kind, v, vl, s_acc[w][a], s_lo[w][a]);
+    uint64_t out;
+    switch (kind) {
+      case A_SUM_F64: out = f64_bits(sum_value(as_f64((uint64_t)v), vl)); break;
+      case A_MIN_F64:
+      case A_MAX_F64: out = (uint64_t)(v ^ (int64_t)(((uint64_t)(v >> 63)) >> 1)); break;
+      case A_MIN_F32:
+      case A_MAX_F32: {
+        const int32_t k32 = (int32_t)v;
+        out = (uint64_t)(uint32_t)(k32 ^ (int32_t)(((uint32_t)(k32 >> 31)) >> 1));
+        break;
+      }
+      default: out = (uint64_t)v;
+    }
+    p.mres[(uint64_t)r * (NA + 1) + a] = (int64_t)out;
+  }
+  if (tid == 0) p.mres[(uint64_t)r * (NA + 1) + NA] = s_fl[0] | s_fl[1] | s_fl[2] | s_fl[3];
 }
 
 }  // namespace oxr

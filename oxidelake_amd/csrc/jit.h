@@ -56,6 +56,9 @@ struct AggJitShape {
   std::vector<uint8_t> kinds;
   std::vector<const oxr_expr *> inputs;
   std::vector<uint8_t> nullable;
+  // > 0: generate only the batched-request kernels (aggregate_multi) for
+  // batches of up to this many requests; literals are always parameters
+  int multi = 0;
 };
 
 struct JitAgg {
@@ -69,12 +72,30 @@ struct JitAgg {
   int spill_words = 0;  // 32-bit words per spill record
   int rows_per_lane = 0;
   std::vector<uint64_t> lits;  // AggJitArgs::lit (prepared mode)
+  // batched requests (shape.multi): kernels for up to kMultiK[i] requests
+  int multi_rows = 0;
+  hipFunction_t multi_fn[4] = {}, multi_final[4] = {};
   std::string source;
 };
+
+// Request counts the batched-request kernels are instantiated for.
+constexpr int kMultiK[4] = {1, 4, 8, 16};
+// The largest batch a shape of `naggs` aggregates takes (each request keeps
+// its accumulators in registers).
+inline int multi_max_requests(int naggs) {
+  const int cap = 48 / (naggs > 0 ? naggs : 1);
+  int k = 1;
+  for (int b : kMultiK)
+    if (b <= cap) k = b;
+  return k;
+}
 
 // Finds or builds the aggregation kernel of this shape for `dev`;
 // OXR_EUNSUPPORTED (no error recorded): the interpreter kernel runs it.
 int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJitShape &shape, JitAgg *out);
+// Generates the source (and, in prepared mode, the literal values) of this
+// shape without compiling or loading anything.
+int jit_aggregate_generate(const std::vector<DevCol> &cols, const AggJitShape &shape, JitAgg *out);
 // The generated source (compiled for gfx950 when asked), no device needed.
 int jit_aggregate_source(const std::vector<DevCol> &cols, const AggJitShape &shape, bool do_compile,
                          std::string *src);

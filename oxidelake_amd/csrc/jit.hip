@@ -476,7 +476,7 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   std::vector<char> val_used(cols.size(), 0), valid_used(cols.size(), 0);
   JitFilter scratch;
   Gen g{cols, &scratch, val_used, valid_used};
-  if (tune_get("jit.params", 0) > 0) g.params = &ja->lits;
+  if (tune_get("jit.params", 0) > 0 || sh.multi > 0) g.params = &ja->lits;
   std::string pred_code = "    return true;\n";
   if (sh.filter && sh.filter->count > 0) {
     g.e = sh.filter;
@@ -572,6 +572,22 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   src += "  __device__ static inline void contrib(const uint64_t *L, const Row &x, uint32_t j, bool sel, "
          "int64_t (&c)[kAggs], uint32_t &cm, bool &err) {\n" + contrib + "  }\n";
   src += "};\n}  // namespace\n";
+  if (sh.multi > 0) {
+    // batched requests: rows per lane from the knob agg.multi_rows
+    const int64_t mr = tune_get("agg.multi_rows", 4);
+    ja->multi_rows = (int)(mr > 0 ? mr : 4);
+    for (int i = 0; i < 4; i++) {
+      if (kMultiK[i] > sh.multi) break;
+      const std::string k = std::to_string(kMultiK[i]);
+      src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_multi" + k +
+             "(const AggJitArgs p) {\n  aggregate_multi<Q, " + std::to_string(ja->multi_rows) + ", " + k +
+             ">(p);\n}\n";
+      src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_multi_final" + k +
+             "(const AggJitArgs p) {\n  aggregate_multi_final<Q, " + k + ">(p);\n}\n";
+    }
+    ja->source = src;
+    return OXR_OK;
+  }
   src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_jit(const AggJitArgs p) {\n";
   src += "  aggregate_tiles<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";
   // high-cardinality skewed keys: the hot-group kernel (aggregate_hot), for
@@ -613,6 +629,15 @@ int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJit
   JitAgg ja;
   const int rc = generate_aggregate(cols, shape, &ja);
   if (rc != OXR_OK) return rc;
+  if (shape.multi > 0) {
+    for (int i = 0; i < 4 && kMultiK[i] <= shape.multi; i++) {
+      const std::string k = std::to_string(kMultiK[i]);
+      OXR_TRY(compile(dev, ja.source, ("oxr_aggregate_multi" + k).c_str(), &ja.multi_fn[i]));
+      OXR_TRY(compile(dev, ja.source, ("oxr_aggregate_multi_final" + k).c_str(), &ja.multi_final[i]));
+    }
+    *out = std::move(ja);
+    return OXR_OK;
+  }
   OXR_TRY(compile(dev, ja.source, "oxr_aggregate_jit", &ja.fn));
   if (ja.dense) {
     OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense16", &ja.dense16));
@@ -625,6 +650,10 @@ int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJit
   }
   *out = std::move(ja);
   return OXR_OK;
+}
+
+int jit_aggregate_generate(const std::vector<DevCol> &cols, const AggJitShape &shape, JitAgg *out) {
+  return generate_aggregate(cols, shape, out);
 }
 
 int jit_aggregate_source(const std::vector<DevCol> &cols, const AggJitShape &shape, bool do_compile,

@@ -5,15 +5,18 @@ Three request kinds, over tables that stay resident for the server's life:
   - "vector": top-k of 1..64 queries over a vector index (BF16 scan + FP32
     rerank by default);
   - "filter_agg": COUNT(*), SUM(a), SUM(c*d) FROM t WHERE b < T over the
-    200M-row filter table (prepared: one compiled kernel for every T);
+    200M-row filter table: every request waiting when the worker becomes
+    free (up to 16) runs as one pass over the table, the run's chain
+    replayed from a captured hipGraph (RocmBackend.prepare_agg_batch);
   - "join_agg": SUM(qty), SUM(qty*price), COUNT(*) FROM fact JOIN dim ON
     fk = pk WHERE fk in [lo, hi) GROUP BY grp.
 
 Every kind has its own worker thread and HIP stream, so requests of
 different kinds run at the same time on the device. Requests are admitted
 continuously: a vector worker takes every vector request waiting when it
-becomes free (up to 256 queries) into one scan, so requests that arrive
-while a scan runs join the next one instead of queueing one by one. A
+becomes free (up to 256 queries) into one scan, and the filter_agg worker
+every waiting filter_agg request into one shared scan, so requests that
+arrive while a batch runs join the next one instead of queueing one by one. A
 memory gate admits a batch only when its estimated device footprint fits in
 the budget beside everything in flight. The one-at-a-time path
 (`run_one_at_a_time`) runs the same requests in arrival order, each alone,
@@ -38,7 +41,7 @@ import numpy as np
 import pyarrow as pa
 
 from . import autotune, plan
-from .backend import AggState, JoinTable, RocmBackend, VectorIndex
+from .backend import AggBatch, AggState, JoinTable, RocmBackend, VectorIndex
 from .plan import Aggregate, AggregateSpec
 from .plan import AggregateFunction as F
 from .plan import Comparison as Cmp
@@ -73,6 +76,15 @@ class Tables:
     vec_dim: int = 768        # the index's vector size
     k: int = 10
     keep: list = field(default_factory=list)  # anything the tables view into
+    filter_batch: Optional[AggBatch] = None   # batched filter_agg requests (prepare_filter_batch)
+
+
+def prepare_filter_batch(be: RocmBackend, stream, tables: Tables) -> AggBatch:
+    """Prepares the filter_agg shape over the filter table for batched runs
+    (the Server's filter_agg worker uses it when present)."""
+    serving_knobs()
+    tables.filter_batch = be.prepare_agg_batch(stream, tables.filter_table, filter_agg_spec(1))
+    return tables.filter_batch
 
 
 def filter_agg_spec(threshold: int) -> AggregateSpec:
@@ -163,8 +175,10 @@ def footprint(tables: Tables, kind: str, reqs: list[Request]) -> int:
     return 1 << 20
 
 
-def execute(be: RocmBackend, tables: Tables, stream, kind: str, reqs: list[Request]) -> list:
-    """Runs one batch on `stream`; returns one result per request."""
+def execute(be: RocmBackend, tables: Tables, stream, kind: str, reqs: list[Request], plain: bool = False) -> list:
+    """Runs one batch on `stream`; returns one result per request. `plain`:
+    aggregate requests through the plain create/update/finish chain, one
+    by one, even where a batched form is prepared."""
     if kind == "vector":
         qs = np.concatenate([r.payload for r in reqs])
         rows, dist = tables.index.search(stream, qs, tables.k, tables.candidates)
@@ -174,6 +188,8 @@ def execute(be: RocmBackend, tables: Tables, stream, kind: str, reqs: list[Reque
             out.append((rows[at:at + q], dist[at:at + q]))
             at += q
         return out
+    if kind == "filter_agg" and tables.filter_batch is not None and not plain:
+        return tables.filter_batch.run(stream, [filter_agg_spec(int(r.payload)) for r in reqs])
     out = []
     for r in reqs:
         if kind == "filter_agg":
@@ -213,6 +229,16 @@ class Server:
         if first is None:
             return None
         batch = [first]
+        if kind == "filter_agg" and self.tables.filter_batch is not None:
+            while len(batch) < self.tables.filter_batch.max_requests:  # every one waiting now
+                try:
+                    nxt = q.get_nowait()
+                except queue.Empty:
+                    break
+                if nxt is None:
+                    q.put(None)
+                    break
+                batch.append(nxt)
         if kind == "vector":
             nq = len(first.payload)
             while True:  # every vector request waiting now, up to max_q queries
@@ -280,7 +306,7 @@ def run_one_at_a_time(be: RocmBackend, tables: Tables, requests: list[Request]) 
         if wait > 0:
             time.sleep(wait)
         r.start = time.perf_counter() - t0
-        r.result = execute(be, tables, stream, r.kind, [r])[0]
+        r.result = execute(be, tables, stream, r.kind, [r], plain=True)[0]
         r.done = time.perf_counter() - t0
         r.batch = 1
     return stats(requests, time.perf_counter() - t0)

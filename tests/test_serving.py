@@ -30,13 +30,19 @@ class _Tables:
     vec_rows = 1000
     vec_dim = 8
     candidates = 10
+    filter_batch = None  # filter_agg requests one by one
 
     class dim:
         num_rows = 10
 
 
+class _BatchedTables(_Tables):
+    class filter_batch:  # a prepared batch of at most 8 filter_agg requests
+        max_requests = 8
+
+
 def _fake_execute(log, delay=0.002):
-    def run(be, tables, stream, kind, reqs):
+    def run(be, tables, stream, kind, reqs, plain=False):
         log.append((kind, [r.id for r in reqs]))
         time.sleep(delay)
         return [("done", r.id) for r in reqs]
@@ -73,6 +79,21 @@ def test_every_request_answered_once_and_vector_batches_capped(monkeypatch):
     assert any(len(ids) > 1 for k, ids in log if k == "vector")
     for r in reqs:
         assert r.result == ("done", r.id) and r.done >= r.start >= 0
+
+
+def test_filter_agg_requests_batched_when_prepared(monkeypatch):
+    log = []
+    monkeypatch.setattr(serving, "execute", _fake_execute(log, delay=0.01))
+    reqs = _mix(300, rate=20000.0, seed=5)
+    server = serving.Server(_Backend(), _BatchedTables(), max_vector_queries=40)
+    server.run(reqs)
+    assert not server.errors
+    assert sorted(i for _, ids in log for i in ids) == list(range(300))
+    fa = [ids for k, ids in log if k == "filter_agg"]
+    assert all(1 <= len(ids) <= 8 for ids in fa) and any(len(ids) > 1 for ids in fa)
+    order = [i for ids in fa for i in ids]
+    assert order == sorted(order)
+    assert all(len(ids) == 1 for k, ids in log if k == "join_agg")
 
 
 def test_one_at_a_time_runs_in_arrival_order(monkeypatch):

@@ -1,6 +1,7 @@
 """The serving loop on the device: every request's result is the same when
-it is served in the mix (batched with other vector queries, beside other
-request kinds on other streams) as when it runs alone; and the aggregate
+it is served in the mix (batched with other vector queries or filter_agg
+requests, beside other request kinds on other streams) as when it runs
+alone through the plain path; and the aggregate
 requests agree with the oracle (integers exact, Float64 sums within 1e-12
 relative of its compensated sum)."""
 import copy
@@ -44,6 +45,7 @@ def served(backend):
     index = VectorIndex(s, rows, 0, plan.DistanceMetric.Cosine, VEC_BF16)
     s.synchronize()
     tables = serving.Tables(ftab, fact, dim, join, index, VEC_ROWS, candidates=100, k=10, keep=[emb, rows, qv])
+    serving.prepare_filter_batch(backend, s, tables)  # the loop batches filter_agg requests
     yield tables, pool
     index.free()
 
@@ -58,6 +60,7 @@ def test_same_alone_and_in_the_mix(backend, served):
     assert not server.errors, server.errors[:1]
     assert st["requests"] == 150
     assert max(r.batch for r in reqs if r.kind == "vector") > 1
+    assert max(r.batch for r in reqs if r.kind == "filter_agg") > 1
     serving.run_one_at_a_time(backend, tables, alone)
     bad = [r.id for r, a in zip(reqs, alone) if not serving.same_result(r, a)]
     assert not bad, f"requests {bad[:10]} differ between the mix and alone"

@@ -307,6 +307,7 @@ const KnobSpec kKnobs[] = {
     {"agg.multi_grid", 1, 16, 0},              // batched requests: workgroups per CU (default 2)
     {"agg.batch_graph", 0, 1, kBool},          // batched requests: replay a captured hipGraph (default 1)
     {"vector.resident_queries", 0, 1, kBool},
+    {"vector.gemm", 0, 1, kBool},              // batches of 129-256 queries: the GEMM-tiled scan
     {"vector.wide", 0, 3, 0},
     {"vector.nt8", 0, 1, kBool},
     {"vector.pilot", 0, 1, kBool},

@@ -650,6 +650,192 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
   }
 }
 
+// ------------------------------------------------------------- GEMM scan
+// Large batches (129-256 queries) as one GEMM tile per block: 256 rows x
+// 256 queries, eight waves (two per SIMD) in a 4 x 2 grid, each wave 2 x 4
+// tiles of 32 x 32 (64 rows x 128 queries, 128 accumulators). The k axis
+// is walked a 128-byte line at a time: the line of each of the block's 256
+// rows and of each of the 256 queries is copied into LDS by the direct
+// global-to-LDS loads ([256][8] 16-byte units each), and read from there by
+// the waves that need it, so every row is read from HBM once per 256
+// queries (the resident 128-query kernel reads it twice) and no wave loads
+// what another loads. Even lines go to one pair of LDS arrays and odd lines
+// to another (the host runs this kernel for an even number of lines), so
+// the compiler can tell the line being multiplied from the one in flight
+// and waits for the copy only at the line's barrier: the next line's copy
+// overlaps this line's products (the block's last line copies the next
+// block's first). Unit u of row r sits at u ^ (r & 7): a fragment read's
+// eight lanes of one LDS cycle (eight consecutive rows at one unit) fall in
+// eight different 16-byte bank groups. The k group loop stays rolled (its
+// two unrolled copies' fragments spilled registers for FP8). Row factors
+// are double-buffered by block parity; the epilogue is the resident
+// kernels' (scores or appends).
+constexpr int kGemmRows = 256, kGemmQueries = 256, kGemmThreads = 512;
+
+template <int P, int WR, int RT, int NT>
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void gemm_scan_kernel(const ScanArgs a) {
+  constexpr int EB = Prec<P>::EB, KG = Prec<P>::KG, LB = Prec<P>::LB;
+  constexpr int U = LB / 16;                // 16-byte units per lane per k group
+  constexpr int R = 128 / (KG * EB);        // k groups per line
+  constexpr int WQ = 8 / WR;
+  static_assert(WR * RT * 32 == kGemmRows && WQ * NT * 32 == kGemmQueries, "block tile");
+  __shared__ uint4 s_a0[kGemmRows * 8], s_q0[kGemmRows * 8], s_a1[kGemmRows * 8], s_q1[kGemmRows * 8];
+  __shared__ __align__(16) float s_mul[2][kGemmRows];
+  __shared__ __align__(16) float s_bias[2][kGemmRows];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 31, h = lane >> 5;
+  const int wr = wave / WQ, wq = wave % WQ;  // the wave's row slice and query slice
+  const uint64_t rstride = (uint64_t)a.dimp * EB;
+  const int nlines = (int)(rstride / 128);
+  const uint8_t *xb = (const uint8_t *)a.x;
+  const uint8_t *qb = (const uint8_t *)a.qm + (uint64_t)a.qoff * rstride;
+  const uint64_t tile_end = min(a.n, a.tile0 + a.tile_rows);
+  // DMA staging: wave w moves rows [32 w, 32 w + 32) of the rows and of
+  // the queries, 8 rows per instruction; lane l writes LDS unit l of the
+  // instruction's 64, i.e. slot l & 7 of row l >> 3, which holds unit
+  // (l & 7) ^ (row & 7) of the line (the row's low bits are l >> 3: the
+  // instruction's first row is a multiple of 8)
+  const uint32_t lane_off = (uint32_t)((lane >> 3) * rstride) + (uint32_t)((((lane & 7) ^ (lane >> 3)) & 7) * 16);
+  auto dma_stage = [&](uint64_t blk, int line, uint4 *sa, uint4 *sq) {
+    const uint64_t row_w = a.tile0 + blk * (uint64_t)kGemmRows + (uint64_t)wave * (kGemmRows / 8);
+#pragma unroll
+    for (int j = 0; j < kGemmRows / 8 / 8; j++) {
+      const int r0 = wave * (kGemmRows / 8) + j * 8;
+      const uint64_t row0 = row_w + (uint64_t)j * 8;
+      // rows past the tile read the tile's last row (dropped by its bias)
+      const uint8_t *src = row0 + 8 <= tile_end ? xb + row0 * rstride + (uint64_t)line * 128 + lane_off
+                                                 : xb + min(row0 + (uint64_t)(lane >> 3), tile_end - 1) * rstride +
+                                                       (uint64_t)line * 128 + (lane_off % (uint32_t)rstride);
+      __builtin_amdgcn_global_load_lds((const void *)src, (void *)(sa + r0 * 8), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(qb + (uint64_t)r0 * rstride + (uint64_t)line * 128 + lane_off),
+                                       (void *)(sq + r0 * 8), 16, 0, 0);
+    }
+  };
+  auto fill_rows = [&](uint64_t blk, int par) {
+    if (tid >= kGemmRows) return;
+    const uint64_t blockrow = a.tile0 + blk * (uint64_t)kGemmRows;
+    const uint64_t row = blockrow + (uint64_t)tid;
+    float mul = 0.f, bias = INFINITY;  // NULL, zero norm, past the tile
+    if (row < tile_end && valid_row(a.valid, row)) {
+      const float ri = a.rowinfo[row];
+      const float scl = P == P_FP8 ? a.rscale[row] : 1.0f;
+      if (a.metric != OXR_COSINE) {
+        mul = 2.0f * scl;
+        bias = ri;
+      } else if (ri > 0.f) {
+        mul = scl * ri;
+        bias = 0.f;
+      }
+    }
+    s_mul[par][tid] = mul;
+    s_bias[par][tid] = bias;
+  };
+
+  uint64_t blk = blockIdx.x;
+  if (blk < a.nblocks) {
+    fill_rows(blk, 0);
+    dma_stage(blk, 0, s_a0, s_q0);
+  }
+  __syncthreads();
+  int par = 0;
+  for (; blk < a.nblocks; blk += gridDim.x, par ^= 1) {
+    const uint64_t blockrow = a.tile0 + blk * (uint64_t)kGemmRows;
+    f32x16 acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[r][t] = f32x16{};
+    auto compute = [&](const uint4 *sa, const uint4 *sq) {
+#pragma unroll 1
+      for (int gg = 0; gg < R; gg++) {
+        Frag<P> fa[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+          const int row = wr * RT * 32 + r * 32 + i;
+#pragma unroll
+          for (int u = 0; u < U; u++) fa[r].v[u] = sa[row * 8 + ((h * 4 + gg * U + u) ^ (row & 7))];
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          Frag<P> fq;
+          const int qr = wq * NT * 32 + t * 32 + i;
+#pragma unroll
+          for (int u = 0; u < U; u++) fq.v[u] = sq[qr * 8 + ((h * 4 + gg * U + u) ^ (qr & 7))];
+#pragma unroll
+          for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], fa[r], fq);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    // lines in pairs: even lines in buffer 0, odd lines in buffer 1 (the
+    // host runs this kernel only for an even number of lines)
+    for (int line = 0; line < nlines; line += 2) {
+      dma_stage(blk, line + 1, s_a1, s_q1);
+      compute(s_a0, s_q0);
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
+      const uint64_t nblk = line + 2 < nlines ? blk : blk + gridDim.x;
+      if (nblk < a.nblocks) dma_stage(nblk, line + 2 < nlines ? line + 2 : 0, s_a0, s_q0);
+      compute(s_a1, s_q1);
+      __syncthreads();
+    }
+    {
+      const uint64_t nblk = blk + gridDim.x;
+      const bool has_next = nblk < a.nblocks;
+        // epilogue (row factors of this block: s_mul[par], filled at least
+        // one barrier ago)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const int q = a.qoff + wq * NT * 32 + t * 32 + i;
+          if (q >= a.nq) continue;
+          const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
+          const float tau =
+              a.list ? from_ordered((uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32)) : -INFINITY;
+#pragma unroll
+          for (int r = 0; r < RT; r++) {
+#pragma unroll
+            for (int jg = 0; jg < 4; jg++) {
+              const int m0 = 8 * jg + 4 * h;
+              const int br = wr * RT * 32 + r * 32 + m0;
+              const float4 mul4 = *(const float4 *)&s_mul[par][br];
+              const float4 bias4 = *(const float4 *)&s_bias[par][br];
+              const float mulv[4] = {mul4.x, mul4.y, mul4.z, mul4.w};
+              const float biasv[4] = {bias4.x, bias4.y, bias4.z, bias4.w};
+              const uint64_t row0 = blockrow + br;
+              float sc[4];
+#pragma unroll
+              for (int e4 = 0; e4 < 4; e4++) {
+                const float sv = (acc[r][t][jg * 4 + e4] * qs) * mulv[e4] - biasv[e4];
+                sc[e4] = biasv[e4] == INFINITY ? -INFINITY : sv;
+              }
+              if (a.list) {
+#pragma unroll
+                for (int e4 = 0; e4 < 4; e4++) {
+                  if (!(sc[e4] > -INFINITY) || sc[e4] < tau) continue;
+                  const uint32_t pos = atomicAdd(&a.cnt[q], 1u);
+                  if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sc[e4], row0 + e4);
+                }
+              } else {
+                float *out = a.scores + (uint64_t)q * a.tile_rows + (row0 - a.tile0);
+                if (row0 + 4 <= tile_end) {
+                  *(float4 *)out = make_float4(sc[0], sc[1], sc[2], sc[3]);
+                } else {
+#pragma unroll
+                  for (int e4 = 0; e4 < 4; e4++)
+                    if (row0 + e4 < tile_end) out[e4] = sc[e4];
+                }
+              }
+              __builtin_amdgcn_sched_barrier(0);
+            }
+          }
+        }
+        if (has_next) fill_rows(nblk, par ^ 1);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ select
 
 constexpr int kBuf = 4096;   // LDS candidate buffer (keys)
@@ -892,7 +1078,8 @@ struct ScanCfg {
   int nt, rt, qw;
   bool bres;
   int qpl;
-  uint64_t rpb() const { return (uint64_t)(4 / qw) * rt * 32; }
+  bool gemm = false;  // gemm_scan_kernel: 256 rows x 256 queries per block
+  uint64_t rpb() const { return gemm ? (uint64_t)kGemmRows : (uint64_t)(4 / qw) * rt * 32; }
 };
 
 // A block's query tile of at most this many bytes (every k group) can sit
@@ -913,6 +1100,10 @@ ScanCfg pick_scan(int prec, int nqc, int dimp) {
   if (nqc <= 32) return {1, prec == P_FP32 ? 4 : prec == P_BF16 ? 1 : 2, 1, fits(32, kResidentQueryBytes), 32};
   // (a 64-query tile resident at 96 KB measured slower than streamed)
   if (nqc <= 64) return {2, 2, 1, fits(64, 64 * 1024), 64};
+  // large batches: the GEMM tile (256 rows x 256 queries per block, rows
+  // and queries staged through LDS a line at a time), knob vector.gemm
+  if (nqc > 128 && tune_get("vector.gemm", 1) > 0 && ((size_t)dimp * eb / 128) % 2 == 0)
+    return ScanCfg{8, 4, 1, false, kGemmQueries, true};
   // large FP8 batches: the block's 128 queries resident (96 KB), each wave
   // 64 rows x 128 queries, so every row is loaded by one wave (13.6 ms at
   // 256 queries over 20M x 768). vector.wide = 2: two query groups of 64
@@ -949,6 +1140,11 @@ void launch_scan_cfg(oxr_device *dev, uint64_t units, hipStream_t s, const ScanA
 
 template <int P>
 void launch_scan_p(const ScanCfg &c, oxr_device *dev, uint64_t units, hipStream_t s, const ScanArgs &a) {
+  if (c.gemm) {
+    // one block per CU: 128 KB of stage buffers
+    hipLaunchKernelGGL((gemm_scan_kernel<P, 4, 2, 4>), dim3(grid_for(dev, units, 1)), dim3(kGemmThreads), 0, s, a);
+    return;
+  }
   if (P == P_FP8 && c.qw == 2) {
     launch_scan_cfg<P, 2, 4, true, 2>(dev, units, s, a);
     return;

@@ -253,15 +253,19 @@ def test_pilot_bound_matches_plain_scan(backend, stream, precision, overflow):
         assert scans == chunks * (3 if overflow else 2), scans
 
 
-@pytest.mark.parametrize("precision,knob,value", [(VEC_FP8, "vector.wide", 1), (VEC_BF16, "vector.wide", 3),
-                                                  (VEC_FP32, "vector.nt8", 1), (VEC_BF16, "vector.nt8", 1)])
-def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision, knob, value):
+@pytest.mark.parametrize("precision,knobs", [
+    (VEC_FP8, {"vector.gemm": 0, "vector.wide": 1}), (VEC_BF16, {"vector.gemm": 0, "vector.wide": 3}),
+    (VEC_FP32, {"vector.gemm": 0, "vector.nt8": 1}), (VEC_BF16, {"vector.gemm": 0, "vector.nt8": 1}),
+    (VEC_FP8, {"vector.gemm": 1}), (VEC_BF16, {"vector.gemm": 1}), (VEC_FP32, {"vector.gemm": 1})])
+def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision, knobs):
     # other tile shapes for large batches against the streamed 4-tile kernel
-    # in launches of 128 (vector.wide = 0): FP8 with 128 queries resident,
-    # 2 row tiles x 4 query tiles per wave; BF16 with 64 resident, 4 x 2 per
-    # wave, a six-deep load rotation; the 8-tile streamed kernel in one
-    # launch of 256. The same keys, so the same results bit for bit, and
-    # recall against the FP64 oracle
+    # in launches of 128 (vector.gemm = 0, vector.wide = 0): FP8 with 128
+    # queries resident, 2 row tiles x 4 query tiles per wave; BF16 with 64
+    # resident, 4 x 2 per wave, a six-deep load rotation; the 8-tile
+    # streamed kernel in one launch of 256; the GEMM tile (256 rows x 256
+    # queries per block, rows and queries staged through LDS). The same keys
+    # (the same k groups in the same order into the same products), so the
+    # same results bit for bit, and recall against the FP64 oracle
     from oxidelake_amd import autotune
 
     n, dim, nq, k = 40_000, 768, 200, 10
@@ -273,18 +277,56 @@ def test_wide_query_blocks_match_streamed(backend, stream, oracle, precision, kn
     cand = {VEC_BF16: 100, VEC_FP8: 1000, VEC_FP32: k}[precision]
     autotune.set_knob("vector.pilot", 0)
     try:
-        autotune.set_knob(knob, value)
+        for kn, v in knobs.items():
+            autotune.set_knob(kn, v)
         wide = ix.search(stream, qs, k, candidates=cand)
-        autotune.set_knob(knob, -1)
+        for kn in knobs:
+            autotune.set_knob(kn, -1)
+        autotune.set_knob("vector.gemm", 0)
         autotune.set_knob("vector.wide", 0)
         streamed = ix.search(stream, qs, k, candidates=cand)
     finally:
-        autotune.set_knob("vector.pilot", -1)
-        autotune.set_knob("vector.wide", -1)
-        autotune.set_knob(knob, -1)
+        for kn in ("vector.pilot", "vector.wide", "vector.gemm", *knobs):
+            autotune.set_knob(kn, -1)
     assert np.array_equal(wide[0], streamed[0]) and np.array_equal(wide[1], streamed[1])
     hit = 0
     for i in range(0, nq, 10):
         want, _ = oracle.topk64(vecs, None, qs[i], M.Cosine, k)
         hit += len(set(want.tolist()) & set(wide[0][i].tolist()))
     assert hit / (len(range(0, nq, 10)) * k) >= 0.99
+
+
+@pytest.mark.parametrize("precision", [VEC_FP32, VEC_BF16, VEC_FP8])
+@pytest.mark.parametrize("metric", [M.L2, M.Cosine])
+def test_gemm_scan_matches_streamed(backend, stream, precision, metric):
+    """The GEMM-tiled scan (129-256 queries) against the streamed 4-tile
+    kernel, bit for bit: a ragged row count (the last block partly past the
+    end), NULL and zero rows, 129 and 256 queries, in score mode (plain scan)
+    and in append mode (a small pilot bounds the rest)."""
+    from oxidelake_amd import autotune
+
+    n, dim, k = 70_001, 768, 10
+    rng = np.random.default_rng(31 + precision)
+    vecs = rng.normal(size=(n, dim)).astype(np.float32)
+    vecs[5] = 0.0
+    valid = rng.random(n) > 0.03
+    dev = backend.upload(stream, vec_batch(vecs, valid))
+    ix = VectorIndex(stream, dev, 1, metric, precision)
+    cand = {VEC_BF16: 100, VEC_FP8: 300, VEC_FP32: k}[precision]
+    for nq in (129, 256):
+        qs = (vecs[rng.integers(0, n, nq)] + rng.normal(scale=0.5, size=(nq, dim))).astype(np.float32)
+        for pilot in (0, 1):
+            got = {}
+            for gemm in (1, 0):
+                autotune.set_knob("vector.gemm", gemm)
+                autotune.set_knob("vector.wide", 1 if gemm else 0)
+                autotune.set_knob("vector.pilot", pilot)
+                autotune.set_knob("vector.pilot_rows", 8192)
+                try:
+                    got[gemm] = ix.search(stream, qs, k, candidates=cand)
+                finally:
+                    for kn in ("vector.gemm", "vector.wide", "vector.pilot", "vector.pilot_rows"):
+                        autotune.set_knob(kn, -1)
+            assert np.array_equal(got[1][0], got[0][0]), (nq, pilot)
+            assert np.array_equal(got[1][1], got[0][1], equal_nan=True), (nq, pilot)
+            assert not np.isin(got[1][0], np.nonzero(~valid)[0]).any()

@@ -84,6 +84,10 @@ def parse():
                     help="vector: also check the certified top-10 of this many queries against a full "
                          "FP64 scan of every row on the host")
     ap.add_argument("--check", action="store_true", help="join: full-size check against numpy")
+    ap.add_argument("--vec-paths", default="fp32_exact,bf16_rerank100,fp8_rerank1000",
+                    help="vector: the search paths to time (comma separated)")
+    ap.add_argument("--vec-batches", default="1,32,256",
+                    help="vector workload: the query batch sizes to time (comma separated)")
     ap.add_argument("--serve-rates", default="50,200,400",
                     help="serving: offered request rates (requests/s), comma separated")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
@@ -724,6 +728,7 @@ def exec_vector(args, be, stream, batch_sizes=(256,), reps_of=None) -> dict:
     # FP8 e4m3 keeps ~4 significant bits: it needs a deeper rerank than BF16
     # for recall@10 >= 0.99 (recall_vs_rerank_depth below)
     paths = [("fp32_exact", "fp32", k), ("bf16_rerank100", "bf16", 100), ("fp8_rerank1000", "fp8", 1000)]
+    paths = [p for p in paths if p[0] in args.vec_paths.split(",")]
     results = {}
     for pname, iname, cand in paths:
         ix = indexes[iname]
@@ -1101,7 +1106,8 @@ def main():
     be = RocmBackend(0)
     stream = be.create_stream()
     if args.workload == "vector":
-        print(json.dumps(_line(exec_vector(args, be, stream, (1, 32, 256)), VEC_DATA)), flush=True)
+        batches = tuple(int(b) for b in args.vec_batches.split(","))
+        print(json.dumps(_line(exec_vector(args, be, stream, batches), VEC_DATA)), flush=True)
         return
     if args.workload == "aggregate":
         d = exec_aggregate(args, be, stream, args.steps, args.warmup)

@@ -664,9 +664,12 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 // the compiler can tell the line being multiplied from the one in flight
 // and waits for the copy only at the line's barrier: the next line's copy
 // overlaps this line's products (the block's last line copies the next
-// block's first). Unit u of row r sits at u ^ (r & 7): a fragment read's
-// eight lanes of one LDS cycle (eight consecutive rows at one unit) fall in
-// eight different 16-byte bank groups. The k group loop stays rolled (its
+// block's first). Unit u of row r sits at u ^ ((r >> 1) & 7): the LDS
+// serves a 16-byte read to 16 lanes a clock (256 bytes, 64 banks), and a
+// fragment read's 16 lanes (16 consecutive rows at one unit) then fall in
+// 16 different 16-byte bank slots (with u ^ (r & 7), rows r and r + 8
+// collided: 41% of the LDS's active cycles were bank conflicts). The k
+// group loop stays rolled (its
 // two unrolled copies' fragments spilled registers for FP8). Row factors
 // are double-buffered by block parity; the epilogue is the resident
 // kernels' (scores or appends).
@@ -683,6 +686,8 @@ void gemm_scan_kernel(const ScanArgs a) {
   __shared__ uint4 s_a0[kGemmRows * 8], s_q0[kGemmRows * 8], s_a1[kGemmRows * 8], s_q1[kGemmRows * 8];
   __shared__ __align__(16) float s_mul[2][kGemmRows];
   __shared__ __align__(16) float s_bias[2][kGemmRows];
+  // the bias lowered by 4e-6 of its magnitude (the append pre-test below)
+  __shared__ __align__(16) float s_blo[2][kGemmRows];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 31, h = lane >> 5;
@@ -694,22 +699,26 @@ void gemm_scan_kernel(const ScanArgs a) {
   const uint64_t tile_end = min(a.n, a.tile0 + a.tile_rows);
   // DMA staging: wave w moves rows [32 w, 32 w + 32) of the rows and of
   // the queries, 8 rows per instruction; lane l writes LDS unit l of the
-  // instruction's 64, i.e. slot l & 7 of row l >> 3, which holds unit
-  // (l & 7) ^ (row & 7) of the line (the row's low bits are l >> 3: the
-  // instruction's first row is a multiple of 8)
-  const uint32_t lane_off = (uint32_t)((lane >> 3) * rstride) + (uint32_t)((((lane & 7) ^ (lane >> 3)) & 7) * 16);
+  // instruction's 64, i.e. slot l & 7 of row r = r0 + (l >> 3), which holds
+  // unit (l & 7) ^ ((r >> 1) & 7) of the line; r0 = 32 w + 8 j, so
+  // (r >> 1) & 7 = (4 j + (l >> 4)) & 7: two lane offsets, for even and odd j
+  uint32_t lane_off[2];
+#pragma unroll
+  for (int jp = 0; jp < 2; jp++)
+    lane_off[jp] = (uint32_t)((lane >> 3) * rstride) + (uint32_t)((((lane & 7) ^ ((4 * jp + (lane >> 4)) & 7))) * 16);
   auto dma_stage = [&](uint64_t blk, int line, uint4 *sa, uint4 *sq) {
     const uint64_t row_w = a.tile0 + blk * (uint64_t)kGemmRows + (uint64_t)wave * (kGemmRows / 8);
 #pragma unroll
     for (int j = 0; j < kGemmRows / 8 / 8; j++) {
       const int r0 = wave * (kGemmRows / 8) + j * 8;
       const uint64_t row0 = row_w + (uint64_t)j * 8;
+      const uint32_t lo = lane_off[j & 1];
       // rows past the tile read the tile's last row (dropped by its bias)
-      const uint8_t *src = row0 + 8 <= tile_end ? xb + row0 * rstride + (uint64_t)line * 128 + lane_off
+      const uint8_t *src = row0 + 8 <= tile_end ? xb + row0 * rstride + (uint64_t)line * 128 + lo
                                                  : xb + min(row0 + (uint64_t)(lane >> 3), tile_end - 1) * rstride +
-                                                       (uint64_t)line * 128 + (lane_off % (uint32_t)rstride);
+                                                       (uint64_t)line * 128 + (lo % (uint32_t)rstride);
       __builtin_amdgcn_global_load_lds((const void *)src, (void *)(sa + r0 * 8), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void *)(qb + (uint64_t)r0 * rstride + (uint64_t)line * 128 + lane_off),
+      __builtin_amdgcn_global_load_lds((const void *)(qb + (uint64_t)r0 * rstride + (uint64_t)line * 128 + lo),
                                        (void *)(sq + r0 * 8), 16, 0, 0);
     }
   };
@@ -731,6 +740,7 @@ void gemm_scan_kernel(const ScanArgs a) {
     }
     s_mul[par][tid] = mul;
     s_bias[par][tid] = bias;
+    s_blo[par][tid] = bias == INFINITY ? INFINITY : bias - 4e-6f * fabsf(bias);
   };
 
   uint64_t blk = blockIdx.x;
@@ -755,14 +765,14 @@ void gemm_scan_kernel(const ScanArgs a) {
         for (int r = 0; r < RT; r++) {
           const int row = wr * RT * 32 + r * 32 + i;
 #pragma unroll
-          for (int u = 0; u < U; u++) fa[r].v[u] = sa[row * 8 + ((h * 4 + gg * U + u) ^ (row & 7))];
+          for (int u = 0; u < U; u++) fa[r].v[u] = sa[row * 8 + ((h * 4 + gg * U + u) ^ ((row >> 1) & 7))];
         }
 #pragma unroll
         for (int t = 0; t < NT; t++) {
           Frag<P> fq;
           const int qr = wq * NT * 32 + t * 32 + i;
 #pragma unroll
-          for (int u = 0; u < U; u++) fq.v[u] = sq[qr * 8 + ((h * 4 + gg * U + u) ^ (qr & 7))];
+          for (int u = 0; u < U; u++) fq.v[u] = sq[qr * 8 + ((h * 4 + gg * U + u) ^ ((qr >> 1) & 7))];
 #pragma unroll
           for (int r = 0; r < RT; r++) mfma_group<P>(acc[r][t], fa[r], fq);
         }
@@ -793,6 +803,14 @@ void gemm_scan_kernel(const ScanArgs a) {
           const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
           const float tau =
               a.list ? from_ordered((uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32)) : -INFINITY;
+          // append mode: a score sc = (acc * qs) * mul - bias is kept when
+          // sc >= tau; the pre-test acc * (qs * mul) >= tau_lo + blo, with
+          // tau and the bias lowered by 4e-6 of their magnitudes, passes
+          // every kept score (the two products differ by a few ulps of a
+          // value that, near the bound, is at most |tau| + |bias|), so the
+          // exact score is computed only for the few that pass it. A NaN
+          // bound (a pilot with fewer than C rows) keeps every score.
+          const float tau_lo = tau != tau ? -INFINITY : tau - 4e-6f * fabsf(tau);
 #pragma unroll
           for (int r = 0; r < RT; r++) {
 #pragma unroll
@@ -804,20 +822,26 @@ void gemm_scan_kernel(const ScanArgs a) {
               const float mulv[4] = {mul4.x, mul4.y, mul4.z, mul4.w};
               const float biasv[4] = {bias4.x, bias4.y, bias4.z, bias4.w};
               const uint64_t row0 = blockrow + br;
-              float sc[4];
-#pragma unroll
-              for (int e4 = 0; e4 < 4; e4++) {
-                const float sv = (acc[r][t][jg * 4 + e4] * qs) * mulv[e4] - biasv[e4];
-                sc[e4] = biasv[e4] == INFINITY ? -INFINITY : sv;
-              }
               if (a.list) {
+                const float4 blo4 = *(const float4 *)&s_blo[par][br];
+                const float blov[4] = {blo4.x, blo4.y, blo4.z, blo4.w};
 #pragma unroll
                 for (int e4 = 0; e4 < 4; e4++) {
-                  if (!(sc[e4] > -INFINITY) || sc[e4] < tau) continue;
+                  const float av = acc[r][t][jg * 4 + e4];
+                  if (!(av * (qs * mulv[e4]) >= tau_lo + blov[e4])) continue;
+                  const float sv = (av * qs) * mulv[e4] - biasv[e4];
+                  const float sce = biasv[e4] == INFINITY ? -INFINITY : sv;
+                  if (!(sce > -INFINITY) || sce < tau) continue;
                   const uint32_t pos = atomicAdd(&a.cnt[q], 1u);
-                  if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sc[e4], row0 + e4);
+                  if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sce, row0 + e4);
                 }
               } else {
+                float sc[4];
+#pragma unroll
+                for (int e4 = 0; e4 < 4; e4++) {
+                  const float sv = (acc[r][t][jg * 4 + e4] * qs) * mulv[e4] - biasv[e4];
+                  sc[e4] = biasv[e4] == INFINITY ? -INFINITY : sv;
+                }
                 float *out = a.scores + (uint64_t)q * a.tile_rows + (row0 - a.tile0);
                 if (row0 + 4 <= tile_end) {
                   *(float4 *)out = make_float4(sc[0], sc[1], sc[2], sc[3]);

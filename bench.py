@@ -507,7 +507,7 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
     import numpy as np
     import pyarrow as pa
 
-    from oxidelake_amd import datagen
+    from oxidelake_amd import autotune, datagen
     from oxidelake_amd._capi import lib
     from oxidelake_amd.backend import AggState
 
@@ -527,6 +527,9 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
         st.update_join(stream, table, ddim, 1, dfact, 0)
         return st.finish(stream)
 
+    # spill bucket size and hot-kernel grid picked by the project autotuner
+    tuned = autotune.tune_join_aggregate(step, reps=2)
+    log(f"join + aggregate shape: {tuned['knobs']}")
     out = be.download(stream, step())
     ok = None
     if check:
@@ -612,6 +615,7 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
         del dim_host, fsample
     alg = nf * (8 + 4 + 8)  # fk, qty, price: the probe side's compulsory bytes
     del table, dfact, ddim
+    autotune.reset(["agg.spill_pct", "agg.hot_grid"])
     return {
         "metric": "join + aggregate probe rows/s (GpuHashJoinExec + GpuAggregateExec, HBM-resident)",
         "value": nf / (elapsed / steps), "unit": "rows/s", "steps": steps, "warmup": warmup,
@@ -620,7 +624,7 @@ def exec_join(args, be, stream, steps: int, warmup: int, check: bool) -> dict:
                    "fact_rows": nf, "dim_rows": nd, "groups": int(out.num_rows)},
         "kernel_ms": total_ms, "probe_aggregate_ms": kms, "spill_ms": spill_ms,
         "spill_parts_ms": {"scatter": scatter_ms, "fold": fold_ms}, "device_ms_per_step": parts,
-        "build_s": build_s,
+        "build_s": build_s, "autotune": tuned,
         "parity_full_size": ok,
         "roofline": _roofline_hbm(alg, total_ms, recorded_traffic("join_aggregate", fact_rows=nf, dim_rows=nd,
                                                                   groups=args.groups)),
@@ -704,7 +708,7 @@ def exec_vector(args, be, stream, batch_sizes=(256,), reps_of=None) -> dict:
     top-10."""
     import numpy as np
 
-    from oxidelake_amd import datagen, plan
+    from oxidelake_amd import autotune, datagen, plan
     from oxidelake_amd._capi import lib
     from oxidelake_amd.backend import VEC_BF16, VEC_FP8, VEC_FP32, VectorIndex
 
@@ -730,11 +734,19 @@ def exec_vector(args, be, stream, batch_sizes=(256,), reps_of=None) -> dict:
     paths = [("fp32_exact", "fp32", k), ("bf16_rerank100", "bf16", 100), ("fp8_rerank1000", "fp8", 1000)]
     paths = [p for p in paths if p[0] in args.vec_paths.split(",")]
     results = {}
+    tuning = {}
     for pname, iname, cand in paths:
         ix = indexes[iname]
         per_q = {}
         for nq in batch_sizes:
             reps = reps_of[nq]
+            # scan kernel and pilot size picked by the project autotuner for
+            # this batch size (the GEMM tile only runs batches over 128)
+            cands = autotune.VECTOR_LARGE if nq > 128 else [c for c in autotune.VECTOR_LARGE if c[0] == 1]
+            tuning.setdefault(pname, {})[str(nq)] = autotune.sweep(
+                ("vector_scan", "vector_select", "vector_merge", "vector_rerank"),
+                ["vector.gemm", "vector.pilot_rows"], cands, lambda: ix.search(stream, queries[:nq], k, cand),
+                reps=2)
             for _ in range(2):
                 ix.search(stream, queries[:nq], k, cand)
             lat = []
@@ -763,6 +775,7 @@ def exec_vector(args, be, stream, batch_sizes=(256,), reps_of=None) -> dict:
                 "scan_gbs": n * dim * eb / (scan / 1000) / 1e9,
             }
         results[pname] = per_q
+    autotune.reset(["vector.gemm", "vector.pilot_rows"])
     # recall@10 and distance error over every query, against the certified
     # FP64 top-10
     truth, d64, certified, vecs = certified_top10(be, stream, indexes["fp32"], base_ptr, dim, queries, k)
@@ -857,7 +870,7 @@ def exec_vector(args, be, stream, batch_sizes=(256,), reps_of=None) -> dict:
         "paths": results, "index_build_s": build_s,
         "recall": {"queries": qmax, "certified_queries": nc,
                    "truth": "FP64 top-10 certified from the FP32 path's 100 candidates per query"},
-        "recall_vs_rerank_depth": recall_vs_depth, "bruteforce_check": brute,
+        "recall_vs_rerank_depth": recall_vs_depth, "bruteforce_check": brute, "autotune": tuning,
         "roofline": {"bound": "mfma", "achieved": scan_tf, "peak": MFMA_PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
                      "frac": (scan_tf / MFMA_PEAK_TFLOPS[dtype]) if scan_tf else None, "traffic": None},
         "cpu_baseline": cpu, "total_s": time.perf_counter() - t_start,
@@ -935,7 +948,7 @@ def bench_serving(args):
 
     import numpy as np
 
-    from oxidelake_amd import datagen, serving
+    from oxidelake_amd import autotune, datagen, serving
     from oxidelake_amd._capi import lib
     from oxidelake_amd.backend import VEC_BF16, RocmBackend, VectorIndex
     from oxidelake_amd import plan
@@ -961,9 +974,24 @@ def bench_serving(args):
     # compile each kind's kernels once (prepared: every literal is an argument)
     warm = serving.poisson_mix(1000.0, 30, queries, nd, seed=1, max_queries=args.serve_max_queries)
     serving.run_one_at_a_time(be, tables, warm)
+    # the project autotuner picks the vector scan kernel and pilot size for
+    # a full batch of the serving index, and the batched filter_agg kernels'
+    # shape (each candidate prepares its own batch: the grid is fixed then)
+    serving.serving_knobs()
+    tuning = {"vector_256": autotune.sweep(("vector_scan", "vector_select", "vector_merge", "vector_rerank"),
+                                           ["vector.gemm", "vector.pilot_rows"], autotune.VECTOR_LARGE,
+                                           lambda: index.search(stream, queries[:256], 10, 100), reps=2)}
     if not args.serve_no_batch_agg:
+        thresholds = [1 + 31 * i for i in range(16)]
+
+        def batch_run():
+            b = be.prepare_agg_batch(stream, ftab, serving.filter_agg_spec(1))
+            b.run(stream, [serving.filter_agg_spec(t) for t in thresholds])
+
+        tuning["filter_agg_batch_16"] = autotune.tune_agg_batch(batch_run, reps=2)
         serving.prepare_filter_batch(be, stream, tables)  # filter_agg requests batched, hipGraph replays
         serving.Server(be, tables).run(copy.deepcopy(warm))
+    log(f"serving shapes: { {k: v['knobs'] for k, v in tuning.items()} }")
 
     per_rate, same = {}, True
     mismatches = 0
@@ -1011,7 +1039,7 @@ def bench_serving(args):
                                f"{nf} fact rows JOIN {nd} dimension rows WHERE fk in a 1-10% range",
                    "rates_rps": args.serve_rates, "seconds_per_rate": args.serve_seconds},
         "rates": per_rate, "same_alone_and_in_mix": same, "mismatches": mismatches,
-        "filter_agg_batched": tables.filter_batch is not None,
+        "filter_agg_batched": tables.filter_batch is not None, "autotune": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "vector_scan (BF16): every search's rows over the time of every scan launch "

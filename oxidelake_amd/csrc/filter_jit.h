@@ -393,4 +393,162 @@ __device__ __forceinline__ void filter_persistent(const JitArgs &p) {
   if (err) atomicOr((unsigned long long *)&p.result[0], 1ull);
 }
 
+
+// One workgroup per batch (many-batch inputs): the workgroups take batches
+// blockIdx.x, blockIdx.x + gridDim.x, ... and walk each batch's tiles in row
+// order, so a tile's place in its batch is the running count of the tiles
+// before it: no look-back and no ticket. Two register sets of S rows: the
+// next tile's loads are issued before the current tile is ranked, projected
+// and stored, so the loads of one workgroup run under its own compaction.
+// Needs as many batches as there are workgroups to fill the device with;
+// the host picks it for inputs of many batches (knob filter.wg).
+template <class Q, int S, int W = 4>
+__device__ __forceinline__ void filter_batch_wg(const JitArgs &p) {
+  static_assert(S >= 1 && S <= kMaxRowsPerLane, "rows per lane: one kept bit per row in a 32-bit register");
+  static_assert(W >= 1 && W <= 16, "waves per workgroup");
+  constexpr int B = 64 * W;  // threads per workgroup
+  constexpr int NC = S * W;
+  constexpr int PL = (NC + 63) / 64;
+  constexpr int kTile = B * S;
+  constexpr int kWords = kTile / 32 + 2;
+  constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
+  __shared__ uint32_t s_total;
+  __shared__ uint32_t s_cnt[NC];
+  __shared__ uint32_t s_off[NC];
+  __shared__ uint64_t s_mask[NC];
+  __shared__ uint32_t s_bits[kNV * kWords];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const uint64_t nb = (p.n + p.batch_rows - 1) / p.batch_rows;
+  const uint64_t T = p.tiles_per_batch;
+  uint64_t b = blockIdx.x;
+  if (b >= nb) return;  // uniform
+  if (Q::kValidOuts > 0)
+    for (int i = tid; i < kNV * kWords; i += B) s_bits[i] = 0;
+
+  // the rows [base, base + lim) of tile lt of batch bb
+  auto tile_rows = [&](uint64_t bb, uint64_t lt, uint64_t &base, uint32_t &lim) {
+    const uint64_t b0 = bb * p.batch_rows;
+    const uint64_t bend = b0 + p.batch_rows;
+    const uint64_t end = p.n < bend ? p.n : bend;
+    base = b0 + lt * (uint64_t)kTile;
+    lim = end > base ? (uint32_t)(end - base < (uint64_t)kTile ? end - base : kTile) : 0u;
+  };
+  auto load_tile = [&](uint64_t bb, uint64_t lt, typename Q::Row (&x)[S]) {
+    uint64_t base;
+    uint32_t lim;
+    tile_rows(bb, lt, base, lim);
+    const typename Q::In in_cols = Q::ins(p, base);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * B + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+  };
+  bool err = false;
+  uint64_t run = 0;  // kept rows of the batch's tiles before the current one
+  uint64_t lt = 0;
+  // compacts tile lt of batch b from rows x, the next tile's loads going
+  // into xn (past the workgroup's last tile they read the current tile
+  // again, from L2: unconditional loads let the compiler wait on x's loads
+  // alone while xn's stay in flight); advances (b, lt, run)
+  auto step = [&](typename Q::Row (&x)[S], typename Q::Row (&xn)[S]) {
+    const bool last_tile = lt + 1 == T;
+    const uint64_t nbb = last_tile ? b + gridDim.x : b;
+    const uint64_t nlt = last_tile ? 0 : lt + 1;
+    if (nbb < nb) load_tile(nbb, nlt, xn);
+    else load_tile(b, lt, xn);
+    uint64_t base;
+    uint32_t lim;
+    tile_rows(b, lt, base, lim);
+    const uint64_t b0 = b * p.batch_rows;
+    uint32_t selbits = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * B + tid);
+      const bool in = j < lim;
+      const bool k = Q::pred(p.lit, x[s], j, in, err) && in;
+      selbits |= (uint32_t)k << s;
+      const uint64_t mk = __ballot(k);
+      if (lane == 0) {
+        s_cnt[s * W + wave] = (uint32_t)__popcll(mk);
+        s_mask[s * W + wave] = mk;
+      }
+    }
+    __syncthreads();  // ballots in LDS
+    if (wave == 0) {
+      uint32_t v[PL], sum = 0;
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        const int c = lane * PL + k;
+        v[k] = c < NC ? s_cnt[c] : 0u;
+        sum += v[k];
+      }
+      const uint32_t incl = wave_incl_scan(sum, lane);
+      uint32_t r = incl - sum;
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        const int c = lane * PL + k;
+        if (c < NC) s_off[c] = r;
+        r += v[k];
+      }
+      const uint32_t tot = __shfl(incl, 63, 64);  // the whole wave: lane 63's total
+      if (lane == 0) s_total = tot;
+    }
+    __syncthreads();  // offsets and total visible
+    const uint64_t prefix = b0 + run;  // a batch's output starts where its input does
+    const uint64_t w0 = prefix >> 5;
+    const uint32_t bit0 = (uint32_t)(prefix & 31);
+    const uint64_t below = (1ull << lane) - 1;
+    const typename Q::Out o = Q::outs(p, base, prefix);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * B + tid);
+      const bool k = (selbits >> s) & 1u;
+      const uint32_t q = s_off[s * W + wave] + (uint32_t)__popcll(s_mask[s * W + wave] & below);
+      if (k) Q::project(p.lit, o, x[s], j, q, bit0 + q, s_bits, kWords, err);
+      if (Q::kVecOuts > 0) Q::vectors(o, k, j, q, lane);
+    }
+    const uint32_t tot = s_total;
+    __syncthreads();  // every output bit set; every read of s_off, s_mask, s_total done
+    if (Q::kValidOuts > 0 && tot > 0) {
+      const uint64_t last = (prefix + tot - 1) >> 5;
+      const int nw = (int)(last - w0 + 1);
+      for (int k = 0; k < p.nouts; k++) {
+        const int v = p.vslot[k];
+        if (v < 0) continue;
+        uint32_t *g = p.outv[k];
+        for (int w = tid; w < nw; w += B) {
+          const uint32_t bits = s_bits[v * kWords + w];
+          s_bits[v * kWords + w] = 0;  // clear for the next tile (read only by this thread)
+          if (w == 0 || w == nw - 1) {
+            if (bits) atomicOr(&g[w0 + w], bits);
+          } else {
+            g[w0 + w] = bits;
+          }
+        }
+      }
+    }
+    run += tot;
+    if (last_tile) {
+      if (tid == 0) p.counts[b] = run;
+      run = 0;
+    }
+    b = nbb;
+    lt = nlt;
+  };
+  typename Q::Row xa[S], xb[S];
+  load_tile(b, 0, xa);
+  __syncthreads();  // s_bits cleared
+  while (true) {
+    step(xa, xb);
+    if (b >= nb) break;
+    step(xb, xa);
+    if (b >= nb) break;
+  }
+  if (err) atomicOr((unsigned long long *)&p.result[0], 1ull);
+}
+
 }  // namespace oxr

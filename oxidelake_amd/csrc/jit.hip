@@ -387,6 +387,16 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
          ") void oxr_filter_jit(const JitArgs p) {\n";
   src += std::string("  ") + (jf->persistent ? "filter_persistent" : "filter_tile") + "<Q, " +
          std::to_string(jf->rows_per_lane) + ", " + std::to_string(jf->waves) + ">(p);\n}\n";
+  // one workgroup per batch (many-batch inputs, knob filter.wg): two
+  // register sets of wg_rows rows per lane, wg_waves waves (knobs
+  // filter.wg_rows, filter.wg_waves)
+  jf->wg_rows = (int)tune_get("filter.wg_rows", 8);
+  jf->wg_waves = (int)tune_get("filter.wg_waves", 8);
+  const int wg_per_simd = jf->wg_waves / 4;
+  src += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(64 * jf->wg_waves) +
+         ") __attribute__((amdgpu_waves_per_eu(" + std::to_string(wg_per_simd) + ", " +
+         std::to_string(wg_per_simd) + "))) void oxr_filter_wg(const JitArgs p) {\n  filter_batch_wg<Q, " +
+         std::to_string(jf->wg_rows) + ", " + std::to_string(jf->wg_waves) + ">(p);\n}\n";
   jf->source = src;
   return OXR_OK;
 }
@@ -679,6 +689,7 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
   const int rc = generate(cols, pred, proj, nproj, vps, &jf);
   if (rc != OXR_OK) return rc;
   OXR_TRY(compile(dev, jf.source, "oxr_filter_jit", &jf.fn, cached_only));
+  OXR_TRY(compile(dev, jf.source, "oxr_filter_wg", &jf.wg_fn, cached_only));
   *out = std::move(jf);
   return OXR_OK;
 }

@@ -282,6 +282,10 @@ const KnobSpec kKnobs[] = {
     {"filter.persistent", 0, 1, kBool},      // generated filter: persistent grid, tiles by ticket
     {"filter.grid", 1, 16, 0},               // persistent filter: workgroups per CU (default: what fits)
     {"filter.waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 12) | (1u << 16)},  // generated filter: waves per workgroup
+    {"filter.wg", 0, 2, 0},                  // one workgroup per batch: 0 never (default), 1 many batches, 2 always
+    {"filter.wg_rows", 1, 32, 0},            // its rows per lane per register set (default 8)
+    {"filter.wg_grid", 1, 8, 0},             // its workgroups per CU (default 1)
+    {"filter.wg_waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 16)},  // its waves per workgroup (default 8)
     {"filter.v", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
     {"filter.g", 1, 4, (1u << 1) | (1u << 2) | (1u << 4)},
     {"jit.params", 0, 1, kBool},

@@ -328,6 +328,34 @@ def test_persistent_grid_takes_many_tiles_per_workgroup(backend, stream, oracle,
             autotune.set_knob(k, -1)
 
 
+@pytest.mark.parametrize("rows,waves,batch_rows,prepared", [(8, 8, 300_000, 0), (4, 4, 4_096, 0),
+                                                          (16, 4, 65_536, 1), (8, 8, 32, 1)])
+def test_batch_per_workgroup_is_bit_exact(backend, stream, oracle, rows, waves, batch_rows, prepared):
+    # one workgroup per batch (filter.wg = 2: whatever the batch count),
+    # walking its batches' tiles in order with a running offset; more
+    # batches than workgroups at the smaller batch sizes, a ragged last
+    # batch, and the query at its 1% and 50% selectivity points
+    autotune.set_knob("filter.wg", 2)
+    autotune.set_knob("filter.wg_rows", rows)
+    autotune.set_knob("filter.wg_waves", waves)
+    autotune.set_knob("jit.params", prepared)
+    try:
+        n = 700_003
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=13), n)
+        host = backend.download(stream, dev)
+        for b_thr, c_thr, modulus in [(50, 0.5, 7), (30, 0.5, 1000), (470, -1.0, 5)]:
+            pred, proj = datagen.filter_query(b_thr, c_thr, modulus)
+            outs = backend.filter_project_batches(stream, dev, batch_rows, pred, proj)
+            assert len(outs) == -(-n // batch_rows)
+            check = range(len(outs)) if len(outs) <= 64 else list(range(0, len(outs), 97)) + [len(outs) - 1]
+            for i in check:
+                want = oracle.filter_project(host.slice(i * batch_rows, batch_rows), pred, proj)
+                assert_batches_equal(backend.download(stream, outs[i]), want, names=False)
+    finally:
+        for k in ("filter.wg", "filter.wg_rows", "filter.wg_waves", "jit.params"):
+            autotune.set_knob(k, -1)
+
+
 def test_generated_kernel_is_the_one_that_runs(backend, stream):
     # the per-query kernel, not the interpreter, runs a query in its grammar
     from oxidelake_amd._capi import lib

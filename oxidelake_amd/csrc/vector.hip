@@ -674,6 +674,7 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 // are double-buffered by block parity; the epilogue is the resident
 // kernels' (scores or appends).
 constexpr int kGemmRows = 256, kGemmQueries = 256, kGemmThreads = 512;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <int P, int WR, int RT, int NT>
 __global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))

@@ -317,13 +317,13 @@ def test_gemm_scan_matches_streamed(backend, stream, precision, metric):
         qs = (vecs[rng.integers(0, n, nq)] + rng.normal(scale=0.5, size=(nq, dim))).astype(np.float32)
         for pilot in (0, 1):
             got = {}
-            for gemm in (1, 0):
-                autotune.set_knob("vector.gemm", gemm)
-                autotune.set_knob("vector.wide", 1 if gemm else 0)
+            for g in (1, 0):
+                autotune.set_knob("vector.gemm", g)
+                autotune.set_knob("vector.wide", 1 if g else 0)
                 autotune.set_knob("vector.pilot", pilot)
                 autotune.set_knob("vector.pilot_rows", 8192)
                 try:
-                    got[gemm] = ix.search(stream, qs, k, candidates=cand)
+                    got[g] = ix.search(stream, qs, k, candidates=cand)
                 finally:
                     for kn in ("vector.gemm", "vector.wide", "vector.pilot", "vector.pilot_rows"):
                         autotune.set_knob(kn, -1)

@@ -315,6 +315,7 @@ const KnobSpec kKnobs[] = {
     {"vector.wide", 0, 3, 0},
     {"vector.nt8", 0, 1, kBool},
     {"vector.pilot", 0, 1, kBool},
+    {"vector.select_sampled", 0, 1, kBool},   // selection: sampled bound, then one sort (default 1)
     {"vector.pilot_rows", 1, (int64_t)1 << 40, 0},
     {"vector.pilot_cap", 1, (int64_t)1 << 32, 0},
 };

@@ -954,12 +954,89 @@ __device__ void top_c_ahead(Fetch fetch, Conv conv, uint64_t len, int C, uint64_
   compact(buf, count, tau, C);
 }
 
+// The same result as top_c_ahead in about two sorts instead of one per
+// kBuf - kChunk keys kept: 2048 keys sampled at even strides are sorted,
+// and the sample ranked at 2 C x (2048 / len) bounds the keys worth
+// keeping (about 2 C of them pass); every key above the bound is gathered
+// in one pass and the buffer sorted once. A bound that keeps fewer than C
+// keys, or more than the buffer holds, falls back to top_c_ahead, so the
+// kept keys are always the best C of all `len`.
+template <class R, class Fetch, class Conv>
+__device__ void top_c_sampled(Fetch fetch, Conv conv, uint64_t len, int C, uint64_t *buf, int *count,
+                              uint64_t *tau) {
+  constexpr int kSample = kBuf / 2;
+  __shared__ int s_overflow;
+  if (len <= (uint64_t)(4 * kSample)) {  // short inputs: the sample would be most of them
+    top_c_ahead<R>(fetch, conv, len, C, buf, count, tau);
+    return;
+  }
+  for (int t = threadIdx.x; t < kBuf; t += kBlock) {
+    uint64_t key = 0;
+    if (t < kSample) {
+      const uint64_t idx = (uint64_t)t * len / kSample;
+      key = conv(fetch(idx), idx);
+    }
+    buf[t] = key;
+  }
+  __syncthreads();
+  sort_desc(buf);
+  // about 2 C keys pass: a sample rank r keeps r len / 2048 +- sqrt(r) len /
+  // 2048 keys, which stays within [C, kBuf] for C up to 1024 and every
+  // segment length the host makes (at least 32 C rows)
+  const int rank = min(kSample - 1, max(4, (int)((2ull * (uint64_t)C * kSample + len / 2) / len)));
+  const uint64_t bound = buf[rank];
+  __syncthreads();  // every thread has read the bound before the buffer is reused
+  if (threadIdx.x == 0) {
+    *count = 0;
+    s_overflow = 0;
+  }
+  __syncthreads();
+  if (bound != 0) {
+    constexpr int KPT = kChunk / kBlock;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1;
+    for (uint64_t base = 0; base < len; base += kChunk) {
+      uint64_t keys[KPT];
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
+        keys[j] = idx < len ? conv(fetch(idx), idx) : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        const bool pass = keys[j] > bound;
+        const uint64_t m = __ballot(pass);
+        if (m == 0) continue;
+        int at = 0;
+        if (lane == 0) at = atomicAdd(count, __popcll(m));
+        at = __shfl(at, 0, 64);
+        const int pos = at + __popcll(m & below);
+        if (pass) {
+          if (pos < kBuf) buf[pos] = keys[j];
+          else s_overflow = 1;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int cnt = *count;
+  const bool fallback = bound == 0 || s_overflow != 0 || cnt < C;
+  __syncthreads();  // every thread has read the count and flag
+  if (fallback) {
+    top_c_ahead<R>(fetch, conv, len, C, buf, count, tau);
+    return;
+  }
+  compact(buf, count, tau, C);
+}
+
 struct SelectArgs {
   const float *scores;  // [nq][tile_rows]
   uint64_t tile0, tile_len, tile_rows, seg_rows;
   int32_t nq, C, seg_first, segs_total;
   uint64_t *cand;       // [nq][segs_total][C]
+  int32_t sampled;      // top_c_sampled (knob vector.select_sampled), else top_c_ahead
 };
+__device__ inline bool tune_get_select_sampled(const SelectArgs &a) { return a.sampled != 0; }
 
 
 
@@ -974,9 +1051,12 @@ __global__ __launch_bounds__(kBlock) void select_tile_kernel(const SelectArgs a)
   const uint64_t len = min(a.seg_rows, a.tile_len - lo);
   const float *sp = a.scores + (uint64_t)q * a.tile_rows + lo;
   const uint64_t row0 = a.tile0 + lo;
-  top_c_ahead<float>([&](uint64_t idx) -> float { return sp[idx]; },
-                     [&](float s, uint64_t idx) -> uint64_t { return s > -INFINITY ? score_key(s, row0 + idx) : 0; },
-                     len, a.C, s_buf, &s_count, &s_tau);
+  auto fetch = [&](uint64_t idx) -> float { return sp[idx]; };
+  auto conv = [&](float s, uint64_t idx) -> uint64_t { return s > -INFINITY ? score_key(s, row0 + idx) : 0; };
+  if (tune_get_select_sampled(a) && a.C >= 128)  // few candidates: the streaming buffer rarely sorts
+    top_c_sampled<float>(fetch, conv, len, a.C, s_buf, &s_count, &s_tau);
+  else
+    top_c_ahead<float>(fetch, conv, len, a.C, s_buf, &s_count, &s_tau);
   uint64_t *out = a.cand + ((uint64_t)q * a.segs_total + a.seg_first + blockIdx.x) * a.C;
   for (int j = threadIdx.x; j < a.C; j += kBlock) out[j] = s_buf[j];
 }
@@ -1347,7 +1427,8 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
           launch_scan(ix->precision, cfg, nqc, ix->dev, a.nblocks, st, a);
           OXR_HIP(hipGetLastError());
         }
-        SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand};
+        SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand,
+                      (int32_t)(tune_get("vector.select_sampled", 1) > 0)};
         const uint64_t segs_here = (tlen + seg_rows - 1) / seg_rows;
         {
           ProfScope prof(st, "vector_select");

@@ -330,3 +330,41 @@ def test_gemm_scan_matches_streamed(backend, stream, precision, metric):
             assert np.array_equal(got[1][0], got[0][0]), (nq, pilot)
             assert np.array_equal(got[1][1], got[0][1], equal_nan=True), (nq, pilot)
             assert not np.isin(got[1][0], np.nonzero(~valid)[0]).any()
+
+
+@pytest.mark.parametrize("precision", [VEC_FP32, VEC_BF16, VEC_FP8])
+@pytest.mark.parametrize("table", ["mixture", "sparse", "ties"])
+def test_sampled_selection_matches_streaming(backend, stream, precision, table):
+    """Selection by a sampled bound then one sort (vector.select_sampled = 1)
+    against the streaming selection, bit for bit, on the plain scan and the
+    pilot path: a mixture table; one with 97% NULL rows (the sample finds no
+    bound: the streaming fallback); one of identical rows (every score ties,
+    so the bound keeps more keys than the buffer holds: the fallback)."""
+    from oxidelake_amd import autotune
+
+    rng = np.random.default_rng(17 + precision)
+    n, dim, k = 200_003, 64, 10
+    vecs = rng.normal(size=(n, dim)).astype(np.float32)
+    valid = np.ones(n, bool)
+    if table == "sparse":
+        valid = rng.random(n) < 0.03
+    elif table == "ties":
+        vecs[:] = vecs[0]
+    dev = backend.upload(stream, vec_batch(vecs, valid))
+    ix = VectorIndex(stream, dev, 1, M.Cosine, precision)
+    cand = {VEC_BF16: 100, VEC_FP8: 1000, VEC_FP32: k}[precision]
+    for nq in (1, 40, 256):
+        qs = (vecs[rng.integers(0, n, nq)] + rng.normal(scale=0.3, size=(nq, dim))).astype(np.float32)
+        for pilot in (0, 1):
+            got = {}
+            for sampled in (1, 0):
+                autotune.set_knob("vector.select_sampled", sampled)
+                autotune.set_knob("vector.pilot", pilot)
+                autotune.set_knob("vector.pilot_rows", 16384)
+                try:
+                    got[sampled] = ix.search(stream, qs, k, candidates=cand)
+                finally:
+                    for kn in ("vector.select_sampled", "vector.pilot", "vector.pilot_rows"):
+                        autotune.set_knob(kn, -1)
+            assert np.array_equal(got[1][0], got[0][0]), (nq, pilot)
+            assert np.array_equal(got[1][1], got[0][1], equal_nan=True), (nq, pilot)

@@ -25,6 +25,7 @@ struct JitFilter {
   int rows_per_lane = 0;       // S: a tile is 256 * S rows
   bool persistent = false;     // filter_persistent: a grid that fits at once, tiles by ticket
   int waves = 4;               // W: a workgroup is 64 * W threads
+  bool has_wg = false;         // the source holds filter_batch_wg (knob filter.wg set)
   hipFunction_t wg_fn = nullptr;  // filter_batch_wg: one workgroup per batch
   int wg_rows = 8;             // its rows per lane per register set
   int wg_waves = 8;            // its waves per workgroup

@@ -392,8 +392,10 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   // filter.wg_rows, filter.wg_waves)
   jf->wg_rows = (int)tune_get("filter.wg_rows", 8);
   jf->wg_waves = (int)tune_get("filter.wg_waves", 8);
+  jf->has_wg = tune_get("filter.wg", 0) > 0;  // emitted only when it may run (hipRTC time)
   const int wg_per_simd = jf->wg_waves / 4;
-  src += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(64 * jf->wg_waves) +
+  if (jf->has_wg)
+    src += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(64 * jf->wg_waves) +
          ") __attribute__((amdgpu_waves_per_eu(" + std::to_string(wg_per_simd) + ", " +
          std::to_string(wg_per_simd) + "))) void oxr_filter_wg(const JitArgs p) {\n  filter_batch_wg<Q, " +
          std::to_string(jf->wg_rows) + ", " + std::to_string(jf->wg_waves) + ">(p);\n}\n";
@@ -689,7 +691,7 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
   const int rc = generate(cols, pred, proj, nproj, vps, &jf);
   if (rc != OXR_OK) return rc;
   OXR_TRY(compile(dev, jf.source, "oxr_filter_jit", &jf.fn, cached_only));
-  OXR_TRY(compile(dev, jf.source, "oxr_filter_wg", &jf.wg_fn, cached_only));
+  if (jf.has_wg) OXR_TRY(compile(dev, jf.source, "oxr_filter_wg", &jf.wg_fn, cached_only));
   *out = std::move(jf);
   return OXR_OK;
 }

@@ -1222,7 +1222,7 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     }
     if (rc == OXR_OK && e == hipSuccess) {
       ProfScope p3(s->s, "aggregate_fold");
-      rc = jit_launch_raw(ja.fold_fn, nparts, s->s, &sa, sizeof sa, 256);
+      rc = jit_launch_raw(ja.fold_fn, nparts, s->s, &sa, sizeof sa, (unsigned)kFoldThreads);
     }
     if (rc == OXR_OK && e != hipSuccess) rc = fail_hip(e, "aggregate spill passes");
     stream_release(s, tmp);

@@ -515,7 +515,11 @@ __host__ __device__ constexpr int null_index(int a) {
 
 // Workgroup p folds partition p's records into an LDS table of 2^slog2
 // groups (accumulators, used and non-NULL flags: plain byte stores), then
-// adds each used group's accumulators to the HBM table.
+// adds each used group's accumulators to the HBM table. kFoldThreads
+// threads each keep kFoldRecs records' loads in flight (a partition is a
+// few MB of records and there are about two partitions per CU: 256 threads
+// x 4 records left the fold waiting on memory).
+constexpr int kFoldThreads = 512, kFoldRecs = 8;
 template <class Q>
 __device__ __forceinline__ void spill_fold(const SpillArgs &sa) {
   using SR = SpillRec<Q>;
@@ -529,7 +533,7 @@ __device__ __forceinline__ void spill_fold(const SpillArgs &sa) {
     if (threadIdx.x == 0) atomicOr(&sa.error[1], 1u);
     return;
   }
-  for (uint32_t i = threadIdx.x; i < G; i += 256) {
+  for (uint32_t i = threadIdx.x; i < G; i += kFoldThreads) {
 #pragma unroll
     for (int a = 0; a < NA; a++) {
       s_acc[a][i] = acc_init(Q::kKind[a]);
@@ -540,19 +544,19 @@ __device__ __forceinline__ void spill_fold(const SpillArgs &sa) {
   __syncthreads();
   const uint32_t part = blockIdx.x;
   const uint64_t lo = sa.offs[(uint64_t)part * sa.B], hi = sa.offs[(uint64_t)(part + 1) * sa.B];
-  for (uint64_t r0 = lo + threadIdx.x; r0 < hi; r0 += 4 * 256) {
-    uint32_t rec[4][R];
+  for (uint64_t r0 = lo + threadIdx.x; r0 < hi; r0 += (uint64_t)kFoldRecs * kFoldThreads) {
+    uint32_t rec[kFoldRecs][R];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const uint64_t r = r0 + k * 256;
+    for (int k = 0; k < kFoldRecs; k++) {
+      const uint64_t r = r0 + (uint64_t)k * kFoldThreads;
       if (r < hi) load_words<R>(sa.recs + r * R, rec[k]);
       else
 #pragma unroll
         for (int x = 0; x < R; x++) rec[k][x] = 0;
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (r0 + k * 256 >= hi) continue;
+    for (int k = 0; k < kFoldRecs; k++) {
+      if (r0 + (uint64_t)k * kFoldThreads >= hi) continue;
       const uint32_t g = rec[k][0] & (G - 1), cm = SR::mask(rec[k]);
       s_used[g] = 1;
 #pragma unroll
@@ -565,7 +569,7 @@ __device__ __forceinline__ void spill_fold(const SpillArgs &sa) {
   }
   __syncthreads();
   bool ok = true;
-  for (uint32_t i = threadIdx.x; i < G && ok; i += 256) {
+  for (uint32_t i = threadIdx.x; i < G && ok; i += kFoldThreads) {
     if (!s_used[i]) continue;
     const int64_t key = (int64_t)((uint64_t)sa.smin + (uint64_t)part * G + i);
     const uint64_t gs = slot_of(sa.t, true, key, &ok);

@@ -611,7 +611,7 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   if (ja->spill)
     src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_spill_scatter(const SpillArgs a) {\n"
            "  spill_scatter<Q>(a);\n}\n"
-           "extern \"C\" __global__ __launch_bounds__(256) void oxr_spill_fold(const SpillArgs a) {\n"
+           "extern \"C\" __global__ __launch_bounds__(512) void oxr_spill_fold(const SpillArgs a) {\n"
            "  spill_fold<Q>(a);\n}\n";
   ja->spill_words = spill_words + (any_null ? 1 : 0);
   ja->hot = ja->spill;

@@ -1076,6 +1076,8 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
       A.dense_null = in->cols[st->key_col].validity ? 1 : 0;
     }
   }
+  // a global aggregate: the keyless kernel (knob agg.keyless)
+  if (ja.keyless && !js && tune_get("agg.keyless", 1) > 0) fn = ja.keyless_fn;
   // spill the cold tail of a high-cardinality key (see spill_fold):
   // a measured key range that the fold's partitions cover, every aggregate
   // in the block table, plain FP64 sums, and enough rows to pay for it
@@ -1102,8 +1104,9 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   // more resident blocks (knob agg.hot_grid), the others take agg.jit_grid
   const uint64_t tile = 256ull * ja.rows_per_lane;
   const uint64_t ntiles = (in->rows + tile - 1) / tile;
-  const uint64_t per_cu =
-      (uint64_t)std::max<int64_t>(1, hot_try ? tune_get("agg.hot_grid", 8) : tune_get("agg.jit_grid", 4));
+  const uint64_t per_cu = (uint64_t)std::max<int64_t>(
+      1, hot_try ? tune_get("agg.hot_grid", 8)
+                 : fn == ja.keyless_fn && fn ? tune_get("agg.keyless_grid", 4) : tune_get("agg.jit_grid", 4));
   const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)std::max(1, s->dev->cu_count) * per_cu);
   const uint32_t words = (uint32_t)ja.spill_words;  // 32-bit words per record
   void *spill_mem = nullptr;
@@ -1171,7 +1174,10 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
     }
   }
   if (rc == OXR_OK) {
-    ProfScope prof(s->s, fn == ja.fn ? "aggregate" : fn == ja.hot_fn ? "aggregate_hot" : "aggregate_dense");
+    ProfScope prof(s->s, fn == ja.fn ? "aggregate"
+                         : fn == ja.hot_fn ? "aggregate_hot"
+                         : fn == ja.keyless_fn ? "aggregate_keyless"
+                                               : "aggregate_dense");
     rc = jit_launch_raw(fn, (unsigned)grid, s->s, &A, sizeof A);
   }
   if (rc == OXR_OK && nhot) {

@@ -1308,4 +1308,96 @@ __device__ __forceinline__ void aggregate_multi_final(const AggJitArgs &p) {
   if (tid == 0) p.mres[(uint64_t)r * (NA + 1) + NA] = s_fl[0] | s_fl[1] | s_fl[2] | s_fl[3];
 }
 
+// ------------------------------------------------- keyless aggregation
+// A global aggregate (no group key, no join) has one group, so the block
+// table and its per-wave matching are pure overhead: each lane keeps its
+// own accumulators in registers over every tile it takes (tile +=
+// gridDim.x), FP64 sums as (hi, lo) pairs in exact mode; at the end the
+// block folds its waves by a fixed tree and then in order, and applies one
+// update per aggregate to the HBM table's NULL-key slot (the global group),
+// the FP64 sums compensated there as everywhere in exact mode. The slot is
+// claimed (seen[0]) only by a block that selected a row, so a global
+// aggregate over no selected row still reads COUNT 0 and NULLs.
+constexpr uint32_t kKeylessSel = 1u << 31;  // flag bit: the block selected a row
+
+template <class Q, int S>
+__device__ __forceinline__ void aggregate_keyless(const AggJitArgs &p) {
+  constexpr int NA = Q::kAggs;
+  constexpr int kTile = 256 * S;
+  __shared__ int64_t s_acc[4][NA];
+  __shared__ double s_lo[4][NA];
+  __shared__ uint32_t s_fl[4];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  int64_t acc[NA];
+  double lo[NA];
+  uint32_t fl = 0;
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    acc[a] = acc_init(Q::kKind[a]);
+    lo[a] = 0.0;
+  }
+  bool err = false;
+  const uint64_t ntiles = (p.n + kTile - 1) / kTile;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    const typename Q::In in_cols = Q::ins(p, base);
+    typename Q::Row x[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      Q::load(in_cols, j, j < lim, x[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t j = (uint32_t)(s * 256 + tid);
+      const bool in = j < lim;
+      const bool sel = in && Q::pred(p.lit, x[s], j, in, err);
+      int64_t c[NA];
+      uint32_t cm = 0;
+      Q::contrib(p.lit, x[s], j, sel, c, cm, err);
+      if (sel) fl |= kKeylessSel;
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (!((cm >> a) & 1u)) continue;
+        if (Q::kKind[a] == A_SUM_F64 && !Q::kExactF64)
+          acc[a] = combine_kind(A_SUM_F64, acc[a], c[a]);
+        else
+          multi_combine(Q::kKind[a], acc[a], lo[a], c[a], 0.0);
+        fl |= 1u << a;
+      }
+    }
+  }
+  multi_wave<Q>(acc, lo, fl);
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      s_acc[wave][a] = acc[a];
+      s_lo[wave][a] = lo[a];
+    }
+    s_fl[wave] = fl;
+  }
+  if (err) atomicOr(p.error, 1u);
+  __syncthreads();
+  const uint32_t f = s_fl[0] | s_fl[1] | s_fl[2] | s_fl[3];
+  if (tid < NA && (f & kKeylessSel)) {
+    const int a = tid;
+    const uint8_t kind = Q::kKind[a];
+    int64_t v = s_acc[0][a];
+    double vl = s_lo[0][a];
+    for (int w = 1; w < 4; w++) multi_combine(kind, v, vl, s_acc[w][a], s_lo[w][a]);
+    bool ok = true;
+    const uint64_t gs = slot_of(p.t, false, 0, &ok);  // the global group
+    if ((f >> a) & 1u) {
+      if (Q::kExactF64 && kind == A_SUM_F64)
+        sum_f64(&p.t.acc[a][gs], &p.t.lo[a][gs], as_f64((uint64_t)v), vl);
+      else if (kind == A_SUM_F64)
+        acc_apply_global(kind, &p.t.acc[a][gs], (int64_t)f64_bits(sum_value(as_f64((uint64_t)v), vl)));
+      else
+        acc_apply_global(kind, &p.t.acc[a][gs], v);
+      if (p.t.has[a]) p.t.has[a][gs] = 1;
+    }
+  }
+}
+
 }  // namespace oxr

@@ -73,6 +73,8 @@ struct JitAgg {
   hipFunction_t hot_fn = nullptr;
   bool spill = false;  // rows may spill: the spill's kernels below exist
   hipFunction_t scatter_fn = nullptr, fold_fn = nullptr;
+  bool keyless = false;  // a global aggregate: the keyless kernel below exists
+  hipFunction_t keyless_fn = nullptr;
   int spill_words = 0;  // 32-bit words per spill record
   int rows_per_lane = 0;
   std::vector<uint64_t> lits;  // AggJitArgs::lit (prepared mode)

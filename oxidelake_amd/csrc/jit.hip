@@ -623,6 +623,11 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
     src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_hot(const AggJitArgs p) {\n"
            "  aggregate_hot<Q, " + std::to_string(hot_rows) + ">(p);\n}\n";
   }
+  // a global aggregate: per-lane register accumulators (aggregate_keyless)
+  ja->keyless = sh.key_col == -1 && !sh.join;
+  if (ja->keyless)
+    src += "extern \"C\" __global__ __launch_bounds__(256) void oxr_aggregate_keyless(const AggJitArgs p) {\n"
+           "  aggregate_keyless<Q, " + std::to_string(ja->rows_per_lane) + ">(p);\n}\n";
   ja->dense = sh.key_col >= 0 && !sh.join;
   if (ja->dense) {
     // small key ranges: per-lane LDS accumulators (aggregate_dense)
@@ -656,6 +661,7 @@ int jit_aggregate(oxr_device *dev, const std::vector<DevCol> &cols, const AggJit
     OXR_TRY(compile(dev, ja.source, "oxr_aggregate_dense32", &ja.dense32));
   }
   if (ja.hot) OXR_TRY(compile(dev, ja.source, "oxr_aggregate_hot", &ja.hot_fn));
+  if (ja.keyless) OXR_TRY(compile(dev, ja.source, "oxr_aggregate_keyless", &ja.keyless_fn));
   if (ja.spill) {
     OXR_TRY(compile(dev, ja.source, "oxr_spill_scatter", &ja.scatter_fn));
     OXR_TRY(compile(dev, ja.source, "oxr_spill_fold", &ja.fold_fn));

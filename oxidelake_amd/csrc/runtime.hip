@@ -307,6 +307,8 @@ const KnobSpec kKnobs[] = {
     {"agg.hot_rows", 1, (int64_t)1 << 48, 0},  // rows from which an update samples for hot groups
     {"agg.hot_lane_rows", 1, 32, 0},           // hot-group kernel: rows per lane (default: the shape's)
     {"agg.hot_grid", 1, 64, 0},                // hot-group kernel: workgroups per CU (default 8)
+    {"agg.keyless", 0, 1, kBool},              // global aggregates: the keyless kernel (default 1)
+    {"agg.keyless_grid", 1, 64, 0},            // keyless kernel: workgroups per CU (default 4)
     {"agg.multi_rows", 1, 16, 0},              // batched requests: rows per lane (default 4)
     {"agg.multi_grid", 1, 16, 0},              // batched requests: workgroups per CU (default 2)
     {"agg.batch_graph", 0, 1, kBool},          // batched requests: replay a captured hipGraph (default 1)

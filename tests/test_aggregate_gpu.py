@@ -215,6 +215,80 @@ def test_global_aggregate_over_no_rows_is_one_row(backend, stream, oracle):
     assert_agg_equal(got, oracle.aggregate(b, spec), spec)
 
 
+# global aggregates (no key): every aggregate kind, filtered, over NULLs and
+# the float specials of random_batch (i Int64, j Int32, f Float32, g Float64)
+GLOBAL_SPECS = [
+    AggregateSpec(-1, [Aggregate(F.CountStar), Aggregate(F.Sum, col(0)), Aggregate(F.Sum, plan.mul(col(2), col(3))),
+                       Aggregate(F.Min, col(2)), Aggregate(F.Max, col(1)), Aggregate(F.Count, col(2)),
+                       Aggregate(F.Max, col(3)), Aggregate(F.Min, col(0))],
+                  filter=plan.cmp(col(1), Cmp.Lt, Lit.int32(3))),
+    AggregateSpec(-1, [Aggregate(F.Sum, col(3)), Aggregate(F.Sum, col(1)), Aggregate(F.Min, col(3))]),
+    AggregateSpec(-1, [Aggregate(F.Count, col(0)), Aggregate(F.Sum, col(2))],
+                  filter=plan.cmp(col(1), Cmp.Gt, Lit.int32(100))),  # no row selected
+]
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 2_000_003])
+def test_global_aggregates_take_the_keyless_kernel(backend, stream, oracle, path, n):
+    """A global aggregate runs the keyless kernel (per-lane accumulators,
+    one table update per block) on every generated path."""
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    batch = datagen.random_batch(np.random.default_rng(n + 5), n).select([0, 1, 2, 3])
+    dev = backend.upload(stream, batch)
+    for spec in GLOBAL_SPECS:
+        lib.oxr_profile_enable(1)
+        lib.oxr_profile_reset()
+        try:
+            got = backend.download(stream, backend.aggregate(stream, dev, spec))
+            # (no launch at all over no rows)
+            assert autotune.launches("aggregate_keyless") == (0 if path == "interpreter" or n == 0 else 1)
+        finally:
+            lib.oxr_profile_enable(0)
+            lib.oxr_profile_reset()
+        assert_agg_equal(got, oracle.aggregate(batch, spec, sum_mode=1), spec)
+
+
+def test_keyless_streams_and_matches_the_block_table(backend, stream, oracle, path):
+    """The keyless kernel over a resident state fed in uneven batches, and
+    against the block-table kernel (knob agg.keyless 0) on the same input:
+    integers and float MIN/MAX bit for bit, FP64 sums within 1e-12."""
+    from oxidelake_amd import autotune
+
+    n = 400_000
+    batch = datagen.random_batch(np.random.default_rng(77), n).select([0, 1, 2, 3])
+    spec = GLOBAL_SPECS[0]
+    want = oracle.aggregate(batch, spec, sum_mode=1)
+    parts = [backend.upload(stream, batch.slice(a, b - a)) for a, b in [(0, 3), (3, 70_000), (70_000, n)]]
+    st = AggState(backend, stream, parts[0], spec)
+    for d in parts:
+        st.update(stream, d)
+    assert_agg_equal(backend.download(stream, st.finish(stream)), want, spec)
+    dev = backend.upload(stream, batch)
+    keyless = backend.download(stream, backend.aggregate(stream, dev, spec))
+    autotune.set_knob("agg.keyless", 0)
+    try:
+        table = backend.download(stream, backend.aggregate(stream, dev, spec))
+    finally:
+        autotune.set_knob("agg.keyless", -1)
+    assert_agg_equal(keyless, table, spec)
+
+
+def test_keyless_division_by_zero_fails(backend, stream, oracle, path):
+    from oxidelake_amd.backend import ExecutionError
+
+    b = pa.record_batch({"x": pa.array([5, 7, 9, 11], pa.int64()), "y": pa.array([1, 2, 0, 3], pa.int64())})
+    spec = AggregateSpec(-1, [Aggregate(F.Sum, plan.mod(col(0), col(1)))])
+    with pytest.raises(ZeroDivisionError):
+        oracle.aggregate(b, spec)
+    with pytest.raises(ExecutionError):
+        device_aggregate(backend, stream, b, spec)
+    # the same expression over rows the filter leaves out of the zero
+    ok = AggregateSpec(-1, spec.aggregates, filter=plan.cmp(col(1), Cmp.NotEq, Lit.int64(0)))
+    assert_agg_equal(device_aggregate(backend, stream, b, ok), oracle.aggregate(b, ok), ok)
+
+
 def test_hot_keys_fold_across_the_wave(backend, stream, oracle):
     # one key on most rows (the matching round folds it per wave), NULL keys
     # and NULL inputs mixed in, a cold tail of distinct keys

@@ -129,6 +129,24 @@ def test_groupby_and_star_kernels():
     assert "spill_scatter<Q>(a)" in src and "spill_fold<Q>(a)" in src
 
 
+@pytest.mark.parametrize("exact", [0, 1])
+def test_keyless_kernel_only_for_global_aggregates(exact):
+    autotune.set_knob("agg.exact_f64", exact)
+    try:
+        glob = [s for s in AGG_SPECS if s.group_by < 0]
+        keyed = [s for s in AGG_SPECS if s.group_by >= 0]
+        assert glob and keyed
+        for spec in glob:
+            src = backend.aggregate_kernel_source(AGG_COLS, spec, compile=True)
+            assert "oxr_aggregate_keyless" in src and "aggregate_keyless<Q, " in src
+        for spec in keyed:
+            assert "oxr_aggregate_keyless" not in backend.aggregate_kernel_source(AGG_COLS, spec)
+        c3 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False)]
+        assert "keyless" not in backend.aggregate_kernel_source(c3, datagen.star_query_spec(), probe_key=0)
+    finally:
+        autotune.set_knob("agg.exact_f64", -1)
+
+
 def test_aggregation_without_aggregates_uses_the_interpreter():
     from oxidelake_amd.backend import Unsupported
 

@@ -1845,6 +1845,10 @@ int oxr_agg_batch_run(oxr_stream *s, oxr_agg_batch *b, const oxr_agg_spec *reque
   std::lock_guard<std::mutex> lock(b->mu);
   b->hlits[0] = (uint64_t)n;
   std::copy(lits.begin(), lits.end(), b->hlits + 1);
+  // the kernel runs all of its K request slots: the unused ones repeat the
+  // last request's literals (their results are never read)
+  for (int r = n; r < kMultiK[ki]; r++)
+    std::copy(lits.end() - b->nlit, lits.end(), b->hlits + 1 + (size_t)r * b->nlit);
   if (tune_get("agg.batch_graph", 1) > 0) {
     OXR_TRY(batch_graph(b, ki));
     ProfScope prof(s->s, "aggregate_batch");

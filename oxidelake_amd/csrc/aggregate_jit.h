@@ -154,7 +154,11 @@ __device__ inline void wave_sum_f64(double &hi, double &lo) {
 }
 
 // hi + lo as the accumulator's value (lo == 0 keeps hi's sign of zero).
-__host__ __device__ inline double sum_value(double hi, double lo) { return lo != 0.0 ? hi + lo : hi; }
+// A non-finite hi is the sum (inf or NaN absorbs every later addition, and
+// lo may then hold the NaN of an unguarded error term).
+__host__ __device__ inline double sum_value(double hi, double lo) {
+  return lo != 0.0 && __builtin_isfinite(hi) ? hi + lo : hi;
+}
 
 // The per-row contribution of a value of `in_type` (widened to 64 bits:
 // ints sign-extended, floats as their bits) to an accumulator.
@@ -1137,6 +1141,30 @@ __device__ inline void multi_combine(uint8_t kind, int64_t &acc, double &lo, int
   }
 }
 
+// One row's contribution v (when `use`) to a lane's (acc, lo), branch-free:
+// an unused row adds the kind's identity. FP64 sums take TwoSum without the
+// finiteness guard: once hi is not finite it stays so, and sum_value and the
+// pair additions below ignore lo then.
+__device__ inline void multi_row(uint8_t kind, int64_t &acc, double &lo, bool use, int64_t v) {
+  switch (kind) {
+    case A_SUM_F64: {
+      const double hi = as_f64((uint64_t)acc);
+      const double x = use ? as_f64((uint64_t)v) : 0.0;  // hi is never -0.0: +0.0 is exact
+      const double s = hi + x;
+      lo += two_sum_err(hi, x, s);
+      acc = (int64_t)f64_bits(s);
+      break;
+    }
+    case A_MIN_I64:
+    case A_MIN_F64:
+    case A_MIN_F32: acc = use && v < acc ? v : acc; break;
+    case A_MAX_I64:
+    case A_MAX_F64:
+    case A_MAX_F32: acc = use && v > acc ? v : acc; break;
+    default: acc = (int64_t)((uint64_t)acc + (uint64_t)(use ? v : 0)); break;
+  }
+}
+
 // Combines (acc, lo, fl) over the wave by a fixed xor tree: every lane ends
 // with the same values whatever the order its partners' were made in
 // (addition is commutative and TwoSum's error exact).
@@ -1159,6 +1187,7 @@ __device__ __forceinline__ void aggregate_multi(const AggJitArgs &p) {
   constexpr int NA = Q::kAggs;
   constexpr int W = 2 * NA + 1;
   constexpr int kTile = 256 * S;
+  constexpr int NL = Q::kLits > 0 ? Q::kLits : 1;
   __shared__ int64_t s_acc[4][K][NA];
   __shared__ double s_lo[4][K][NA];
   __shared__ uint32_t s_fl[4][K];
@@ -1187,28 +1216,50 @@ __device__ __forceinline__ void aggregate_multi(const AggJitArgs &p) {
       const uint32_t j = (uint32_t)(s * 256 + tid);
       Q::load(in_cols, j, j < lim, x[s]);
     }
+    // contributions that use no literal: once per row for every request,
+    // with the row's error (a division by zero) kept apart for the requests
+    // that select the row
+    int64_t c0[S][NA];
+    uint32_t cm0[S];
+    bool e0[S];
+    if constexpr (!Q::kContribLits) {
+#pragma unroll
+      for (int s = 0; s < S; s++) {
+        const uint32_t j = (uint32_t)(s * 256 + tid);
+        cm0[s] = 0;
+        e0[s] = false;
+        Q::contrib(p.mlits, x[s], j, j < lim, c0[s], cm0[s], e0[s]);
+      }
+    }
+    // every request slot runs, straight-line: the host fills the slots past
+    // the run's requests with the last request's literals, and those
+    // results are never read
 #pragma unroll
     for (int r = 0; r < K; r++) {
-      if ((uint32_t)r < nreq) {  // uniform
-        const uint64_t *L = p.mlits + 1 + (uint64_t)r * p.mnlit;
-        bool err = false;
+      uint64_t L[NL];
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-          const uint32_t j = (uint32_t)(s * 256 + tid);
-          const bool in = j < lim;
-          const bool sel = in && Q::pred(L, x[s], j, in, err);
-          int64_t c[NA];
-          uint32_t cm = 0;
+      for (int i = 0; i < Q::kLits; i++) L[i] = p.mlits[1 + (uint64_t)r * p.mnlit + i];
+      bool err = false;
+#pragma unroll
+      for (int s = 0; s < S; s++) {
+        const uint32_t j = (uint32_t)(s * 256 + tid);
+        const bool in = j < lim;
+        const bool sel = in && Q::pred(L, x[s], j, in, err);
+        int64_t c[NA];
+        uint32_t cm = 0;
+        if constexpr (Q::kContribLits) {
           Q::contrib(L, x[s], j, sel, c, cm, err);
+        } else {
 #pragma unroll
-          for (int a = 0; a < NA; a++) {
-            if (!((cm >> a) & 1u)) continue;
-            multi_combine(Q::kKind[a], acc[r][a], lo[r][a], c[a], 0.0);
-            fl[r] |= 1u << a;
-          }
+          for (int a = 0; a < NA; a++) c[a] = c0[s][a];
+          cm = sel ? cm0[s] : 0u;
+          err |= sel && e0[s];
         }
-        if (err) fl[r] |= kMultiErr;
+#pragma unroll
+        for (int a = 0; a < NA; a++) multi_row(Q::kKind[a], acc[r][a], lo[r][a], (cm >> a) & 1u, c[a]);
+        fl[r] |= cm;
       }
+      if (err) fl[r] |= kMultiErr;
     }
   }
   // the block's partial of each request: waves by a fixed tree, then the

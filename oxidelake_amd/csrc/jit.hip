@@ -572,6 +572,10 @@ int generate_aggregate(const std::vector<DevCol> &cols, const AggJitShape &sh, J
   src += std::string("  static constexpr bool kAnyNullable = ") + (any_null ? "true" : "false") + ";\n";
   src += std::string("  static constexpr bool kExactF64 = ") + (tune_get("agg.exact_f64", 0) > 0 ? "true" : "false") +
          ";\n";
+  // prepared mode: the literal count, and whether an aggregate input reads one
+  src += "  static constexpr int kLits = " + std::to_string(ja->lits.size()) + ";\n";
+  src += std::string("  static constexpr bool kContribLits = ") +
+         (contrib.find("L[") != std::string::npos ? "true" : "false") + ";\n";
   src += "  struct In {\n" + in_fields + "  };\n";
   src += "  __device__ static inline In ins(const AggJitArgs &p, uint64_t base) {\n    In i;\n" + in_init +
          "    return i;\n  }\n";

@@ -1206,29 +1206,40 @@ __device__ __forceinline__ void aggregate_multi(const AggJitArgs &p) {
     }
   }
   const uint64_t ntiles = (p.n + kTile - 1) / kTile;
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // the rows of the block's next tile load while this one's are evaluated
+  auto load_tile = [&](uint64_t tile, typename Q::Row (&x)[S]) {
     const uint64_t base = tile * (uint64_t)kTile;
     const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
     const typename Q::In in_cols = Q::ins(p, base);
-    typename Q::Row x[S];
 #pragma unroll
     for (int s = 0; s < S; s++) {
       const uint32_t j = (uint32_t)(s * 256 + tid);
       Q::load(in_cols, j, j < lim, x[s]);
     }
+  };
+  typename Q::Row x[S], xn[S];
+  if (blockIdx.x < ntiles) load_tile(blockIdx.x, x);
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = tile * (uint64_t)kTile;
+    const uint32_t lim = (uint32_t)(p.n - base < (uint64_t)kTile ? p.n - base : kTile);
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x, xn);
     // contributions that use no literal: once per row for every request,
     // with the row's error (a division by zero) kept apart for the requests
     // that select the row
+    // (v0: input a of the row is not NULL, as lane masks the requests'
+    // selections AND with)
     int64_t c0[S][NA];
-    uint32_t cm0[S];
+    bool v0[S][NA];
     bool e0[S];
     if constexpr (!Q::kContribLits) {
 #pragma unroll
       for (int s = 0; s < S; s++) {
         const uint32_t j = (uint32_t)(s * 256 + tid);
-        cm0[s] = 0;
+        uint32_t cm0 = 0;
         e0[s] = false;
-        Q::contrib(p.mlits, x[s], j, j < lim, c0[s], cm0[s], e0[s]);
+        Q::contrib(p.mlits, x[s], j, j < lim, c0[s], cm0, e0[s]);
+#pragma unroll
+        for (int a = 0; a < NA; a++) v0[s][a] = (cm0 >> a) & 1u;
       }
     }
     // every request slot runs, straight-line: the host fills the slots past
@@ -1240,27 +1251,40 @@ __device__ __forceinline__ void aggregate_multi(const AggJitArgs &p) {
 #pragma unroll
       for (int i = 0; i < Q::kLits; i++) L[i] = p.mlits[1 + (uint64_t)r * p.mnlit + i];
       bool err = false;
+      bool took[NA];
+#pragma unroll
+      for (int a = 0; a < NA; a++) took[a] = false;
 #pragma unroll
       for (int s = 0; s < S; s++) {
         const uint32_t j = (uint32_t)(s * 256 + tid);
         const bool in = j < lim;
         const bool sel = in && Q::pred(L, x[s], j, in, err);
-        int64_t c[NA];
-        uint32_t cm = 0;
         if constexpr (Q::kContribLits) {
+          int64_t c[NA];
+          uint32_t cm = 0;
           Q::contrib(L, x[s], j, sel, c, cm, err);
+#pragma unroll
+          for (int a = 0; a < NA; a++) {
+            const bool use = (cm >> a) & 1u;
+            multi_row(Q::kKind[a], acc[r][a], lo[r][a], use, c[a]);
+            took[a] |= use;
+          }
         } else {
 #pragma unroll
-          for (int a = 0; a < NA; a++) c[a] = c0[s][a];
-          cm = sel ? cm0[s] : 0u;
+          for (int a = 0; a < NA; a++) {
+            const bool use = sel && v0[s][a];
+            multi_row(Q::kKind[a], acc[r][a], lo[r][a], use, c0[s][a]);
+            took[a] |= use;
+          }
           err |= sel && e0[s];
         }
-#pragma unroll
-        for (int a = 0; a < NA; a++) multi_row(Q::kKind[a], acc[r][a], lo[r][a], (cm >> a) & 1u, c[a]);
-        fl[r] |= cm;
       }
+#pragma unroll
+      for (int a = 0; a < NA; a++) fl[r] |= took[a] ? 1u << a : 0u;
       if (err) fl[r] |= kMultiErr;
     }
+#pragma unroll
+    for (int s = 0; s < S; s++) x[s] = xn[s];
   }
   // the block's partial of each request: waves by a fixed tree, then the
   // four waves in order

@@ -676,6 +676,114 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(const ScanArgs a) {
 constexpr int kGemmRows = 256, kGemmQueries = 256, kGemmThreads = 512;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Row factors of block `blk` into LDS (threads 0..255, one row each):
+// scores are (acc * qscale) * mul - bias; bias +inf drops the row (NULL,
+// zero norm, past the tile); blo is the bias lowered by 4e-6 of its
+// magnitude (the append pre-test of gemm_epilogue).
+template <int P>
+__device__ __forceinline__ void gemm_fill_rows(const ScanArgs &a, uint64_t blk, int tid, uint64_t tile_end,
+                                               float *s_mul, float *s_bias, float *s_blo) {
+  if (tid >= kGemmRows) return;
+  const uint64_t row = a.tile0 + blk * (uint64_t)kGemmRows + (uint64_t)tid;
+  float mul = 0.f, bias = INFINITY;
+  if (row < tile_end && valid_row(a.valid, row)) {
+    const float ri = a.rowinfo[row];
+    const float scl = P == P_FP8 ? a.rscale[row] : 1.0f;
+    if (a.metric != OXR_COSINE) {
+      mul = 2.0f * scl;
+      bias = ri;
+    } else if (ri > 0.f) {
+      mul = scl * ri;
+      bias = 0.f;
+    }
+  }
+  s_mul[tid] = mul;
+  s_bias[tid] = bias;
+  s_blo[tid] = bias == INFINITY ? INFINITY : bias - 4e-6f * fabsf(bias);
+}
+
+// Per query tile t of a wave (queries wq): the query's scale and, in append
+// mode, its bound as a float. Fixed for the launch, so loaded once.
+template <int P, int NT>
+__device__ __forceinline__ void gemm_query_factors(const ScanArgs &a, int wq, int i, float (&qs)[NT],
+                                                   float (&tau)[NT]) {
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int q = a.qoff + wq * NT * 32 + t * 32 + i;
+    qs[t] = P == P_FP8 && q < a.nq ? a.qscale[q] : 1.0f;
+    tau[t] = a.list && q < a.nq ? from_ordered((uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32))
+                                : -INFINITY;
+  }
+}
+
+// The GEMM tile's epilogue for one wave (rows wr, queries wq of the block
+// at `blockrow`): scores into a.scores, or appends past each query's bound.
+template <int P, int RT, int NT>
+__device__ __forceinline__ void gemm_epilogue(const ScanArgs &a, uint64_t blockrow, int wr, int wq, int i, int h,
+                                              const f32x16 (&acc)[RT][NT], const float (&qsv)[NT],
+                                              const float (&tauv)[NT], const float *s_mul, const float *s_bias,
+                                              const float *s_blo) {
+  const uint64_t tile_end = min(a.n, a.tile0 + a.tile_rows);
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int q = a.qoff + wq * NT * 32 + t * 32 + i;
+    if (q >= a.nq) continue;
+    const float qs = qsv[t];
+    const float tau = tauv[t];
+    // append mode: a score sc = (acc * qs) * mul - bias is kept when
+    // sc >= tau; the pre-test acc * (qs * mul) >= tau_lo + blo, with
+    // tau and the bias lowered by 4e-6 of their magnitudes, passes
+    // every kept score (the two products differ by a few ulps of a
+    // value that, near the bound, is at most |tau| + |bias|), so the
+    // exact score is computed only for the few that pass it. A NaN
+    // bound (a pilot with fewer than C rows) keeps every score.
+    const float tau_lo = tau != tau ? -INFINITY : tau - 4e-6f * fabsf(tau);
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+#pragma unroll
+      for (int jg = 0; jg < 4; jg++) {
+        const int m0 = 8 * jg + 4 * h;
+        const int br = wr * RT * 32 + r * 32 + m0;
+        const float4 mul4 = *(const float4 *)&s_mul[br];
+        const float4 bias4 = *(const float4 *)&s_bias[br];
+        const float mulv[4] = {mul4.x, mul4.y, mul4.z, mul4.w};
+        const float biasv[4] = {bias4.x, bias4.y, bias4.z, bias4.w};
+        const uint64_t row0 = blockrow + br;
+        if (a.list) {
+          const float4 blo4 = *(const float4 *)&s_blo[br];
+          const float blov[4] = {blo4.x, blo4.y, blo4.z, blo4.w};
+#pragma unroll
+          for (int e4 = 0; e4 < 4; e4++) {
+            const float av = acc[r][t][jg * 4 + e4];
+            if (!(av * (qs * mulv[e4]) >= tau_lo + blov[e4])) continue;
+            const float sv = (av * qs) * mulv[e4] - biasv[e4];
+            const float sce = biasv[e4] == INFINITY ? -INFINITY : sv;
+            if (!(sce > -INFINITY) || sce < tau) continue;
+            const uint32_t pos = atomicAdd(&a.cnt[q], 1u);
+            if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sce, row0 + e4);
+          }
+        } else {
+          float sc[4];
+#pragma unroll
+          for (int e4 = 0; e4 < 4; e4++) {
+            const float sv = (acc[r][t][jg * 4 + e4] * qs) * mulv[e4] - biasv[e4];
+            sc[e4] = biasv[e4] == INFINITY ? -INFINITY : sv;
+          }
+          float *out = a.scores + (uint64_t)q * a.tile_rows + (row0 - a.tile0);
+          if (row0 + 4 <= tile_end) {
+            *(float4 *)out = make_float4(sc[0], sc[1], sc[2], sc[3]);
+          } else {
+#pragma unroll
+            for (int e4 = 0; e4 < 4; e4++)
+              if (row0 + e4 < tile_end) out[e4] = sc[e4];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+}
+
 template <int P, int WR, int RT, int NT>
 __global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void gemm_scan_kernel(const ScanArgs a) {
@@ -723,30 +831,11 @@ void gemm_scan_kernel(const ScanArgs a) {
                                        (void *)(sq + r0 * 8), 16, 0, 0);
     }
   };
-  auto fill_rows = [&](uint64_t blk, int par) {
-    if (tid >= kGemmRows) return;
-    const uint64_t blockrow = a.tile0 + blk * (uint64_t)kGemmRows;
-    const uint64_t row = blockrow + (uint64_t)tid;
-    float mul = 0.f, bias = INFINITY;  // NULL, zero norm, past the tile
-    if (row < tile_end && valid_row(a.valid, row)) {
-      const float ri = a.rowinfo[row];
-      const float scl = P == P_FP8 ? a.rscale[row] : 1.0f;
-      if (a.metric != OXR_COSINE) {
-        mul = 2.0f * scl;
-        bias = ri;
-      } else if (ri > 0.f) {
-        mul = scl * ri;
-        bias = 0.f;
-      }
-    }
-    s_mul[par][tid] = mul;
-    s_bias[par][tid] = bias;
-    s_blo[par][tid] = bias == INFINITY ? INFINITY : bias - 4e-6f * fabsf(bias);
-  };
-
+  float qsv[NT], tauv[NT];
+  gemm_query_factors<P, NT>(a, wq, i, qsv, tauv);
   uint64_t blk = blockIdx.x;
   if (blk < a.nblocks) {
-    fill_rows(blk, 0);
+    gemm_fill_rows<P>(a, blk, tid, tile_end, s_mul[0], s_bias[0], s_blo[0]);
     dma_stage(blk, 0, s_a0, s_q0);
   }
   __syncthreads();
@@ -792,72 +881,9 @@ void gemm_scan_kernel(const ScanArgs a) {
       compute(s_a1, s_q1);
       __syncthreads();
     }
-    {
-      const uint64_t nblk = blk + gridDim.x;
-      const bool has_next = nblk < a.nblocks;
-        // epilogue (row factors of this block: s_mul[par], filled at least
-        // one barrier ago)
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const int q = a.qoff + wq * NT * 32 + t * 32 + i;
-          if (q >= a.nq) continue;
-          const float qs = P == P_FP8 ? a.qscale[q] : 1.0f;
-          const float tau =
-              a.list ? from_ordered((uint32_t)(a.tau_keys[(uint64_t)q * a.C + a.C - 1] >> 32)) : -INFINITY;
-          // append mode: a score sc = (acc * qs) * mul - bias is kept when
-          // sc >= tau; the pre-test acc * (qs * mul) >= tau_lo + blo, with
-          // tau and the bias lowered by 4e-6 of their magnitudes, passes
-          // every kept score (the two products differ by a few ulps of a
-          // value that, near the bound, is at most |tau| + |bias|), so the
-          // exact score is computed only for the few that pass it. A NaN
-          // bound (a pilot with fewer than C rows) keeps every score.
-          const float tau_lo = tau != tau ? -INFINITY : tau - 4e-6f * fabsf(tau);
-#pragma unroll
-          for (int r = 0; r < RT; r++) {
-#pragma unroll
-            for (int jg = 0; jg < 4; jg++) {
-              const int m0 = 8 * jg + 4 * h;
-              const int br = wr * RT * 32 + r * 32 + m0;
-              const float4 mul4 = *(const float4 *)&s_mul[par][br];
-              const float4 bias4 = *(const float4 *)&s_bias[par][br];
-              const float mulv[4] = {mul4.x, mul4.y, mul4.z, mul4.w};
-              const float biasv[4] = {bias4.x, bias4.y, bias4.z, bias4.w};
-              const uint64_t row0 = blockrow + br;
-              if (a.list) {
-                const float4 blo4 = *(const float4 *)&s_blo[par][br];
-                const float blov[4] = {blo4.x, blo4.y, blo4.z, blo4.w};
-#pragma unroll
-                for (int e4 = 0; e4 < 4; e4++) {
-                  const float av = acc[r][t][jg * 4 + e4];
-                  if (!(av * (qs * mulv[e4]) >= tau_lo + blov[e4])) continue;
-                  const float sv = (av * qs) * mulv[e4] - biasv[e4];
-                  const float sce = biasv[e4] == INFINITY ? -INFINITY : sv;
-                  if (!(sce > -INFINITY) || sce < tau) continue;
-                  const uint32_t pos = atomicAdd(&a.cnt[q], 1u);
-                  if (pos < a.cap) a.list[(uint64_t)q * a.cap + pos] = score_key(sce, row0 + e4);
-                }
-              } else {
-                float sc[4];
-#pragma unroll
-                for (int e4 = 0; e4 < 4; e4++) {
-                  const float sv = (acc[r][t][jg * 4 + e4] * qs) * mulv[e4] - biasv[e4];
-                  sc[e4] = biasv[e4] == INFINITY ? -INFINITY : sv;
-                }
-                float *out = a.scores + (uint64_t)q * a.tile_rows + (row0 - a.tile0);
-                if (row0 + 4 <= tile_end) {
-                  *(float4 *)out = make_float4(sc[0], sc[1], sc[2], sc[3]);
-                } else {
-#pragma unroll
-                  for (int e4 = 0; e4 < 4; e4++)
-                    if (row0 + e4 < tile_end) out[e4] = sc[e4];
-                }
-              }
-              __builtin_amdgcn_sched_barrier(0);
-            }
-          }
-        }
-        if (has_next) fill_rows(nblk, par ^ 1);
-    }
+    gemm_epilogue<P, RT, NT>(a, blockrow, wr, wq, i, h, acc, qsv, tauv, s_mul[par], s_bias[par], s_blo[par]);
+    const uint64_t nblk = blk + gridDim.x;
+    if (nblk < a.nblocks) gemm_fill_rows<P>(a, nblk, tid, tile_end, s_mul[par ^ 1], s_bias[par ^ 1], s_blo[par ^ 1]);
   }
 }
 

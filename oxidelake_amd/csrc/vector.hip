@@ -892,18 +892,32 @@ void gemm_scan_kernel(const ScanArgs a) {
 constexpr int kBuf = 4096;   // LDS candidate buffer (keys)
 constexpr int kChunk = 2048; // keys read per round (8 per thread)
 
-// Bitonic sort of s[0..kBuf) in descending order by the whole block.
+// Bitonic sort of s[0..N) in descending order by the whole block. Each
+// stage's compare-exchanges of a thread (N / 2 / kBlock pairs) read all
+// their pairs first, then write: one LDS round trip per stage instead of
+// one per pair (the network, and so the result, is the plain bitonic one).
+template <int N = kBuf>
 __device__ void sort_desc(uint64_t *s) {
-  for (int size = 2; size <= kBuf; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < kBuf / 2; t += kBlock) {
-        const int lo = 2 * stride * (t / stride) + (t % stride);
-        const int hi = lo + stride;
-        const bool desc = (lo & size) == 0;
-        const uint64_t x = s[lo], y = s[hi];
-        if ((x < y) == desc) {
-          s[lo] = y;
-          s[hi] = x;
+  constexpr int P = N / 2 / kBlock;
+  static_assert(P >= 1 && P * 2 * kBlock == N, "sort size");
+  for (int ls1 = 1; (1 << ls1) <= N; ls1++) {  // size = 2^ls1
+    for (int ls = ls1 - 1; ls >= 0; ls--) {      // stride = 2^ls
+      const int stride = 1 << ls;
+      int lo[P];
+      uint64_t x[P], y[P];
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        const int t = threadIdx.x + p * kBlock;
+        lo[p] = ((t >> ls) << (ls + 1)) | (t & (stride - 1));
+        x[p] = s[lo[p]];
+        y[p] = s[lo[p] + stride];
+      }
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        const bool desc = ((lo[p] >> ls1) & 1) == 0;
+        if ((x[p] < y[p]) == desc) {
+          s[lo[p]] = y[p];
+          s[lo[p] + stride] = x[p];
         }
       }
       __syncthreads();
@@ -917,7 +931,10 @@ __device__ void compact(uint64_t *buf, int *count, uint64_t *tau, int C) {
   for (int t = threadIdx.x; t < kBuf; t += kBlock)
     if (t >= n) buf[t] = 0;
   __syncthreads();
-  sort_desc(buf);
+  if (n <= kBuf / 2)
+    sort_desc<kBuf / 2>(buf);  // the zeros after the keys stay last
+  else
+    sort_desc(buf);
   if (threadIdx.x == 0) {
     const int keep = n < C ? n : C;
     *count = keep;
@@ -1005,7 +1022,7 @@ __device__ void top_c_sampled(Fetch fetch, Conv conv, uint64_t len, int C, uint6
     buf[t] = key;
   }
   __syncthreads();
-  sort_desc(buf);
+  sort_desc<kSample>(buf);  // the samples; the zeros after them stay last
   // about 2 C keys pass: a sample rank r keeps r len / 2048 +- sqrt(r) len /
   // 2048 keys, which stays within [C, kBuf] for C up to 1024 and every
   // segment length the host makes (at least 32 C rows)
@@ -1021,13 +1038,24 @@ __device__ void top_c_sampled(Fetch fetch, Conv conv, uint64_t len, int C, uint6
     constexpr int KPT = kChunk / kBlock;
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1;
+    // the next chunk's values load while this chunk's keys are placed
+    R raw[KPT];
+    auto load = [&](uint64_t base) {
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
+        if (idx < len) raw[j] = fetch(idx);
+      }
+    };
+    load(0);
     for (uint64_t base = 0; base < len; base += kChunk) {
       uint64_t keys[KPT];
 #pragma unroll
       for (int j = 0; j < KPT; j++) {
         const uint64_t idx = base + (uint64_t)(j * kBlock + threadIdx.x);
-        keys[j] = idx < len ? conv(fetch(idx), idx) : 0;
+        keys[j] = idx < len ? conv(raw[j], idx) : 0;
       }
+      if (base + kChunk < len) load(base + kChunk);
 #pragma unroll
       for (int j = 0; j < KPT; j++) {
         const bool pass = keys[j] > bound;

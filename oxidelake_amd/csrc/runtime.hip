@@ -317,7 +317,7 @@ const KnobSpec kKnobs[] = {
     {"vector.wide", 0, 3, 0},
     {"vector.nt8", 0, 1, kBool},
     {"vector.pilot", 0, 1, kBool},
-    {"vector.select_sampled", 0, 1, kBool},   // selection: sampled bound, then one sort (default 1)
+    {"vector.select_sampled", 0, 4096, 0},    // selection: sampled bound then one sort for C at least this (default 128; 0 off)
     {"vector.pilot_rows", 1, (int64_t)1 << 40, 0},
     {"vector.pilot_cap", 1, (int64_t)1 << 32, 0},
 };

@@ -1088,9 +1088,9 @@ struct SelectArgs {
   uint64_t tile0, tile_len, tile_rows, seg_rows;
   int32_t nq, C, seg_first, segs_total;
   uint64_t *cand;       // [nq][segs_total][C]
-  int32_t sampled;      // top_c_sampled (knob vector.select_sampled), else top_c_ahead
+  int32_t sampled;      // top_c_sampled for C at least this (knob vector.select_sampled), else top_c_ahead
 };
-__device__ inline bool tune_get_select_sampled(const SelectArgs &a) { return a.sampled != 0; }
+__device__ inline bool tune_get_select_sampled(const SelectArgs &a) { return a.sampled > 0; }
 
 
 
@@ -1107,7 +1107,7 @@ __global__ __launch_bounds__(kBlock) void select_tile_kernel(const SelectArgs a)
   const uint64_t row0 = a.tile0 + lo;
   auto fetch = [&](uint64_t idx) -> float { return sp[idx]; };
   auto conv = [&](float s, uint64_t idx) -> uint64_t { return s > -INFINITY ? score_key(s, row0 + idx) : 0; };
-  if (tune_get_select_sampled(a) && a.C >= 128)  // few candidates: the streaming buffer rarely sorts
+  if (tune_get_select_sampled(a) && a.C >= a.sampled)  // few candidates: the streaming buffer rarely sorts
     top_c_sampled<float>(fetch, conv, len, a.C, s_buf, &s_count, &s_tau);
   else
     top_c_ahead<float>(fetch, conv, len, a.C, s_buf, &s_count, &s_tau);
@@ -1482,7 +1482,7 @@ int search(oxr_stream *s, const oxr_vector_index *ix, const float *queries, int3
           OXR_HIP(hipGetLastError());
         }
         SelectArgs sa{scores, t0, tlen, tile, seg_rows, nqc, C, (int)(ti * segs_per_tile), (int)segs_total, cand,
-                      (int32_t)(tune_get("vector.select_sampled", 1) > 0)};
+                      (int32_t)tune_get("vector.select_sampled", 128)};
         const uint64_t segs_here = (tlen + seg_rows - 1) / seg_rows;
         {
           ProfScope prof(st, "vector_select");

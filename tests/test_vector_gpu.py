@@ -335,8 +335,9 @@ def test_gemm_scan_matches_streamed(backend, stream, precision, metric):
 @pytest.mark.parametrize("precision", [VEC_FP32, VEC_BF16, VEC_FP8])
 @pytest.mark.parametrize("table", ["mixture", "sparse", "ties"])
 def test_sampled_selection_matches_streaming(backend, stream, precision, table):
-    """Selection by a sampled bound then one sort (vector.select_sampled = 1)
-    against the streaming selection, bit for bit, on the plain scan and the
+    """Selection by a sampled bound then one sort (vector.select_sampled = 1:
+    for every candidate count) against the streaming selection, bit for bit,
+    on the plain scan and the
     pilot path: a mixture table; one with 97% NULL rows (the sample finds no
     bound: the streaming fallback); one of identical rows (every score ties,
     so the bound keeps more keys than the buffer holds: the fallback)."""

@@ -110,8 +110,12 @@ def sweep(kernel, knobs: Sequence[str], candidates: Iterable[Sequence[int]],
 
 
 def tune_filter(run: Callable[[], object], reps: int = 3) -> dict:
-    """Tiles of the generated filter kernel (the path queries take)."""
-    return sweep("filter_project", ["filter.rows"], [(r,) for r in FILTER_ROWS], run, reps)
+    """Tiles of the generated filter kernel (the path queries take), with
+    plain or nontemporal projection stores (on the 200M-row query the
+    nontemporal ones measured ~1.5% faster on one box, within the spread of
+    another: the sweep keeps whichever is faster here)."""
+    return sweep("filter_project", ["filter.rows", "jit.stream_stores"],
+                 [(r, st) for st in (0, 1) for r in FILTER_ROWS], run, reps)
 
 
 def tune_filter_interpreter(run: Callable[[], object], reps: int = 3) -> dict:

@@ -281,6 +281,34 @@ def test_every_rows_per_lane_is_bit_exact(backend, stream, oracle, rows, persist
         autotune.set_knob("filter.persistent", -1)
 
 
+@pytest.mark.parametrize("loads,stores", [(0, 0), (1, 1), (0, 1)])
+@pytest.mark.parametrize("rows", [16, 24])
+def test_nontemporal_loads_and_stores_are_bit_exact(backend, stream, oracle, rows, loads, stores):
+    # knobs jit.stream_loads / jit.stream_stores (the autotuner may pick
+    # nontemporal projection stores): the same bits either way
+    autotune.set_knob("filter.rows", rows)
+    autotune.set_knob("jit.stream_loads", loads)
+    autotune.set_knob("jit.stream_stores", stores)
+    try:
+        rng = np.random.default_rng(90 + rows)
+        for n in (1, 1000, 20_001):
+            batch = datagen.random_batch(rng, n)
+            for pred, proj in [(PREDICATES[10], PROJECTIONS[2]), (PREDICATES[12], PROJECTIONS[3])]:
+                got, want = run_both(backend, stream, oracle, batch, pred, proj)
+                assert_batches_equal(got, want, names=False)
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=7), 300_001)
+        host = backend.download(stream, dev)
+        for t in (5, 50, 470):
+            pred, proj = datagen.filter_query(t)
+            outs = backend.filter_project_batches(stream, dev, 65_536, pred, proj)
+            for i, o in enumerate(outs):
+                want = oracle.filter_project(host.slice(i * 65_536, 65_536), pred, proj)
+                assert_batches_equal(backend.download(stream, o), want, names=False)
+    finally:
+        for k in ("filter.rows", "jit.stream_loads", "jit.stream_stores"):
+            autotune.set_knob(k, -1)
+
+
 @pytest.mark.parametrize("waves,rows", [(8, 16), (12, 16), (16, 8), (8, 24), (12, 3), (16, 1)])
 def test_every_waves_per_workgroup_is_bit_exact(backend, stream, oracle, waves, rows):
     # tiles of 64 * W * S rows; S * W above 64 (step, wave) counts scans two

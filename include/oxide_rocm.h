@@ -174,7 +174,10 @@ int oxr_upload(oxr_stream *s, const struct ArrowSchema *schema,
 int oxr_download(oxr_stream *s, const oxr_batch *batch,
                  struct ArrowSchema *out_schema, struct ArrowArray *out_array);
 
-/* A batch over device memory the caller owns (no copy). `names` may be NULL. */
+/* A batch over device memory the caller owns (no copy). `names` may be NULL.
+ * As for every batch, the memory must not change while the batch lives:
+ * statistics measured on a column (an aggregation key's value range) are
+ * kept with the batch. */
 int oxr_batch_wrap(oxr_device *dev, const oxr_column *columns,
                    const char *const *names, int32_t ncols, uint64_t rows,
                    oxr_batch **out);

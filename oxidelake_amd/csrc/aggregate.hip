@@ -613,6 +613,28 @@ int key_range(oxr_stream *s, const DevCol &c, uint64_t n, KeyRange *out) {
   return OXR_OK;
 }
 
+// key_range of column `col` of batch `b`, measured on the first call and
+// kept with the batch (knob agg.range_cache: 0 measures every time).
+int key_range_of(oxr_stream *s, const oxr_batch *b, int col, KeyRange *out) {
+  const bool cache = tune_get("agg.range_cache", 1) > 0;
+  if (cache) {
+    std::lock_guard<std::mutex> lock(b->stats_mu);
+    for (const oxr_batch::KeyStats &k : b->key_stats)
+      if (k.col == col) {
+        out->known = true;
+        out->min = k.min;
+        out->count = k.count;
+        return OXR_OK;
+      }
+  }
+  OXR_TRY(key_range(s, b->cols[col], b->rows, out));
+  if (cache) {
+    std::lock_guard<std::mutex> lock(b->stats_mu);
+    b->key_stats.push_back({col, out->min, out->count});
+  }
+  return OXR_OK;
+}
+
 // Grows the table so that `more` further distinct keys keep the load <= 1/2.
 int ensure_capacity(oxr_stream *s, oxr_agg_state *st, uint64_t more) {
   uint64_t want = next_pow2(2 * (st->groups + more));
@@ -1261,8 +1283,8 @@ int run_update(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, const Join
   // could overflow the table part way through an update)
   KeyRange kr;
   if (st->key_col != -1 && n > (uint64_t)std::max<int64_t>(0, tune_get("agg.range_rows", kRangeCheckRows))) {
-    if (js) OXR_TRY(key_range(s, js->build->cols[js->build_group], js->build->rows, &kr));
-    else OXR_TRY(key_range(s, in->cols[st->key_col], n, &kr));
+    if (js) OXR_TRY(key_range_of(s, js->build, js->build_group, &kr));
+    else OXR_TRY(key_range_of(s, in, st->key_col, &kr));
     more = std::min<uint64_t>(more, std::max<uint64_t>(kr.count, 1));
   }
   OXR_TRY(ensure_capacity(s, st, more));

@@ -173,6 +173,16 @@ struct oxr_batch {
   // buffers this batch's columns point into but does not own (views of
   // another result, or of an input whose buffers it shares)
   std::vector<std::shared_ptr<void>> holders;
+  // Column statistics measured once and kept with the batch (a batch's
+  // memory does not change while it lives): the value range of a key
+  // column, for the aggregation's table sizing and kernel choice.
+  struct KeyStats {
+    int32_t col = -1;
+    int64_t min = 0;
+    uint64_t count = 0;
+  };
+  mutable std::mutex stats_mu;
+  mutable std::vector<KeyStats> key_stats;
 };
 
 namespace oxr {

@@ -300,6 +300,7 @@ const KnobSpec kKnobs[] = {
     {"agg.exact_f64", 0, 1, kBool},
     {"agg.dense", 0, 1, kBool},
     {"agg.range_rows", 0, (int64_t)1 << 48, 0},
+    {"agg.range_cache", 0, 1, kBool},          // key ranges kept with the batch (default 1)
     {"agg.spill", 0, 1, kBool},
     {"agg.spill_rows", 0, (int64_t)1 << 48, 0},
     {"agg.spill_pct", 1, 400, 0},

@@ -289,6 +289,41 @@ def test_keyless_division_by_zero_fails(backend, stream, oracle, path):
     assert_agg_equal(device_aggregate(backend, stream, b, ok), oracle.aggregate(b, ok), ok)
 
 
+def test_key_range_is_measured_once_per_batch(backend, stream, oracle, path):
+    """The key column's value range (table sizing, kernel choice) is kept
+    with the batch: measured by the first update over a batch, again for a
+    new batch, every time with knob agg.range_cache 0; the same results."""
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    autotune.set_knob("agg.range_rows", 0)  # measure at any size (reset by the fixture)
+    rng = np.random.default_rng(44)
+    n = 120_000
+    b = pa.record_batch({"k": pa.array(rng.integers(-20, 30, n).astype(np.int64), mask=rng.random(n) < 0.05),
+                         "v": pa.array(rng.normal(size=n))})
+    spec = AggregateSpec(0, [Aggregate(F.Sum, col(1)), Aggregate(F.CountStar), Aggregate(F.Max, col(1))])
+    want = oracle.aggregate(b, spec, sum_mode=1)
+    dev = backend.upload(stream, b)
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        outs = [backend.download(stream, backend.aggregate(stream, dev, spec)) for _ in range(3)]
+        assert autotune.launches("key_range") == 1
+        other = backend.upload(stream, b)
+        outs.append(backend.download(stream, backend.aggregate(stream, other, spec)))
+        assert autotune.launches("key_range") == 2
+        autotune.set_knob("agg.range_cache", 0)
+        outs.append(backend.download(stream, backend.aggregate(stream, dev, spec)))
+        outs.append(backend.download(stream, backend.aggregate(stream, dev, spec)))
+        assert autotune.launches("key_range") == 4
+    finally:
+        autotune.set_knob("agg.range_cache", -1)
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
+    for o in outs:
+        assert_agg_equal(o, want, spec)
+
+
 def test_hot_keys_fold_across_the_wave(backend, stream, oracle):
     # one key on most rows (the matching round folds it per wave), NULL keys
     # and NULL inputs mixed in, a cold tail of distinct keys

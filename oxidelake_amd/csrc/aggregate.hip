@@ -963,6 +963,32 @@ int build_hot_map(oxr_stream *s, const oxr_agg_state *st, const oxr_batch *in, c
   const uint64_t samples = std::min<uint64_t>(n, 1ull << 20);
   const uint32_t nh = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(cap, srange), samples);
   const uint64_t stride = n / samples;  // (samples - 1) * stride < n
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  // The map depends only on the batch's key column (through the join), the
+  // key range and its size, not on the aggregation's filter or inputs: it is
+  // kept with the batch and every later update over it takes it (knob
+  // agg.hot_cache). Each update still gets its own partials and flags.
+  const bool cache = tune_get("agg.hot_cache", 1) > 0;
+  const uint64_t jser = js ? js->table->serial : 0;
+  const int32_t kcol = js ? js->probe_key : st->key_col, bgrp = js ? js->build_group : -1;
+  const size_t b_part = up((size_t)grid * na * nh * 8), b_flags = up((size_t)grid * nh);
+  if (cache) {
+    std::lock_guard<std::mutex> lock(in->stats_mu);
+    for (const oxr_batch::HotMap &h : in->hot_maps)
+      if (h.join_serial == jser && h.key_col == kcol && h.build_group == bgrp && h.smin == kr->min &&
+          h.srange == srange && h.nh == nh) {
+        // built on another stream, perhaps: its kernels come first
+        OXR_HIP(hipStreamWaitEvent(s->s, h.ready, 0));
+        OXR_TRY(stream_alloc(s, b_part + b_flags, mem));
+        A.hot_part = (int64_t *)*mem;
+        A.hot_flags = (uint8_t *)((char *)*mem + b_part);
+        A.hotmap = h.hotmap;
+        A.nhot = nh;
+        *hotkeys = h.hotkeys;
+        *nhot = nh;
+        return OXR_OK;
+      }
+  }
   // the sampled groups, sorted, run-length encoded into (group, count)
   // runs, the runs sorted by count (descending): no atomics
   size_t b_sort1 = 0, b_rle = 0, b_sort2 = 0;
@@ -977,20 +1003,27 @@ int build_hot_map(oxr_stream *s, const oxr_agg_state *st, const oxr_batch *in, c
                                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, ns, 0, 32,
                                                      s->s);
   if (e != hipSuccess) return fail_hip(e, "hot map sizes");
-  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-  const size_t b_s = up(samples * 4), b_part = up((size_t)grid * na * nh * 8), b_flags = up((size_t)grid * nh);
+  const size_t b_s = up(samples * 4);
   const size_t b_tmp = up(std::max(b_sort1, std::max(b_rle, b_sort2)));
-  const size_t total = 6 * b_s + 256 + up(srange * 4) + up((size_t)nh * 8) + b_part + b_flags + b_tmp;
-  OXR_TRY(stream_alloc(s, total, mem));
+  const size_t b_map = up(srange * 4) + up((size_t)nh * 8);  // the map and the hot keys
+  // the map in an allocation of its own when it is kept with the batch
+  void *pmem = nullptr;
+  if (cache) OXR_TRY(stream_alloc(s, b_map, &pmem));
+  const size_t total = 6 * b_s + 256 + (cache ? 0 : b_map) + b_part + b_flags + b_tmp;
+  int arc = stream_alloc(s, total, mem);
+  if (arc != OXR_OK) {
+    if (pmem) stream_release(s, pmem);
+    return arc;
+  }
   char *q = (char *)*mem;
   uint32_t *grp = (uint32_t *)q, *grp_sorted = (uint32_t *)(q + b_s), *runs = (uint32_t *)(q + 2 * b_s),
            *counts = (uint32_t *)(q + 3 * b_s), *cnt_sorted = (uint32_t *)(q + 4 * b_s),
            *run_sorted = (uint32_t *)(q + 5 * b_s), *nruns = (uint32_t *)(q + 6 * b_s);
   q += 6 * b_s + 256;
-  int32_t *hotmap = (int32_t *)q;
-  q += up(srange * 4);
-  *hotkeys = (int64_t *)q;
-  q += up((size_t)nh * 8);
+  char *mq = cache ? (char *)pmem : q;
+  if (!cache) q += b_map;
+  int32_t *hotmap = (int32_t *)mq;
+  *hotkeys = (int64_t *)(mq + up(srange * 4));
   A.hot_part = (int64_t *)q;
   q += b_part;
   A.hot_flags = (uint8_t *)q;
@@ -998,8 +1031,12 @@ int build_hot_map(oxr_stream *s, const oxr_agg_state *st, const oxr_batch *in, c
   void *tmp = q;
   // counts past the run count stay 0 (sorted last, never hot)
   int rc = fill_zero(s, counts, samples * 4);
-  if (rc != OXR_OK) return rc;
+  if (rc != OXR_OK) {
+    if (pmem) stream_release(s, pmem);
+    return rc;
+  }
   const unsigned tb = 256;
+  ProfScope prof(s->s, "aggregate_hot_map");  // a map built (not one taken from the batch)
   hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<uint64_t>((srange + tb - 1) / tb, 4096)), dim3(tb), 0,
                      s->s, hotmap, srange, -1);
   HotKeySrc src{};
@@ -1032,10 +1069,39 @@ int build_hot_map(oxr_stream *s, const oxr_agg_state *st, const oxr_batch *in, c
                        2u, kr->min, srange, hotmap, *hotkeys);
     e = hipGetLastError();
   }
-  if (e != hipSuccess) return fail_hip(e, "hot map");
+  if (e != hipSuccess) {
+    if (pmem) stream_release(s, pmem);
+    return fail_hip(e, "hot map");
+  }
   A.hotmap = hotmap;
   A.nhot = nh;
   *nhot = nh;
+  if (cache) {
+    hipEvent_t ready = nullptr;
+    e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ready, s->s);
+    if (e != hipSuccess) {
+      if (ready) (void)hipEventDestroy(ready);
+      // not kept: this update's map is released with its scratch
+      (void)hipGetLastError();
+      stream_release(s, pmem);
+      return fail_hip(e, "hot map event");
+    }
+    std::lock_guard<std::mutex> lock(in->stats_mu);
+    oxr_batch::HotMap h;
+    h.ready = ready;
+    h.join_serial = jser;
+    h.key_col = kcol;
+    h.build_group = bgrp;
+    h.smin = kr->min;
+    h.srange = srange;
+    h.nh = nh;
+    h.mem = pmem;
+    h.hotmap = hotmap;
+    h.hotkeys = *hotkeys;
+    h.core = retain(s->core);
+    in->hot_maps.push_back(h);
+  }
   return OXR_OK;
 }
 
@@ -1181,7 +1247,6 @@ int run_update_generated(oxr_stream *s, oxr_agg_state *st, const oxr_batch *in, 
   uint32_t nhot = 0;
   int rc = OXR_OK;
   if (nparts && hot_try) {
-    ProfScope prof(s->s, "aggregate_hot_map");
     rc = build_hot_map(s, st, in, js, kr, A, (uint32_t)grid, hot_cap, &hot_mem, &hotkeys, &nhot);
     if (rc == OXR_OK && nhot) fn = ja.hot_fn;
     if (rc == OXR_OK && nhot && js) {

@@ -183,6 +183,23 @@ struct oxr_batch {
   };
   mutable std::mutex stats_mu;
   mutable std::vector<KeyStats> key_stats;
+  // the aggregation's hot-group map of this batch's key column (through a
+  // join table when join_serial != 0): which groups the hot kernel keeps in
+  // LDS, from a sample of the key column; device memory from the arena,
+  // released on `core`'s stream with the batch
+  struct HotMap {
+    uint64_t join_serial = 0;
+    int32_t key_col = -1, build_group = -1;
+    int64_t smin = 0;
+    uint64_t srange = 0;
+    uint32_t nh = 0;
+    void *mem = nullptr;
+    int32_t *hotmap = nullptr;
+    int64_t *hotkeys = nullptr;
+    oxr::StreamCore *core = nullptr;
+    hipEvent_t ready = nullptr;  // recorded on core's stream after the map is built
+  };
+  mutable std::vector<HotMap> hot_maps;
 };
 
 namespace oxr {

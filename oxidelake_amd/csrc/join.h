@@ -18,6 +18,7 @@ struct oxr_join_table {
   uint64_t nrows = 0;        // build rows with a non-NULL key
   uint64_t build_rows = 0;   // rows of the build batch
   int32_t key_type = OXR_INT64;
+  uint64_t serial = 0;       // unique per table of the process (cache keys)
 };
 
 struct oxr_join_matches {

@@ -242,6 +242,8 @@ int oxr_join_build(oxr_stream *s, const oxr_batch *build, int32_t key, oxr_join_
   if (n >= (1ull << 29))
     return fail(OXR_EUNSUPPORTED, "rocm.join", "build side of 2^29 rows or more");
   std::unique_ptr<oxr_join_table> jt(new oxr_join_table());
+  static std::atomic<uint64_t> next_serial{0};
+  jt->serial = ++next_serial;
   jt->dev = s->dev;
   jt->owner = retain(s->core);
   jt->build_rows = n;

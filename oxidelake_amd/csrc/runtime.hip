@@ -301,6 +301,7 @@ const KnobSpec kKnobs[] = {
     {"agg.dense", 0, 1, kBool},
     {"agg.range_rows", 0, (int64_t)1 << 48, 0},
     {"agg.range_cache", 0, 1, kBool},          // key ranges kept with the batch (default 1)
+    {"agg.hot_cache", 0, 1, kBool},            // hot-group maps kept with the batch (default 1)
     {"agg.spill", 0, 1, kBool},
     {"agg.spill_rows", 0, (int64_t)1 << 48, 0},
     {"agg.spill_pct", 1, 400, 0},
@@ -1142,6 +1143,11 @@ void oxr_batch_free(oxr_batch *b) {
     if (!c.owned) continue;
     b->dev->arena->release(c.values, s);
     b->dev->arena->release(c.validity, s);
+  }
+  for (auto &h : b->hot_maps) {
+    b->dev->arena->release(h.mem, h.core->s);
+    if (h.ready) (void)hipEventDestroy(h.ready);
+    release(h.core);
   }
   release(b->owner);
   delete b;

@@ -125,6 +125,55 @@ def test_star_query_with_zipf_skew(backend, stream, oracle, path):
     assert counts.sum() == nf and counts.max() > nf // 20
 
 
+def test_hot_map_is_kept_with_the_probe_batch(backend, stream, oracle, path):
+    """The hot-group map of a probe batch through a join table is built by
+    the first update and taken by every later one, whatever its filter or
+    stream; another join table builds its own; knob agg.hot_cache 0 builds
+    one per update. Every result against the oracle."""
+    if path != "hot":
+        pytest.skip("hot-group kernel only")
+    from oxidelake_amd import autotune
+    from oxidelake_amd._capi import lib
+
+    # (groups past 4x the 1,536 hot slots: the hot kernel's condition)
+    nf, nd, groups = 1_000_000, 100_000, 10_000
+    fact_cols, dim_cols = datagen.star_columns(nf, nd, groups, seed=9)
+    dfact = datagen.table(backend, stream, fact_cols, nf)
+    ddim = datagen.table(backend, stream, dim_cols, nd)
+    fact, dim = backend.download(stream, dfact), backend.download(stream, ddim)
+    aggs = datagen.star_query_spec().aggregates
+    filt = plan.cmp(col(1), Cmp.Gt, Lit.int32(50))
+    table = backend.prepare_join_build(stream, ddim, 0)
+    other_stream = backend.create_stream()
+
+    def run(tbl, f, strm):
+        st = AggState(backend, strm, dfact, AggregateSpec(-1, aggs, filter=f))
+        st.update_join(strm, tbl, ddim, 1, dfact, 0)
+        return backend.download(strm, st.finish(strm))
+
+    lib.oxr_profile_enable(1)
+    lib.oxr_profile_reset()
+    try:
+        got = [(run(table, None, stream), None), (run(table, filt, stream), filt),
+               (run(table, None, other_stream), None)]
+        assert autotune.launches("aggregate_hot_map") == 1
+        assert autotune.launches("aggregate_hot") == 3
+        table2 = backend.prepare_join_build(stream, ddim, 0)
+        got.append((run(table2, filt, stream), filt))
+        assert autotune.launches("aggregate_hot_map") == 2
+        autotune.set_knob("agg.hot_cache", 0)
+        got.append((run(table, None, stream), None))
+        got.append((run(table, None, stream), None))
+        assert autotune.launches("aggregate_hot_map") == 4
+    finally:
+        autotune.set_knob("agg.hot_cache", -1)
+        lib.oxr_profile_enable(0)
+        lib.oxr_profile_reset()
+    for g, f in got:
+        want, spec = oracle_join_agg(oracle, fact, dim, aggs, f)
+        assert_agg_equal(g, want, spec)
+
+
 def test_plain_and_joined_updates_do_not_mix(backend, stream):
     rng = np.random.default_rng(4)
     fact, dim = tables(rng, 100, 10, 3, nulls=False)

@@ -25,7 +25,7 @@ AGG_GRIDS = [2, 4, 8, 16]
 # tile or the resident/streamed kernels, and how many first rows bound the
 # rest (a small pilot costs little selection but lets more keys through the
 # bound; a large one the reverse)
-VECTOR_LARGE = [(1, 1 << 19), (1, 1 << 20), (1, 1 << 21), (1, 1 << 22), (0, 1 << 21)]
+VECTOR_LARGE = [(1, 1 << 19), (1, 3 << 18), (1, 1 << 20), (1, 3 << 19), (1, 1 << 21), (1, 1 << 22), (0, 1 << 21)]
 # join + aggregate over a wide skewed key range: (agg.spill_pct, agg.hot_grid)
 # -- spill bucket size in percent of a block's rows, hot-kernel workgroups
 # per CU

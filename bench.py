@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--e2e", action="store_true",
                     help="filter: also time the plain drop-in path over host batches (PCIe included)")
     ap.add_argument("--filter-rows", type=int, default=0,
-                    help="rows per lane of the generated kernel (default: autotuned)")
+                    help="rows per lane of the generated tile kernel, the staged kernel off (default: autotuned)")
     ap.add_argument("--workload", choices=["filter", "vector", "aggregate", "join", "serving", "copy", "agg_batch"],
                     default="filter",
                     help="filter: the 200M-row filter+projection with every operator under `execs` "
@@ -319,8 +319,9 @@ def bench_filter(args, be, stream) -> dict:
 
     # launch shape picked by the project autotuner on this device and table
     if args.filter_rows:
+        autotune.set_knob("filter.staged", 0)
         autotune.set_knob("filter.rows", args.filter_rows)
-        tuned = {"knobs": {"filter.rows": args.filter_rows}, "ms": None}
+        tuned = {"knobs": {"filter.staged": 0, "filter.rows": args.filter_rows}, "ms": None}
     else:
         tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
     log(f"launch shape: {tuned['knobs']}")

@@ -20,6 +20,9 @@ CACHE = Path(os.environ.get("OXR_TUNE_CACHE", Path.home() / ".cache" / "oxidelak
 
 FILTER_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (4, 1)]  # interpreter kernel
 FILTER_ROWS = [8, 16, 24, 32]  # generated kernels: rows per lane
+# the staged filter kernel: (waves per workgroup, chunks of 4 rows per lane,
+# kept rows its LDS stage holds per tile)
+FILTER_STAGED = [(8, 8, 3072), (8, 6, 2560), (4, 8, 1536)]
 AGG_GRIDS = [2, 4, 8, 16]
 # vector search, large batches: (vector.gemm, vector.pilot_rows) -- the GEMM
 # tile or the resident/streamed kernels, and how many first rows bound the
@@ -110,12 +113,13 @@ def sweep(kernel, knobs: Sequence[str], candidates: Iterable[Sequence[int]],
 
 
 def tune_filter(run: Callable[[], object], reps: int = 3) -> dict:
-    """Tiles of the generated filter kernel (the path queries take), with
-    plain or nontemporal projection stores (on the 200M-row query the
-    nontemporal ones measured ~1.5% faster on one box, within the spread of
-    another: the sweep keeps whichever is faster here)."""
-    return sweep("filter_project", ["filter.rows", "jit.stream_stores"],
-                 [(r, st) for st in (0, 1) for r in FILTER_ROWS], run, reps)
+    """The generated filter's two kernels on the device and table at hand:
+    the tile kernel at each rows per lane (the staged kernel off), and the
+    staged kernel (kept rows parked in LDS) at each shape of FILTER_STAGED,
+    which the query takes while its kept fraction fits the stage."""
+    knobs = ["filter.staged", "filter.rows", "filter.staged_waves", "filter.chunks", "filter.stage_rows"]
+    cands = [(0, r, -1, -1, -1) for r in FILTER_ROWS] + [(1, 24, w, c, st) for w, c, st in FILTER_STAGED]
+    return sweep("filter_project", knobs, cands, run, reps)
 
 
 def tune_filter_interpreter(run: Callable[[], object], reps: int = 3) -> dict:

@@ -326,9 +326,30 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
   const int64_t wg_knob = tune_get("filter.wg", 0);
   const bool use_wg = use_jit && !jf.persistent && jf.wg_fn && !tgt &&
                       (wg_knob >= 2 || (wg_knob == 1 && nbatches * 4 >= (uint64_t)std::max(1, s->dev->cu_count) * 3));
-  const uint64_t tile_rows = use_wg    ? (uint64_t)64 * jf.wg_waves * jf.wg_rows
-                             : use_jit ? (uint64_t)64 * jf.waves * jf.rows_per_lane
-                                       : (uint64_t)kBlock * shape.v * shape.g;
+  // the staged kernel (kept rows parked in LDS) unless the last launch of
+  // this query shape kept more rows than its stage holds with a margin:
+  // tiles past the stage read their later chunks twice (at 50% selectivity
+  // 2.2 ms against the tile kernel's 1.7 on the 200M-row table)
+  // (inputs under four staged tiles take the tile kernel; knob filter.staged
+  // = 2 takes the staged kernel whatever the input)
+  bool use_staged = false;
+  if (use_jit && jf.staged && !use_wg && !jf.persistent) {
+    const double kept = jit_kept_fraction(jf.fn);
+    const uint64_t staged_tile = (uint64_t)64 * jf.staged_waves * jf.staged_rows;
+    const double share = (double)jf.stage_rows / (double)staged_tile;
+    use_staged = tune_get("filter.staged", 1) >= 2 ||
+                 (n >= 4 * staged_tile && (kept < 0.0 || kept <= 0.9 * share));
+    if (use_staged) {
+      // no-allocation calls run what an earlier call compiled, else the tile kernel
+      const int lrc = jit_filter_staged(s->dev, &jf, tgt != nullptr);
+      if (lrc == OXR_EUNSUPPORTED && tgt) use_staged = false;
+      else if (lrc != OXR_OK) return lrc;
+    }
+  }
+  const uint64_t tile_rows = use_wg       ? (uint64_t)64 * jf.wg_waves * jf.wg_rows
+                             : use_staged ? (uint64_t)64 * jf.staged_waves * jf.staged_rows
+                             : use_jit    ? (uint64_t)64 * jf.waves * jf.rows_per_lane
+                                          : (uint64_t)kBlock * shape.v * shape.g;
   std::unique_ptr<Params> guard(new Params());
   Params &P = *guard;
   std::memset(&P, 0, sizeof P);
@@ -476,7 +497,10 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
       }
       grid = std::min<uint64_t>(P.ntiles, (uint64_t)std::max(1, s->dev->cu_count) * (uint64_t)per_cu);
     }
-    if (rc == OXR_OK && !use_wg) {
+    if (rc == OXR_OK && use_staged) {
+      ProfScope prof(s->s, "filter_project");
+      rc = jit_launch(jf.staged_fn, (unsigned)grid, s->s, A, (unsigned)(64 * jf.staged_waves));
+    } else if (rc == OXR_OK && !use_wg) {
       ProfScope prof(s->s, "filter_project");
       rc = jit_launch(jf.fn, (unsigned)grid, s->s, A, (unsigned)(64 * jf.waves));
     }
@@ -507,6 +531,11 @@ int run_filter(oxr_stream *s, const oxr_batch *input, uint64_t batch_rows, const
       } else {
         std::memcpy(&err, host, 8);
         if (nbatches) std::memcpy(counts->data(), (const char *)host + 64, nbatches * 8);
+        if (use_jit && jf.staged && n > 0 && !use_wg && !jf.persistent) {
+          uint64_t kept = 0;
+          for (uint64_t c : *counts) kept += c;
+          jit_note_kept(jf.fn, (double)kept / (double)n);
+        }
       }
     }
   }

@@ -230,6 +230,190 @@ __device__ __forceinline__ void filter_tile(const JitArgs &p) {
   }
 }
 
+// Staged form of filter_tile. The tile's S = C * NCH rows per lane pass
+// through registers in NCH chunks of C (the next chunk's loads in flight
+// while the current one is ranked), and the kept rows' projections wait in
+// LDS at their rank inside the tile rather than in registers: the registers
+// hold two chunks instead of the whole tile, so more workgroups share a CU
+// and some are loading while others wait on their look-backs. After the
+// look-back the staged rows go out as contiguous runs. The stage holds CAP
+// rows; from the first chunk whose ranks would pass CAP, the chunks are
+// stored directly after the look-back, their rows read a second time (the
+// kept flags stay in a register, so the predicate is not evaluated again).
+// Output validity bits are collected at their tile rank and shifted to the
+// tile's output position when written. Row layout, ranks and results are
+// those of filter_tile<Q, C * NCH, W>.
+template <class Q, int C, int NCH, int W, int CAP>
+__device__ __forceinline__ void filter_staged(const JitArgs &p) {
+  constexpr int S = C * NCH;
+  static_assert(S >= 1 && S <= kMaxRowsPerLane, "rows per lane: one kept bit per row in a 32-bit register");
+  static_assert(W >= 1 && W <= 16, "waves per workgroup");
+  static_assert(C * W <= 64, "a chunk's (step, wave) counts: one per lane of one wave");
+  static_assert(CAP % 8 == 0, "stage arrays start 8-byte aligned");
+  static_assert(Q::kVecOuts == 0, "vector projections are copied by filter_tile");
+  constexpr int B = 64 * W;
+  constexpr int CC = C * W;
+  constexpr int NC = S * W;
+  constexpr int kTile = B * S;
+  constexpr int kWords = kTile / 32 + 2;
+  constexpr int kNV = Q::kValidOuts > 0 ? Q::kValidOuts : 1;
+  constexpr int kStage = Q::kStageBytes * CAP > 0 ? Q::kStageBytes * CAP : 8;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_run;     // kept rows of the chunks ranked so far
+  __shared__ uint32_t s_staged;  // once the tile passes CAP: ranks below this are in the stage
+  __shared__ uint32_t s_cnt[NC];
+  __shared__ uint32_t s_off[NC];
+  __shared__ uint64_t s_mask[NC];
+  __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_bits[kNV * kWords];
+  __shared__ __attribute__((aligned(16))) char s_stage[kStage];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(p.counter, 1u);
+  if (Q::kValidOuts > 0)
+    for (int i = tid; i < kNV * kWords; i += B) s_bits[i] = 0;
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t batch = tile / p.tiles_per_batch;
+  const uint64_t lt = tile - batch * p.tiles_per_batch;
+  const uint64_t b0 = batch * p.batch_rows;
+  const uint64_t bend = b0 + p.batch_rows;
+  const uint64_t end = p.n < bend ? p.n : bend;
+  const uint64_t base = b0 + lt * (uint64_t)kTile;
+  const uint32_t lim = end > base ? (uint32_t)(end - base < (uint64_t)kTile ? end - base : kTile) : 0u;
+  const uint64_t below = (1ull << lane) - 1;
+
+  const typename Q::In in_cols = Q::ins(p, base);
+  const typename Q::Out st = Q::outs_stage(s_stage, CAP);
+  typename Q::Row x[C], xn[C];
+#pragma unroll
+  for (int s = 0; s < C; s++) {
+    const uint32_t j = (uint32_t)(s * B + tid);
+    Q::load(in_cols, j, j < lim, x[s]);
+  }
+  bool err = false;
+  uint32_t selbits = 0;
+  uint32_t run = 0;  // wave 0: kept rows of the chunks ranked so far
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
+    // predicate and ballots of the chunk's C steps
+#pragma unroll
+    for (int s = 0; s < C; s++) {
+      const int step = ch * C + s;
+      const uint32_t j = (uint32_t)(step * B + tid);
+      const bool in = j < lim;
+      const bool k = Q::pred(p.lit, x[s], j, in, err) && in;
+      selbits |= (uint32_t)k << step;
+      const uint64_t mk = __ballot(k);
+      if (lane == 0) {
+        s_cnt[step * W + wave] = (uint32_t)__popcll(mk);
+        s_mask[step * W + wave] = mk;
+      }
+    }
+    // the next chunk's loads, in flight over this chunk's ranking and
+    // staging (issued after the predicate, so the loads a wait covers are
+    // one chunk's, within what the load counter tracks)
+    if (ch + 1 < NCH) {
+#pragma unroll
+      for (int s = 0; s < C; s++) {
+        const uint32_t j = (uint32_t)(((ch + 1) * C + s) * B + tid);
+        Q::load(in_cols, j, j < lim, xn[s]);
+      }
+    }
+    __syncthreads();
+    // the chunk's ranks, continuing the tile's running count
+    if (wave == 0) {
+      const uint32_t v = lane < CC ? s_cnt[ch * CC + lane] : 0u;
+      const uint32_t incl = wave_incl_scan(v, lane);
+      if (lane < CC) s_off[ch * CC + lane] = run + incl - v;
+      const uint32_t prev = run;
+      run += __shfl(incl, 63, 64);
+      if (lane == 0) {
+        s_run = run;
+        // the first chunk past the stage starts where the stage ends
+        if (prev <= (uint32_t)CAP && run > (uint32_t)CAP) s_staged = prev;
+      }
+    }
+    __syncthreads();
+    // a chunk goes to the stage when all its ranks fit (ranks only grow,
+    // so the staged chunks are a prefix of the tile)
+    if (s_run <= (uint32_t)CAP) {
+#pragma unroll
+      for (int s = 0; s < C; s++) {
+        const int step = ch * C + s;
+        if ((selbits >> step) & 1u) {
+          const uint32_t j = (uint32_t)(step * B + tid);
+          const uint32_t q = s_off[step * W + wave] + (uint32_t)__popcll(s_mask[step * W + wave] & below);
+          Q::project(p.lit, st, x[s], j, q, q, s_bits, kWords, err);
+        }
+      }
+    }
+    if (ch + 1 < NCH) {
+#pragma unroll
+      for (int s = 0; s < C; s++) x[s] = xn[s];
+    }
+  }
+  // look-back over the tile's total
+  if (wave == 0) {
+    const uint64_t pre = lookback(p.status, tile, tile - lt, run, lane);
+    if (lane == 0) {
+      s_prefix = b0 + pre;
+      if (lt == p.tiles_per_batch - 1) p.counts[batch] = pre + run;
+    }
+  }
+  __syncthreads();
+  const uint64_t prefix = s_prefix;
+  const uint32_t tot = s_run;
+  const uint32_t staged = tot <= (uint32_t)CAP ? tot : s_staged;
+  const typename Q::Out o = Q::outs(p, base, prefix);
+  // staged rows out as contiguous runs
+  Q::copy_out(o, s_stage, CAP, staged, tid, B);
+  // chunks past the stage: rows read again, kept rows stored directly
+  if (staged < tot) {
+    for (int ch = 0; ch < NCH; ch++) {
+      if (s_off[ch * CC] < staged) continue;
+#pragma unroll
+      for (int s = 0; s < C; s++) {
+        const uint32_t j = (uint32_t)((ch * C + s) * B + tid);
+        Q::load(in_cols, j, j < lim, x[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < C; s++) {
+        const int step = ch * C + s;
+        if ((selbits >> step) & 1u) {
+          const uint32_t j = (uint32_t)(step * B + tid);
+          const uint32_t q = s_off[step * W + wave] + (uint32_t)__popcll(s_mask[step * W + wave] & below);
+          Q::project(p.lit, o, x[s], j, q, q, s_bits, kWords, err);
+        }
+      }
+    }
+  }
+  if (err) atomicOr((unsigned long long *)&p.result[0], 1ull);
+  // output validity words: bit q of the tile is output bit prefix + q
+  if (Q::kValidOuts == 0 || tot == 0) return;
+  __syncthreads();
+  const uint64_t w0 = prefix >> 5;
+  const uint32_t bit0 = (uint32_t)(prefix & 31);
+  const int nw = (int)(((prefix + tot - 1) >> 5) - w0 + 1);
+  for (int k = 0; k < p.nouts; k++) {
+    const int v = p.vslot[k];
+    if (v < 0) continue;
+    uint32_t *g = p.outv[k];
+    const uint32_t *t = s_bits + v * kWords;
+    for (int w = tid; w < nw; w += B) {
+      const uint32_t hi = t[w];
+      const uint32_t bits = bit0 == 0 ? hi : (hi << bit0) | (w > 0 ? t[w - 1] >> (32 - bit0) : 0u);
+      if (w == 0 || w == nw - 1) {
+        if (bits) atomicOr(&g[w0 + w], bits);
+      } else {
+        g[w0 + w] = bits;
+      }
+    }
+  }
+}
+
 // Persistent form of filter_tile: a grid of about as many workgroups as fit
 // on the device at once, each taking tiles by ticket until none is left.
 // Two register sets of S rows: the next tile's loads are issued before the

@@ -29,9 +29,15 @@ struct JitFilter {
   hipFunction_t wg_fn = nullptr;  // filter_batch_wg: one workgroup per batch
   int wg_rows = 8;             // its rows per lane per register set
   int wg_waves = 8;            // its waves per workgroup
+  bool staged = false;         // staged_source holds filter_staged (knob filter.staged, default on)
+  hipFunction_t staged_fn = nullptr;  // loaded by jit_filter_staged
+  int staged_rows = 0;         // its rows per lane (chunk rows x chunks)
+  int staged_waves = 8;        // its waves per workgroup
+  int stage_rows = 0;          // kept rows its LDS stage holds per tile
   std::vector<int32_t> vslot;  // JitArgs::vslot per projection
   std::vector<uint64_t> lits;  // JitArgs::lit (prepared mode, knob jit.params)
   std::string source;
+  std::string staged_source;
 };
 
 // Finds or builds the filter kernel of this query shape for `dev`. `vps`
@@ -46,6 +52,13 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
 
 // Launches a generated kernel: `grid` blocks of `block` threads.
 int jit_launch(hipFunction_t fn, unsigned grid, hipStream_t s, const JitArgs &args, unsigned block = kJitBlock);
+// Loads the staged kernel of a query shape (compiling its module unless
+// cached_only, when OXR_EUNSUPPORTED means it is not compiled yet).
+int jit_filter_staged(oxr_device *dev, JitFilter *jf, bool cached_only);
+// Kept fraction of the last launch of a query shape, by its tile kernel (-1:
+// none yet), which picks between the staged and the tile kernel.
+double jit_kept_fraction(hipFunction_t fn);
+void jit_note_kept(hipFunction_t fn, double fraction);
 int jit_launch_raw(hipFunction_t fn, unsigned grid, hipStream_t s, void *args, size_t size,
                    unsigned block = kJitBlock);
 

@@ -308,10 +308,21 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
     const std::string res = g.boolean(0, pred->count);
     pred_code = g.code + "    return " + res + ";\n";
   }
+  // the staged kernel (knob filter.staged, emitted beside the tile kernel)
+  // parks kept rows' projections in LDS; vector projections keep the tile
+  // kernel's whole-wave copies
+  bool has_vec = false;
+  for (int k = 0; k < nproj; k++)
+    if (proj[k].count == 1 && proj[k].nodes[0].kind == OXR_X_COLUMN && proj[k].nodes[0].column >= 0 &&
+        proj[k].nodes[0].column < (int)cols.size() && cols[proj[k].nodes[0].column].type == OXR_F32_VECTOR)
+      has_vec = true;
+  const bool persistent = tune_get("filter.persistent", 0) > 0;
+  const bool staged = tune_get("filter.staged", 1) > 0 && !persistent && !has_vec;
+  const bool stream_stores = tune_get("jit.stream_stores", 0) > 0;
   // projections
   std::string proj_code = "    const bool in = true;\n    (void)in;\n";
-  std::string vec_code, out_fields, out_init;
-  int nvalid = 0, nvec = 0;
+  std::string vec_code, out_fields, out_init, stage_init, copy_code;
+  int nvalid = 0, nvec = 0, stage_bytes = 0;
   jf->vslot.assign(nproj, -1);
   for (int k = 0; k < nproj; k++) {
     const oxr_expr &pe = proj[k];
@@ -341,7 +352,15 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
     proj_code += "    {\n" + g.code;
     out_fields += std::string("    ") + cname(v.t) + " *o" + ks + ";\n";
     out_init += std::string("    o.o") + ks + " = (" + cname(v.t) + " *)p.out[" + ks + "] + at;\n";
-    if (tune_get("jit.stream_stores", 0) > 0)
+    // staged: the stage's array for output k starts at cap * (bytes of the
+    // outputs before it); the copy out streams when the knob says so
+    const std::string tk = cname(v.t);
+    stage_init += "    o.o" + ks + " = (" + tk + " *)(stage + (uint64_t)cap * " + std::to_string(stage_bytes) + ");\n";
+    copy_code += "    {\n      const " + tk + " *t = (const " + tk + " *)(stage + (uint64_t)cap * " +
+                 std::to_string(stage_bytes) + ");\n      for (uint32_t i = (uint32_t)tid; i < n; i += (uint32_t)nt) " +
+                 (stream_stores ? "st_stream(&o.o" + ks + "[i], t[i]);" : "o.o" + ks + "[i] = t[i];") + "\n    }\n";
+    stage_bytes += (v.t == OXR_INT32 || v.t == OXR_FLOAT32) ? 4 : 8;
+    if (stream_stores)
       proj_code += "    st_stream(&o.o" + ks + "[q], " + v.v + ");\n";
     else
       proj_code += "    o.o" + ks + "[q] = " + v.v + ";\n";
@@ -355,11 +374,32 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   // the persistent kernel (knob filter.persistent, default off: 3x slower
   // on the 200M-row query, its look-backs walk back through whole rounds of
   // tiles compacted in step) holds two tiles' rows: half the rows per lane
-  jf->persistent = tune_get("filter.persistent", 0) > 0;
+  jf->persistent = persistent;
   // waves per workgroup (knob filter.waves): a tile of 64 * W * S rows, so
   // more waves mean fewer tiles and fewer look-backs for the same registers
   jf->waves = (int)tune_get("filter.waves", 4);
   jf->rows_per_lane = pick_rows(std::max(vgprs, 1) * (jf->persistent ? 2 : 1));
+  // staged: W' waves, chunks of C rows per lane, NCH chunks per tile, a
+  // stage of CAP kept rows (knobs filter.staged_waves, filter.chunk_rows,
+  // filter.chunks, filter.stage_rows; on the 200M-row query 8 x 4 x 8 with
+  // 3072 rows measured 1.19 ms against 1.31 for the tile kernel)
+  int chunk_rows = 0, chunks = 0, cap = 0;
+  if (staged) {
+    jf->staged_waves = (int)tune_get("filter.staged_waves", 8);
+    chunk_rows = (int)tune_get("filter.chunk_rows", 4);
+    chunks = (int)tune_get("filter.chunks", 8);
+    cap = (int)(tune_get("filter.stage_rows", 3072) & ~int64_t(7));
+    if (chunk_rows * jf->staged_waves > 64)
+      return fail(OXR_EPLAN, "rocm.jit", "filter.chunk_rows x filter.staged_waves must be at most 64");
+    if (chunk_rows * chunks > kMaxRowsPerLane)
+      return fail(OXR_EPLAN, "rocm.jit", "filter.chunk_rows x filter.chunks must be at most 32");
+    if ((int64_t)cap * stage_bytes > 128 * 1024)
+      return fail(OXR_EPLAN, "rocm.jit",
+                  "filter.stage_rows x " + std::to_string(stage_bytes) + " projected bytes per row exceeds 128 KiB of LDS");
+    jf->staged = true;
+    jf->staged_rows = chunk_rows * chunks;
+    jf->stage_rows = cap;
+  }
   std::string src;
   src += "// generated filter + projection kernel\n";
   src += "#include \"filter_jit.h\"\nusing namespace oxr;\nnamespace {\nstruct Q {\n";
@@ -371,6 +411,14 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   src += "  struct Out {\n" + out_fields + "  };\n";
   src += "  __device__ static inline Out outs(const JitArgs &p, uint64_t base, uint64_t at) {\n    Out o;\n" +
          out_init + "    return o;\n  }\n";
+  if (staged) {
+    src += "  static constexpr int kStageBytes = " + std::to_string(stage_bytes) + ";\n";
+    src += "  __device__ static inline Out outs_stage(char *stage, uint32_t cap) {\n    Out o;\n" + stage_init +
+           "    (void)stage;\n    (void)cap;\n    return o;\n  }\n";
+    src += "  __device__ static inline void copy_out(const Out &o, const char *stage, uint32_t cap, uint32_t n, "
+           "int tid, int nt) {\n" +
+           copy_code + "    (void)o;\n    (void)stage;\n    (void)cap;\n    (void)n;\n    (void)tid;\n    (void)nt;\n  }\n";
+  }
   src += row;
   src += "  __device__ static inline void load(const In &i, uint32_t j, bool in, Row &x) {\n" + load +
          "  }\n";
@@ -383,6 +431,13 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
          "int lane) {\n" +
          vec_code + "  }\n";
   src += "};\n}  // namespace\n";
+  // the staged kernel is a module of its own, compiled when a launch first
+  // takes it (small inputs and high kept fractions never do)
+  if (staged)
+    jf->staged_source = src + "extern \"C\" __global__ __launch_bounds__(" + std::to_string(64 * jf->staged_waves) +
+                        ") void oxr_filter_staged(const JitArgs p) {\n  filter_staged<Q, " +
+                        std::to_string(chunk_rows) + ", " + std::to_string(chunks) + ", " +
+                        std::to_string(jf->staged_waves) + ", " + std::to_string(cap) + ">(p);\n}\n";
   src += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(64 * jf->waves) +
          ") void oxr_filter_jit(const JitArgs p) {\n";
   src += std::string("  ") + (jf->persistent ? "filter_persistent" : "filter_tile") + "<Q, " +
@@ -706,6 +761,28 @@ int jit_filter(oxr_device *dev, const std::vector<DevCol> &cols, const oxr_expr 
   return OXR_OK;
 }
 
+namespace {
+std::mutex g_kept_mu;
+std::unordered_map<hipFunction_t, double> g_kept;  // staged kernel -> kept fraction of its last launch
+}  // namespace
+
+double jit_kept_fraction(hipFunction_t fn) {
+  std::lock_guard<std::mutex> lock(g_kept_mu);
+  auto it = g_kept.find(fn);
+  return it == g_kept.end() ? -1.0 : it->second;
+}
+
+void jit_note_kept(hipFunction_t fn, double fraction) {
+  std::lock_guard<std::mutex> lock(g_kept_mu);
+  g_kept[fn] = fraction;
+}
+
+int jit_filter_staged(oxr_device *dev, JitFilter *jf, bool cached_only) {
+  if (jf->staged_fn) return OXR_OK;
+  if (!jf->staged) return OXR_EUNSUPPORTED;
+  return compile(dev, jf->staged_source, "oxr_filter_staged", &jf->staged_fn, cached_only);
+}
+
 int jit_launch_raw(hipFunction_t fn, unsigned grid, hipStream_t s, void *args, size_t size, unsigned block) {
   void *cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                  HIP_LAUNCH_PARAM_END};
@@ -741,12 +818,16 @@ extern "C" int oxr_filter_kernel_source(const oxr_column *cols, int32_t ncols, c
   for (int i = 0; i < nproj; i++) OXR_TRY(compile_value(&projection[i], dc, &prog, &vps[i]));
   JitFilter jf;
   const int rc = generate(dc, predicate, projection, nproj, vps, &jf);
+  if (rc == OXR_EPLAN) return rc;  // a knob asks for a shape the kernels do not take
   if (rc != OXR_OK)
     return fail(OXR_EUNSUPPORTED, "rocm.jit", "outside what generated kernels cover (the interpreter runs it)");
+  // the tile kernel's module, then the staged kernel's (a module of its own)
   if (compile) {
     std::vector<char> code;
     OXR_TRY(rtc_compile("gfx950", jf.source, &code));
+    if (jf.staged) OXR_TRY(rtc_compile("gfx950", jf.staged_source, &code));
   }
+  jf.source += jf.staged_source;
   if (len) *len = jf.source.size();
   if (buf && cap) {
     const size_t n = std::min(cap - 1, jf.source.size());

@@ -283,6 +283,11 @@ const KnobSpec kKnobs[] = {
     {"filter.grid", 1, 16, 0},               // persistent filter: workgroups per CU (default: what fits)
     {"filter.waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 12) | (1u << 16)},  // generated filter: waves per workgroup
     {"filter.wg", 0, 2, 0},                  // one workgroup per batch: 0 never (default), 1 many batches, 2 always
+    {"filter.staged", 0, 2, 0},              // generated filter, kept rows staged in LDS: 0 never, 1 by the shape's kept fraction (default), 2 always
+    {"filter.staged_waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 12) | (1u << 16)},  // its waves per workgroup (default 8)
+    {"filter.chunk_rows", 1, 16, 0},         // its rows per lane per chunk (default 4; times waves at most 64)
+    {"filter.chunks", 1, 32, 0},             // its chunks per tile (default 8; times chunk rows at most 32)
+    {"filter.stage_rows", 8, 8192, 0},       // its LDS stage in kept rows (default 3072; rounded down to 8)
     {"filter.wg_rows", 1, 32, 0},            // its rows per lane per register set (default 8)
     {"filter.wg_grid", 1, 8, 0},             // its workgroups per CU (default 1)
     {"filter.wg_waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 16)},  // its waves per workgroup (default 8)

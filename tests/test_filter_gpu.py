@@ -309,6 +309,57 @@ def test_nontemporal_loads_and_stores_are_bit_exact(backend, stream, oracle, row
             autotune.set_knob(k, -1)
 
 
+STAGED_KNOBS = ("filter.staged", "filter.chunk_rows", "filter.chunks", "filter.stage_rows", "filter.staged_waves")
+
+
+@pytest.mark.parametrize("chunk_rows,chunks,stage_rows,waves", [
+    (4, 8, 3072, 8),   # the default staged shape
+    (8, 3, 2048, 4),
+    (4, 6, 2048, 4),
+    (12, 2, 1024, 4),
+    (16, 2, 4096, 4),
+    (8, 3, 64, 4),     # a 64-row stage: most tiles store their later chunks directly
+    (8, 2, 8, 8),      # 8 waves, an 8-row stage
+    (1, 1, 8, 4),
+    (3, 5, 16, 16),
+])
+def test_staged_kernel_is_bit_exact(backend, stream, oracle, chunk_rows, chunks, stage_rows, waves):
+    # kept rows' projections parked in LDS, rows loaded in chunks; tiles that
+    # keep more rows than the stage holds store their later chunks directly
+    # (filter.staged = 2: the staged kernel whatever the kept fraction)
+    for k, v in zip(STAGED_KNOBS, (2, chunk_rows, chunks, stage_rows, waves)):
+        autotune.set_knob(k, v)
+    try:
+        rng = np.random.default_rng(70 + chunk_rows * chunks)
+        for n in (1, 1000, 30_001):
+            batch = datagen.random_batch(rng, n)
+            for pred, proj in [(PREDICATES[10], PROJECTIONS[2]), (PREDICATES[12], PROJECTIONS[1]),
+                               (None, PROJECTIONS[3]), (PREDICATES[0], [4, 0])]:
+                got, want = run_both(backend, stream, oracle, batch, pred, proj)
+                assert_batches_equal(got, want, names=False)
+        dev = datagen.table(backend, stream, datagen.filter_table_columns(seed=5), 1_000_003)
+        host = backend.download(stream, dev)
+        for t in (5, 50, 470):
+            pred, proj = datagen.filter_query(t)
+            for batch_rows in (65_536, 1_000_000):
+                outs = backend.filter_project_batches(stream, dev, batch_rows, pred, proj)
+                for i, o in enumerate(outs):
+                    want = oracle.filter_project(host.slice(i * batch_rows, batch_rows), pred, proj)
+                    assert_batches_equal(backend.download(stream, o), want, names=False)
+        # a zero divisor in the last row of a large batch: with every row
+        # kept, that row's chunk is past any stage and stored directly
+        n = 200_000
+        y = np.ones(n, np.int64)
+        y[-1] = 0
+        b = pa.record_batch({"x": pa.array(np.arange(n, dtype=np.int64)), "y": pa.array(y)})
+        dev = backend.upload(stream, b)
+        with pytest.raises(ExecutionError):
+            backend.filter_project(stream, dev, None, [plan.div(col(0), col(1))])
+    finally:
+        for k in STAGED_KNOBS:
+            autotune.set_knob(k, -1)
+
+
 @pytest.mark.parametrize("waves,rows", [(8, 16), (12, 16), (16, 8), (8, 24), (12, 3), (16, 1)])
 def test_every_waves_per_workgroup_is_bit_exact(backend, stream, oracle, waves, rows):
     # tiles of 64 * W * S rows; S * W above 64 (step, wave) counts scans two

@@ -93,6 +93,46 @@ def test_every_rows_per_lane_compiles(rows, persistent):
         autotune.set_knob("filter.persistent", -1)
 
 
+@pytest.mark.parametrize("chunk_rows,chunks,stage_rows", [(8, 3, 2048), (4, 6, 1536), (16, 2, 4096), (1, 1, 8)])
+def test_staged_kernel_compiles(chunk_rows, chunks, stage_rows):
+    knobs = {"filter.staged": 1, "filter.chunk_rows": chunk_rows, "filter.chunks": chunks,
+             "filter.stage_rows": stage_rows, "filter.staged_waves": 4}
+    for k, v in knobs.items():
+        autotune.set_knob(k, v)
+    try:
+        pred, proj = datagen.filter_query()
+        src = backend.filter_kernel_source(FILTER_TABLE, pred, proj, compile=True)
+        assert f"filter_staged<Q, {chunk_rows}, {chunks}, 4, {stage_rows}>" in src
+        # beside the tile kernel, which keeps its own shape
+        assert "filter_tile<Q, 4, 4>" in src
+        # three 8-byte projections per kept row in the stage
+        assert "kStageBytes = 24;" in src
+        # vector projections keep the tile kernel's whole-wave copies
+        src = backend.filter_kernel_source(RANDOM, PREDICATES[0], [4, 0], compile=True)
+        assert "filter_tile<Q, 4, 4>" in src and "filter_staged<" not in src
+    finally:
+        for k in knobs:
+            autotune.set_knob(k, -1)
+
+
+@pytest.mark.parametrize("knobs", [
+    {"filter.chunk_rows": 8, "filter.staged_waves": 16},  # 128 (step, wave) counts in a chunk
+    {"filter.chunk_rows": 16, "filter.chunks": 3},   # 48 rows per lane
+    {"filter.stage_rows": 8192},                     # 192 KiB of stage
+])
+def test_staged_shapes_outside_the_kernel_are_refused(knobs):
+    autotune.set_knob("filter.staged", 1)
+    for k, v in knobs.items():
+        autotune.set_knob(k, v)
+    try:
+        pred, proj = datagen.filter_query()
+        with pytest.raises(PlanError):
+            backend.filter_kernel_source(FILTER_TABLE, pred, proj)
+    finally:
+        for k in ("filter.staged", *knobs):
+            autotune.set_knob(k, -1)
+
+
 def test_plan_errors_match_the_interpreter():
     with pytest.raises(PlanError):
         backend.filter_kernel_source(FILTER_TABLE, Predicate.compare(9, Cmp.Eq, Lit.int64(1)), [0])

@@ -67,6 +67,8 @@ def parse():
                     help="rows of the CPU baselines' samples (filter, aggregate)")
     ap.add_argument("--e2e", action="store_true",
                     help="filter: also time the plain drop-in path over host batches (PCIe included)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="filter: the library's default launch shape, no autotuner sweep (for counter runs)")
     ap.add_argument("--filter-rows", type=int, default=0,
                     help="rows per lane of the generated tile kernel, the staged kernel off (default: autotuned)")
     ap.add_argument("--workload", choices=["filter", "vector", "aggregate", "join", "serving", "copy", "agg_batch"],
@@ -322,6 +324,8 @@ def bench_filter(args, be, stream) -> dict:
         autotune.set_knob("filter.staged", 0)
         autotune.set_knob("filter.rows", args.filter_rows)
         tuned = {"knobs": {"filter.staged": 0, "filter.rows": args.filter_rows}, "ms": None}
+    elif args.no_tune:
+        tuned = {"knobs": {}, "ms": None}
     else:
         tuned = autotune.tune_filter(lambda: run(args.b_threshold), reps=3)
     log(f"launch shape: {tuned['knobs']}")

@@ -319,6 +319,9 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
   const bool persistent = tune_get("filter.persistent", 0) > 0;
   const bool staged = tune_get("filter.staged", 1) > 0 && !persistent && !has_vec;
   const bool stream_stores = tune_get("jit.stream_stores", 0) > 0;
+  // the staged kernel's copy out of LDS writes whole lines: nontemporal
+  // (knob filter.stage_nt; 1.19 -> 1.11-1.13 ms on the 200M-row query)
+  const bool stage_nt = tune_get("filter.stage_nt", 1) > 0;
   // projections
   std::string proj_code = "    const bool in = true;\n    (void)in;\n";
   std::string vec_code, out_fields, out_init, stage_init, copy_code;
@@ -353,12 +356,12 @@ int generate(const std::vector<DevCol> &cols, const oxr_expr *pred, const oxr_ex
     out_fields += std::string("    ") + cname(v.t) + " *o" + ks + ";\n";
     out_init += std::string("    o.o") + ks + " = (" + cname(v.t) + " *)p.out[" + ks + "] + at;\n";
     // staged: the stage's array for output k starts at cap * (bytes of the
-    // outputs before it); the copy out streams when the knob says so
+    // outputs before it)
     const std::string tk = cname(v.t);
     stage_init += "    o.o" + ks + " = (" + tk + " *)(stage + (uint64_t)cap * " + std::to_string(stage_bytes) + ");\n";
     copy_code += "    {\n      const " + tk + " *t = (const " + tk + " *)(stage + (uint64_t)cap * " +
                  std::to_string(stage_bytes) + ");\n      for (uint32_t i = (uint32_t)tid; i < n; i += (uint32_t)nt) " +
-                 (stream_stores ? "st_stream(&o.o" + ks + "[i], t[i]);" : "o.o" + ks + "[i] = t[i];") + "\n    }\n";
+                 (stage_nt ? "st_stream(&o.o" + ks + "[i], t[i]);" : "o.o" + ks + "[i] = t[i];") + "\n    }\n";
     stage_bytes += (v.t == OXR_INT32 || v.t == OXR_FLOAT32) ? 4 : 8;
     if (stream_stores)
       proj_code += "    st_stream(&o.o" + ks + "[q], " + v.v + ");\n";

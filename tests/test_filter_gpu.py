@@ -309,25 +309,27 @@ def test_nontemporal_loads_and_stores_are_bit_exact(backend, stream, oracle, row
             autotune.set_knob(k, -1)
 
 
-STAGED_KNOBS = ("filter.staged", "filter.chunk_rows", "filter.chunks", "filter.stage_rows", "filter.staged_waves")
+STAGED_KNOBS = ("filter.staged", "filter.chunk_rows", "filter.chunks", "filter.stage_rows", "filter.staged_waves",
+                "filter.stage_nt")
 
 
-@pytest.mark.parametrize("chunk_rows,chunks,stage_rows,waves", [
-    (4, 8, 3072, 8),   # the default staged shape
-    (8, 3, 2048, 4),
-    (4, 6, 2048, 4),
-    (12, 2, 1024, 4),
-    (16, 2, 4096, 4),
-    (8, 3, 64, 4),     # a 64-row stage: most tiles store their later chunks directly
-    (8, 2, 8, 8),      # 8 waves, an 8-row stage
-    (1, 1, 8, 4),
-    (3, 5, 16, 16),
+@pytest.mark.parametrize("chunk_rows,chunks,stage_rows,waves,nt", [
+    (4, 8, 3072, 8, 1),   # the default staged shape
+    (4, 8, 3072, 8, 0),   # its copy out with plain stores
+    (8, 3, 2048, 4, 1),
+    (4, 6, 2048, 4, 1),
+    (12, 2, 1024, 4, 1),
+    (16, 2, 4096, 4, 1),
+    (8, 3, 64, 4, 1),     # a 64-row stage: most tiles store their later chunks directly
+    (8, 2, 8, 8, 1),      # 8 waves, an 8-row stage
+    (1, 1, 8, 4, 1),
+    (3, 5, 16, 16, 0),
 ])
-def test_staged_kernel_is_bit_exact(backend, stream, oracle, chunk_rows, chunks, stage_rows, waves):
+def test_staged_kernel_is_bit_exact(backend, stream, oracle, chunk_rows, chunks, stage_rows, waves, nt):
     # kept rows' projections parked in LDS, rows loaded in chunks; tiles that
     # keep more rows than the stage holds store their later chunks directly
     # (filter.staged = 2: the staged kernel whatever the kept fraction)
-    for k, v in zip(STAGED_KNOBS, (2, chunk_rows, chunks, stage_rows, waves)):
+    for k, v in zip(STAGED_KNOBS, (2, chunk_rows, chunks, stage_rows, waves, nt)):
         autotune.set_knob(k, v)
     try:
         rng = np.random.default_rng(70 + chunk_rows * chunks)

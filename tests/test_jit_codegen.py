@@ -105,8 +105,10 @@ def test_staged_kernel_compiles(chunk_rows, chunks, stage_rows):
         assert f"filter_staged<Q, {chunk_rows}, {chunks}, 4, {stage_rows}>" in src
         # beside the tile kernel, which keeps its own shape
         assert "filter_tile<Q, 4, 4>" in src
-        # three 8-byte projections per kept row in the stage
+        # three 8-byte projections per kept row in the stage, copied out
+        # with nontemporal stores (knob filter.stage_nt, default 1)
         assert "kStageBytes = 24;" in src
+        assert "st_stream(&o.o2[i], t[i]);" in src
         # vector projections keep the tile kernel's whole-wave copies
         src = backend.filter_kernel_source(RANDOM, PREDICATES[0], [4, 0], compile=True)
         assert "filter_tile<Q, 4, 4>" in src and "filter_staged<" not in src

@@ -400,6 +400,32 @@ __device__ __forceinline__ void store_words(uint32_t *dst, const uint32_t (&w)[R
     for (int i = 0; i < R; i++) dst[i] = w[i];
   }
 }
+typedef uint32_t u32x4_nt __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_nt __attribute__((ext_vector_type(2)));
+// A record's words read once: nontemporal loads.
+template <int R>
+__device__ __forceinline__ void load_words_nt(const uint32_t *src, uint32_t (&w)[R]) {
+  if constexpr (R % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < R / 4; i++) {
+      const u32x4_nt v = __builtin_nontemporal_load((const u32x4_nt *)src + i);
+      w[4 * i] = v.x;
+      w[4 * i + 1] = v.y;
+      w[4 * i + 2] = v.z;
+      w[4 * i + 3] = v.w;
+    }
+  } else if constexpr (R % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      const u32x2_nt v = __builtin_nontemporal_load((const u32x2_nt *)src + i);
+      w[2 * i] = v.x;
+      w[2 * i + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; i++) w[i] = __builtin_nontemporal_load(src + i);
+  }
+}
 template <int R>
 __device__ __forceinline__ void load_words(const uint32_t *src, uint32_t (&w)[R]) {
   if constexpr (R % 4 == 0) {
@@ -495,7 +521,10 @@ __device__ __forceinline__ void spill_scatter(const SpillArgs &a) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const uint64_t r = r0 + k * 256;
-      if (r < n) load_words<R>(src + r * R, rec[k]);
+      if (r < n) {
+        if constexpr (Q::kSpillNT) load_words_nt<R>(src + r * R, rec[k]);
+        else load_words<R>(src + r * R, rec[k]);
+      }
       else
 #pragma unroll
         for (int x = 0; x < R; x++) rec[k][x] = 0;
@@ -553,7 +582,10 @@ __device__ __forceinline__ void spill_fold(const SpillArgs &sa) {
 #pragma unroll
     for (int k = 0; k < kFoldRecs; k++) {
       const uint64_t r = r0 + (uint64_t)k * kFoldThreads;
-      if (r < hi) load_words<R>(sa.recs + r * R, rec[k]);
+      if (r < hi) {
+        if constexpr (Q::kSpillNT) load_words_nt<R>(sa.recs + r * R, rec[k]);
+        else load_words<R>(sa.recs + r * R, rec[k]);
+      }
       else
 #pragma unroll
         for (int x = 0; x < R; x++) rec[k][x] = 0;

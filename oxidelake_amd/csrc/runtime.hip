@@ -289,6 +289,7 @@ const KnobSpec kKnobs[] = {
     {"filter.chunks", 1, 32, 0},             // its chunks per tile (default 8; times chunk rows at most 32)
     {"filter.stage_rows", 8, 8192, 0},       // its LDS stage in kept rows (default 3072; rounded down to 8)
     {"filter.stage_nt", 0, 1, kBool},        // its copy out of the stage with nontemporal stores (default 1)
+    {"agg.spill_nt", 0, 1, kBool},           // aggregation spill: scatter and fold read records with nontemporal loads (default 1)
     {"filter.wg_rows", 1, 32, 0},            // its rows per lane per register set (default 8)
     {"filter.wg_grid", 1, 8, 0},             // its workgroups per CU (default 1)
     {"filter.wg_waves", 4, 16, (1u << 4) | (1u << 8) | (1u << 16)},  // its waves per workgroup (default 8)

@@ -135,7 +135,20 @@ def test_staged_shapes_outside_the_kernel_are_refused(knobs):
             autotune.set_knob(k, -1)
 
 
-def test_plan_errors_match_the_interpreter():
+def test_spill_records_read_nontemporal_compiles():
+    # knob agg.spill_nt (default 1): the scatter and the fold read the
+    # spill's records with nontemporal loads (4-word records, uint4 loads)
+    c3 = [(plan.INT64, False), (plan.INT32, False), (plan.FLOAT64, False)]
+    assert "kSpillNT = true;" in backend.aggregate_kernel_source(c3, datagen.star_query_spec(), probe_key=0)
+    autotune.set_knob("agg.spill_nt", 0)
+    try:
+        src = backend.aggregate_kernel_source(c3, datagen.star_query_spec(), probe_key=0, compile=True)
+        assert "kSpillNT = false;" in src
+    finally:
+        autotune.set_knob("agg.spill_nt", -1)
+
+
+
     with pytest.raises(PlanError):
         backend.filter_kernel_source(FILTER_TABLE, Predicate.compare(9, Cmp.Eq, Lit.int64(1)), [0])
     with pytest.raises(Unsupported):

@@ -303,9 +303,12 @@ int oxr_filter_project_noalloc(oxr_stream *s, const oxr_batch *input, const oxr_
  * kernels are fixed per operator, cuda/ops.rs:143-373). Only the type, dim
  * and validity (NULL or not) of each of the `ncols` columns are read; no
  * device is needed. With `compile` set it is also compiled for gfx950.
+ * The text is the tile kernel's module followed, for shapes the staged
+ * kernel takes (knob filter.staged), by the staged kernel's module.
  * Writes the NUL-terminated source to buf (truncated to cap-1 bytes) and its
  * full length to *len. OXR_EUNSUPPORTED: the query runs on the interpreter
- * kernel instead of a generated one. */
+ * kernel instead of a generated one; OXR_EPLAN: a knob asks for a staged
+ * shape the kernel does not take. */
 int oxr_filter_kernel_source(const oxr_column *cols, int32_t ncols, const oxr_expr *predicate,
                              const oxr_expr *projection, int32_t nproj, int32_t compile,
                              char *buf, size_t cap, size_t *len);

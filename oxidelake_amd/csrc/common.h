@@ -148,6 +148,8 @@ int fill_zero(oxr_stream *s, void *p, size_t bytes);
 // Tuning knobs (oxr_tune_set): launch shapes and table sizes chosen by the
 // autotuner; `dflt` when a knob was never set.
 int64_t tune_get(const char *knob, int64_t dflt);
+// Bumped by every oxr_tune_set: what was derived from the knobs before is stale.
+uint64_t tune_epoch();
 
 // Kernel timing (oxr_profile_*): brackets one launch with HIP events when
 // profiling is enabled; a no-op otherwise.

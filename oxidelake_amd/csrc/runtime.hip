@@ -346,6 +346,9 @@ int tune_check(const char *knob, int64_t value) {
   return fail(OXR_EPLAN, "rocm.tune", std::string("unknown knob ") + knob);
 }
 
+std::atomic<uint64_t> g_tune_epoch{0};
+uint64_t tune_epoch() { return g_tune_epoch.load(); }
+
 int64_t tune_get(const char *knob, int64_t dflt) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   auto it = g_tune.find(knob);
@@ -1103,6 +1106,7 @@ int oxr_tune_set(const char *knob, int64_t value) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   if (value < 0) g_tune.erase(knob);
   else g_tune[knob] = value;
+  g_tune_epoch.fetch_add(1);
   return OXR_OK;
 }
 
